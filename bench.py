@@ -1,0 +1,206 @@
+#!/usr/bin/env python3
+"""Benchmark: exact touch-configuration search on one MI355X.
+
+Workload (one "step"): 65,536 segments that start at exact-verified free
+states of the torus scene (env torus R=0.3, r=0.08, 2,048 triangles; robot
+torus R=0.1, r=0.02, 1,024 triangles), head in uniform random SE(3)
+directions for a length ~ U[0.05, 0.5], are checked at discrete steps of
+1/1024 with early exit at the first contact (transitStateToWithContact), and
+refine each contact with 20 bisection levels.  A query is one state validity
+decision made under the reference's semantics: every evaluated discrete step
+plus every bisection probe.  Inputs sit in HBM before the timed region.
+
+Prints one JSON line (metric, value in queries/s, roofline of the segment
+kernel, CPU-oracle baseline on a bounded sample).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "exact-verified SE(3) validity queries/sec (0 mismatches) + free samples/sec w/ surrogate"
+N_SEG = 65536
+VD = 1.0 / 1024
+LEVELS = 20
+# Straight-line binary64 operation counts of the oracle's node-pair box test
+# and 17-axis triangle test (oracle/oracle.c obb_disjoint / or_tri_intersect;
+# adds, subtracts, multiplies and square roots).
+C_BV = 286
+C_TRI = 753
+PEAK_FP32_TFLOPS = 157.3
+
+
+def build_workload(uw, W, n_seg, seed=0):
+    starts = []
+    have = 0
+    k = 0
+    while have < n_seg:
+        qs = W.random_states(2 * n_seg, seed=1000 + seed + k)
+        v = uw.is_valid_states(qs)
+        starts.append(qs[v])
+        have += int(v.sum())
+        k += 1
+    starts = np.concatenate(starts)[:n_seg]
+    ends = W.segment_ends(starts, seed=2000 + seed)
+    return np.ascontiguousarray(starts), np.ascontiguousarray(ends)
+
+
+def cpu_baseline(ow, starts, ends, gpu_res, target_s):
+    """Time the oracle on a bounded sample of the same segments (all threads
+    OMP allows) and check the GPU's answers for that sample."""
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    n = 256
+    t0 = time.perf_counter()
+    o = ow.touch_batch(starts[:n], ends[:n], VD, LEVELS, nthreads=threads)
+    dt = time.perf_counter() - t0
+    if dt < target_s:
+        n = int(min(len(starts), max(n, n * target_s / max(dt, 1e-3))))
+        t0 = time.perf_counter()
+        o = ow.touch_batch(starts[:n], ends[:n], VD, LEVELS, nthreads=threads)
+        dt = time.perf_counter() - t0
+    q = int(o["queries"].sum())
+    mism = 0
+    for key, gk in (("done", "done"), ("lo", "lo"), ("hi", "hi"), ("queries", "steps")):
+        a = o[key]
+        b = gpu_res[gk][:n]
+        if key == "done":
+            a = a.astype(bool)
+        mism += int((a != b).sum())
+    return dict(value=q / dt, unit="queries/s", cores=threads, kind="port",
+                sample=f"first {n} of the {len(starts)} touch segments ({q} queries, {dt:.1f} s)"), mism, n
+
+
+def oracle_flops_per_query(ow, starts, ends, n):
+    """Mean algorithmic flops per query from the oracle's traversal counters
+    over the states the first n segments evaluate."""
+    from oracle import oracle as O
+    qs = []
+    for i in range(n):
+        lf, st, done, lt, t, ns = ow.transit(starts[i], ends[i], VD)
+        dist = O.distance(starts[i], ends[i])
+        d = VD
+        for k in range(ns):
+            qs.append(O.interpolate(starts[i], ends[i], d * (1.0 / dist)))
+            if d < dist:
+                d = min(d + VD, dist)
+            else:
+                d += VD
+    qs = np.array(qs)
+    _, nbv, ntri = ow.valid_states(qs, counters=True)
+    return float(C_BV * nbv.mean() + C_TRI * ntri.mean()), float(nbv.mean()), float(ntri.mean())
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--segments", type=int, default=N_SEG)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(each query is independent; see DESIGN.md for how the path shards).", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from puzzletunneldiscovery_amd import pyosr, workloads as W
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.set_device(0)
+    env, rob = W.torus_scene(0)
+    uw = pyosr.create_unit_world_from_arrays(env, rob)
+    starts, ends = build_workload(uw, W, args.segments)
+    n = starts.shape[0]
+    dev = torch.device("cuda:0")
+    d_from = torch.from_numpy(starts).to(dev)
+    d_to = torch.from_numpy(ends).to(dev)
+    d_done = torch.zeros(n, dtype=torch.uint8, device=dev)
+    d_lt = torch.zeros(n, dtype=torch.float64, device=dev)
+    d_t = torch.zeros(n, dtype=torch.float64, device=dev)
+    d_lo = torch.zeros(n, dtype=torch.float64, device=dev)
+    d_hi = torch.zeros(n, dtype=torch.float64, device=dev)
+    d_steps = torch.zeros(n, dtype=torch.int32, device=dev)
+    # a dedicated (default-priority) stream so the HIP events bracket exactly
+    # the engine's kernels
+    stream = torch.cuda.Stream()
+    torch.cuda.set_stream(stream)
+    sp = stream.cuda_stream
+
+    def step():
+        uw.touch_search_device(d_from.data_ptr(), d_to.data_ptr(), n, VD, LEVELS, d_done.data_ptr(),
+                               d_lt.data_ptr(), d_t.data_ptr(), d_lo.data_ptr(), d_hi.data_ptr(),
+                               d_steps.data_ptr(), sp)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    uw.check_device()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        step()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    uw.check_device()
+    wall = t1 - t0
+    kern_ms = ev0.elapsed_time(ev1) / args.steps
+    res = dict(done=d_done.cpu().numpy().astype(bool), lo=d_lo.cpu().numpy(), hi=d_hi.cpu().numpy(),
+               steps=d_steps.cpu().numpy())
+    queries = int(res["steps"].astype(np.int64).sum())
+    value = queries * args.steps / wall
+
+    cpu = None
+    mism = None
+    flops_q = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        O.set_libm(False)
+        ow = O.OracleWorld().setup(*env, *rob)
+        cpu, mism, n_checked = cpu_baseline(ow, starts, ends, res, args.cpu_seconds)
+        flops_q, nbv, ntri = oracle_flops_per_query(ow, starts, ends, min(256, n))
+    roof = None
+    if flops_q is not None:
+        achieved = queries * flops_q / (kern_ms * 1e-3) / 1e12
+        roof = dict(bound="fp32-valu", achieved=round(achieved, 3), peak=PEAK_FP32_TFLOPS, unit="TFLOP/s",
+                    frac=round(achieved / PEAK_FP32_TFLOPS, 5), traffic=None,
+                    flops_per_query=round(flops_q, 1), kernel="k_segments<touch>",
+                    kernel_ms=round(kern_ms, 4))
+    line = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 (exact leaf tests) + f32 (conservative culling)",
+        "data": "synthetic (seeded torus scene, Shoemake states)",
+        "config": {"workload": "touch-config search: torus env 2,048 tris / robot 1,024 tris, "
+                               f"{n} segments, step 1/1024, {LEVELS} bisection levels",
+                   "segments": n, "queries_per_step": queries,
+                   "contacts": int((~res["done"]).sum())},
+        "mismatches": mism,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,165 @@
+/*
+ * se3cq.h -- C ABI of the MI355X (gfx950) SE(3) collision-query engine.
+ *
+ * The engine replaces the configuration-space validity path of
+ * PuzzleTunnelDiscovery's pyosr.UnitWorld (lib/pyosr/module.cc:127-250 ->
+ * lib/osr/unit_world.cc -> lib/osr/cdmodel.cc -> FCL).  Each entry point
+ * names the reference member it stands in for.  Plain pointers and sizes only;
+ * states are binary64 rows [tx, ty, tz, qw, qx, qy, qz] (lib/osr/osr_state.h:20-33).
+ *
+ * Conventions
+ *   - Every function returns 0 on success and a negative status on failure;
+ *     se3cq_last_error() then returns a message for the calling thread.
+ *     Nothing throws across the boundary.
+ *   - "_host" entry points take host buffers and block until results are
+ *     back in them.  "_device" entry points take device pointers (HBM) and
+ *     enqueue on `stream` (a hipStream_t; NULL = the world's own stream)
+ *     without synchronising; call se3cq_check(w) after synchronising to
+ *     surface device-side errors.
+ *   - A world is immutable after se3cq_angle_model; calls on one world are
+ *     serialised internally, calls on different worlds may run concurrently.
+ *   - geo: 0 = environment (UnitWorld::GEO_ENV), 1 = robot (GEO_ROB).
+ */
+#ifndef SE3CQ_H
+#define SE3CQ_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct se3cq_world se3cq_world;
+
+#define SE3CQ_OK 0
+#define SE3CQ_EINVAL (-1)
+#define SE3CQ_ESTATE (-2)
+#define SE3CQ_EIO (-3)
+#define SE3CQ_EDEVICE (-4)
+#define SE3CQ_ENOMEM (-5)
+
+const char* se3cq_last_error(void);
+const char* se3cq_version(void);
+
+/* ---- world lifecycle ------------------------------------------------- */
+/* UnitWorld::UnitWorld (lib/osr/unit_world.cc:65-70): identity perturbation. */
+int se3cq_world_create(se3cq_world** out);
+int se3cq_world_destroy(se3cq_world* w);
+/* UnitWorld::copyFrom (unit_world.cc:76-92), pyosr "setupFrom". */
+int se3cq_world_copy_from(se3cq_world* dst, const se3cq_world* src);
+
+/* ---- geometry ingest and unit-world normalisation ---------------------- */
+/* UnitWorld::loadModelFromFile / loadRobotFromFile (unit_world.cc:94-110) ->
+ * Scene::load (lib/osr/scene.cc:78-156).  OBJ files only. */
+int se3cq_load_file(se3cq_world* w, int geo, const char* path);
+/* Same, from in-memory binary32 vertices [nv,3] and triangle indices [nf,3]. */
+int se3cq_load_mesh(se3cq_world* w, int geo, const float* V, int64_t nv, const uint32_t* F, int64_t nf);
+/* Scene::getCenter / overrideCenter (scene.h:63-65), UnitWorld::enforceRobotCenter (unit_world.cc:113-117). */
+int se3cq_get_center(const se3cq_world* w, int geo, float* c3);
+int se3cq_get_ompl_center(const se3cq_world* w, int geo, float* c3);
+int se3cq_enforce_robot_center(se3cq_world* w, const double* c3);
+/* UnitWorld::scaleToUnit (unit_world.cc:123-131). */
+int se3cq_scale_to_unit(se3cq_world* w);
+/* UnitWorld::angleModel (unit_world.cc:137-161): builds both BVHs and uploads
+ * them to the current HIP device.  Both angles must be 0. */
+int se3cq_angle_model(se3cq_world* w, float latitude, float longitude);
+/* scene_scale property (module.cc:248). */
+int se3cq_get_scene_scale(const se3cq_world* w, float* scale);
+/* scene_matrix / robot_matrix properties (module.cc:249-250), row-major 4x4. */
+int se3cq_get_matrix(const se3cq_world* w, int geo, double* m16);
+/* Mesh sizes after ingest and the unit-world (collision) vertices,
+ * binary64 [nv,3] (CDModel::addVF, lib/osr/cdmodel.cc:120-160). */
+int se3cq_get_mesh_size(const se3cq_world* w, int geo, int64_t* nv, int64_t* nf);
+int se3cq_get_mesh(const se3cq_world* w, int geo, float* V, uint32_t* F);
+int se3cq_get_cd_vertices(const se3cq_world* w, int geo, double* V);
+/* Depth of the engine's BVH over one model (diagnostics). */
+int se3cq_get_bvh_info(const se3cq_world* w, int geo, int32_t* depth, int64_t* nnodes);
+/* set_perturbation / perturbation (module.cc:135-136, unit_world.cc:164-175). */
+int se3cq_set_perturbation(se3cq_world* w, const double* q7);
+int se3cq_get_perturbation(const se3cq_world* w, double* q7);
+/* recommended_cres property (module.cc:247). */
+int se3cq_set_recommended_cres(se3cq_world* w, double cres);
+int se3cq_get_recommended_cres(const se3cq_world* w, double* cres);
+
+/* ---- the hot path ------------------------------------------------------ */
+/* UnitWorld::isValid (unit_world.cc:215-247), batched: out[i] = 1 if state i
+ * is collision free.  Without both models loaded every state is valid
+ * (unit_world.cc:218-219). */
+int se3cq_valid_states_host(se3cq_world* w, const double* q, int64_t n, uint8_t* out);
+int se3cq_valid_states_device(se3cq_world* w, const double* d_q, int64_t n, uint8_t* d_out, void* stream);
+
+/* UnitWorld::transitStateToWithContact (unit_world.cc:405-474), batched over
+ * n (from, to) rows with one verify_delta.  Outputs per row: last_free [7],
+ * first colliding state [7], done (1 = free), last_tau and tau in binary64
+ * (the reference narrows both to binary32 in its tuple), steps = isValid
+ * calls made.  last_free/state/steps may be NULL. */
+int se3cq_transit_host(se3cq_world* w, const double* from, const double* to, int64_t n, double verify_delta,
+                       double* last_free, double* state, uint8_t* done, double* last_tau, double* tau,
+                       int32_t* steps);
+int se3cq_transit_device(se3cq_world* w, const double* d_from, const double* d_to, int64_t n,
+                         double verify_delta, double* d_last_free, double* d_state, uint8_t* d_done,
+                         double* d_last_tau, double* d_tau, int32_t* d_steps, void* stream);
+
+/* UnitWorld::isValidTransition (unit_world.cc:477-486), batched. */
+int se3cq_valid_transitions_host(se3cq_world* w, const double* from, const double* to, int64_t n,
+                                 double initial_verify_delta, uint8_t* out);
+
+/* Touch-configuration search: transitStateToWithContact followed, for rows
+ * that hit an obstacle, by `levels` bisection halvings of [last_tau, tau]
+ * (mid = 0.5*(lo+hi); lo = mid if interpolate(from,to,mid) is valid, else
+ * hi = mid).  No reference counterpart: it extends the touch search of
+ * src/GP/pipeline/touchq_util.py:8-27.  lo/hi: final bracket (free rows get
+ * lo = last_tau, hi = 1).  steps counts isValid calls plus probes. */
+int se3cq_touch_search_host(se3cq_world* w, const double* from, const double* to, int64_t n,
+                            double verify_delta, int levels, uint8_t* done, double* last_tau, double* tau,
+                            double* lo, double* hi, int32_t* steps);
+int se3cq_touch_search_device(se3cq_world* w, const double* d_from, const double* d_to, int64_t n,
+                              double verify_delta, int levels, uint8_t* d_done, double* d_last_tau,
+                              double* d_tau, double* d_lo, double* d_hi, int32_t* d_steps, void* stream);
+
+/* Bisection alone on given brackets (one per row). */
+int se3cq_bisect_contact_host(se3cq_world* w, const double* from, const double* to, int64_t n,
+                              const double* lo_in, const double* hi_in, int levels, double* lo, double* hi);
+
+/* UnitWorld::calculateVisibilityMatrix (unit_world.cc:740-762): int8 n x n,
+ * symmetric, diagonal 0 (the reference leaves it unwritten). */
+int se3cq_visibility_matrix_host(se3cq_world* w, const double* qs, int64_t n, int qs_are_unit,
+                                 double verify_magnitude, int8_t* out);
+/* calculateVisibilityMatrix2 (unit_world.cc:764-797): int8 m x n. */
+int se3cq_visibility_matrix2_host(se3cq_world* w, const double* qs0, int64_t m, int qs0_are_unit,
+                                  const double* qs1, int64_t n, int qs1_are_unit, double verify_magnitude,
+                                  int8_t* out);
+/* calculateVisibilityPair (unit_world.cc:799-840): row i of qs0 against row i
+ * of qs1; both must have n rows. */
+int se3cq_visibility_pair_host(se3cq_world* w, const double* qs0, int qs0_are_unit, const double* qs1,
+                               int qs1_are_unit, int64_t n, double verify_magnitude, int8_t* out);
+/* Bit-packed symmetric visibility matrix on the device: bit (i*n+j) of the
+ * uint32 words d_bits (ceil(n*n/32) words, zeroed by the call) is 1 when the
+ * segment i -> j (i < j, mirrored) is free.  States must be unit states. */
+int se3cq_visibility_bits_device(se3cq_world* w, const double* d_qs, int64_t n, double verify_magnitude,
+                                 uint32_t* d_bits, int32_t* d_steps, void* stream);
+
+/* Surface device-side failures of earlier _device calls (after sync). */
+int se3cq_check(se3cq_world* w);
+
+/* ---- state arithmetic (host) ------------------------------------------ */
+/* translateToUnitState (unit_world.cc:526-539) / translateFromUnitState (541-555). */
+int se3cq_translate_to_unit(const se3cq_world* w, const double* q, int64_t n, double* out);
+int se3cq_translate_from_unit(const se3cq_world* w, const double* q, int64_t n, double* out);
+/* distance (lib/osr/osr_state.cc:130-154) and multi_distance (156-166). */
+int se3cq_distance(const double* a7, const double* b7, double* out);
+int se3cq_multi_distance(const double* origin7, const double* targets, int64_t n, double* out);
+/* interpolate (osr_state.cc:27-44). */
+int se3cq_interpolate(const double* a7, const double* b7, double tau, double* out7);
+/* apply (osr_state.cc:243-259). */
+int se3cq_apply(const double* from7, const double* tr3, const double* aa3, double* out7);
+/* decompose_2 (osr_state.cc:94-99): translation and rotation matrix (row-major). */
+int se3cq_decompose_2(const double* q7, double* t3, double* R9);
+/* compose_from_angleaxis (osr_state.cc:121-128). */
+int se3cq_compose_from_angleaxis(const double* t3, double angle, const double* axis3, double* out7);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,1233 @@
+/*
+ * oracle.c -- CPU oracle for the SE(3) collision-query path.
+ *
+ * TEST INFRASTRUCTURE ONLY (see oracle.h).  The shipped library never links
+ * or calls this file.  Build: oracle/Makefile (gcc -O2 -ffp-contract=off
+ * -fopenmp), output oracle/_build/liboracle.so.
+ *
+ * The transcendental functions come from the library's se3_math.h (portable,
+ * bit-identical on CPU and GPU) unless or_set_libm(1) selects glibc; every
+ * other piece of arithmetic below is restated here independently of the
+ * library's own sources.
+ */
+#define _GNU_SOURCE
+#include "oracle.h"
+#include "../puzzletunneldiscovery_amd/csrc/se3_math.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ------------------------------------------------------------------ */
+/* libm selection                                                     */
+/* ------------------------------------------------------------------ */
+static int g_glibc = 0;
+void or_set_libm(int use_glibc) { g_glibc = use_glibc ? 1 : 0; }
+int or_get_libm(void) { return g_glibc; }
+static double o_sin(double x) { return g_glibc ? sin(x) : se3_sin(x); }
+static double o_cos(double x) { return g_glibc ? cos(x) : se3_cos(x); }
+static double o_acos(double x) { return g_glibc ? acos(x) : se3_acos(x); }
+static double o_atan2(double y, double x) { return g_glibc ? atan2(y, x) : se3_atan2(y, x); }
+
+/* ------------------------------------------------------------------ */
+/* Data structures                                                    */
+/* ------------------------------------------------------------------ */
+typedef struct {
+	float* V;
+	int64_t nv;
+	uint32_t* F;
+	int64_t nf;
+	float center[3];   /* Scene::center_ (OMPL centre unless overridden) */
+	float bbox[6];     /* left, right, bottom, top, back, front (origin-seeded) */
+	int loaded;
+} OrMesh;
+
+typedef struct {
+	double ax[9]; /* row-major 3x3; column k is box axis k */
+	double c[3];
+	double e[3];
+	int first;    /* children at first, first+1; -1 for a leaf */
+	int tri;      /* leaf triangle index */
+} ONode;
+
+typedef struct {
+	double* V; /* unit-world vertices, nv*3 */
+	int64_t nv;
+	uint32_t* F;
+	int64_t nf;
+	ONode* nodes;
+	int64_t nn;
+	int depth;
+} OrModel;
+
+struct OrWorld {
+	OrMesh mesh[2];
+	float scale;
+	float xf[2][16]; /* column-major binary32 4x4 (glm layout): xf[col*4+row] */
+	OrModel cd[2];
+	int has_cd[2];
+	double pert[7];
+	double pR[9], pt[3];
+	double calib[16]; /* row-major binary64 copy of the env calibration matrix */
+};
+
+/* ------------------------------------------------------------------ */
+/* Small linear algebra (left-to-right association throughout).        */
+/* ------------------------------------------------------------------ */
+static inline double dot3(const double* a, const double* b)
+{
+	return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2];
+}
+static inline void cross3(const double* a, const double* b, double* r)
+{
+	r[0] = a[1] * b[2] - a[2] * b[1];
+	r[1] = a[2] * b[0] - a[0] * b[2];
+	r[2] = a[0] * b[1] - a[1] * b[0];
+}
+static inline double smax(double a, double b) { return (a < b) ? b : a; }
+static inline double smin(double a, double b) { return (b < a) ? b : a; }
+static inline void sub3(const double* a, const double* b, double* r)
+{
+	r[0] = a[0] - b[0];
+	r[1] = a[1] - b[1];
+	r[2] = a[2] - b[2];
+}
+
+/* ------------------------------------------------------------------ */
+/* Quaternion helpers, Eigen 3.3 (x86-64 SSE2 build) semantics.        */
+/* q = (w, x, y, z).                                                    */
+/* ------------------------------------------------------------------ */
+static void q_normalize(double* q)
+{
+	/* squaredNorm over coeffs (x,y,z,w): packets (x,y),(z,w) -> (x2+z2)+(y2+w2) */
+	double n2 = (q[1] * q[1] + q[3] * q[3]) + (q[2] * q[2] + q[0] * q[0]);
+	if (n2 > 0.0) {
+		double n = sqrt(n2);
+		for (int i = 0; i < 4; i++)
+			q[i] = q[i] / n;
+	}
+}
+static double q_dot(const double* a, const double* b)
+{
+	return (a[1] * b[1] + a[3] * b[3]) + (a[2] * b[2] + a[0] * b[0]);
+}
+/* Hamilton product, grouped as Eigen's SSE2 binary64 quat_product. */
+static void q_mul(const double* a, const double* b, double* r)
+{
+	double wa = a[0], xa = a[1], ya = a[2], za = a[3];
+	double wb = b[0], xb = b[1], yb = b[2], zb = b[3];
+	double t1x = wa * xb + ya * zb, t1y = wa * yb + ya * wb;
+	double t2x = za * yb - xa * wb, t2y = za * xb - xa * zb;
+	double t1z = wa * zb - ya * xb, t1w = wa * wb - ya * yb;
+	double t2z = za * wb + xa * yb, t2w = za * zb + xa * xb;
+	r[1] = t1x - t2x;
+	r[2] = t1y + t2y;
+	r[3] = t1z + t2z;
+	r[0] = t1w - t2w;
+}
+/* Quaternion::toRotationMatrix, on the quaternion as given. Row-major. */
+static void q_to_R(const double* q, double* R)
+{
+	double w = q[0], x = q[1], y = q[2], z = q[3];
+	double tx = 2.0 * x, ty = 2.0 * y, tz = 2.0 * z;
+	double twx = tx * w, twy = ty * w, twz = tz * w;
+	double txx = tx * x, txy = ty * x, txz = tz * x;
+	double tyy = ty * y, tyz = tz * y, tzz = tz * z;
+	R[0] = 1.0 - (tyy + tzz);
+	R[1] = txy - twz;
+	R[2] = txz + twy;
+	R[3] = txy + twz;
+	R[4] = 1.0 - (txx + tzz);
+	R[5] = tyz - twx;
+	R[6] = txz - twy;
+	R[7] = tyz + twx;
+	R[8] = 1.0 - (txx + tyy);
+}
+
+/* translate_state_to_transform (lib/osr/osr_state.cc:16-25). */
+void or_state_to_transform(const double* q7, double* R9, double* t3)
+{
+	double q[4] = {q7[3], q7[4], q7[5], q7[6]};
+	q_to_R(q, R9);
+	t3[0] = q7[0];
+	t3[1] = q7[1];
+	t3[2] = q7[2];
+}
+
+/* distance (lib/osr/osr_state.cc:130-154). */
+double or_distance(const double* a, const double* b)
+{
+	double qa[4] = {a[3], a[4], a[5], a[6]}, qb[4] = {b[3], b[4], b[5], b[6]};
+	q_normalize(qa);
+	q_normalize(qb);
+	double d[3] = {a[0] - b[0], a[1] - b[1], a[2] - b[2]};
+	double trdist = sqrt(dot3(d, d));
+	/* Eigen angularDistance: d = qa * qb.conjugate(); 2*atan2(|d.vec|, |d.w|) */
+	double cb[4] = {qb[0], -qb[1], -qb[2], -qb[3]};
+	double r[4];
+	q_mul(qa, cb, r);
+	double v[3] = {r[1], r[2], r[3]};
+	double rotdist = 2.0 * o_atan2(sqrt(dot3(v, v)), fabs(r[0]));
+	return trdist + rotdist;
+}
+
+/* interpolate (lib/osr/osr_state.cc:27-44) with Eigen's slerp. */
+void or_interpolate(const double* a, const double* b, double tau, double* out)
+{
+	double omt = 1.0 - tau;
+	for (int i = 0; i < 3; i++)
+		out[i] = a[i] * omt + b[i] * tau;
+	double q0[4] = {a[3], a[4], a[5], a[6]}, q1[4] = {b[3], b[4], b[5], b[6]};
+	q_normalize(q0);
+	q_normalize(q1);
+	const double one = 1.0 - 0x1p-52;
+	double d = q_dot(q0, q1);
+	double absd = fabs(d);
+	double s0, s1;
+	if (absd >= one) {
+		s0 = 1.0 - tau;
+		s1 = tau;
+	} else {
+		double theta = o_acos(absd);
+		double st = o_sin(theta);
+		s0 = o_sin((1.0 - tau) * theta) / st;
+		s1 = o_sin(tau * theta) / st;
+	}
+	if (d < 0.0)
+		s1 = -s1;
+	for (int i = 0; i < 4; i++)
+		out[3 + i] = s0 * q0[i] + s1 * q1[i];
+}
+
+/* apply (lib/osr/osr_state.cc:243-259). */
+void or_apply(const double* from, const double* tr, const double* aa, double* out)
+{
+	double rot[4] = {from[3], from[4], from[5], from[6]};
+	q_normalize(rot);
+	double t[3] = {from[0] + tr[0], from[1] + tr[1], from[2] + tr[2]};
+	double angle = sqrt(dot3(aa, aa));
+	double axis[3];
+	if (angle == 0.0) {
+		axis[0] = 1.0;
+		axis[1] = 0.0;
+		axis[2] = 0.0;
+	} else {
+		double n = sqrt(dot3(aa, aa));
+		axis[0] = aa[0] / n;
+		axis[1] = aa[1] / n;
+		axis[2] = aa[2] / n;
+	}
+	/* Quaternion(AngleAxis): w = cos(a/2), vec = sin(a/2) * axis */
+	double ha = 0.5 * angle;
+	double sh = o_sin(ha);
+	double qa[4] = {o_cos(ha), sh * axis[0], sh * axis[1], sh * axis[2]};
+	double r[4];
+	q_mul(qa, rot, r);
+	q_normalize(r);
+	out[0] = t[0];
+	out[1] = t[1];
+	out[2] = t[2];
+	for (int i = 0; i < 4; i++)
+		out[3 + i] = r[i];
+}
+
+/* ------------------------------------------------------------------ */
+/* Mesh ingest                                                        */
+/* ------------------------------------------------------------------ */
+static void mesh_free(OrMesh* m)
+{
+	free(m->V);
+	free(m->F);
+	memset(m, 0, sizeof(*m));
+}
+
+/* Scene::load bookkeeping: origin-seeded bbox (bbox.h:22,41-51) and the
+ * binary32 mean of vertices (scene.cc:40-50, assimp 4.1 aiVector3D /=). */
+static void mesh_finish(OrMesh* m)
+{
+	float l = 0, r = 0, b = 0, t = 0, k = 0, f = 0;
+	float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+	for (int64_t i = 0; i < m->nv; i++) {
+		float x = m->V[3 * i], y = m->V[3 * i + 1], z = m->V[3 * i + 2];
+		r = (x > r) ? x : r;  /* std::max(v.x, right) */
+		l = (x < l) ? x : l;
+		t = (y > t) ? y : t;
+		b = (y < b) ? y : b;
+		f = (z > f) ? z : f;
+		k = (z < k) ? z : k;
+		sx += x;
+		sy += y;
+		sz += z;
+	}
+	m->bbox[0] = l;
+	m->bbox[1] = r;
+	m->bbox[2] = b;
+	m->bbox[3] = t;
+	m->bbox[4] = k;
+	m->bbox[5] = f;
+	float n = (float)m->nv;
+	m->center[0] = sx / n;
+	m->center[1] = sy / n;
+	m->center[2] = sz / n;
+	m->loaded = 1;
+}
+
+int or_load_arrays(OrWorld* w, int geo, const float* V, int64_t nv, const uint32_t* F, int64_t nf)
+{
+	if (geo < 0 || geo > 1 || nv <= 0 || nf <= 0)
+		return -1;
+	for (int64_t i = 0; i < 3 * nf; i++)
+		if (F[i] >= (uint64_t)nv)
+			return -2;
+	OrMesh* m = &w->mesh[geo];
+	mesh_free(m);
+	m->V = (float*)malloc(sizeof(float) * 3 * nv);
+	m->F = (uint32_t*)malloc(sizeof(uint32_t) * 3 * nf);
+	memcpy(m->V, V, sizeof(float) * 3 * nv);
+	memcpy(m->F, F, sizeof(uint32_t) * 3 * nf);
+	m->nv = nv;
+	m->nf = nf;
+	mesh_finish(m);
+	w->has_cd[geo] = 0;
+	return 0;
+}
+
+/* OBJ ingest, restating what assimp's importer with Triangulate |
+ * JoinIdenticalVertices hands Mesh::Mesh (lib/osr/scene.cc:99-105,
+ * lib/osr/mesh.cc:24-63): one vertex per distinct (position, uv, normal)
+ * value tuple in first-use order over the face corners, polygons split into
+ * fans.  Coordinates are parsed with strtof (correctly rounded). */
+typedef struct { float a[8]; } Corner; /* pos3, uv2, nrm3 */
+
+static uint64_t corner_hash(const Corner* c)
+{
+	uint64_t h = 1469598103934665603ULL;
+	const unsigned char* p = (const unsigned char*)c->a;
+	for (size_t i = 0; i < sizeof(c->a); i++) {
+		h ^= p[i];
+		h *= 1099511628211ULL;
+	}
+	return h;
+}
+
+int or_load_obj(OrWorld* w, int geo, const char* path)
+{
+	FILE* fp = fopen(path, "r");
+	if (!fp)
+		return -1;
+	size_t cap_v = 1024, cap_t = 1024, cap_n = 1024, cap_c = 4096, cap_f = 4096;
+	size_t nv = 0, nt = 0, nn = 0, nc = 0, nfc = 0;
+	float* pv = malloc(sizeof(float) * 3 * cap_v);
+	float* pt = malloc(sizeof(float) * 2 * cap_t);
+	float* pn = malloc(sizeof(float) * 3 * cap_n);
+	Corner* corners = malloc(sizeof(Corner) * cap_c);
+	int64_t* faces = malloc(sizeof(int64_t) * 3 * cap_f); /* corner ids */
+	char line[4096];
+	while (fgets(line, sizeof(line), fp)) {
+		char* s = line;
+		while (*s == ' ' || *s == '\t')
+			s++;
+		if (s[0] == 'v' && (s[1] == ' ' || s[1] == '\t')) {
+			if (nv == cap_v) { cap_v *= 2; pv = realloc(pv, sizeof(float) * 3 * cap_v); }
+			char* e = s + 1;
+			for (int k = 0; k < 3; k++)
+				pv[3 * nv + k] = strtof(e, &e);
+			nv++;
+		} else if (s[0] == 'v' && s[1] == 't') {
+			if (nt == cap_t) { cap_t *= 2; pt = realloc(pt, sizeof(float) * 2 * cap_t); }
+			char* e = s + 2;
+			for (int k = 0; k < 2; k++)
+				pt[2 * nt + k] = strtof(e, &e);
+			nt++;
+		} else if (s[0] == 'v' && s[1] == 'n') {
+			if (nn == cap_n) { cap_n *= 2; pn = realloc(pn, sizeof(float) * 3 * cap_n); }
+			char* e = s + 2;
+			for (int k = 0; k < 3; k++)
+				pn[3 * nn + k] = strtof(e, &e);
+			nn++;
+		} else if (s[0] == 'f' && (s[1] == ' ' || s[1] == '\t')) {
+			int64_t ids[64];
+			int cnt = 0;
+			char* e = s + 1;
+			while (cnt < 64) {
+				while (*e == ' ' || *e == '\t')
+					e++;
+				if (*e == '\0' || *e == '\n' || *e == '\r')
+					break;
+				long vi = strtol(e, &e, 10), ti = 0, ni = 0;
+				if (*e == '/') {
+					e++;
+					if (*e != '/')
+						ti = strtol(e, &e, 10);
+					if (*e == '/') {
+						e++;
+						ni = strtol(e, &e, 10);
+					}
+				}
+				while (*e && *e != ' ' && *e != '\t' && *e != '\n' && *e != '\r')
+					e++;
+				if (vi < 0) vi = (long)nv + 1 + vi;
+				if (ti < 0) ti = (long)nt + 1 + ti;
+				if (ni < 0) ni = (long)nn + 1 + ni;
+				Corner c;
+				memset(&c, 0, sizeof(c));
+				if (vi >= 1 && (size_t)vi <= nv)
+					memcpy(c.a, pv + 3 * (vi - 1), 3 * sizeof(float));
+				if (ti >= 1 && (size_t)ti <= nt)
+					memcpy(c.a + 3, pt + 2 * (ti - 1), 2 * sizeof(float));
+				if (ni >= 1 && (size_t)ni <= nn)
+					memcpy(c.a + 5, pn + 3 * (ni - 1), 3 * sizeof(float));
+				if (nc == cap_c) { cap_c *= 2; corners = realloc(corners, sizeof(Corner) * cap_c); }
+				corners[nc] = c;
+				ids[cnt++] = (int64_t)nc;
+				nc++;
+			}
+			for (int k = 1; k + 1 < cnt; k++) {
+				if (nfc == cap_f) { cap_f *= 2; faces = realloc(faces, sizeof(int64_t) * 3 * cap_f); }
+				faces[3 * nfc] = ids[0];
+				faces[3 * nfc + 1] = ids[k];
+				faces[3 * nfc + 2] = ids[k + 1];
+				nfc++;
+			}
+		}
+	}
+	fclose(fp);
+	/* Join identical corners (first use wins) with an open-addressing table. */
+	size_t tsz = 1;
+	while (tsz < 2 * nc + 2)
+		tsz <<= 1;
+	int64_t* table = malloc(sizeof(int64_t) * tsz);
+	for (size_t i = 0; i < tsz; i++)
+		table[i] = -1;
+	int64_t* remap = malloc(sizeof(int64_t) * (nc ? nc : 1));
+	float* V = malloc(sizeof(float) * 3 * (nc ? nc : 1));
+	int64_t nuniq = 0;
+	int64_t* uniq_src = malloc(sizeof(int64_t) * (nc ? nc : 1));
+	for (size_t i = 0; i < nc; i++) {
+		uint64_t h = corner_hash(&corners[i]) & (tsz - 1);
+		for (;;) {
+			int64_t u = table[h];
+			if (u < 0) {
+				table[h] = nuniq;
+				uniq_src[nuniq] = (int64_t)i;
+				memcpy(V + 3 * nuniq, corners[i].a, 3 * sizeof(float));
+				remap[i] = nuniq++;
+				break;
+			}
+			if (memcmp(corners[uniq_src[u]].a, corners[i].a, sizeof(corners[i].a)) == 0) {
+				remap[i] = u;
+				break;
+			}
+			h = (h + 1) & (tsz - 1);
+		}
+	}
+	uint32_t* F = malloc(sizeof(uint32_t) * 3 * (nfc ? nfc : 1));
+	for (size_t i = 0; i < 3 * nfc; i++)
+		F[i] = (uint32_t)remap[faces[i]];
+	int rc = -3;
+	if (nuniq > 0 && nfc > 0)
+		rc = or_load_arrays(w, geo, V, nuniq, F, (int64_t)nfc);
+	free(pv); free(pt); free(pn); free(corners); free(faces);
+	free(table); free(remap); free(V); free(uniq_src); free(F);
+	return rc;
+}
+
+int64_t or_mesh_nv(const OrWorld* w, int geo) { return w->mesh[geo].nv; }
+int64_t or_mesh_nf(const OrWorld* w, int geo) { return w->mesh[geo].nf; }
+void or_mesh_copy(const OrWorld* w, int geo, float* V, uint32_t* F)
+{
+	memcpy(V, w->mesh[geo].V, sizeof(float) * 3 * w->mesh[geo].nv);
+	memcpy(F, w->mesh[geo].F, sizeof(uint32_t) * 3 * w->mesh[geo].nf);
+}
+void or_mesh_center(const OrWorld* w, int geo, float* c3)
+{
+	memcpy(c3, w->mesh[geo].center, 3 * sizeof(float));
+}
+
+/* ------------------------------------------------------------------ */
+/* World                                                              */
+/* ------------------------------------------------------------------ */
+OrWorld* or_world_new(void)
+{
+	OrWorld* w = (OrWorld*)calloc(1, sizeof(OrWorld));
+	w->scale = 1.0f;
+	w->pert[3] = 1.0;
+	double R[9], t[3];
+	or_state_to_transform(w->pert, R, t);
+	memcpy(w->pR, R, sizeof(R));
+	memcpy(w->pt, t, sizeof(t));
+	for (int i = 0; i < 16; i++)
+		w->calib[i] = (i % 5 == 0) ? 1.0 : 0.0;
+	return w;
+}
+
+static void model_free(OrModel* m)
+{
+	free(m->V);
+	free(m->F);
+	free(m->nodes);
+	memset(m, 0, sizeof(*m));
+}
+
+void or_world_free(OrWorld* w)
+{
+	if (!w)
+		return;
+	for (int g = 0; g < 2; g++) {
+		mesh_free(&w->mesh[g]);
+		model_free(&w->cd[g]);
+	}
+	free(w);
+}
+
+void or_enforce_robot_center(OrWorld* w, const double* c3)
+{
+	/* Scene::overrideCenter takes a glm::vec3 (binary32). */
+	for (int i = 0; i < 3; i++)
+		w->mesh[1].center[i] = (float)c3[i];
+}
+
+static float bbox_span(const float* b)
+{
+	float sx = b[1] - b[0], sy = b[3] - b[2], sz = b[5] - b[4];
+	float m = (sx < sy) ? sy : sx;
+	return (m < sz) ? sz : m;
+}
+
+/* UnitWorld::scaleToUnit (lib/osr/unit_world.cc:123-131). */
+void or_scale_to_unit(OrWorld* w)
+{
+	float se = bbox_span(w->mesh[0].bbox);
+	float sr = 1.0f;
+	if (w->mesh[1].loaded)
+		sr = bbox_span(w->mesh[1].bbox);
+	float m = (se < sr) ? sr : se;
+	w->scale = (float)(1.0 / (double)m);
+}
+
+float or_scene_scale(const OrWorld* w) { return w->scale; }
+
+/* ---- BVH (FCL-style: PCA-fitted OBB per node, mean split, 1 tri/leaf) ---- */
+static void jacobi3(double A[3][3], double Vv[3][3], double ev[3])
+{
+	for (int i = 0; i < 3; i++)
+		for (int j = 0; j < 3; j++)
+			Vv[i][j] = (i == j) ? 1.0 : 0.0;
+	for (int sweep = 0; sweep < 64; sweep++) {
+		double off = fabs(A[0][1]) + fabs(A[0][2]) + fabs(A[1][2]);
+		double diag = fabs(A[0][0]) + fabs(A[1][1]) + fabs(A[2][2]);
+		if (off <= 1e-30 * diag || off == 0.0)
+			break;
+		for (int p = 0; p < 2; p++) {
+			for (int q = p + 1; q < 3; q++) {
+				if (A[p][q] == 0.0)
+					continue;
+				double theta = (A[q][q] - A[p][p]) / (2.0 * A[p][q]);
+				double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+				double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+				for (int k = 0; k < 3; k++) {
+					double akp = A[k][p], akq = A[k][q];
+					A[k][p] = c * akp - s * akq;
+					A[k][q] = s * akp + c * akq;
+				}
+				for (int k = 0; k < 3; k++) {
+					double apk = A[p][k], aqk = A[q][k];
+					A[p][k] = c * apk - s * aqk;
+					A[q][k] = s * apk + c * aqk;
+				}
+				for (int k = 0; k < 3; k++) {
+					double vkp = Vv[k][p], vkq = Vv[k][q];
+					Vv[k][p] = c * vkp - s * vkq;
+					Vv[k][q] = s * vkp + c * vkq;
+				}
+			}
+		}
+	}
+	for (int i = 0; i < 3; i++)
+		ev[i] = A[i][i];
+}
+
+typedef struct {
+	OrModel* m;
+	int* tri; /* permutation */
+	double* cen; /* triangle centroids */
+	int64_t next;
+	int maxdepth;
+} Builder;
+
+static void fit_obb(const OrModel* m, const int* tri, int n, ONode* nd)
+{
+	double mean[3] = {0, 0, 0};
+	int cnt = 3 * n;
+	for (int i = 0; i < n; i++)
+		for (int k = 0; k < 3; k++) {
+			const double* v = m->V + 3 * m->F[3 * tri[i] + k];
+			mean[0] += v[0];
+			mean[1] += v[1];
+			mean[2] += v[2];
+		}
+	for (int d = 0; d < 3; d++)
+		mean[d] /= cnt;
+	double C[3][3] = {{0}};
+	for (int i = 0; i < n; i++)
+		for (int k = 0; k < 3; k++) {
+			const double* v = m->V + 3 * m->F[3 * tri[i] + k];
+			double d[3] = {v[0] - mean[0], v[1] - mean[1], v[2] - mean[2]};
+			for (int r = 0; r < 3; r++)
+				for (int c = 0; c < 3; c++)
+					C[r][c] += d[r] * d[c];
+		}
+	double E[3][3], ev[3];
+	jacobi3(C, E, ev);
+	int order[3] = {0, 1, 2};
+	for (int i = 0; i < 3; i++)
+		for (int j = i + 1; j < 3; j++)
+			if (ev[order[j]] > ev[order[i]]) {
+				int t = order[i];
+				order[i] = order[j];
+				order[j] = t;
+			}
+	double a[3][3];
+	for (int k = 0; k < 2; k++) {
+		double v[3] = {E[0][order[k]], E[1][order[k]], E[2][order[k]]};
+		double n2 = sqrt(dot3(v, v));
+		for (int r = 0; r < 3; r++)
+			a[k][r] = v[r] / n2;
+	}
+	cross3(a[0], a[1], a[2]);
+	{
+		double n2 = sqrt(dot3(a[2], a[2]));
+		for (int r = 0; r < 3; r++)
+			a[2][r] /= n2;
+	}
+	double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+	for (int i = 0; i < n; i++)
+		for (int k = 0; k < 3; k++) {
+			const double* v = m->V + 3 * m->F[3 * tri[i] + k];
+			for (int ax = 0; ax < 3; ax++) {
+				double p = dot3(a[ax], v);
+				if (p < lo[ax]) lo[ax] = p;
+				if (p > hi[ax]) hi[ax] = p;
+			}
+		}
+	double mid[3];
+	for (int ax = 0; ax < 3; ax++) {
+		mid[ax] = 0.5 * (lo[ax] + hi[ax]);
+		nd->e[ax] = 0.5 * (hi[ax] - lo[ax]);
+	}
+	for (int r = 0; r < 3; r++) {
+		nd->c[r] = (a[0][r] * mid[0] + a[1][r] * mid[1]) + a[2][r] * mid[2];
+		for (int k = 0; k < 3; k++)
+			nd->ax[3 * r + k] = a[k][r];
+	}
+}
+
+static void build_rec(Builder* b, int64_t ni, int lo, int hi, int depth)
+{
+	OrModel* m = b->m;
+	ONode* nd = &m->nodes[ni];
+	if (depth > b->maxdepth)
+		b->maxdepth = depth;
+	fit_obb(m, b->tri + lo, hi - lo, nd);
+	if (hi - lo == 1) {
+		nd->first = -1;
+		nd->tri = b->tri[lo];
+		return;
+	}
+	/* split along the box's principal axis at the mean centroid projection */
+	double ax[3] = {nd->ax[0], nd->ax[3], nd->ax[6]};
+	double sum = 0.0;
+	for (int i = lo; i < hi; i++)
+		sum += dot3(ax, b->cen + 3 * b->tri[i]);
+	double split = sum / (hi - lo);
+	int i = lo, j = hi - 1;
+	while (i <= j) {
+		if (dot3(ax, b->cen + 3 * b->tri[i]) < split)
+			i++;
+		else {
+			int t = b->tri[i];
+			b->tri[i] = b->tri[j];
+			b->tri[j] = t;
+			j--;
+		}
+	}
+	int mid = i;
+	if (mid == lo || mid == hi)
+		mid = lo + (hi - lo) / 2;
+	int64_t first = b->next;
+	b->next += 2;
+	nd->first = (int)first;
+	nd->tri = -1;
+	build_rec(b, first, lo, mid, depth + 1);
+	build_rec(b, first + 1, mid, hi, depth + 1);
+}
+
+static void model_build(OrModel* m)
+{
+	Builder b;
+	b.m = m;
+	b.tri = (int*)malloc(sizeof(int) * m->nf);
+	b.cen = (double*)malloc(sizeof(double) * 3 * m->nf);
+	for (int64_t i = 0; i < m->nf; i++) {
+		b.tri[i] = (int)i;
+		for (int d = 0; d < 3; d++) {
+			double s = m->V[3 * m->F[3 * i] + d] + m->V[3 * m->F[3 * i + 1] + d];
+			b.cen[3 * i + d] = (s + m->V[3 * m->F[3 * i + 2] + d]) / 3.0;
+		}
+	}
+	m->nn = 2 * m->nf - 1;
+	m->nodes = (ONode*)calloc(m->nn, sizeof(ONode));
+	b.next = 1;
+	b.maxdepth = 0;
+	build_rec(&b, 0, 0, (int)m->nf, 0);
+	m->depth = b.maxdepth;
+	free(b.tri);
+	free(b.cen);
+}
+
+/* glm::dmat4(m) * glm::dvec4(v, 1): (m0*v0 + m1*v1) + (m2*v2 + m3*1). */
+static void xform_vertex(const float* M, const float* v, double* out)
+{
+	for (int r = 0; r < 3; r++) {
+		double a = (double)M[0 * 4 + r] * (double)v[0] + (double)M[1 * 4 + r] * (double)v[1];
+		double b = (double)M[2 * 4 + r] * (double)v[2] + (double)M[3 * 4 + r] * 1.0;
+		out[r] = a + b;
+	}
+}
+
+/* UnitWorld::angleModel(0,0) (lib/osr/unit_world.cc:137-161) with
+ * Scene::scale/moveToCenter (scene.cc:211-230) and CDModel::addVF
+ * (cdmodel.cc:120-160). */
+int or_angle_model(OrWorld* w, float latitude, float longitude)
+{
+	if (latitude != 0.0f || longitude != 0.0f)
+		return -1;
+	if (!w->mesh[0].loaded)
+		return -2;
+	float s = w->scale;
+	for (int g = 0; g < 2; g++) {
+		float* M = w->xf[g];
+		memset(M, 0, 16 * sizeof(float));
+		M[0] = s;
+		M[5] = s;
+		M[10] = s;
+		M[15] = 1.0f;
+		if (g == 1) {
+			/* scale(s) * translate(-center): column 3 = s * (-c) in binary32 */
+			const float* c = w->mesh[1].center;
+			M[12] = s * (-c[0]);
+			M[13] = s * (-c[1]);
+			M[14] = s * (-c[2]);
+		}
+	}
+	for (int i = 0; i < 4; i++)
+		for (int j = 0; j < 4; j++)
+			w->calib[4 * i + j] = (double)w->xf[0][4 * j + i];
+	for (int g = 0; g < 2; g++) {
+		model_free(&w->cd[g]);
+		w->has_cd[g] = 0;
+		OrMesh* me = &w->mesh[g];
+		if (!me->loaded)
+			continue;
+		OrModel* m = &w->cd[g];
+		m->nv = me->nv;
+		m->nf = me->nf;
+		m->V = (double*)malloc(sizeof(double) * 3 * m->nv);
+		m->F = (uint32_t*)malloc(sizeof(uint32_t) * 3 * m->nf);
+		memcpy(m->F, me->F, sizeof(uint32_t) * 3 * m->nf);
+		for (int64_t i = 0; i < m->nv; i++)
+			xform_vertex(w->xf[g], me->V + 3 * i, m->V + 3 * i);
+		model_build(m);
+		w->has_cd[g] = 1;
+	}
+	return 0;
+}
+
+void or_cd_vertices(const OrWorld* w, int geo, double* out)
+{
+	memcpy(out, w->cd[geo].V, sizeof(double) * 3 * w->cd[geo].nv);
+}
+
+void or_calibration(const OrWorld* w, double* m16) { memcpy(m16, w->calib, 16 * sizeof(double)); }
+int or_bvh_depth(const OrWorld* w, int geo) { return w->cd[geo].depth; }
+
+void or_set_perturbation(OrWorld* w, const double* q7)
+{
+	memcpy(w->pert, q7, 7 * sizeof(double));
+	or_state_to_transform(q7, w->pR, w->pt);
+}
+
+/* ------------------------------------------------------------------ */
+/* Narrow phase: FCL/PQP triangle-triangle SAT (17 axes).              */
+/* ------------------------------------------------------------------ */
+static int project6(const double* ax, const double* p1, const double* p2, const double* p3,
+                    const double* q1, const double* q2, const double* q3)
+{
+	double P1 = dot3(ax, p1), P2 = dot3(ax, p2), P3 = dot3(ax, p3);
+	double Q1 = dot3(ax, q1), Q2 = dot3(ax, q2), Q3 = dot3(ax, q3);
+	/* std::max(a, std::max(b, c)) and std::min likewise */
+	double mx1 = smax(P1, smax(P2, P3)), mn1 = smin(P1, smin(P2, P3));
+	double mx2 = smax(Q1, smax(Q2, Q3)), mn2 = smin(Q1, smin(Q2, Q3));
+	if (mn1 > mx2)
+		return 0;
+	if (mn2 > mx1)
+		return 0;
+	return 1;
+}
+
+int or_tri_intersect(const double* P, const double* Q)
+{
+	double p1[3], p2[3], p3[3], q1[3], q2[3], q3[3];
+	sub3(P, P, p1);
+	sub3(P + 3, P, p2);
+	sub3(P + 6, P, p3);
+	sub3(Q, P, q1);
+	sub3(Q + 3, P, q2);
+	sub3(Q + 6, P, q3);
+	double e1[3], e2[3], e3[3], f1[3], f2[3], f3[3], n1[3], m1[3], a[3];
+	sub3(p2, p1, e1);
+	sub3(p3, p2, e2);
+	sub3(p1, p3, e3);
+	sub3(q2, q1, f1);
+	sub3(q3, q2, f2);
+	sub3(q1, q3, f3);
+	cross3(e1, e2, n1);
+	if (!project6(n1, p1, p2, p3, q1, q2, q3)) return 0;
+	cross3(f1, f2, m1);
+	if (!project6(m1, p1, p2, p3, q1, q2, q3)) return 0;
+	const double* E[3] = {e1, e2, e3};
+	const double* Fv[3] = {f1, f2, f3};
+	for (int i = 0; i < 3; i++)
+		for (int j = 0; j < 3; j++) {
+			cross3(E[i], Fv[j], a);
+			if (!project6(a, p1, p2, p3, q1, q2, q3)) return 0;
+		}
+	for (int i = 0; i < 3; i++) {
+		cross3(E[i], n1, a);
+		if (!project6(a, p1, p2, p3, q1, q2, q3)) return 0;
+	}
+	for (int i = 0; i < 3; i++) {
+		cross3(Fv[i], m1, a);
+		if (!project6(a, p1, p2, p3, q1, q2, q3)) return 0;
+	}
+	return 1;
+}
+
+/* Robot vertex into the env frame: R*q + T, rows left to right. */
+static inline void rt_apply(const double* R, const double* T, const double* q, double* out)
+{
+	for (int r = 0; r < 3; r++)
+		out[r] = ((R[3 * r] * q[0] + R[3 * r + 1] * q[1]) + R[3 * r + 2] * q[2]) + T[r];
+}
+
+static int leaf_test(const OrModel* env, const OrModel* rob, int te, int tr,
+                     const double* R, const double* T)
+{
+	double P[9], Q[9];
+	for (int k = 0; k < 3; k++) {
+		memcpy(P + 3 * k, env->V + 3 * env->F[3 * te + k], 3 * sizeof(double));
+		rt_apply(R, T, rob->V + 3 * rob->F[3 * tr + k], Q + 3 * k);
+	}
+	return or_tri_intersect(P, Q);
+}
+
+/*
+ * Gottschalk 15-axis OBB disjointness with FCL's 1e-6 slack on |B|.
+ * O: orthonormal env<-robot rotation used for the robot box's axes.
+ * L, T: the actual (possibly non-orthonormal) linear map and translation that
+ * the leaf test applies to robot vertices; L maps the box centre.
+ * infl: ||L - O||_F.  L(B - c) lies in O(B - c) plus a ball of radius
+ * infl*|e|_2, so the robot extents grow by that radius and the test stays
+ * conservative for quaternions that are not exactly unit length.
+ */
+static int obb_disjoint(const ONode* A, const ONode* B, const double* O, const double* L,
+                        const double* T, double infl)
+{
+	double Bw[9], cb[3];
+	for (int r = 0; r < 3; r++)
+		for (int k = 0; k < 3; k++)
+			Bw[3 * r + k] = (O[3 * r] * B->ax[k] + O[3 * r + 1] * B->ax[3 + k]) + O[3 * r + 2] * B->ax[6 + k];
+	rt_apply(L, T, B->c, cb);
+	double Rr[3][3], Bf[3][3], t[3], d[3];
+	sub3(cb, A->c, d);
+	for (int i = 0; i < 3; i++) {
+		double ai[3] = {A->ax[i], A->ax[3 + i], A->ax[6 + i]};
+		t[i] = dot3(ai, d);
+		for (int j = 0; j < 3; j++) {
+			double bj[3] = {Bw[j], Bw[3 + j], Bw[6 + j]};
+			Rr[i][j] = dot3(ai, bj);
+			Bf[i][j] = fabs(Rr[i][j]) + 1e-6;
+		}
+	}
+	const double* a = A->e;
+	double grow = infl * sqrt(dot3(B->e, B->e));
+	double b[3] = {B->e[0] + grow, B->e[1] + grow, B->e[2] + grow};
+	for (int i = 0; i < 3; i++)
+		if (fabs(t[i]) > a[i] + (b[0] * Bf[i][0] + b[1] * Bf[i][1] + b[2] * Bf[i][2]))
+			return 1;
+	for (int j = 0; j < 3; j++) {
+		double s = t[0] * Rr[0][j] + t[1] * Rr[1][j] + t[2] * Rr[2][j];
+		if (fabs(s) > b[j] + (a[0] * Bf[0][j] + a[1] * Bf[1][j] + a[2] * Bf[2][j]))
+			return 1;
+	}
+	for (int i = 0; i < 3; i++) {
+		int i1 = (i + 1) % 3, i2 = (i + 2) % 3;
+		for (int j = 0; j < 3; j++) {
+			int j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+			double s = t[i2] * Rr[i1][j] - t[i1] * Rr[i2][j];
+			double r = a[i1] * Bf[i2][j] + a[i2] * Bf[i1][j] + b[j1] * Bf[i][j2] + b[j2] * Bf[i][j1];
+			if (fabs(s) > r)
+				return 1;
+		}
+	}
+	return 0;
+}
+
+/* Relative transform env<-robot from the two CD transforms
+ * (FCL relativeTransform: R = Re^T Rr, T = Re^T (tr - te)). */
+static void relative_tf_raw(const double* pert, const double* q7, double* R, double* T)
+{
+	double Re[9], te[3], Rr[9], tr[3];
+	or_state_to_transform(pert, Re, te);
+	or_state_to_transform(q7, Rr, tr);
+	double d[3] = {tr[0] - te[0], tr[1] - te[1], tr[2] - te[2]};
+	for (int i = 0; i < 3; i++) {
+		for (int j = 0; j < 3; j++)
+			R[3 * i + j] = (Re[i] * Rr[j] + Re[3 + i] * Rr[3 + j]) + Re[6 + i] * Rr[6 + j];
+		T[i] = (Re[i] * d[0] + Re[3 + i] * d[1]) + Re[6 + i] * d[2];
+	}
+}
+
+typedef struct {
+	double R[9], T[3]; /* leaf-test transform */
+	double O[9];       /* orthonormal rotation for box axes */
+	double infl;
+} QueryTf;
+
+static void query_tf(const OrWorld* w, const double* q7, QueryTf* qt)
+{
+	relative_tf_raw(w->pert, q7, qt->R, qt->T);
+	double pn[7], qn[7], dummy[3];
+	memcpy(pn, w->pert, sizeof(pn));
+	memcpy(qn, q7, sizeof(qn));
+	q_normalize(pn + 3);
+	q_normalize(qn + 3);
+	relative_tf_raw(pn, qn, qt->O, dummy);
+	double s = 0.0;
+	for (int i = 0; i < 9; i++) {
+		double d = qt->R[i] - qt->O[i];
+		s += d * d;
+	}
+	qt->infl = sqrt(s) * (1.0 + 1e-9) + 1e-15;
+}
+
+static int collide(const OrWorld* w, const QueryTf* qt, int64_t* nbv, int64_t* ntri)
+{
+	const OrModel* env = &w->cd[0];
+	const OrModel* rob = &w->cd[1];
+	enum { CAP = 4096 };
+	int stack_a[CAP], stack_b[CAP];
+	int sp = 0;
+	stack_a[sp] = 0;
+	stack_b[sp] = 0;
+	sp++;
+	int64_t cbv = 0, ctri = 0;
+	int hit = 0;
+	while (sp > 0) {
+		sp--;
+		int a = stack_a[sp], b = stack_b[sp];
+		const ONode* A = &env->nodes[a];
+		const ONode* B = &rob->nodes[b];
+		cbv++;
+		if (obb_disjoint(A, B, qt->O, qt->R, qt->T, qt->infl))
+			continue;
+		int la = A->first < 0, lb = B->first < 0;
+		if (la && lb) {
+			ctri++;
+			if (leaf_test(env, rob, A->tri, B->tri, qt->R, qt->T)) {
+				hit = 1;
+				break;
+			}
+			continue;
+		}
+		if (sp + 2 > CAP) {
+			fprintf(stderr, "oracle: traversal stack overflow\n");
+			abort();
+		}
+		double sa = dot3(A->e, A->e), sb = dot3(B->e, B->e);
+		/* FCL firstOverSecond: descend the env node if the robot node is a
+		 * leaf or the env box is larger. */
+		if (!la && (lb || sa > sb)) {
+			stack_a[sp] = A->first + 1; stack_b[sp] = b; sp++;
+			stack_a[sp] = A->first;     stack_b[sp] = b; sp++;
+		} else {
+			stack_a[sp] = a; stack_b[sp] = B->first + 1; sp++;
+			stack_a[sp] = a; stack_b[sp] = B->first;     sp++;
+		}
+	}
+	if (nbv) *nbv = cbv;
+	if (ntri) *ntri = ctri;
+	return hit;
+}
+
+int or_is_valid_counted(const OrWorld* w, const double* q7, int64_t* n_bv, int64_t* n_tri)
+{
+	if (n_bv) *n_bv = 0;
+	if (n_tri) *n_tri = 0;
+	if (!w->has_cd[0] || !w->has_cd[1])
+		return 1; /* isValid without models is true (unit_world.cc:218-219) */
+	QueryTf qt;
+	query_tf(w, q7, &qt);
+	return !collide(w, &qt, n_bv, n_tri);
+}
+
+int or_is_valid(const OrWorld* w, const double* q7) { return or_is_valid_counted(w, q7, NULL, NULL); }
+
+int or_is_valid_bruteforce(const OrWorld* w, const double* q7)
+{
+	if (!w->has_cd[0] || !w->has_cd[1])
+		return 1;
+	double R[9], T[3];
+	relative_tf_raw(w->pert, q7, R, T);
+	const OrModel* env = &w->cd[0];
+	const OrModel* rob = &w->cd[1];
+	for (int64_t i = 0; i < env->nf; i++)
+		for (int64_t j = 0; j < rob->nf; j++)
+			if (leaf_test(env, rob, (int)i, (int)j, R, T))
+				return 0;
+	return 1;
+}
+
+void or_valid_states(const OrWorld* w, const double* q, int64_t n, uint8_t* out,
+                     int64_t* n_bv, int64_t* n_tri, int nthreads)
+{
+#ifdef _OPENMP
+	if (nthreads > 0)
+		omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 64)
+#endif
+	for (int64_t i = 0; i < n; i++) {
+		int64_t a = 0, b = 0;
+		out[i] = (uint8_t)or_is_valid_counted(w, q + 7 * i, &a, &b);
+		if (n_bv) n_bv[i] = a;
+		if (n_tri) n_tri[i] = b;
+	}
+}
+
+/* ------------------------------------------------------------------ */
+/* Segments                                                           */
+/* ------------------------------------------------------------------ */
+/* transitStateToWithContact (lib/osr/unit_world.cc:405-474). */
+void or_transit(const OrWorld* w, const double* from, const double* to, double vd,
+                double* last_free, double* state, int* done, double* last_tau, double* tau_out,
+                int64_t* nsteps)
+{
+	double dist = or_distance(from, to);
+	int64_t steps = 0;
+	if (vd >= dist) {
+		memcpy(last_free, from, 7 * sizeof(double));
+		memcpy(state, to, 7 * sizeof(double));
+		*done = 0;
+		*last_tau = 0.0;
+		*tau_out = 1.0;
+		*nsteps = 0;
+		return;
+	}
+	double lf[7], st[7];
+	memcpy(lf, from, sizeof(lf));
+	memcpy(st, from, sizeof(st));
+	double delta = vd;
+	double inv_dist = 1.0 / dist;
+	double ltau = 0.0;
+	while (delta <= dist) {
+		double tau = delta * inv_dist;
+		or_interpolate(from, to, tau, st);
+		steps++;
+		if (!or_is_valid(w, st)) {
+			memcpy(last_free, lf, sizeof(lf));
+			memcpy(state, st, sizeof(st));
+			*done = 0;
+			*last_tau = ltau;
+			*tau_out = tau;
+			*nsteps = steps;
+			return;
+		}
+		ltau = tau;
+		memcpy(lf, st, sizeof(st));
+		if (delta < dist) {
+			delta += vd;
+			delta = (delta < dist) ? delta : dist; /* std::min(dist, delta) */
+		} else {
+			delta += vd;
+		}
+	}
+	memcpy(last_free, lf, sizeof(lf));
+	memcpy(state, st, sizeof(st));
+	*done = delta > dist;
+	*last_tau = ltau;
+	*tau_out = 0.0; /* the outer tau is shadowed in the loop */
+	*nsteps = steps;
+}
+
+/* isValidTransition (lib/osr/unit_world.cc:477-486). */
+int or_is_valid_transition(const OrWorld* w, const double* from, const double* to,
+                           double initial_vd, int64_t* nsteps)
+{
+	double d = or_distance(from, to);
+	double half = d / 2.0;
+	double vd = (initial_vd < half) ? initial_vd : half; /* std::min(d/2, ivd) */
+	double lf[7], st[7], lt, t;
+	int done;
+	int64_t ns;
+	or_transit(w, from, to, vd, lf, st, &done, &lt, &t, &ns);
+	if (nsteps)
+		*nsteps = ns;
+	return done;
+}
+
+void or_bisect(const OrWorld* w, const double* from, const double* to,
+               double lo, double hi, int levels, double* lo_out, double* hi_out)
+{
+	double st[7];
+	for (int i = 0; i < levels; i++) {
+		double mid = 0.5 * (lo + hi);
+		or_interpolate(from, to, mid, st);
+		if (or_is_valid(w, st))
+			lo = mid;
+		else
+			hi = mid;
+	}
+	*lo_out = lo;
+	*hi_out = hi;
+}
+
+void or_touch_batch(const OrWorld* w, const double* from, const double* to, int64_t n,
+                    double vd, int levels, uint8_t* done, double* last_tau, double* tau,
+                    double* lo, double* hi, int64_t* queries, int nthreads)
+{
+#ifdef _OPENMP
+	if (nthreads > 0)
+		omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 16)
+#endif
+	for (int64_t i = 0; i < n; i++) {
+		double lf[7], st[7], lt, t;
+		int dn;
+		int64_t ns;
+		or_transit(w, from + 7 * i, to + 7 * i, vd, lf, st, &dn, &lt, &t, &ns);
+		done[i] = (uint8_t)dn;
+		last_tau[i] = lt;
+		tau[i] = t;
+		/* free rows: bracket [last_tau, 1]; early-return rows: [0, 1] */
+		double l = lt, h = dn ? 1.0 : t;
+		int64_t q = ns;
+		if (!dn && ns > 0) {
+			or_bisect(w, from + 7 * i, to + 7 * i, lt, t, levels, &l, &h);
+			q += levels;
+		}
+		lo[i] = l;
+		hi[i] = h;
+		queries[i] = q;
+	}
+}
+
+/* ------------------------------------------------------------------ */
+/* Unit-state translation and visibility                              */
+/* ------------------------------------------------------------------ */
+/* translateToUnitState (unit_world.cc:526-539) + applyPertubation (713-724). */
+void or_translate_to_unit_state(const OrWorld* w, const double* q, double* out)
+{
+	const double* M = w->calib;
+	double t4[4] = {q[0], q[1], q[2], 1.0};
+	double nt[3];
+	for (int r = 0; r < 3; r++)
+		nt[r] = ((M[4 * r] * t4[0] + M[4 * r + 1] * t4[1]) + M[4 * r + 2] * t4[2]) + M[4 * r + 3] * t4[3];
+	double rot[4] = {q[3], q[4], q[5], q[6]};
+	q_normalize(rot);
+	double prot[4] = {w->pert[3], w->pert[4], w->pert[5], w->pert[6]};
+	q_normalize(prot);
+	double qret[4];
+	q_mul(prot, rot, qret);
+	double PR[9];
+	q_to_R(prot, PR);
+	double tret[3];
+	for (int r = 0; r < 3; r++)
+		tret[r] = w->pert[r] + ((PR[3 * r] * nt[0] + PR[3 * r + 1] * nt[1]) + PR[3 * r + 2] * nt[2]);
+	q_normalize(qret);
+	out[0] = tret[0];
+	out[1] = tret[1];
+	out[2] = tret[2];
+	for (int i = 0; i < 4; i++)
+		out[3 + i] = qret[i];
+}
+
+static double* to_unit_rows(const OrWorld* w, const double* qs, int64_t n, int is_unit)
+{
+	double* u = (double*)malloc(sizeof(double) * 7 * (n ? n : 1));
+	for (int64_t i = 0; i < n; i++) {
+		if (is_unit)
+			memcpy(u + 7 * i, qs + 7 * i, 7 * sizeof(double));
+		else
+			or_translate_to_unit_state(w, qs + 7 * i, u + 7 * i);
+	}
+	return u;
+}
+
+/* calculateVisibilityMatrix (unit_world.cc:740-762); the diagonal, which the
+ * reference leaves unwritten, is 0 here. */
+void or_visibility_matrix(const OrWorld* w, const double* qs, int64_t n, int is_unit,
+                          double vd, int8_t* out, int nthreads)
+{
+	double* u = to_unit_rows(w, qs, n, is_unit);
+	memset(out, 0, (size_t)(n * n));
+#ifdef _OPENMP
+	if (nthreads > 0)
+		omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1)
+#endif
+	for (int64_t fi = 0; fi < n; fi++)
+		for (int64_t ti = fi + 1; ti < n; ti++) {
+			int8_t v = (int8_t)(or_is_valid_transition(w, u + 7 * fi, u + 7 * ti, vd, NULL) != 0);
+			out[fi * n + ti] = v;
+			out[ti * n + fi] = v;
+		}
+	free(u);
+}
+
+void or_visibility_matrix2(const OrWorld* w, const double* qs0, int64_t m, int unit0,
+                           const double* qs1, int64_t n, int unit1, double vd,
+                           int8_t* out, int nthreads)
+{
+	double* u0 = to_unit_rows(w, qs0, m, unit0);
+	double* u1 = to_unit_rows(w, qs1, n, unit1);
+#ifdef _OPENMP
+	if (nthreads > 0)
+		omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1)
+#endif
+	for (int64_t fi = 0; fi < m; fi++)
+		for (int64_t ti = 0; ti < n; ti++)
+			out[fi * n + ti] = (int8_t)or_is_valid_transition(w, u0 + 7 * fi, u1 + 7 * ti, vd, NULL);
+	free(u0);
+	free(u1);
+}
+
+void or_visibility_pair(const OrWorld* w, const double* qs0, const double* qs1, int64_t n,
+                        int unit0, int unit1, double vd, int8_t* out, int nthreads)
+{
+	double* u0 = to_unit_rows(w, qs0, n, unit0);
+	double* u1 = to_unit_rows(w, qs1, n, unit1);
+#ifdef _OPENMP
+	if (nthreads > 0)
+		omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 16)
+#endif
+	for (int64_t i = 0; i < n; i++)
+		out[i] = (int8_t)or_is_valid_transition(w, u0 + 7 * i, u1 + 7 * i, vd, NULL);
+	free(u0);
+	free(u1);
+}

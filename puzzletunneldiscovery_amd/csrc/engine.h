@@ -1,0 +1,66 @@
+/*
+ * engine.h -- internal data layout shared by the host-side world
+ * (world.cpp) and the gfx950 kernels (kernels.hip).
+ *
+ * HBM layout per model (environment or robot), built once by angle_model:
+ *   nodes : GNode[2*nf-1], 64 B each, children of an internal node are
+ *           adjacent (first, first+1), nodes in breadth-first order so the
+ *           top levels shared by every query sit in a few cache lines.
+ *   tris  : binary64 xyz of the three vertices of every triangle in the
+ *           mesh's own face order (72 B per triangle); the exact leaf test
+ *           reads these.
+ * Node boxes are binary32 and conservative: every vertex of every triangle
+ * under a node lies inside the box defined by the stored binary32 centre,
+ * axes and extents, checked in binary64 at build time.
+ */
+#ifndef SE3CQ_ENGINE_H
+#define SE3CQ_ENGINE_H
+
+#include <stdint.h>
+
+#if defined(__HIPCC__) || defined(__HIP__)
+#include <hip/hip_runtime.h>
+#else
+struct alignas(16) float4 {
+	float x, y, z, w;
+};
+#endif
+
+namespace se3cq {
+
+/* c_ex = (cx, cy, cz, ex), a0_ey = (axis0, ey), a1_ez = (axis1, ez),
+ * a2_child = (axis2, child) with child >= 0 an internal node whose children
+ * are child and child+1, and child < 0 a leaf holding triangle ~child. */
+struct alignas(16) GNode {
+	float4 c_ex;
+	float4 a0_ey;
+	float4 a1_ez;
+	float4 a2_child;
+};
+
+struct DevModel {
+	const GNode* nodes;
+	const double* tris;
+	int32_t nnodes;
+	int32_t ntris;
+	int32_t depth;
+	int32_t pad;
+};
+
+/* Everything a kernel needs about the scene, passed by value. */
+struct DevScene {
+	DevModel env;
+	DevModel rob;
+	double Re[9];    /* perturbation rotation (raw quaternion), row-major */
+	double te[3];    /* perturbation translation */
+	double ReHat[9]; /* perturbation rotation from the normalised quaternion */
+	float rob_reach; /* max over robot nodes of |centre| + |extents|, rounded up */
+	float scene_mag; /* max |centre|_inf + max |extent| over both models */
+	int32_t stack_depth; /* per-thread traversal stack entries */
+	int32_t wide;        /* 1 if node ids need more than 16 bits */
+	uint32_t* error_flag; /* device word, non-zero after a traversal overflow */
+};
+
+} // namespace se3cq
+
+#endif

@@ -1,0 +1,40 @@
+/*
+ * launch.h -- host-callable launch wrappers of the gfx950 kernels.
+ */
+#ifndef SE3CQ_LAUNCH_H
+#define SE3CQ_LAUNCH_H
+
+#include <hip/hip_runtime.h>
+
+#include "engine.h"
+
+namespace se3cq {
+
+/* Per-segment outputs; any pointer may be null except done. */
+struct SegOut {
+	uint8_t* done;     /* 1 = collision free */
+	double* last_tau;  /* binary64 */
+	double* tau;       /* binary64 */
+	double* last_free; /* [n,7] */
+	double* state;     /* [n,7] */
+	double* lo;        /* touch search: bisection bracket */
+	double* hi;
+	int32_t* steps;    /* isValid calls under the reference's semantics (+ probes) */
+};
+
+hipError_t launch_valid_states(const DevScene& s, const double* q, int64_t n, uint8_t* out, hipStream_t st);
+/* mode: 0 transit (fixed verify_delta), 1 transition (min(d/2, ivd)), 2 touch (transit + bisection) */
+hipError_t launch_segments(const DevScene& s, int mode, const double* from, const double* to, int64_t n,
+                           double vd, int levels, const SegOut& o, hipStream_t st);
+/* vis: 0 pair (row i vs row i), 1 symmetric matrix (fi < ti, mirrored), 2 full m x n matrix */
+hipError_t launch_visibility(const DevScene& s, int vis, const double* qs0, const double* qs1, int64_t m,
+                             int64_t n, double ivd, int8_t* out8, uint32_t* bits, int32_t* steps, hipStream_t st);
+
+/* Fill kernels (used instead of hipMemsetAsync so callers may capture the
+ * device entry points in a HIP graph). */
+hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t st);
+hipError_t launch_fill_u8(uint8_t* p, int64_t n, uint8_t v, hipStream_t st);
+
+} // namespace se3cq
+
+#endif

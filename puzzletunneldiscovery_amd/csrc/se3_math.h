@@ -1,0 +1,380 @@
+/*
+ * se3_math.h -- deterministic SE(3) state arithmetic shared by host C/C++ and
+ * gfx950 device code.
+ *
+ * Every function here is written so that the x86-64 host build (gcc or clang,
+ * -ffp-contract=off) and the gfx950 device build produce bit-identical
+ * results: only IEEE-754 binary64 +, -, *, /, sqrt, fabs, floor and
+ * comparisons are used, with an explicit association order and FMA
+ * contraction disabled.  Division and sqrt are correctly rounded on both
+ * targets.  That makes every interpolated segment state computed on the GPU
+ * equal, bit for bit, to the same state computed on the CPU.
+ *
+ * The transcendental functions (sin, cos, acos, atan2) are this library's own
+ * ("se3_sin" etc.): Cody-Waite reduction plus Taylor polynomials evaluated by
+ * Horner's rule, accurate to about 1 ulp on the domains the state
+ * arithmetic uses.  They replace the C library's sin/acos/atan2 that Eigen
+ * calls inside Quaternion::slerp and Quaternion::angularDistance
+ * (lib/osr/osr_state.cc:39 and :135); glibc's versions are not bit-portable to
+ * the GPU.
+ *
+ * State layout (lib/osr/osr_state.h:44-50, lib/osr/unit_world.h:44-48):
+ *   q = [tx, ty, tz, qw, qx, qy, qz] in binary64.
+ *
+ * Association orders follow Eigen 3.3 as built for x86-64 with SSE2 (the
+ * reference's Release build): 4-element quaternion reductions pair
+ * (x,z)+(y,w); 3-element vector reductions run left to right.
+ */
+#ifndef SE3_MATH_H
+#define SE3_MATH_H
+
+#include <math.h>
+
+#if defined(__HIPCC__) || defined(__HIP__)
+#define SE3_HD __host__ __device__
+#else
+#define SE3_HD
+#endif
+
+#ifdef __clang__
+#pragma clang fp contract(off)
+#endif
+
+/* ---------------------------------------------------------------------- */
+/* Constants (binary64, hi+lo splits where a second word is needed).      */
+/* ---------------------------------------------------------------------- */
+#define SE3_PI_HI 0x1.921fb54442d18p+1
+#define SE3_PI_LO 0x1.1a62633145c07p-53
+#define SE3_PIO2_HI 0x1.921fb54442d18p+0
+#define SE3_PIO2_LO 0x1.1a62633145c07p-54
+#define SE3_PIO4_HI 0x1.921fb54442d18p-1
+#define SE3_PIO4_LO 0x1.1a62633145c07p-55
+#define SE3_TWO_OVER_PI 0x1.45f306dc9c883p-1
+/* pi/2 = P1 + P2 + P3; P1 and P2 carry 33 significant bits so k*P1 and k*P2
+ * are exact for |k| < 2^20. */
+#define SE3_RED_P1 0x1.921fb54400000p+0
+#define SE3_RED_P2 0x1.0b4611a600000p-34
+#define SE3_RED_P3 0x1.3198a2e037073p-69
+#define SE3_TAN_PI_8 0x1.a827999fcef32p-2
+/* Largest |x| handled by the deterministic argument reduction. */
+#define SE3_RED_MAX 0x1p20
+
+/* sin(r) on |r| <= pi/4 (+ a little): r + r^3 * P(r^2), Taylor to r^21. */
+SE3_HD static inline double se3_sin_kernel(double r)
+{
+	double r2 = r * r;
+	double p = 0x1.71b8ef6dcf572p-66;
+	p = -0x1.2f49b46814157p-57 + r2 * p;
+	p = 0x1.952c77030ad4ap-49 + r2 * p;
+	p = -0x1.ae7f3e733b81fp-41 + r2 * p;
+	p = 0x1.6124613a86d09p-33 + r2 * p;
+	p = -0x1.ae64567f544e4p-26 + r2 * p;
+	p = 0x1.71de3a556c734p-19 + r2 * p;
+	p = -0x1.a01a01a01a01ap-13 + r2 * p;
+	p = 0x1.1111111111111p-7 + r2 * p;
+	p = -0x1.5555555555555p-3 + r2 * p;
+	return r + (r2 * r) * p;
+}
+
+/* cos(r) on |r| <= pi/4 (+ a little): 1 - r^2/2 + r^4 * Q(r^2), Taylor to r^20. */
+SE3_HD static inline double se3_cos_kernel(double r)
+{
+	double r2 = r * r;
+	double p = 0x1.e542ba4020225p-62;
+	p = -0x1.6827863b97d97p-53 + r2 * p;
+	p = 0x1.ae7f3e733b81fp-45 + r2 * p;
+	p = -0x1.93974a8c07c9dp-37 + r2 * p;
+	p = 0x1.1eed8eff8d898p-29 + r2 * p;
+	p = -0x1.27e4fb7789f5cp-22 + r2 * p;
+	p = 0x1.a01a01a01a01ap-16 + r2 * p;
+	p = -0x1.6c16c16c16c17p-10 + r2 * p;
+	p = 0x1.5555555555555p-5 + r2 * p;
+	return 1.0 + r2 * (-0.5 + r2 * p);
+}
+
+/* Reduce x to r in [-pi/4, pi/4] (approximately) and a quadrant q = k mod 4. */
+SE3_HD static inline double se3_reduce_pio2(double x, int* quadrant)
+{
+	double k = floor(x * SE3_TWO_OVER_PI + 0.5);
+	double r = ((x - k * SE3_RED_P1) - k * SE3_RED_P2) - k * SE3_RED_P3;
+	*quadrant = ((int)k) & 3;
+	return r;
+}
+
+SE3_HD static inline double se3_sin(double x)
+{
+	double ax = fabs(x);
+	if (ax <= SE3_PIO4_HI)
+		return se3_sin_kernel(x);
+	if (!(ax <= SE3_RED_MAX))
+		return (x != x) ? x + x : sin(x); /* NaN, or outside the state domain */
+	int q;
+	double r = se3_reduce_pio2(x, &q);
+	switch (q) {
+	case 0: return se3_sin_kernel(r);
+	case 1: return se3_cos_kernel(r);
+	case 2: return -se3_sin_kernel(r);
+	default: return -se3_cos_kernel(r);
+	}
+}
+
+SE3_HD static inline double se3_cos(double x)
+{
+	double ax = fabs(x);
+	if (ax <= SE3_PIO4_HI)
+		return se3_cos_kernel(x);
+	if (!(ax <= SE3_RED_MAX))
+		return (x != x) ? x + x : cos(x);
+	int q;
+	double r = se3_reduce_pio2(x, &q);
+	switch (q) {
+	case 0: return se3_cos_kernel(r);
+	case 1: return -se3_sin_kernel(r);
+	case 2: return -se3_cos_kernel(r);
+	default: return se3_sin_kernel(r);
+	}
+}
+
+/* asin(x) for |x| <= 0.5: x + x^3 * A(x^2), Maclaurin series to x^49. */
+SE3_HD static inline double se3_asin_kernel(double x)
+{
+	double t = x * x;
+	double p = 0x1.3275586c5f2f0p-9;
+	p = 0x1.464c0950f7d47p-9 + t * p;
+	p = 0x1.5c5f56efaaaabp-9 + t * p;
+	p = 0x1.750de64d7d05fp-9 + t * p;
+	p = 0x1.90cb77f60c7cep-9 + t * p;
+	p = 0x1.b026f57b13b14p-9 + t * p;
+	p = 0x1.d3d2a8e0dd67dp-9 + t * p;
+	p = 0x1.fcaf8fb6db6dbp-9 + t * p;
+	p = 0x1.15ee9d45d1746p-8 + t * p;
+	p = 0x1.31683bdef7bdfp-8 + t * p;
+	p = 0x1.51ba308d3dcb1p-8 + t * p;
+	p = 0x1.782dda12f684cp-8 + t * p;
+	p = 0x1.a6863d70a3d71p-8 + t * p;
+	p = 0x1.df3bd37a6f4dfp-8 + t * p;
+	p = 0x1.12ef3cf3cf3cfp-7 + t * p;
+	p = 0x1.3fde50d79435ep-7 + t * p;
+	p = 0x1.7a87878787878p-7 + t * p;
+	p = 0x1.c99999999999ap-7 + t * p;
+	p = 0x1.1c4ec4ec4ec4fp-6 + t * p;
+	p = 0x1.6e8ba2e8ba2e9p-6 + t * p;
+	p = 0x1.f1c71c71c71c7p-6 + t * p;
+	p = 0x1.6db6db6db6db7p-5 + t * p;
+	p = 0x1.3333333333333p-4 + t * p;
+	p = 0x1.5555555555555p-3 + t * p;
+	return x + (t * x) * p;
+}
+
+SE3_HD static inline double se3_acos(double x)
+{
+	if (x != x)
+		return x + x;
+	if (x >= 0.5) {
+		/* 1 - x is exact on [0.5, 1] (Sterbenz); * 0.5 is exact. */
+		double s = sqrt((1.0 - x) * 0.5);
+		return 2.0 * se3_asin_kernel(s);
+	}
+	if (x <= -0.5) {
+		double s = sqrt((1.0 + x) * 0.5);
+		return SE3_PI_HI - (2.0 * se3_asin_kernel(s) - SE3_PI_LO);
+	}
+	return SE3_PIO2_HI - (se3_asin_kernel(x) - SE3_PIO2_LO);
+}
+
+/* atan(w) for |w| <= tan(pi/8): w + w^3 * B(w^2), Maclaurin series to w^43. */
+SE3_HD static inline double se3_atan_kernel(double w)
+{
+	double t = w * w;
+	double p = -0x1.7d05f417d05f4p-6;
+	p = 0x1.8f9c18f9c18fap-6 + t * p;
+	p = -0x1.a41a41a41a41ap-6 + t * p;
+	p = 0x1.bacf914c1bad0p-6 + t * p;
+	p = -0x1.d41d41d41d41dp-6 + t * p;
+	p = 0x1.f07c1f07c1f08p-6 + t * p;
+	p = -0x1.0842108421084p-5 + t * p;
+	p = 0x1.1a7b9611a7b96p-5 + t * p;
+	p = -0x1.2f684bda12f68p-5 + t * p;
+	p = 0x1.47ae147ae147bp-5 + t * p;
+	p = -0x1.642c8590b2164p-5 + t * p;
+	p = 0x1.8618618618618p-5 + t * p;
+	p = -0x1.af286bca1af28p-5 + t * p;
+	p = 0x1.e1e1e1e1e1e1ep-5 + t * p;
+	p = -0x1.1111111111111p-4 + t * p;
+	p = 0x1.3b13b13b13b14p-4 + t * p;
+	p = -0x1.745d1745d1746p-4 + t * p;
+	p = 0x1.c71c71c71c71cp-4 + t * p;
+	p = -0x1.2492492492492p-3 + t * p;
+	p = 0x1.999999999999ap-3 + t * p;
+	p = -0x1.5555555555555p-2 + t * p;
+	return w + (t * w) * p;
+}
+
+/* atan(z) for 0 <= z <= 1. */
+SE3_HD static inline double se3_atan01(double z)
+{
+	if (z > SE3_TAN_PI_8) {
+		double w = (z - 1.0) / (z + 1.0);
+		return SE3_PIO4_HI + (se3_atan_kernel(w) + SE3_PIO4_LO);
+	}
+	return se3_atan_kernel(z);
+}
+
+SE3_HD static inline double se3_atan2(double y, double x)
+{
+	if (x != x || y != y)
+		return x + y;
+	double ax = fabs(x), ay = fabs(y);
+	double a;
+	if (ay == 0.0 && ax == 0.0)
+		a = 0.0;
+	else if (ay <= ax)
+		a = se3_atan01(ay / ax);
+	else
+		a = SE3_PIO2_HI - (se3_atan01(ax / ay) - SE3_PIO2_LO);
+	if (x < 0.0 || (x == 0.0 && signbit(x)))
+		a = SE3_PI_HI - (a - SE3_PI_LO);
+	return signbit(y) ? -a : a;
+}
+
+/* ---------------------------------------------------------------------- */
+/* Quaternion and state arithmetic (Eigen 3.3 semantics, see file header). */
+/* Quaternions are passed as (w, x, y, z).                                */
+/* ---------------------------------------------------------------------- */
+
+/* Eigen QuaternionBase::normalize: coeffs /= sqrt(squaredNorm) if > 0. */
+SE3_HD static inline void se3_quat_normalize(double* q)
+{
+	double w = q[0], x = q[1], y = q[2], z = q[3];
+	double n2 = (x * x + z * z) + (y * y + w * w);
+	if (n2 > 0.0) {
+		double n = sqrt(n2);
+		q[0] = w / n;
+		q[1] = x / n;
+		q[2] = y / n;
+		q[3] = z / n;
+	}
+}
+
+/* Eigen quaternion dot: coeffs (x,y,z,w) reduced as (x,z)+(y,w). */
+SE3_HD static inline double se3_quat_dot(const double* a, const double* b)
+{
+	return (a[1] * b[1] + a[3] * b[3]) + (a[2] * b[2] + a[0] * b[0]);
+}
+
+/* Hamilton product a*b with Eigen's SSE2 binary64 grouping. */
+SE3_HD static inline void se3_quat_mul(const double* a, const double* b, double* r)
+{
+	double wa = a[0], xa = a[1], ya = a[2], za = a[3];
+	double wb = b[0], xb = b[1], yb = b[2], zb = b[3];
+	double rx = (wa * xb + ya * zb) - (za * yb - xa * wb);
+	double ry = (wa * yb + ya * wb) + (za * xb - xa * zb);
+	double rz = (wa * zb - ya * xb) + (za * wb + xa * yb);
+	double rw = (wa * wb - ya * yb) - (za * zb + xa * xb);
+	r[0] = rw;
+	r[1] = rx;
+	r[2] = ry;
+	r[3] = rz;
+}
+
+/*
+ * Eigen QuaternionBase::toRotationMatrix on the quaternion exactly as given
+ * (not normalised), row-major 3x3.  translate_state_to_transform
+ * (lib/osr/osr_state.cc:16-25) feeds the raw state quaternion to it.
+ */
+SE3_HD static inline void se3_quat_to_rotmat(double w, double x, double y, double z, double* R)
+{
+	double tx = 2.0 * x, ty = 2.0 * y, tz = 2.0 * z;
+	double twx = tx * w, twy = ty * w, twz = tz * w;
+	double txx = tx * x, txy = ty * x, txz = tz * x;
+	double tyy = ty * y, tyz = tz * y, tzz = tz * z;
+	R[0] = 1.0 - (tyy + tzz);
+	R[1] = txy - twz;
+	R[2] = txz + twy;
+	R[3] = txy + twz;
+	R[4] = 1.0 - (txx + tzz);
+	R[5] = tyz - twx;
+	R[6] = txz - twy;
+	R[7] = tyz + twx;
+	R[8] = 1.0 - (txx + tyy);
+}
+
+/* distance() of lib/osr/osr_state.cc:130-154: |t0-t1| + angularDistance. */
+SE3_HD static inline double se3_distance(const double* a, const double* b)
+{
+	double qa[4] = {a[3], a[4], a[5], a[6]};
+	double qb[4] = {b[3], b[4], b[5], b[6]};
+	se3_quat_normalize(qa);
+	se3_quat_normalize(qb);
+	double dx = a[0] - b[0], dy = a[1] - b[1], dz = a[2] - b[2];
+	double trdist = sqrt((dx * dx + dy * dy) + dz * dz);
+	/* angularDistance: d = qa * conj(qb); 2 atan2(|d.vec|, |d.w|). */
+	double cb[4] = {qb[0], -qb[1], -qb[2], -qb[3]};
+	double d[4];
+	se3_quat_mul(qa, cb, d);
+	double vn = sqrt((d[1] * d[1] + d[2] * d[2]) + d[3] * d[3]);
+	double rotdist = 2.0 * se3_atan2(vn, fabs(d[0]));
+	return trdist + rotdist;
+}
+
+/*
+ * Segment interpolation, interpolate() of lib/osr/osr_state.cc:27-44, split
+ * into a per-segment part (normalised endpoint quaternions and the slerp
+ * angle, which Eigen's slerp recomputes identically on every call) and a
+ * per-tau part.
+ */
+typedef struct se3_segment {
+	double p0[3], p1[3];
+	double q0[4], q1[4]; /* normalised, (w,x,y,z) */
+	double theta, sin_theta;
+	int lerp;  /* |dot| >= 1 - eps: linear weights */
+	int neg;   /* dot < 0: negate the second weight */
+} se3_segment;
+
+SE3_HD static inline void se3_segment_init(se3_segment* s, const double* from, const double* to)
+{
+	for (int i = 0; i < 3; i++) {
+		s->p0[i] = from[i];
+		s->p1[i] = to[i];
+	}
+	for (int i = 0; i < 4; i++) {
+		s->q0[i] = from[3 + i];
+		s->q1[i] = to[3 + i];
+	}
+	se3_quat_normalize(s->q0);
+	se3_quat_normalize(s->q1);
+	const double one = 1.0 - 0x1p-52; /* 1 - NumTraits<double>::epsilon() */
+	double d = se3_quat_dot(s->q0, s->q1);
+	double absd = fabs(d);
+	s->neg = d < 0.0;
+	if (absd >= one) {
+		s->lerp = 1;
+		s->theta = 0.0;
+		s->sin_theta = 0.0;
+	} else {
+		s->lerp = 0;
+		s->theta = se3_acos(absd);
+		s->sin_theta = se3_sin(s->theta);
+	}
+}
+
+SE3_HD static inline void se3_segment_at(const se3_segment* s, double tau, double* out)
+{
+	double omt = 1.0 - tau;
+	for (int i = 0; i < 3; i++)
+		out[i] = s->p0[i] * omt + s->p1[i] * tau;
+	double scale0, scale1;
+	if (s->lerp) {
+		scale0 = 1.0 - tau;
+		scale1 = tau;
+	} else {
+		scale0 = se3_sin((1.0 - tau) * s->theta) / s->sin_theta;
+		scale1 = se3_sin(tau * s->theta) / s->sin_theta;
+	}
+	if (s->neg)
+		scale1 = -scale1;
+	for (int i = 0; i < 4; i++)
+		out[3 + i] = scale0 * s->q0[i] + scale1 * s->q1[i];
+}
+
+#endif

@@ -1,0 +1,1357 @@
+/*
+ * world.cpp -- host side of the SE(3) collision-query engine: geometry
+ * ingest, unit-world normalisation, BVH construction and upload, and the
+ * extern "C" entry points declared in include/se3cq.h.
+ *
+ * Geometry follows the reference bit for bit where it defines the collision
+ * vertices:
+ *   - Scene::load keeps binary32 positions and takes the binary32 mean of the
+ *     vertices as the centre; the bounding box is seeded at the origin
+ *     (lib/osr/scene.cc:40-50,135-145, lib/osr/bbox.h:22-51).
+ *   - scaleToUnit: scale = float(1.0 / max(span_env, span_rob))
+ *     (lib/osr/unit_world.cc:123-131).
+ *   - angleModel(0,0): env matrix diag(s); robot matrix scale(s)*translate(-c)
+ *     built in binary32 (glm), then CDModel::addVF applies it in binary64:
+ *     (m0*v0 + m1*v1) + (m2*v2 + m3) (lib/osr/cdmodel.cc:120-160).
+ * The BVH is this engine's own (median split, PCA boxes, binary32 storage
+ * made conservative against the binary64 vertices); only the exact leaf test
+ * decides a boolean, so the tree shape never changes an answer.
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/se3cq.h"
+#include "engine.h"
+#include "launch.h"
+#include "se3_math.h"
+
+using namespace se3cq;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg)
+{
+	g_err = msg;
+	return code;
+}
+
+struct Mesh {
+	std::vector<float> V;
+	std::vector<uint32_t> F;
+	float center[3] = {0, 0, 0};
+	float ompl_center[3] = {0, 0, 0};
+	float bbox[6] = {0, 0, 0, 0, 0, 0}; /* l r b t k f */
+	bool loaded = false;
+};
+
+struct HostModel {
+	std::vector<double> Vcd;      /* unit-world vertices [nv*3] */
+	std::vector<GNode> nodes;     /* BFS order */
+	std::vector<double> tris;     /* [nf*9] */
+	int depth = 0;
+	float reach = 0.0f;           /* max |c| + |e| over nodes */
+	float mag = 0.0f;             /* max |c|_inf + max extent */
+};
+
+struct DevBuf {
+	void* p = nullptr;
+	size_t n = 0;
+	void release()
+	{
+		if (p)
+			(void)hipFree(p);
+		p = nullptr;
+		n = 0;
+	}
+	bool reserve(size_t bytes)
+	{
+		if (bytes <= n)
+			return true;
+		release();
+		if (hipMalloc(&p, bytes) != hipSuccess) {
+			p = nullptr;
+			return false;
+		}
+		n = bytes;
+		return true;
+	}
+};
+
+} // namespace
+
+struct se3cq_world {
+	Mesh mesh[2];
+	float scale = 1.0f;
+	float xf[2][16];       /* binary32 column-major (glm) */
+	bool has_model[2] = {false, false};
+	HostModel hm[2];
+	double pert[7] = {0, 0, 0, 1, 0, 0, 0};
+	double cres = 0.0;
+	/* device */
+	int device = -1;
+	bool uploaded = false;
+	hipStream_t stream = nullptr;
+	DevBuf d_nodes[2], d_tris[2];
+	DevBuf d_err;
+	DevBuf ws[12];
+	std::mutex mu;
+
+	se3cq_world()
+	{
+		for (int g = 0; g < 2; g++)
+			for (int i = 0; i < 16; i++)
+				xf[g][i] = (i % 5 == 0) ? 1.0f : 0.0f;
+	}
+	~se3cq_world() { release_device(); }
+	void release_device()
+	{
+		for (int g = 0; g < 2; g++) {
+			d_nodes[g].release();
+			d_tris[g].release();
+		}
+		d_err.release();
+		for (auto& b : ws)
+			b.release();
+		if (stream)
+			(void)hipStreamDestroy(stream);
+		stream = nullptr;
+		uploaded = false;
+	}
+	bool ready() const { return has_model[0] && has_model[1]; }
+};
+
+namespace {
+
+/* ---------------------------------------------------------------------- */
+/* Mesh ingest                                                            */
+/* ---------------------------------------------------------------------- */
+void mesh_finish(Mesh& m)
+{
+	float l = 0, r = 0, b = 0, t = 0, k = 0, f = 0;
+	float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+	const size_t nv = m.V.size() / 3;
+	for (size_t i = 0; i < nv; i++) {
+		const float x = m.V[3 * i], y = m.V[3 * i + 1], z = m.V[3 * i + 2];
+		r = std::max(x, r);
+		l = std::min(x, l);
+		t = std::max(y, t);
+		b = std::min(y, b);
+		f = std::max(z, f);
+		k = std::min(z, k);
+		sx += x;
+		sy += y;
+		sz += z;
+	}
+	const float bb[6] = {l, r, b, t, k, f};
+	std::memcpy(m.bbox, bb, sizeof(bb));
+	const float n = (float)nv;
+	m.ompl_center[0] = sx / n;
+	m.ompl_center[1] = sy / n;
+	m.ompl_center[2] = sz / n;
+	std::memcpy(m.center, m.ompl_center, sizeof(m.center));
+	m.loaded = true;
+}
+
+int mesh_from_arrays(Mesh& m, const float* V, int64_t nv, const uint32_t* F, int64_t nf)
+{
+	if (!V || !F || nv <= 0 || nf <= 0)
+		return fail(SE3CQ_EINVAL, "load_mesh: empty mesh");
+	for (int64_t i = 0; i < 3 * nf; i++)
+		if ((int64_t)F[i] >= nv)
+			return fail(SE3CQ_EINVAL, "load_mesh: face index out of range");
+	m = Mesh();
+	m.V.assign(V, V + 3 * nv);
+	m.F.assign(F, F + 3 * nf);
+	mesh_finish(m);
+	return SE3CQ_OK;
+}
+
+/* Wavefront OBJ as assimp (Triangulate | JoinIdenticalVertices) presents it
+ * to Mesh::Mesh (lib/osr/mesh.cc:24-63): polygons become triangle fans and
+ * face corners with identical (position, uv, normal) values share one
+ * vertex, numbered in order of first use. */
+int mesh_from_obj(Mesh& m, const char* path)
+{
+	FILE* fp = std::fopen(path, "r");
+	if (!fp)
+		return fail(SE3CQ_EIO, std::string("cannot open ") + path);
+	std::vector<float> pv, pt, pn;
+	struct Key {
+		float a[8];
+		bool operator==(const Key& o) const { return std::memcmp(a, o.a, sizeof(a)) == 0; }
+	};
+	struct KeyHash {
+		size_t operator()(const Key& k) const
+		{
+			uint64_t h = 1469598103934665603ULL;
+			const unsigned char* p = reinterpret_cast<const unsigned char*>(k.a);
+			for (size_t i = 0; i < sizeof(k.a); i++)
+				h = (h ^ p[i]) * 1099511628211ULL;
+			return (size_t)h;
+		}
+	};
+	std::unordered_map<Key, uint32_t, KeyHash> uniq;
+	std::vector<float> V;
+	std::vector<uint32_t> F;
+	char line[8192];
+	while (std::fgets(line, sizeof(line), fp)) {
+		char* s = line;
+		while (*s == ' ' || *s == '\t')
+			s++;
+		if (s[0] == 'v' && (s[1] == ' ' || s[1] == '\t')) {
+			char* e = s + 1;
+			for (int k = 0; k < 3; k++)
+				pv.push_back(std::strtof(e, &e));
+		} else if (s[0] == 'v' && s[1] == 't') {
+			char* e = s + 2;
+			for (int k = 0; k < 2; k++)
+				pt.push_back(std::strtof(e, &e));
+		} else if (s[0] == 'v' && s[1] == 'n') {
+			char* e = s + 2;
+			for (int k = 0; k < 3; k++)
+				pn.push_back(std::strtof(e, &e));
+		} else if (s[0] == 'f' && (s[1] == ' ' || s[1] == '\t')) {
+			std::vector<uint32_t> ids;
+			char* e = s + 1;
+			for (;;) {
+				while (*e == ' ' || *e == '\t')
+					e++;
+				if (*e == '\0' || *e == '\n' || *e == '\r')
+					break;
+				long vi = std::strtol(e, &e, 10), ti = 0, ni = 0;
+				if (*e == '/') {
+					e++;
+					if (*e != '/')
+						ti = std::strtol(e, &e, 10);
+					if (*e == '/') {
+						e++;
+						ni = std::strtol(e, &e, 10);
+					}
+				}
+				while (*e && *e != ' ' && *e != '\t' && *e != '\n' && *e != '\r')
+					e++;
+				const long nvv = (long)pv.size() / 3, ntt = (long)pt.size() / 2, nnn = (long)pn.size() / 3;
+				if (vi < 0) vi = nvv + 1 + vi;
+				if (ti < 0) ti = ntt + 1 + ti;
+				if (ni < 0) ni = nnn + 1 + ni;
+				Key key;
+				std::memset(key.a, 0, sizeof(key.a));
+				if (vi >= 1 && vi <= nvv)
+					std::memcpy(key.a, &pv[3 * (vi - 1)], 3 * sizeof(float));
+				if (ti >= 1 && ti <= ntt)
+					std::memcpy(key.a + 3, &pt[2 * (ti - 1)], 2 * sizeof(float));
+				if (ni >= 1 && ni <= nnn)
+					std::memcpy(key.a + 5, &pn[3 * (ni - 1)], 3 * sizeof(float));
+				auto it = uniq.find(key);
+				uint32_t id;
+				if (it == uniq.end()) {
+					id = (uint32_t)(V.size() / 3);
+					uniq.emplace(key, id);
+					V.insert(V.end(), key.a, key.a + 3);
+				} else {
+					id = it->second;
+				}
+				ids.push_back(id);
+			}
+			for (size_t k = 1; k + 1 < ids.size(); k++) {
+				F.push_back(ids[0]);
+				F.push_back(ids[k]);
+				F.push_back(ids[k + 1]);
+			}
+		}
+	}
+	std::fclose(fp);
+	if (V.empty() || F.empty())
+		return fail(SE3CQ_EIO, std::string("no triangles in ") + path);
+	return mesh_from_arrays(m, V.data(), (int64_t)V.size() / 3, F.data(), (int64_t)F.size() / 3);
+}
+
+float bbox_span(const float* b)
+{
+	return std::max(std::max(b[1] - b[0], b[3] - b[2]), b[5] - b[4]);
+}
+
+/* ---------------------------------------------------------------------- */
+/* BVH                                                                    */
+/* ---------------------------------------------------------------------- */
+void sym_eigen3(double A[3][3], double E[3][3])
+{
+	/* cyclic Jacobi; E columns are eigenvectors */
+	for (int i = 0; i < 3; i++)
+		for (int j = 0; j < 3; j++)
+			E[i][j] = (i == j) ? 1.0 : 0.0;
+	for (int sweep = 0; sweep < 50; sweep++) {
+		const double off = std::fabs(A[0][1]) + std::fabs(A[0][2]) + std::fabs(A[1][2]);
+		const double dia = std::fabs(A[0][0]) + std::fabs(A[1][1]) + std::fabs(A[2][2]);
+		if (off == 0.0 || off <= 1e-24 * dia)
+			break;
+		for (int p = 0; p < 2; p++)
+			for (int q = p + 1; q < 3; q++) {
+				if (A[p][q] == 0.0)
+					continue;
+				const double th = (A[q][q] - A[p][p]) / (2.0 * A[p][q]);
+				const double t = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
+				const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+				for (int k = 0; k < 3; k++) {
+					const double kp = A[k][p], kq = A[k][q];
+					A[k][p] = c * kp - s * kq;
+					A[k][q] = s * kp + c * kq;
+				}
+				for (int k = 0; k < 3; k++) {
+					const double pk = A[p][k], qk = A[q][k];
+					A[p][k] = c * pk - s * qk;
+					A[q][k] = s * pk + c * qk;
+				}
+				for (int k = 0; k < 3; k++) {
+					const double kp = E[k][p], kq = E[k][q];
+					E[k][p] = c * kp - s * kq;
+					E[k][q] = s * kp + c * kq;
+				}
+			}
+	}
+}
+
+float round_up_f(double x)
+{
+	float f = (float)x;
+	if ((double)f < x)
+		f = std::nextafter(f, INFINITY);
+	return f;
+}
+
+/* Fit a box to the vertices of triangles perm[lo..hi) and store it in
+ * binary32 such that every vertex v satisfies |a_i . (v - c)| <= e_i for the
+ * stored a_i, c, e_i evaluated exactly. */
+void fit_node(const HostModel& m, const uint32_t* F, const std::vector<int>& perm, int lo, int hi, GNode& g,
+              double* c_out, double* e_out)
+{
+	double mean[3] = {0, 0, 0};
+	const int cnt = 3 * (hi - lo);
+	for (int i = lo; i < hi; i++)
+		for (int k = 0; k < 3; k++) {
+			const double* v = &m.Vcd[3 * F[3 * perm[i] + k]];
+			for (int d = 0; d < 3; d++)
+				mean[d] += v[d];
+		}
+	for (int d = 0; d < 3; d++)
+		mean[d] /= cnt;
+	double C[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+	for (int i = lo; i < hi; i++)
+		for (int k = 0; k < 3; k++) {
+			const double* v = &m.Vcd[3 * F[3 * perm[i] + k]];
+			const double d[3] = {v[0] - mean[0], v[1] - mean[1], v[2] - mean[2]};
+			for (int r = 0; r < 3; r++)
+				for (int c = 0; c < 3; c++)
+					C[r][c] += d[r] * d[c];
+		}
+	double E[3][3];
+	sym_eigen3(C, E);
+	int ord[3] = {0, 1, 2};
+	std::sort(ord, ord + 3, [&](int a, int b) { return C[a][a] > C[b][b]; });
+	double ax[3][3];
+	for (int k = 0; k < 2; k++) {
+		double n = 0;
+		for (int r = 0; r < 3; r++)
+			n += E[r][ord[k]] * E[r][ord[k]];
+		n = std::sqrt(n);
+		for (int r = 0; r < 3; r++)
+			ax[k][r] = E[r][ord[k]] / n;
+	}
+	ax[2][0] = ax[0][1] * ax[1][2] - ax[0][2] * ax[1][1];
+	ax[2][1] = ax[0][2] * ax[1][0] - ax[0][0] * ax[1][2];
+	ax[2][2] = ax[0][0] * ax[1][1] - ax[0][1] * ax[1][0];
+	{
+		const double n = std::sqrt(ax[2][0] * ax[2][0] + ax[2][1] * ax[2][1] + ax[2][2] * ax[2][2]);
+		for (int r = 0; r < 3; r++)
+			ax[2][r] /= n;
+	}
+	float af[3][3];
+	for (int i = 0; i < 3; i++)
+		for (int r = 0; r < 3; r++)
+			af[i][r] = (float)ax[i][r];
+	double plo[3] = {INFINITY, INFINITY, INFINITY}, phi[3] = {-INFINITY, -INFINITY, -INFINITY};
+	for (int i = lo; i < hi; i++)
+		for (int k = 0; k < 3; k++) {
+			const double* v = &m.Vcd[3 * F[3 * perm[i] + k]];
+			for (int a = 0; a < 3; a++) {
+				const double p = (double)af[a][0] * v[0] + (double)af[a][1] * v[1] + (double)af[a][2] * v[2];
+				plo[a] = std::min(plo[a], p);
+				phi[a] = std::max(phi[a], p);
+			}
+		}
+	double c[3] = {0, 0, 0};
+	for (int a = 0; a < 3; a++) {
+		const double mid = 0.5 * (plo[a] + phi[a]);
+		for (int r = 0; r < 3; r++)
+			c[r] += mid * (double)af[a][r];
+	}
+	float cf[3] = {(float)c[0], (float)c[1], (float)c[2]};
+	double e[3] = {0, 0, 0};
+	for (int i = lo; i < hi; i++)
+		for (int k = 0; k < 3; k++) {
+			const double* v = &m.Vcd[3 * F[3 * perm[i] + k]];
+			const double d[3] = {v[0] - (double)cf[0], v[1] - (double)cf[1], v[2] - (double)cf[2]};
+			for (int a = 0; a < 3; a++) {
+				const double p = (double)af[a][0] * d[0] + (double)af[a][1] * d[1] + (double)af[a][2] * d[2];
+				e[a] = std::max(e[a], std::fabs(p));
+			}
+		}
+	float ef[3];
+	for (int a = 0; a < 3; a++)
+		ef[a] = round_up_f(e[a] * (1.0 + 0x1p-40) + 0x1p-60);
+	g.c_ex = {cf[0], cf[1], cf[2], ef[0]};
+	g.a0_ey = {af[0][0], af[0][1], af[0][2], ef[1]};
+	g.a1_ez = {af[1][0], af[1][1], af[1][2], ef[2]};
+	g.a2_child = {af[2][0], af[2][1], af[2][2], 0.0f};
+	for (int r = 0; r < 3; r++) {
+		c_out[r] = cf[r];
+		e_out[r] = ef[r];
+	}
+}
+
+void set_child(GNode& g, int32_t child)
+{
+	float f;
+	std::memcpy(&f, &child, sizeof(f));
+	g.a2_child.w = f;
+}
+
+void build_model(HostModel& m, const Mesh& me, const float* M)
+{
+	const size_t nv = me.V.size() / 3, nf = me.F.size() / 3;
+	m.Vcd.resize(3 * nv);
+	for (size_t i = 0; i < nv; i++) {
+		const float* v = &me.V[3 * i];
+		for (int r = 0; r < 3; r++) {
+			const double a = (double)M[0 * 4 + r] * (double)v[0] + (double)M[1 * 4 + r] * (double)v[1];
+			const double b = (double)M[2 * 4 + r] * (double)v[2] + (double)M[3 * 4 + r] * 1.0;
+			m.Vcd[3 * i + r] = a + b;
+		}
+	}
+	m.tris.resize(9 * nf);
+	for (size_t t = 0; t < nf; t++)
+		for (int k = 0; k < 3; k++)
+			for (int r = 0; r < 3; r++)
+				m.tris[9 * t + 3 * k + r] = m.Vcd[3 * me.F[3 * t + k] + r];
+	std::vector<double> cen(3 * nf);
+	for (size_t t = 0; t < nf; t++)
+		for (int r = 0; r < 3; r++)
+			cen[3 * t + r] = (m.tris[9 * t + r] + m.tris[9 * t + 3 + r] + m.tris[9 * t + 6 + r]) / 3.0;
+	std::vector<int> perm(nf);
+	for (size_t t = 0; t < nf; t++)
+		perm[t] = (int)t;
+	m.nodes.assign(2 * nf - 1, GNode());
+	struct Item {
+		int idx, lo, hi, depth;
+	};
+	std::deque<Item> q;
+	q.push_back({0, 0, (int)nf, 0});
+	int next = 1;
+	m.depth = 0;
+	double reach = 0.0, cmax = 0.0, emax = 0.0;
+	while (!q.empty()) {
+		Item it = q.front();
+		q.pop_front();
+		m.depth = std::max(m.depth, it.depth);
+		GNode& g = m.nodes[it.idx];
+		double c[3], e[3];
+		fit_node(m, me.F.data(), perm, it.lo, it.hi, g, c, e);
+		reach = std::max(reach, std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]) +
+		                            std::sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]));
+		cmax = std::max(cmax, std::max(std::fabs(c[0]), std::max(std::fabs(c[1]), std::fabs(c[2]))));
+		emax = std::max(emax, std::max(e[0], std::max(e[1], e[2])));
+		if (it.hi - it.lo == 1) {
+			set_child(g, ~perm[it.lo]);
+			continue;
+		}
+		double blo[3] = {INFINITY, INFINITY, INFINITY}, bhi[3] = {-INFINITY, -INFINITY, -INFINITY};
+		for (int i = it.lo; i < it.hi; i++)
+			for (int r = 0; r < 3; r++) {
+				blo[r] = std::min(blo[r], cen[3 * perm[i] + r]);
+				bhi[r] = std::max(bhi[r], cen[3 * perm[i] + r]);
+			}
+		int axis = 0;
+		for (int r = 1; r < 3; r++)
+			if (bhi[r] - blo[r] > bhi[axis] - blo[axis])
+				axis = r;
+		const int mid = (it.lo + it.hi) / 2;
+		std::nth_element(perm.begin() + it.lo, perm.begin() + mid, perm.begin() + it.hi,
+		                 [&](int a, int b) { return cen[3 * a + axis] < cen[3 * b + axis]; });
+		const int first = next;
+		next += 2;
+		set_child(g, first);
+		q.push_back({first, it.lo, mid, it.depth + 1});
+		q.push_back({first + 1, mid, it.hi, it.depth + 1});
+	}
+	m.reach = round_up_f(reach * (1.0 + 1e-6));
+	m.mag = round_up_f(cmax + emax);
+}
+
+/* ---------------------------------------------------------------------- */
+/* Device scene                                                           */
+/* ---------------------------------------------------------------------- */
+DevScene make_scene(se3cq_world* w)
+{
+	DevScene s;
+	std::memset(&s, 0, sizeof(s));
+	for (int g = 0; g < 2; g++) {
+		DevModel& dm = g ? s.rob : s.env;
+		dm.nodes = (const GNode*)w->d_nodes[g].p;
+		dm.tris = (const double*)w->d_tris[g].p;
+		dm.nnodes = (int32_t)w->hm[g].nodes.size();
+		dm.ntris = (int32_t)(w->hm[g].tris.size() / 9);
+		dm.depth = w->hm[g].depth;
+	}
+	double t[3];
+	se3_quat_to_rotmat(w->pert[3], w->pert[4], w->pert[5], w->pert[6], s.Re);
+	for (int i = 0; i < 3; i++)
+		s.te[i] = w->pert[i];
+	double qn[4] = {w->pert[3], w->pert[4], w->pert[5], w->pert[6]};
+	se3_quat_normalize(qn);
+	se3_quat_to_rotmat(qn[0], qn[1], qn[2], qn[3], s.ReHat);
+	(void)t;
+	s.rob_reach = w->hm[1].reach;
+	s.scene_mag = w->hm[0].mag + w->hm[1].mag;
+	s.stack_depth = w->hm[0].depth + w->hm[1].depth + 2;
+	s.wide = (w->hm[0].nodes.size() > 65536 || w->hm[1].nodes.size() > 65536) ? 1 : 0;
+	s.error_flag = (uint32_t*)w->d_err.p;
+	return s;
+}
+
+int upload(se3cq_world* w)
+{
+	if (hipGetDevice(&w->device) != hipSuccess)
+		return fail(SE3CQ_EDEVICE, "no HIP device");
+	if (!w->stream && hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking) != hipSuccess)
+		return fail(SE3CQ_EDEVICE, "hipStreamCreate failed");
+	for (int g = 0; g < 2; g++) {
+		const HostModel& m = w->hm[g];
+		const size_t nb = m.nodes.size() * sizeof(GNode), tb = m.tris.size() * sizeof(double);
+		if (!w->d_nodes[g].reserve(nb) || !w->d_tris[g].reserve(tb))
+			return fail(SE3CQ_ENOMEM, "device allocation failed");
+		if (hipMemcpy(w->d_nodes[g].p, m.nodes.data(), nb, hipMemcpyHostToDevice) != hipSuccess ||
+		    hipMemcpy(w->d_tris[g].p, m.tris.data(), tb, hipMemcpyHostToDevice) != hipSuccess)
+			return fail(SE3CQ_EDEVICE, "device upload failed");
+	}
+	if (!w->d_err.reserve(sizeof(uint32_t)) || hipMemset(w->d_err.p, 0, sizeof(uint32_t)) != hipSuccess)
+		return fail(SE3CQ_EDEVICE, "device allocation failed");
+	const int need = (int)(w->hm[0].depth + w->hm[1].depth + 2) * 2 * 4 * 64;
+	if (need > 48 * 1024)
+		return fail(SE3CQ_EINVAL, "BVH too deep for the traversal stack");
+	w->uploaded = true;
+	return SE3CQ_OK;
+}
+
+/* Upload the models on first use (angle_model itself needs no device, so the
+ * host-side geometry can be inspected on machines without one).  Queries
+ * without a device fail: there is no CPU fallback. */
+int ensure_device(se3cq_world* w)
+{
+	if (w->uploaded)
+		return SE3CQ_OK;
+	int ndev = 0;
+	if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+		return fail(SE3CQ_EDEVICE, "no HIP device available (the engine has no CPU fallback)");
+	return upload(w);
+}
+
+int check_device(se3cq_world* w)
+{
+	uint32_t flag = 0;
+	if (hipMemcpy(&flag, w->d_err.p, sizeof(flag), hipMemcpyDeviceToHost) != hipSuccess)
+		return fail(SE3CQ_EDEVICE, "device error check failed");
+	if (flag)
+		return fail(SE3CQ_EDEVICE, "traversal stack overflow on the device");
+	return SE3CQ_OK;
+}
+
+hipStream_t pick_stream(se3cq_world* w, void* s) { return s ? (hipStream_t)s : w->stream; }
+
+template <typename T>
+T* ws_get(se3cq_world* w, int slot, size_t count)
+{
+	if (!w->ws[slot].reserve(std::max<size_t>(count, 1) * sizeof(T)))
+		return nullptr;
+	return (T*)w->ws[slot].p;
+}
+
+int sync_and_check(se3cq_world* w, hipStream_t st, hipError_t le)
+{
+	if (le != hipSuccess)
+		return fail(SE3CQ_EDEVICE, std::string("kernel launch failed: ") + hipGetErrorString(le));
+	const hipError_t e = hipStreamSynchronize(st);
+	if (e != hipSuccess)
+		return fail(SE3CQ_EDEVICE, std::string("device error: ") + hipGetErrorString(e));
+	return check_device(w);
+}
+
+#define CHECK_WORLD(w)                                                   \
+	do {                                                                 \
+		if (!(w))                                                        \
+			return fail(SE3CQ_EINVAL, "null world");                     \
+	} while (0)
+
+#define HIPCHK(x)                                                        \
+	do {                                                                 \
+		hipError_t e_ = (x);                                             \
+		if (e_ != hipSuccess)                                            \
+			return fail(SE3CQ_EDEVICE, std::string(#x ": ") + hipGetErrorString(e_)); \
+	} while (0)
+
+void unit_row(const se3cq_world* w, const double* q, double* out)
+{
+	/* translateToUnitState (unit_world.cc:526-539) + applyPertubation (713-724) */
+	double M[16];
+	for (int i = 0; i < 4; i++)
+		for (int j = 0; j < 4; j++)
+			M[4 * i + j] = (double)w->xf[0][4 * j + i];
+	const double t4[4] = {q[0], q[1], q[2], 1.0};
+	double nt[3];
+	for (int r = 0; r < 3; r++)
+		nt[r] = ((M[4 * r] * t4[0] + M[4 * r + 1] * t4[1]) + M[4 * r + 2] * t4[2]) + M[4 * r + 3] * t4[3];
+	double rot[4] = {q[3], q[4], q[5], q[6]};
+	se3_quat_normalize(rot);
+	double prot[4] = {w->pert[3], w->pert[4], w->pert[5], w->pert[6]};
+	se3_quat_normalize(prot);
+	double qret[4];
+	se3_quat_mul(prot, rot, qret);
+	double PR[9];
+	se3_quat_to_rotmat(prot[0], prot[1], prot[2], prot[3], PR);
+	for (int r = 0; r < 3; r++)
+		out[r] = w->pert[r] + ((PR[3 * r] * nt[0] + PR[3 * r + 1] * nt[1]) + PR[3 * r + 2] * nt[2]);
+	se3_quat_normalize(qret);
+	for (int i = 0; i < 4; i++)
+		out[3 + i] = qret[i];
+}
+
+std::vector<double> unit_rows(const se3cq_world* w, const double* qs, int64_t n, int is_unit)
+{
+	std::vector<double> u((size_t)std::max<int64_t>(n, 1) * 7);
+	for (int64_t i = 0; i < n; i++) {
+		if (is_unit)
+			std::memcpy(&u[7 * i], qs + 7 * i, 7 * sizeof(double));
+		else
+			unit_row(w, qs + 7 * i, &u[7 * i]);
+	}
+	return u;
+}
+
+} // namespace
+
+/* ======================================================================= */
+/* extern "C" entry points                                                 */
+/* ======================================================================= */
+extern "C" {
+
+const char* se3cq_last_error(void) { return g_err.c_str(); }
+const char* se3cq_version(void) { return "se3cq 0.1 gfx950"; }
+
+int se3cq_world_create(se3cq_world** out)
+{
+	if (!out)
+		return fail(SE3CQ_EINVAL, "null output");
+	*out = new (std::nothrow) se3cq_world();
+	return *out ? SE3CQ_OK : fail(SE3CQ_ENOMEM, "out of memory");
+}
+
+int se3cq_world_destroy(se3cq_world* w)
+{
+	delete w;
+	return SE3CQ_OK;
+}
+
+int se3cq_world_copy_from(se3cq_world* dst, const se3cq_world* src)
+{
+	CHECK_WORLD(dst);
+	CHECK_WORLD(src);
+	std::lock_guard<std::mutex> lk(dst->mu);
+	for (int g = 0; g < 2; g++) {
+		dst->mesh[g] = src->mesh[g];
+		dst->hm[g] = src->hm[g];
+		dst->has_model[g] = src->has_model[g];
+	}
+	std::memcpy(dst->xf, src->xf, sizeof(dst->xf));
+	dst->scale = src->scale;
+	std::memcpy(dst->pert, src->pert, sizeof(dst->pert));
+	dst->cres = src->cres;
+	if (dst->ready())
+		return upload(dst);
+	return SE3CQ_OK;
+}
+
+int se3cq_load_file(se3cq_world* w, int geo, const char* path)
+{
+	CHECK_WORLD(w);
+	if (geo < 0 || geo > 1 || !path)
+		return fail(SE3CQ_EINVAL, "bad geo or path");
+	std::lock_guard<std::mutex> lk(w->mu);
+	Mesh m;
+	const int rc = mesh_from_obj(m, path);
+	if (rc)
+		return rc;
+	w->mesh[geo] = std::move(m);
+	w->has_model[geo] = false;
+	return SE3CQ_OK;
+}
+
+int se3cq_load_mesh(se3cq_world* w, int geo, const float* V, int64_t nv, const uint32_t* F, int64_t nf)
+{
+	CHECK_WORLD(w);
+	if (geo < 0 || geo > 1)
+		return fail(SE3CQ_EINVAL, "bad geo");
+	std::lock_guard<std::mutex> lk(w->mu);
+	Mesh m;
+	const int rc = mesh_from_arrays(m, V, nv, F, nf);
+	if (rc)
+		return rc;
+	w->mesh[geo] = std::move(m);
+	w->has_model[geo] = false;
+	return SE3CQ_OK;
+}
+
+int se3cq_get_center(const se3cq_world* w, int geo, float* c3)
+{
+	CHECK_WORLD(w);
+	if (geo < 0 || geo > 1 || !w->mesh[geo].loaded)
+		return fail(SE3CQ_ESTATE, "mesh not loaded");
+	std::memcpy(c3, w->mesh[geo].center, 3 * sizeof(float));
+	return SE3CQ_OK;
+}
+
+int se3cq_get_ompl_center(const se3cq_world* w, int geo, float* c3)
+{
+	CHECK_WORLD(w);
+	if (geo < 0 || geo > 1 || !w->mesh[geo].loaded)
+		return fail(SE3CQ_ESTATE, "mesh not loaded");
+	std::memcpy(c3, w->mesh[geo].ompl_center, 3 * sizeof(float));
+	return SE3CQ_OK;
+}
+
+int se3cq_enforce_robot_center(se3cq_world* w, const double* c3)
+{
+	CHECK_WORLD(w);
+	if (!w->mesh[1].loaded)
+		return fail(SE3CQ_ESTATE, "robot not loaded");
+	for (int i = 0; i < 3; i++)
+		w->mesh[1].center[i] = (float)c3[i];
+	return SE3CQ_OK;
+}
+
+int se3cq_scale_to_unit(se3cq_world* w)
+{
+	CHECK_WORLD(w);
+	if (!w->mesh[0].loaded)
+		return fail(SE3CQ_ESTATE, "environment not loaded");
+	const float se = bbox_span(w->mesh[0].bbox);
+	const float sr = w->mesh[1].loaded ? bbox_span(w->mesh[1].bbox) : 1.0f;
+	w->scale = (float)(1.0 / (double)std::max(se, sr));
+	return SE3CQ_OK;
+}
+
+int se3cq_angle_model(se3cq_world* w, float latitude, float longitude)
+{
+	CHECK_WORLD(w);
+	if (latitude != 0.0f || longitude != 0.0f)
+		return fail(SE3CQ_EINVAL,
+		            "The arguments of UnitWorld::angleModel are reserved for compatibility reason. Both values must be 0.");
+	if (!w->mesh[0].loaded)
+		return fail(SE3CQ_ESTATE, "environment not loaded");
+	std::lock_guard<std::mutex> lk(w->mu);
+	const float s = w->scale;
+	for (int g = 0; g < 2; g++) {
+		float* M = w->xf[g];
+		std::memset(M, 0, 16 * sizeof(float));
+		M[0] = M[5] = M[10] = s;
+		M[15] = 1.0f;
+		if (g == 1) {
+			const float* c = w->mesh[1].center;
+			M[12] = s * (-c[0]);
+			M[13] = s * (-c[1]);
+			M[14] = s * (-c[2]);
+		}
+	}
+	for (int g = 0; g < 2; g++) {
+		w->has_model[g] = false;
+		if (!w->mesh[g].loaded)
+			continue;
+		w->hm[g] = HostModel();
+		build_model(w->hm[g], w->mesh[g], w->xf[g]);
+		w->has_model[g] = true;
+	}
+	w->release_device();
+	return SE3CQ_OK;
+}
+
+int se3cq_get_scene_scale(const se3cq_world* w, float* scale)
+{
+	CHECK_WORLD(w);
+	*scale = w->scale;
+	return SE3CQ_OK;
+}
+
+int se3cq_get_matrix(const se3cq_world* w, int geo, double* m16)
+{
+	CHECK_WORLD(w);
+	if (geo < 0 || geo > 1)
+		return fail(SE3CQ_EINVAL, "bad geo");
+	for (int i = 0; i < 4; i++)
+		for (int j = 0; j < 4; j++)
+			m16[4 * i + j] = (double)w->xf[geo][4 * j + i];
+	return SE3CQ_OK;
+}
+
+int se3cq_get_mesh_size(const se3cq_world* w, int geo, int64_t* nv, int64_t* nf)
+{
+	CHECK_WORLD(w);
+	if (geo < 0 || geo > 1)
+		return fail(SE3CQ_EINVAL, "bad geo");
+	*nv = (int64_t)w->mesh[geo].V.size() / 3;
+	*nf = (int64_t)w->mesh[geo].F.size() / 3;
+	return SE3CQ_OK;
+}
+
+int se3cq_get_mesh(const se3cq_world* w, int geo, float* V, uint32_t* F)
+{
+	CHECK_WORLD(w);
+	if (geo < 0 || geo > 1)
+		return fail(SE3CQ_EINVAL, "bad geo");
+	std::memcpy(V, w->mesh[geo].V.data(), w->mesh[geo].V.size() * sizeof(float));
+	std::memcpy(F, w->mesh[geo].F.data(), w->mesh[geo].F.size() * sizeof(uint32_t));
+	return SE3CQ_OK;
+}
+
+int se3cq_get_cd_vertices(const se3cq_world* w, int geo, double* V)
+{
+	CHECK_WORLD(w);
+	if (geo < 0 || geo > 1 || !w->has_model[geo])
+		return fail(SE3CQ_ESTATE, "model not built (call angle_model)");
+	std::memcpy(V, w->hm[geo].Vcd.data(), w->hm[geo].Vcd.size() * sizeof(double));
+	return SE3CQ_OK;
+}
+
+int se3cq_get_bvh_info(const se3cq_world* w, int geo, int32_t* depth, int64_t* nnodes)
+{
+	CHECK_WORLD(w);
+	if (geo < 0 || geo > 1 || !w->has_model[geo])
+		return fail(SE3CQ_ESTATE, "model not built (call angle_model)");
+	*depth = w->hm[geo].depth;
+	*nnodes = (int64_t)w->hm[geo].nodes.size();
+	return SE3CQ_OK;
+}
+
+int se3cq_set_perturbation(se3cq_world* w, const double* q7)
+{
+	CHECK_WORLD(w);
+	std::lock_guard<std::mutex> lk(w->mu);
+	std::memcpy(w->pert, q7, 7 * sizeof(double));
+	return SE3CQ_OK;
+}
+
+int se3cq_get_perturbation(const se3cq_world* w, double* q7)
+{
+	CHECK_WORLD(w);
+	std::memcpy(q7, w->pert, 7 * sizeof(double));
+	return SE3CQ_OK;
+}
+
+int se3cq_set_recommended_cres(se3cq_world* w, double cres)
+{
+	CHECK_WORLD(w);
+	w->cres = cres;
+	return SE3CQ_OK;
+}
+
+int se3cq_get_recommended_cres(const se3cq_world* w, double* cres)
+{
+	CHECK_WORLD(w);
+	*cres = w->cres;
+	return SE3CQ_OK;
+}
+
+int se3cq_check(se3cq_world* w)
+{
+	CHECK_WORLD(w);
+	if (!w->ready() || !w->uploaded)
+		return SE3CQ_OK;
+	return check_device(w);
+}
+
+/* ---- hot path ---- */
+int se3cq_valid_states_device(se3cq_world* w, const double* d_q, int64_t n, uint8_t* d_out, void* stream)
+{
+	CHECK_WORLD(w);
+	if (n < 0)
+		return fail(SE3CQ_EINVAL, "negative count");
+	{
+		std::lock_guard<std::mutex> lk0(w->mu);
+		const int rc0 = ensure_device(w);
+		if (rc0 && w->ready())
+			return rc0;
+	}
+	hipStream_t st = pick_stream(w, stream);
+	if (!w->ready()) {
+		if (n > 0)
+			HIPCHK(launch_fill_u8(d_out, n, 1, st));
+		return SE3CQ_OK;
+	}
+	const hipError_t e = launch_valid_states(make_scene(w), d_q, n, d_out, st);
+	if (e != hipSuccess)
+		return fail(SE3CQ_EDEVICE, std::string("launch failed: ") + hipGetErrorString(e));
+	return SE3CQ_OK;
+}
+
+int se3cq_valid_states_host(se3cq_world* w, const double* q, int64_t n, uint8_t* out)
+{
+	CHECK_WORLD(w);
+	if (n < 0 || (n > 0 && (!q || !out)))
+		return fail(SE3CQ_EINVAL, "bad arguments");
+	if (n == 0)
+		return SE3CQ_OK;
+	if (!w->ready()) {
+		std::memset(out, 1, (size_t)n);
+		return SE3CQ_OK;
+	}
+	std::lock_guard<std::mutex> lk(w->mu);
+	const int rc0 = ensure_device(w);
+	if (rc0)
+		return rc0;
+	double* dq = ws_get<double>(w, 0, 7 * n);
+	uint8_t* dout = ws_get<uint8_t>(w, 1, n);
+	if (!dq || !dout)
+		return fail(SE3CQ_ENOMEM, "device allocation failed");
+	HIPCHK(hipMemcpyAsync(dq, q, 7 * n * sizeof(double), hipMemcpyHostToDevice, w->stream));
+	const hipError_t le = launch_valid_states(make_scene(w), dq, n, dout, w->stream);
+	int rc = sync_and_check(w, w->stream, le);
+	if (rc)
+		return rc;
+	HIPCHK(hipMemcpy(out, dout, (size_t)n, hipMemcpyDeviceToHost));
+	return SE3CQ_OK;
+}
+
+static int seg_device(se3cq_world* w, int mode, const double* d_from, const double* d_to, int64_t n, double vd,
+                      int levels, const SegOut& o, hipStream_t st)
+{
+	const hipError_t e = launch_segments(make_scene(w), mode, d_from, d_to, n, vd, levels, o, st);
+	if (e != hipSuccess)
+		return fail(SE3CQ_EDEVICE, std::string("launch failed: ") + hipGetErrorString(e));
+	return SE3CQ_OK;
+}
+
+int se3cq_transit_device(se3cq_world* w, const double* d_from, const double* d_to, int64_t n, double verify_delta,
+                         double* d_last_free, double* d_state, uint8_t* d_done, double* d_last_tau, double* d_tau,
+                         int32_t* d_steps, void* stream)
+{
+	CHECK_WORLD(w);
+	if (!w->ready())
+		return fail(SE3CQ_ESTATE, "models not loaded");
+	{
+		std::lock_guard<std::mutex> lk0(w->mu);
+		const int rc0 = ensure_device(w);
+		if (rc0)
+			return rc0;
+	}
+	if ((d_last_free == nullptr) != (d_state == nullptr))
+		return fail(SE3CQ_EINVAL, "last_free and state must both be given or both be null");
+	SegOut o{d_done, d_last_tau, d_tau, d_last_free, d_state, nullptr, nullptr, d_steps};
+	return seg_device(w, 0, d_from, d_to, n, verify_delta, 0, o, pick_stream(w, stream));
+}
+
+int se3cq_transit_host(se3cq_world* w, const double* from, const double* to, int64_t n, double verify_delta,
+                       double* last_free, double* state, uint8_t* done, double* last_tau, double* tau,
+                       int32_t* steps)
+{
+	CHECK_WORLD(w);
+	if (!w->ready())
+		return fail(SE3CQ_ESTATE, "models not loaded");
+	{
+		std::lock_guard<std::mutex> lk0(w->mu);
+		const int rc0 = ensure_device(w);
+		if (rc0)
+			return rc0;
+	}
+	if (n <= 0)
+		return n == 0 ? SE3CQ_OK : fail(SE3CQ_EINVAL, "negative count");
+	std::lock_guard<std::mutex> lk(w->mu);
+	double* df = ws_get<double>(w, 0, 7 * n);
+	double* dt = ws_get<double>(w, 2, 7 * n);
+	double* dlf = ws_get<double>(w, 3, 7 * n);
+	double* dst = ws_get<double>(w, 4, 7 * n);
+	uint8_t* dd = ws_get<uint8_t>(w, 1, n);
+	double* dlt = ws_get<double>(w, 5, n);
+	double* dta = ws_get<double>(w, 6, n);
+	int32_t* dsp = ws_get<int32_t>(w, 7, n);
+	if (!df || !dt || !dlf || !dst || !dd || !dlt || !dta || !dsp)
+		return fail(SE3CQ_ENOMEM, "device allocation failed");
+	HIPCHK(hipMemcpyAsync(df, from, 7 * n * sizeof(double), hipMemcpyHostToDevice, w->stream));
+	HIPCHK(hipMemcpyAsync(dt, to, 7 * n * sizeof(double), hipMemcpyHostToDevice, w->stream));
+	SegOut o{dd, dlt, dta, dlf, dst, nullptr, nullptr, dsp};
+	const hipError_t le = launch_segments(make_scene(w), 0, df, dt, n, verify_delta, 0, o, w->stream);
+	int rc = sync_and_check(w, w->stream, le);
+	if (rc)
+		return rc;
+	if (last_free) HIPCHK(hipMemcpy(last_free, dlf, 7 * n * sizeof(double), hipMemcpyDeviceToHost));
+	if (state) HIPCHK(hipMemcpy(state, dst, 7 * n * sizeof(double), hipMemcpyDeviceToHost));
+	if (done) HIPCHK(hipMemcpy(done, dd, n, hipMemcpyDeviceToHost));
+	if (last_tau) HIPCHK(hipMemcpy(last_tau, dlt, n * sizeof(double), hipMemcpyDeviceToHost));
+	if (tau) HIPCHK(hipMemcpy(tau, dta, n * sizeof(double), hipMemcpyDeviceToHost));
+	if (steps) HIPCHK(hipMemcpy(steps, dsp, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+	return SE3CQ_OK;
+}
+
+int se3cq_valid_transitions_host(se3cq_world* w, const double* from, const double* to, int64_t n,
+                                 double initial_verify_delta, uint8_t* out)
+{
+	CHECK_WORLD(w);
+	if (!w->ready())
+		return fail(SE3CQ_ESTATE, "models not loaded");
+	{
+		std::lock_guard<std::mutex> lk0(w->mu);
+		const int rc0 = ensure_device(w);
+		if (rc0)
+			return rc0;
+	}
+	if (n <= 0)
+		return n == 0 ? SE3CQ_OK : fail(SE3CQ_EINVAL, "negative count");
+	std::lock_guard<std::mutex> lk(w->mu);
+	double* df = ws_get<double>(w, 0, 7 * n);
+	double* dt = ws_get<double>(w, 2, 7 * n);
+	uint8_t* dd = ws_get<uint8_t>(w, 1, n);
+	if (!df || !dt || !dd)
+		return fail(SE3CQ_ENOMEM, "device allocation failed");
+	HIPCHK(hipMemcpyAsync(df, from, 7 * n * sizeof(double), hipMemcpyHostToDevice, w->stream));
+	HIPCHK(hipMemcpyAsync(dt, to, 7 * n * sizeof(double), hipMemcpyHostToDevice, w->stream));
+	SegOut o{dd, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+	const hipError_t le = launch_segments(make_scene(w), 1, df, dt, n, initial_verify_delta, 0, o, w->stream);
+	int rc = sync_and_check(w, w->stream, le);
+	if (rc)
+		return rc;
+	HIPCHK(hipMemcpy(out, dd, n, hipMemcpyDeviceToHost));
+	return SE3CQ_OK;
+}
+
+int se3cq_touch_search_device(se3cq_world* w, const double* d_from, const double* d_to, int64_t n,
+                              double verify_delta, int levels, uint8_t* d_done, double* d_last_tau, double* d_tau,
+                              double* d_lo, double* d_hi, int32_t* d_steps, void* stream)
+{
+	CHECK_WORLD(w);
+	if (!w->ready())
+		return fail(SE3CQ_ESTATE, "models not loaded");
+	{
+		std::lock_guard<std::mutex> lk0(w->mu);
+		const int rc0 = ensure_device(w);
+		if (rc0)
+			return rc0;
+	}
+	if (levels < 0 || levels > 60)
+		return fail(SE3CQ_EINVAL, "levels must be in [0, 60]");
+	if (!d_lo || !d_hi)
+		return fail(SE3CQ_EINVAL, "lo and hi outputs are required");
+	SegOut o{d_done, d_last_tau, d_tau, nullptr, nullptr, d_lo, d_hi, d_steps};
+	return seg_device(w, 2, d_from, d_to, n, verify_delta, levels, o, pick_stream(w, stream));
+}
+
+int se3cq_touch_search_host(se3cq_world* w, const double* from, const double* to, int64_t n, double verify_delta,
+                            int levels, uint8_t* done, double* last_tau, double* tau, double* lo, double* hi,
+                            int32_t* steps)
+{
+	CHECK_WORLD(w);
+	if (!w->ready())
+		return fail(SE3CQ_ESTATE, "models not loaded");
+	{
+		std::lock_guard<std::mutex> lk0(w->mu);
+		const int rc0 = ensure_device(w);
+		if (rc0)
+			return rc0;
+	}
+	if (levels < 0 || levels > 60)
+		return fail(SE3CQ_EINVAL, "levels must be in [0, 60]");
+	if (n <= 0)
+		return n == 0 ? SE3CQ_OK : fail(SE3CQ_EINVAL, "negative count");
+	std::lock_guard<std::mutex> lk(w->mu);
+	double* df = ws_get<double>(w, 0, 7 * n);
+	double* dt = ws_get<double>(w, 2, 7 * n);
+	uint8_t* dd = ws_get<uint8_t>(w, 1, n);
+	double* dlt = ws_get<double>(w, 5, n);
+	double* dta = ws_get<double>(w, 6, n);
+	int32_t* dsp = ws_get<int32_t>(w, 7, n);
+	double* dlo = ws_get<double>(w, 8, n);
+	double* dhi = ws_get<double>(w, 9, n);
+	if (!df || !dt || !dd || !dlt || !dta || !dsp || !dlo || !dhi)
+		return fail(SE3CQ_ENOMEM, "device allocation failed");
+	HIPCHK(hipMemcpyAsync(df, from, 7 * n * sizeof(double), hipMemcpyHostToDevice, w->stream));
+	HIPCHK(hipMemcpyAsync(dt, to, 7 * n * sizeof(double), hipMemcpyHostToDevice, w->stream));
+	SegOut o{dd, dlt, dta, nullptr, nullptr, dlo, dhi, dsp};
+	const hipError_t le = launch_segments(make_scene(w), 2, df, dt, n, verify_delta, levels, o, w->stream);
+	int rc = sync_and_check(w, w->stream, le);
+	if (rc)
+		return rc;
+	if (done) HIPCHK(hipMemcpy(done, dd, n, hipMemcpyDeviceToHost));
+	if (last_tau) HIPCHK(hipMemcpy(last_tau, dlt, n * sizeof(double), hipMemcpyDeviceToHost));
+	if (tau) HIPCHK(hipMemcpy(tau, dta, n * sizeof(double), hipMemcpyDeviceToHost));
+	if (lo) HIPCHK(hipMemcpy(lo, dlo, n * sizeof(double), hipMemcpyDeviceToHost));
+	if (hi) HIPCHK(hipMemcpy(hi, dhi, n * sizeof(double), hipMemcpyDeviceToHost));
+	if (steps) HIPCHK(hipMemcpy(steps, dsp, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+	return SE3CQ_OK;
+}
+
+int se3cq_bisect_contact_host(se3cq_world* w, const double* from, const double* to, int64_t n, const double* lo_in,
+                              const double* hi_in, int levels, double* lo, double* hi)
+{
+	/* Bisection on an explicit bracket: evaluate every probe as a batch of
+	 * states per level (n states per launch, `levels` launches). */
+	CHECK_WORLD(w);
+	if (!w->ready())
+		return fail(SE3CQ_ESTATE, "models not loaded");
+	{
+		std::lock_guard<std::mutex> lk0(w->mu);
+		const int rc0 = ensure_device(w);
+		if (rc0)
+			return rc0;
+	}
+	if (n <= 0)
+		return n == 0 ? SE3CQ_OK : fail(SE3CQ_EINVAL, "negative count");
+	std::vector<double> l(lo_in, lo_in + n), h(hi_in, hi_in + n), mids(7 * n);
+	std::vector<uint8_t> ok(n);
+	for (int lv = 0; lv < levels; lv++) {
+		for (int64_t i = 0; i < n; i++) {
+			se3_segment sg;
+			se3_segment_init(&sg, from + 7 * i, to + 7 * i);
+			se3_segment_at(&sg, 0.5 * (l[i] + h[i]), &mids[7 * i]);
+		}
+		int rc = se3cq_valid_states_host(w, mids.data(), n, ok.data());
+		if (rc)
+			return rc;
+		for (int64_t i = 0; i < n; i++) {
+			const double m = 0.5 * (l[i] + h[i]);
+			if (ok[i])
+				l[i] = m;
+			else
+				h[i] = m;
+		}
+	}
+	std::memcpy(lo, l.data(), n * sizeof(double));
+	std::memcpy(hi, h.data(), n * sizeof(double));
+	return SE3CQ_OK;
+}
+
+static int visibility_host(se3cq_world* w, int vis, const std::vector<double>& u0, const std::vector<double>& u1,
+                           int64_t m, int64_t n, double vd, int8_t* out, size_t out_n)
+{
+	std::lock_guard<std::mutex> lk(w->mu);
+	double* d0 = ws_get<double>(w, 0, u0.size());
+	double* d1 = ws_get<double>(w, 2, u1.size());
+	int8_t* dout = ws_get<int8_t>(w, 1, out_n);
+	if (!d0 || !d1 || !dout)
+		return fail(SE3CQ_ENOMEM, "device allocation failed");
+	HIPCHK(hipMemcpyAsync(d0, u0.data(), u0.size() * sizeof(double), hipMemcpyHostToDevice, w->stream));
+	HIPCHK(hipMemcpyAsync(d1, u1.data(), u1.size() * sizeof(double), hipMemcpyHostToDevice, w->stream));
+	HIPCHK(hipMemsetAsync(dout, 0, out_n, w->stream));
+	const hipError_t le = launch_visibility(make_scene(w), vis, d0, d1, m, n, vd, dout, nullptr, nullptr, w->stream);
+	int rc = sync_and_check(w, w->stream, le);
+	if (rc)
+		return rc;
+	HIPCHK(hipMemcpy(out, dout, out_n, hipMemcpyDeviceToHost));
+	return SE3CQ_OK;
+}
+
+int se3cq_visibility_matrix_host(se3cq_world* w, const double* qs, int64_t n, int qs_are_unit,
+                                 double verify_magnitude, int8_t* out)
+{
+	CHECK_WORLD(w);
+	if (!w->ready())
+		return fail(SE3CQ_ESTATE, "models not loaded");
+	{
+		std::lock_guard<std::mutex> lk0(w->mu);
+		const int rc0 = ensure_device(w);
+		if (rc0)
+			return rc0;
+	}
+	if (n <= 0)
+		return n == 0 ? SE3CQ_OK : fail(SE3CQ_EINVAL, "negative count");
+	std::vector<double> u = unit_rows(w, qs, n, qs_are_unit);
+	return visibility_host(w, 1, u, u, n, n, verify_magnitude, out, (size_t)(n * n));
+}
+
+int se3cq_visibility_matrix2_host(se3cq_world* w, const double* qs0, int64_t m, int qs0_are_unit, const double* qs1,
+                                  int64_t n, int qs1_are_unit, double verify_magnitude, int8_t* out)
+{
+	CHECK_WORLD(w);
+	if (!w->ready())
+		return fail(SE3CQ_ESTATE, "models not loaded");
+	{
+		std::lock_guard<std::mutex> lk0(w->mu);
+		const int rc0 = ensure_device(w);
+		if (rc0)
+			return rc0;
+	}
+	if (m < 0 || n < 0)
+		return fail(SE3CQ_EINVAL, "negative count");
+	if (m == 0 || n == 0)
+		return SE3CQ_OK;
+	std::vector<double> u0 = unit_rows(w, qs0, m, qs0_are_unit);
+	std::vector<double> u1 = unit_rows(w, qs1, n, qs1_are_unit);
+	return visibility_host(w, 2, u0, u1, m, n, verify_magnitude, out, (size_t)(m * n));
+}
+
+int se3cq_visibility_pair_host(se3cq_world* w, const double* qs0, int qs0_are_unit, const double* qs1,
+                               int qs1_are_unit, int64_t n, double verify_magnitude, int8_t* out)
+{
+	CHECK_WORLD(w);
+	if (!w->ready())
+		return fail(SE3CQ_ESTATE, "models not loaded");
+	{
+		std::lock_guard<std::mutex> lk0(w->mu);
+		const int rc0 = ensure_device(w);
+		if (rc0)
+			return rc0;
+	}
+	if (n <= 0)
+		return n == 0 ? SE3CQ_OK : fail(SE3CQ_EINVAL, "negative count");
+	std::vector<double> u0 = unit_rows(w, qs0, n, qs0_are_unit);
+	std::vector<double> u1 = unit_rows(w, qs1, n, qs1_are_unit);
+	return visibility_host(w, 0, u0, u1, n, n, verify_magnitude, out, (size_t)n);
+}
+
+int se3cq_visibility_bits_device(se3cq_world* w, const double* d_qs, int64_t n, double verify_magnitude,
+                                 uint32_t* d_bits, int32_t* d_steps, void* stream)
+{
+	CHECK_WORLD(w);
+	if (!w->ready())
+		return fail(SE3CQ_ESTATE, "models not loaded");
+	{
+		std::lock_guard<std::mutex> lk0(w->mu);
+		const int rc0 = ensure_device(w);
+		if (rc0)
+			return rc0;
+	}
+	if (n < 0)
+		return fail(SE3CQ_EINVAL, "negative count");
+	hipStream_t st = pick_stream(w, stream);
+	const size_t words = (size_t)((n * n + 31) / 32);
+	if (words)
+		HIPCHK(launch_fill_u32(d_bits, (int64_t)words, 0u, st));
+	const hipError_t e =
+	    launch_visibility(make_scene(w), 1, d_qs, d_qs, n, n, verify_magnitude, nullptr, d_bits, d_steps, st);
+	if (e != hipSuccess)
+		return fail(SE3CQ_EDEVICE, std::string("launch failed: ") + hipGetErrorString(e));
+	return SE3CQ_OK;
+}
+
+/* ---- state arithmetic ---- */
+int se3cq_translate_to_unit(const se3cq_world* w, const double* q, int64_t n, double* out)
+{
+	CHECK_WORLD(w);
+	for (int64_t i = 0; i < n; i++)
+		unit_row(w, q + 7 * i, out + 7 * i);
+	return SE3CQ_OK;
+}
+
+int se3cq_translate_from_unit(const se3cq_world* w, const double* q, int64_t n, double* out)
+{
+	/* unapplyPertubation (unit_world.cc:726-738) then the inverse calibration
+	 * (diag(1/s)) on the translation (541-555). */
+	CHECK_WORLD(w);
+	for (int64_t i = 0; i < n; i++) {
+		const double* s = q + 7 * i;
+		double rot[4] = {s[3], s[4], s[5], s[6]};
+		se3_quat_normalize(rot);
+		double prot[4] = {w->pert[3], w->pert[4], w->pert[5], w->pert[6]};
+		se3_quat_normalize(prot);
+		double pinv[4] = {prot[0], -prot[1], -prot[2], -prot[3]};
+		double R[9];
+		se3_quat_to_rotmat(pinv[0], pinv[1], pinv[2], pinv[3], R);
+		const double d[3] = {-w->pert[0] + s[0], -w->pert[1] + s[1], -w->pert[2] + s[2]};
+		double t[3];
+		for (int r = 0; r < 3; r++)
+			t[r] = (R[3 * r] * d[0] + R[3 * r + 1] * d[1]) + R[3 * r + 2] * d[2];
+		double qr[4];
+		se3_quat_mul(pinv, rot, qr);
+		se3_quat_normalize(qr);
+		const double inv = 1.0 / (double)w->scale;
+		double* o = out + 7 * i;
+		for (int r = 0; r < 3; r++)
+			o[r] = t[r] * inv;
+		for (int k = 0; k < 4; k++)
+			o[3 + k] = qr[k];
+	}
+	return SE3CQ_OK;
+}
+
+int se3cq_distance(const double* a7, const double* b7, double* out)
+{
+	*out = se3_distance(a7, b7);
+	return SE3CQ_OK;
+}
+
+int se3cq_multi_distance(const double* origin7, const double* targets, int64_t n, double* out)
+{
+	for (int64_t i = 0; i < n; i++)
+		out[i] = se3_distance(origin7, targets + 7 * i);
+	return SE3CQ_OK;
+}
+
+int se3cq_interpolate(const double* a7, const double* b7, double tau, double* out7)
+{
+	se3_segment sg;
+	se3_segment_init(&sg, a7, b7);
+	se3_segment_at(&sg, tau, out7);
+	return SE3CQ_OK;
+}
+
+int se3cq_apply(const double* from7, const double* tr3, const double* aa3, double* out7)
+{
+	double rot[4] = {from7[3], from7[4], from7[5], from7[6]};
+	se3_quat_normalize(rot);
+	const double angle = std::sqrt((aa3[0] * aa3[0] + aa3[1] * aa3[1]) + aa3[2] * aa3[2]);
+	double axis[3] = {1.0, 0.0, 0.0};
+	if (angle != 0.0)
+		for (int k = 0; k < 3; k++)
+			axis[k] = aa3[k] / angle;
+	const double ha = 0.5 * angle;
+	const double sh = se3_sin(ha);
+	const double qa[4] = {se3_cos(ha), sh * axis[0], sh * axis[1], sh * axis[2]};
+	double r[4];
+	se3_quat_mul(qa, rot, r);
+	se3_quat_normalize(r);
+	for (int k = 0; k < 3; k++)
+		out7[k] = from7[k] + tr3[k];
+	for (int k = 0; k < 4; k++)
+		out7[3 + k] = r[k];
+	return SE3CQ_OK;
+}
+
+int se3cq_decompose_2(const double* q7, double* t3, double* R9)
+{
+	double q[4] = {q7[3], q7[4], q7[5], q7[6]};
+	se3_quat_normalize(q);
+	se3_quat_to_rotmat(q[0], q[1], q[2], q[3], R9);
+	for (int k = 0; k < 3; k++)
+		t3[k] = q7[k];
+	return SE3CQ_OK;
+}
+
+int se3cq_compose_from_angleaxis(const double* t3, double angle, const double* axis3, double* out7)
+{
+	const double ha = 0.5 * angle;
+	const double sh = se3_sin(ha);
+	double q[4] = {se3_cos(ha), sh * axis3[0], sh * axis3[1], sh * axis3[2]};
+	se3_quat_normalize(q);
+	for (int k = 0; k < 3; k++)
+		out7[k] = t3[k];
+	for (int k = 0; k < 4; k++)
+		out7[3 + k] = q[k];
+	return SE3CQ_OK;
+}
+
+} /* extern "C" */
