@@ -170,13 +170,14 @@ def main():
         O.set_libm(False)
         ow = O.OracleWorld().setup(*env, *rob)
         cpu, mism, n_checked = cpu_baseline(ow, starts, ends, res, args.cpu_seconds)
-        flops_q, nbv, ntri = oracle_flops_per_query(ow, starts, ends, min(256, n))
+        flops_q, nbv, ntri = oracle_flops_per_query(ow, starts, ends, min(2048, n))
     roof = None
     if flops_q is not None:
         achieved = queries * flops_q / (kern_ms * 1e-3) / 1e12
         roof = dict(bound="fp32-valu", achieved=round(achieved, 3), peak=PEAK_FP32_TFLOPS, unit="TFLOP/s",
                     frac=round(achieved / PEAK_FP32_TFLOPS, 5), traffic=None,
-                    flops_per_query=round(flops_q, 1), kernel="k_segments<touch>",
+                    flops_per_query=round(flops_q, 1), oracle_node_tests_per_query=round(nbv, 2),
+                    oracle_tri_tests_per_query=round(ntri, 3), kernel="k_segments<touch>",
                     kernel_ms=round(kern_ms, 4))
     line = {
         "metric": METRIC,

@@ -53,12 +53,15 @@ struct DevScene {
 	DevModel rob;
 	double Re[9];    /* perturbation rotation (raw quaternion), row-major */
 	double te[3];    /* perturbation translation */
-	double ReHat[9]; /* perturbation rotation from the normalised quaternion */
+	float PeT[9];    /* binary32 transpose of the normalised perturbation rotation */
+	float fro_Re;    /* ||Re||_F, rounded up */
+	float fro_dRe;   /* sqrt(3) * ||Re - R(normalised)||_F, rounded up */
 	float rob_reach; /* max over robot nodes of |centre| + |extents|, rounded up */
 	float scene_mag; /* max |centre|_inf + max |extent| over both models */
-	int32_t stack_depth; /* per-thread traversal stack entries */
-	int32_t wide;        /* 1 if node ids need more than 16 bits */
-	uint32_t* error_flag; /* device word, non-zero after a traversal overflow */
+	int32_t pert_identity; /* perturbation is exactly [0,0,0,1,0,0,0] */
+	int32_t stack_depth;   /* per-thread traversal stack entries */
+	int32_t wide;          /* 1 if node ids need more than 16 bits */
+	uint32_t* error_flag;  /* device word, non-zero after a traversal overflow */
 };
 
 } // namespace se3cq

@@ -22,74 +22,113 @@
 #include <hip/hip_runtime.h>
 
 #include "engine.h"
-#include "se3_math.h"
 #include "launch.h"
+#include "se3_math.h"
+
+/* Minimum resident wavefronts per SIMD the traversal kernels are compiled
+ * for (caps their VGPR budget). */
+#ifndef SE3CQ_MIN_WAVES
+#define SE3CQ_MIN_WAVES 4
+#endif
 
 namespace se3cq {
 
 enum { MODE_TRANSIT = 0, MODE_TRANSITION = 1, MODE_TOUCH = 2 };
 enum { VIS_PAIR = 0, VIS_MATRIX = 1, VIS_MATRIX2 = 2 };
 
-/* Per-query transforms.  R, T: the exact map applied to robot vertices in
- * the leaf test (FCL relative transform Re^T Rr, Re^T (tr - te)).  O: the
- * same rotation from normalised quaternions, rounded to binary32, used to
- * orient robot boxes; infl bounds ||R - O||_F * |box reach| so robot boxes
- * grown by it contain the exact images of their triangles. */
-struct QTf {
-	double R[9], T[3];
-	float O[9], Tf[3];
-	float infl, eta;
+/*
+ * Culling transform of one query (binary32).  O orients robot boxes in the
+ * env frame and T places them.  The exact map the leaf test applies is
+ * L q + T' with L = Re^T R(q) and T' = Re^T (t - te) in binary64, where R(q)
+ * is Eigen's toRotationMatrix of the raw (possibly non-unit) quaternion.
+ * Since R(q) = n^2 R(q/n) + (1 - n^2) I (n = |q|), ||L - O||_F is at most
+ * ||Re||_F * 2.83 * |1 - n^2| + sqrt(3) ||Re - R(pert/|pert|)||_F plus the
+ * binary32 rounding of O (below 3e-5); robot boxes grown by that bound times
+ * their reach (|centre| + |extents|) contain the exact images of their
+ * triangles.  eta absorbs the binary32 rounding of translations.
+ */
+struct QCull {
+	float O[9];
+	float T[3];
+	float infl;
+	float eta;
 };
 
-__device__ __forceinline__ void make_qtf(const DevScene& s, const double* q, QTf& t)
+__device__ __forceinline__ void make_qcull(const DevScene& s, const double* q, QCull& c)
 {
-	double Rr[9];
-	se3_quat_to_rotmat(q[3], q[4], q[5], q[6], Rr);
-	double d[3] = {q[0] - s.te[0], q[1] - s.te[1], q[2] - s.te[2]};
+	const float w = (float)q[3], x = (float)q[4], y = (float)q[5], z = (float)q[6];
+	const float n2 = fmaf(w, w, fmaf(x, x, fmaf(y, y, z * z)));
+	float Rq[9];
+	if (n2 > 0.0f) {
+		const float inv = rsqrtf(n2);
+		const float hw = w * inv, hx = x * inv, hy = y * inv, hz = z * inv;
+		const float tx = 2.0f * hx, ty = 2.0f * hy, tz = 2.0f * hz;
+		Rq[0] = 1.0f - fmaf(ty, hy, tz * hz);
+		Rq[1] = fmaf(ty, hx, -tz * hw);
+		Rq[2] = fmaf(tz, hx, ty * hw);
+		Rq[3] = fmaf(ty, hx, tz * hw);
+		Rq[4] = 1.0f - fmaf(tx, hx, tz * hz);
+		Rq[5] = fmaf(tz, hy, -tx * hw);
+		Rq[6] = fmaf(tz, hx, -ty * hw);
+		Rq[7] = fmaf(tz, hy, tx * hw);
+		Rq[8] = 1.0f - fmaf(tx, hx, ty * hy);
+	} else {
+		/* R(0) is exactly the identity */
 #pragma unroll
-	for (int i = 0; i < 3; i++) {
-#pragma unroll
-		for (int j = 0; j < 3; j++)
-			t.R[3 * i + j] = (s.Re[i] * Rr[j] + s.Re[3 + i] * Rr[3 + j]) + s.Re[6 + i] * Rr[6 + j];
-		t.T[i] = (s.Re[i] * d[0] + s.Re[3 + i] * d[1]) + s.Re[6 + i] * d[2];
+		for (int i = 0; i < 9; i++)
+			Rq[i] = (i % 4 == 0) ? 1.0f : 0.0f;
 	}
-	double qn[4] = {q[3], q[4], q[5], q[6]};
-	se3_quat_normalize(qn);
-	double Rn[9];
-	se3_quat_to_rotmat(qn[0], qn[1], qn[2], qn[3], Rn);
-	double s2 = 0.0;
+	float dl1 = 0.0f;
+	if (s.pert_identity) {
 #pragma unroll
-	for (int i = 0; i < 3; i++)
+		for (int i = 0; i < 9; i++)
+			c.O[i] = Rq[i];
 #pragma unroll
-		for (int j = 0; j < 3; j++) {
-			double o = (s.ReHat[i] * Rn[j] + s.ReHat[3 + i] * Rn[3 + j]) + s.ReHat[6 + i] * Rn[6 + j];
-			float of = (float)o;
-			t.O[3 * i + j] = of;
-			double dd = t.R[3 * i + j] - (double)of;
-			s2 += dd * dd;
+		for (int i = 0; i < 3; i++)
+			c.T[i] = (float)q[i];
+	} else {
+		float d[3];
+#pragma unroll
+		for (int i = 0; i < 3; i++) {
+			d[i] = (float)(q[i] - s.te[i]);
+			dl1 += fabsf(d[i]);
 		}
-	/* (1 + 2^-20) absorbs the binary32 rounding of the product */
-	t.infl = (float)(sqrt(s2) * (double)s.rob_reach * (1.0 + 0x1p-20));
-	float tm = 0.0f;
 #pragma unroll
-	for (int i = 0; i < 3; i++) {
-		t.Tf[i] = (float)t.T[i];
-		tm = fmaxf(tm, fabsf(t.Tf[i]));
+		for (int i = 0; i < 3; i++) {
+#pragma unroll
+			for (int j = 0; j < 3; j++)
+				c.O[3 * i + j] = fmaf(s.PeT[3 * i], Rq[j], fmaf(s.PeT[3 * i + 1], Rq[3 + j], s.PeT[3 * i + 2] * Rq[6 + j]));
+			c.T[i] = fmaf(s.PeT[3 * i], d[0], fmaf(s.PeT[3 * i + 1], d[1], s.PeT[3 * i + 2] * d[2]));
+		}
 	}
-	t.eta = 1e-5f * (tm + s.scene_mag);
+	const float dev = fabsf(1.0f - n2) + 1e-6f * n2;
+	const float delta = fmaf(s.fro_Re * 2.83f, dev, s.fro_dRe + 3e-5f);
+	c.infl = delta * s.rob_reach;
+	const float tm = fmaxf(fabsf(c.T[0]), fmaxf(fabsf(c.T[1]), fabsf(c.T[2])));
+	c.eta = fmaf(1e-5f, tm + s.scene_mag, s.fro_dRe * dl1);
 }
 
-/* Transformed robot box (binary32), cached across consecutive tests that
- * share the robot node. */
+/* Robot box in the env frame, cached across consecutive tests that share the
+ * robot node. */
 struct RobBox {
 	float W[9]; /* column j = robot axis j in the env frame */
 	float c[3];
 	float e[3];
-	float size;
 	int child;
 };
 
-__device__ __forceinline__ void load_rob_box(const GNode& B, const QTf& t, RobBox& r)
+__device__ __forceinline__ GNode load_node(const GNode* __restrict__ nodes, int i)
+{
+	const float4* p = reinterpret_cast<const float4*>(nodes + i);
+	GNode n;
+	n.c_ex = p[0];
+	n.a0_ey = p[1];
+	n.a1_ez = p[2];
+	n.a2_child = p[3];
+	return n;
+}
+
+__device__ __forceinline__ void load_rob_box(const GNode& B, const QCull& t, RobBox& r)
 {
 	const float b[3][3] = {{B.a0_ey.x, B.a0_ey.y, B.a0_ey.z},
 	                       {B.a1_ez.x, B.a1_ez.y, B.a1_ez.z},
@@ -99,12 +138,11 @@ __device__ __forceinline__ void load_rob_box(const GNode& B, const QTf& t, RobBo
 #pragma unroll
 		for (int j = 0; j < 3; j++)
 			r.W[3 * i + j] = fmaf(t.O[3 * i], b[j][0], fmaf(t.O[3 * i + 1], b[j][1], t.O[3 * i + 2] * b[j][2]));
-		r.c[i] = fmaf(t.O[3 * i], B.c_ex.x, fmaf(t.O[3 * i + 1], B.c_ex.y, fmaf(t.O[3 * i + 2], B.c_ex.z, t.Tf[i])));
+		r.c[i] = fmaf(t.O[3 * i], B.c_ex.x, fmaf(t.O[3 * i + 1], B.c_ex.y, fmaf(t.O[3 * i + 2], B.c_ex.z, t.T[i])));
 	}
 	r.e[0] = B.c_ex.w + t.infl;
 	r.e[1] = B.a0_ey.w + t.infl;
 	r.e[2] = B.a1_ez.w + t.infl;
-	r.size = fmaf(B.c_ex.w, B.c_ex.w, fmaf(B.a0_ey.w, B.a0_ey.w, B.a1_ez.w * B.a1_ez.w));
 	r.child = __float_as_int(B.a2_child.w);
 }
 
@@ -114,15 +152,17 @@ __device__ __forceinline__ void load_rob_box(const GNode& B, const QTf& t, RobBo
  * 3e-5 plus eta = 1e-5 * (|T| + scene magnitude): binary32 rounding of
  * these ~10-operation expressions is below 2e-6 relative, so a binary32
  * "disjoint" implies the exact boxes are disjoint.  |R| gets FCL's 1e-6.
+ * The three env-axis tests run first, row by row, so most disjoint pairs
+ * exit before the full 3x3 relative rotation is formed.
  */
 __device__ __forceinline__ bool obb_disjoint(const GNode& A, const RobBox& B, float eta)
 {
+	const float K = 1.00003f;
+	const float d[3] = {B.c[0] - A.c_ex.x, B.c[1] - A.c_ex.y, B.c[2] - A.c_ex.z};
+	const float ea[3] = {A.c_ex.w, A.a0_ey.w, A.a1_ez.w};
 	const float a[3][3] = {{A.a0_ey.x, A.a0_ey.y, A.a0_ey.z},
 	                       {A.a1_ez.x, A.a1_ez.y, A.a1_ez.z},
 	                       {A.a2_child.x, A.a2_child.y, A.a2_child.z}};
-	const float ea[3] = {A.c_ex.w, A.a0_ey.w, A.a1_ez.w};
-	const float d[3] = {B.c[0] - A.c_ex.x, B.c[1] - A.c_ex.y, B.c[2] - A.c_ex.z};
-	const float K = 1.00003f;
 	float t[3], Rr[3][3], Bf[3][3];
 #pragma unroll
 	for (int i = 0; i < 3; i++) {
@@ -132,18 +172,15 @@ __device__ __forceinline__ bool obb_disjoint(const GNode& A, const RobBox& B, fl
 			Rr[i][j] = fmaf(a[i][0], B.W[j], fmaf(a[i][1], B.W[3 + j], a[i][2] * B.W[6 + j]));
 			Bf[i][j] = fabsf(Rr[i][j]) + 1e-6f;
 		}
-	}
-#pragma unroll
-	for (int i = 0; i < 3; i++) {
-		float r = ea[i] + fmaf(B.e[0], Bf[i][0], fmaf(B.e[1], Bf[i][1], B.e[2] * Bf[i][2]));
+		const float r = ea[i] + fmaf(B.e[0], Bf[i][0], fmaf(B.e[1], Bf[i][1], B.e[2] * Bf[i][2]));
 		if (fabsf(t[i]) > fmaf(r, K, eta))
 			return true;
 	}
 #pragma unroll
 	for (int j = 0; j < 3; j++) {
-		float s = fmaf(t[0], Rr[0][j], fmaf(t[1], Rr[1][j], t[2] * Rr[2][j]));
-		float r = B.e[j] + fmaf(ea[0], Bf[0][j], fmaf(ea[1], Bf[1][j], ea[2] * Bf[2][j]));
-		if (fabsf(s) > fmaf(r, K, eta))
+		const float sj = fmaf(t[0], Rr[0][j], fmaf(t[1], Rr[1][j], t[2] * Rr[2][j]));
+		const float r = B.e[j] + fmaf(ea[0], Bf[0][j], fmaf(ea[1], Bf[1][j], ea[2] * Bf[2][j]));
+		if (fabsf(sj) > fmaf(r, K, eta))
 			return true;
 	}
 #pragma unroll
@@ -152,9 +189,9 @@ __device__ __forceinline__ bool obb_disjoint(const GNode& A, const RobBox& B, fl
 #pragma unroll
 		for (int j = 0; j < 3; j++) {
 			const int j1 = (j + 1) % 3, j2 = (j + 2) % 3;
-			float s = fmaf(t[i2], Rr[i1][j], -t[i1] * Rr[i2][j]);
-			float r = fmaf(ea[i1], Bf[i2][j], fmaf(ea[i2], Bf[i1][j], fmaf(B.e[j1], Bf[i][j2], B.e[j2] * Bf[i][j1])));
-			if (fabsf(s) > fmaf(r, K, eta))
+			const float sx = fmaf(t[i2], Rr[i1][j], -t[i1] * Rr[i2][j]);
+			const float r = fmaf(ea[i1], Bf[i2][j], fmaf(ea[i2], Bf[i1][j], fmaf(B.e[j1], Bf[i][j2], B.e[j2] * Bf[i][j1])));
+			if (fabsf(sx) > fmaf(r, K, eta))
 				return true;
 		}
 	}
@@ -172,12 +209,13 @@ __device__ __forceinline__ void cross3d(const double* a, const double* b, double
 	r[1] = a[2] * b[0] - a[0] * b[2];
 	r[2] = a[0] * b[1] - a[1] * b[0];
 }
+/* project6: the two triangles' intervals on axis ax overlap (closed). */
 __device__ __forceinline__ bool overlap6(const double* ax, const double* p1, const double* p2,
                                          const double* p3, const double* q1, const double* q2,
                                          const double* q3)
 {
-	double P1 = dot3d(ax, p1), P2 = dot3d(ax, p2), P3 = dot3d(ax, p3);
-	double Q1 = dot3d(ax, q1), Q2 = dot3d(ax, q2), Q3 = dot3d(ax, q3);
+	const double P1 = dot3d(ax, p1), P2 = dot3d(ax, p2), P3 = dot3d(ax, p3);
+	const double Q1 = dot3d(ax, q1), Q2 = dot3d(ax, q2), Q3 = dot3d(ax, q3);
 	/* std::max / std::min semantics (also for NaN operands) */
 	double t = (P2 < P3) ? P3 : P2;
 	const double mx1 = (P1 < t) ? t : P1;
@@ -190,70 +228,129 @@ __device__ __forceinline__ bool overlap6(const double* ax, const double* p1, con
 	return !(mn1 > mx2) && !(mn2 > mx1);
 }
 
-__device__ bool tri_intersect(const double* __restrict__ P, const double* __restrict__ Q)
+/*
+ * 17-axis SAT on triangles already translated by -P1 (p1 is the zero vector,
+ * as FCL's P1 - P1).  Edges are formed on the fly (e1 = p2 - p1,
+ * e2 = p3 - p2, e3 = p1 - p3, f likewise) so only the six points and the two
+ * face normals stay live.  Axis order and arithmetic follow FCL's
+ * intersect_Triangle; the boolean does not depend on the order.
+ */
+__device__ __forceinline__ bool sat17(const double* p1, const double* p2, const double* p3, const double* q1,
+                                      const double* q2, const double* q3)
 {
-	double p1[3], p2[3], p3[3], q1[3], q2[3], q3[3];
+	double e1[3], e2[3], n1[3], m1[3], ax[3], u[3], v[3];
 #pragma unroll
 	for (int k = 0; k < 3; k++) {
-		p1[k] = P[k] - P[k];
-		p2[k] = P[3 + k] - P[k];
-		p3[k] = P[6 + k] - P[k];
-		q1[k] = Q[k] - P[k];
-		q2[k] = Q[3 + k] - P[k];
-		q3[k] = Q[6 + k] - P[k];
+		e1[k] = p2[k] - p1[k];
+		e2[k] = p3[k] - p2[k];
 	}
-	double e[3][3], f[3][3], n1[3], m1[3], ax[3];
-#pragma unroll
-	for (int k = 0; k < 3; k++) {
-		e[0][k] = p2[k] - p1[k];
-		e[1][k] = p3[k] - p2[k];
-		e[2][k] = p1[k] - p3[k];
-		f[0][k] = q2[k] - q1[k];
-		f[1][k] = q3[k] - q2[k];
-		f[2][k] = q1[k] - q3[k];
-	}
-	cross3d(e[0], e[1], n1);
+	cross3d(e1, e2, n1);
 	if (!overlap6(n1, p1, p2, p3, q1, q2, q3)) return false;
-	cross3d(f[0], f[1], m1);
+#pragma unroll
+	for (int k = 0; k < 3; k++) {
+		u[k] = q2[k] - q1[k];
+		v[k] = q3[k] - q2[k];
+	}
+	cross3d(u, v, m1);
 	if (!overlap6(m1, p1, p2, p3, q1, q2, q3)) return false;
-	for (int i = 0; i < 3; i++)
+#pragma unroll 1
+	for (int i = 0; i < 3; i++) {
+		const double* a0 = (i == 0) ? p2 : (i == 1) ? p3 : p1;
+		const double* a1 = (i == 0) ? p1 : (i == 1) ? p2 : p3;
+#pragma unroll
+		for (int k = 0; k < 3; k++)
+			u[k] = a0[k] - a1[k];
+#pragma unroll 1
 		for (int j = 0; j < 3; j++) {
-			cross3d(e[i], f[j], ax);
+			const double* b0 = (j == 0) ? q2 : (j == 1) ? q3 : q1;
+			const double* b1 = (j == 0) ? q1 : (j == 1) ? q2 : q3;
+#pragma unroll
+			for (int k = 0; k < 3; k++)
+				v[k] = b0[k] - b1[k];
+			cross3d(u, v, ax);
 			if (!overlap6(ax, p1, p2, p3, q1, q2, q3)) return false;
 		}
+	}
+#pragma unroll 1
 	for (int i = 0; i < 3; i++) {
-		cross3d(e[i], n1, ax);
+		const double* a0 = (i == 0) ? p2 : (i == 1) ? p3 : p1;
+		const double* a1 = (i == 0) ? p1 : (i == 1) ? p2 : p3;
+#pragma unroll
+		for (int k = 0; k < 3; k++)
+			u[k] = a0[k] - a1[k];
+		cross3d(u, n1, ax);
 		if (!overlap6(ax, p1, p2, p3, q1, q2, q3)) return false;
 	}
-	for (int i = 0; i < 3; i++) {
-		cross3d(f[i], m1, ax);
+#pragma unroll 1
+	for (int j = 0; j < 3; j++) {
+		const double* b0 = (j == 0) ? q2 : (j == 1) ? q3 : q1;
+		const double* b1 = (j == 0) ? q1 : (j == 1) ? q2 : q3;
+#pragma unroll
+		for (int k = 0; k < 3; k++)
+			v[k] = b0[k] - b1[k];
+		cross3d(v, m1, ax);
 		if (!overlap6(ax, p1, p2, p3, q1, q2, q3)) return false;
 	}
 	return true;
 }
 
-__device__ __forceinline__ bool leaf_test(const DevScene& s, const QTf& t, int te, int tr)
+/* Exact binary64 map of robot vertices into the env frame for state q: the
+ * FCL relative transform R = Re^T R(q), T = Re^T (t - te). */
+struct ExactTf {
+	double R[9], T[3];
+};
+
+__device__ __forceinline__ void make_exact(const DevScene& s, const double* q, ExactTf& x)
 {
-	const double* P = s.env.tris + 9 * (int64_t)te;
-	const double* q = s.rob.tris + 9 * (int64_t)tr;
-	double Pl[9], Q[9];
+	double Rr[9];
+	se3_quat_to_rotmat(q[3], q[4], q[5], q[6], Rr);
+	const double d[3] = {q[0] - s.te[0], q[1] - s.te[1], q[2] - s.te[2]};
 #pragma unroll
-	for (int k = 0; k < 9; k++)
-		Pl[k] = P[k];
+	for (int i = 0; i < 3; i++) {
+#pragma unroll
+		for (int j = 0; j < 3; j++)
+			x.R[3 * i + j] = (s.Re[i] * Rr[j] + s.Re[3 + i] * Rr[3 + j]) + s.Re[6 + i] * Rr[6 + j];
+		x.T[i] = (s.Re[i] * d[0] + s.Re[3 + i] * d[1]) + s.Re[6 + i] * d[2];
+	}
+}
+
+/*
+ * One (env triangle, robot triangle) pair at state q: robot vertices R q + T,
+ * both triangles translated by -P1, then the SAT.  P1 - P1 is the zero vector
+ * for the finite coordinates the host admits, so p1 is the literal 0.
+ */
+__device__ __forceinline__ bool leaf_pair(const DevScene& s, const double* q, int te, int tr)
+{
+	ExactTf x;
+	make_exact(s, q, x);
+	const double* P = s.env.tris + 9 * (int64_t)te;
+	const double* qv = s.rob.tris + 9 * (int64_t)tr;
+	const double P1[3] = {P[0], P[1], P[2]};
+	const double p1[3] = {0.0, 0.0, 0.0};
+	double p2[3], p3[3], q1[3], q2[3], q3[3];
 #pragma unroll
 	for (int k = 0; k < 3; k++) {
-		double q0 = q[3 * k], q1 = q[3 * k + 1], q2 = q[3 * k + 2];
+		p2[k] = P[3 + k] - P1[k];
+		p3[k] = P[6 + k] - P1[k];
+	}
+	double* qq[3] = {q1, q2, q3};
+#pragma unroll
+	for (int v = 0; v < 3; v++) {
+		const double a = qv[3 * v], b = qv[3 * v + 1], c = qv[3 * v + 2];
 #pragma unroll
 		for (int r = 0; r < 3; r++)
-			Q[3 * k + r] = ((t.R[3 * r] * q0 + t.R[3 * r + 1] * q1) + t.R[3 * r + 2] * q2) + t.T[r];
+			qq[v][r] = (((x.R[3 * r] * a + x.R[3 * r + 1] * b) + x.R[3 * r + 2] * c) + x.T[r]) - P1[r];
 	}
-	return tri_intersect(Pl, Q);
+	return sat17(p1, p2, p3, q1, q2, q3);
 }
 
 /* ---- depth-first BVH x BVH traversal, one thread per query ---- */
-/* Stack entries live in LDS, strided by the block size so that lanes at the
- * same depth touch consecutive banks.  Narrow scenes pack (env, rob) node ids
- * into one 32-bit word; wide scenes use two words. */
+/* Stack entries and the pending-leaf buffer live in LDS, strided by the
+ * block size so that lanes at the same depth touch consecutive banks.
+ * Narrow scenes pack (env, rob) ids into one 32-bit word; wide scenes use two
+ * words. */
+enum { LEAF_BUF = 4 };
+
 template <bool WIDE>
 struct PairStack {
 	uint32_t* base;
@@ -273,99 +370,178 @@ struct PairStack {
 			a = (int)base[(2 * k) * stride];
 			b = (int)base[(2 * k + 1) * stride];
 		} else {
-			uint32_t v = base[k * stride];
+			const uint32_t v = base[k * stride];
 			a = (int)(v >> 16);
 			b = (int)(v & 0xffffu);
 		}
 	}
 };
 
-__device__ __forceinline__ GNode load_node(const GNode* __restrict__ nodes, int i)
-{
-	const float4* p = reinterpret_cast<const float4*>(nodes + i);
-	GNode n;
-	n.c_ex = p[0];
-	n.a0_ey = p[1];
-	n.a1_ez = p[2];
-	n.a2_child = p[3];
-	return n;
-}
+/* Where a traversal finds its exact binary64 state when a leaf test needs it. */
+struct StateFromGlobal {
+	const double* q;
+	__device__ __forceinline__ void get(double* o) const
+	{
+#pragma unroll
+		for (int k = 0; k < 7; k++)
+			o[k] = q[k];
+	}
+};
 
-template <bool WIDE>
-__device__ bool collide_dfs(const DevScene& s, const QTf& t, const PairStack<WIDE>& stk)
+struct StateFromSegment {
+	const se3_segment* sg; /* LDS */
+	double tau;
+	__device__ __forceinline__ void get(double* o) const { se3_segment_at(sg, tau, o); }
+};
+
+/*
+ * Box pairs are tested in binary32; leaf-leaf pairs that survive are queued
+ * (up to LEAF_BUF per thread) and tested exactly in a separate phase, so the
+ * traversal's registers are free while the binary64 SAT runs and the lanes
+ * of a wave run their SATs together.  Stops at the first intersecting pair.
+ */
+template <bool WIDE, class Src>
+__device__ bool collide_dfs(const DevScene& s, const Src& src, const PairStack<WIDE>& stk,
+                            const PairStack<WIDE>& leaves)
 {
 	int sp = 0;
 	stk.put(sp++, 0, 0);
-	int cached_b = -1;
-	RobBox B;
 	const int cap = s.stack_depth;
-	while (sp > 0) {
-		int a, b;
-		stk.get(--sp, a, b);
-		GNode A = load_node(s.env.nodes, a);
-		if (b != cached_b) {
-			GNode Bn = load_node(s.rob.nodes, b);
-			load_rob_box(Bn, t, B);
-			cached_b = b;
+	for (;;) {
+		int nleaf = 0;
+		{
+			/* the culling transform lives only through the traversal phase */
+			QCull c;
+			{
+				double q[7];
+				src.get(q);
+				make_qcull(s, q, c);
+			}
+			int cached_b = -1;
+			RobBox B;
+			while (sp > 0 && nleaf < LEAF_BUF) {
+				int a, b;
+				stk.get(--sp, a, b);
+				const GNode A = load_node(s.env.nodes, a);
+				if (b != cached_b) {
+					const GNode Bn = load_node(s.rob.nodes, b);
+					load_rob_box(Bn, c, B);
+					cached_b = b;
+				}
+				if (obb_disjoint(A, B, c.eta))
+					continue;
+				const int ca = __float_as_int(A.a2_child.w);
+				const bool la = ca < 0, lb = B.child < 0;
+				if (la && lb) {
+					leaves.put(nleaf++, ~ca, ~B.child);
+					continue;
+				}
+				if (sp + 2 > cap) {
+					atomicOr(s.error_flag, 1u);
+					return true;
+				}
+				/* FCL's rule: split the env node unless it is a leaf or the
+				 * robot box is the larger one (squared extent norms). */
+				const float sa = fmaf(A.c_ex.w, A.c_ex.w, fmaf(A.a0_ey.w, A.a0_ey.w, A.a1_ez.w * A.a1_ez.w));
+				const float eb0 = B.e[0] - c.infl, eb1 = B.e[1] - c.infl, eb2 = B.e[2] - c.infl;
+				const float sb = fmaf(eb0, eb0, fmaf(eb1, eb1, eb2 * eb2));
+				if (!la && (lb || sa > sb)) {
+					stk.put(sp++, ca + 1, b);
+					stk.put(sp++, ca, b);
+				} else {
+					stk.put(sp++, a, B.child + 1);
+					stk.put(sp++, a, B.child);
+				}
+			}
 		}
-		if (obb_disjoint(A, B, t.eta))
-			continue;
-		const int ca = __float_as_int(A.a2_child.w);
-		const bool la = ca < 0, lb = B.child < 0;
-		if (la && lb) {
-			if (leaf_test(s, t, ~ca, ~B.child))
+		for (int i = 0; i < nleaf; i++) {
+			int te, tr;
+			leaves.get(i, te, tr);
+			double q[7];
+			src.get(q);
+			if (leaf_pair(s, q, te, tr))
 				return true;
-			continue;
 		}
-		if (sp + 2 > cap) {
-			atomicOr(s.error_flag, 1u);
-			return true;
-		}
-		const float sa = fmaf(A.c_ex.w, A.c_ex.w, fmaf(A.a0_ey.w, A.a0_ey.w, A.a1_ez.w * A.a1_ez.w));
-		if (!la && (lb || sa > B.size)) {
-			stk.put(sp++, ca + 1, b);
-			stk.put(sp++, ca, b);
-		} else {
-			stk.put(sp++, a, B.child + 1);
-			stk.put(sp++, a, B.child);
-		}
+		if (sp == 0)
+			return false;
 	}
-	return false;
 }
 
-template <bool WIDE>
-__device__ __forceinline__ bool state_collides(const DevScene& s, const double* q, uint32_t* smem)
+/* LDS words per thread: traversal stack plus the pending-leaf buffer. */
+__host__ __device__ __forceinline__ int lds_words_per_thread(const DevScene& s)
 {
-	QTf t;
-	make_qtf(s, q, t);
-	PairStack<WIDE> stk{smem + threadIdx.x, (int)blockDim.x};
-	return collide_dfs<WIDE>(s, t, stk);
+	return (s.stack_depth + LEAF_BUF) * (s.wide ? 2 : 1);
 }
 
 template <bool WIDE>
-__global__ void __launch_bounds__(256) k_valid_states(DevScene s, const double* __restrict__ qs, int64_t n,
+__device__ __forceinline__ void thread_stacks(const DevScene& s, uint32_t* smem, PairStack<WIDE>& stk,
+                                              PairStack<WIDE>& leaves)
+{
+	const int stride = (int)blockDim.x;
+	stk = PairStack<WIDE>{smem + threadIdx.x, stride};
+	leaves = PairStack<WIDE>{smem + threadIdx.x + (size_t)s.stack_depth * (WIDE ? 2 : 1) * stride, stride};
+}
+
+template <bool WIDE>
+__global__ void __launch_bounds__(256, SE3CQ_MIN_WAVES) k_valid_states(DevScene s, const double* __restrict__ qs, int64_t n,
                                                       uint8_t* __restrict__ out)
 {
 	extern __shared__ uint32_t smem[];
 	const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n)
 		return;
-	double q[7];
-#pragma unroll
-	for (int k = 0; k < 7; k++)
-		q[k] = qs[7 * i + k];
-	out[i] = state_collides<WIDE>(s, q, smem) ? 0 : 1;
+	const double* qp = qs + 7 * i;
+	PairStack<WIDE> stk, leaves;
+	thread_stacks<WIDE>(s, smem, stk, leaves);
+	out[i] = collide_dfs<WIDE>(s, StateFromGlobal{qp}, stk, leaves) ? 0 : 1;
 }
 
 /* ---- segments: one wavefront per segment ---- */
 __device__ __forceinline__ double shfl_d(double v, int src) { return __shfl(v, src, 64); }
+
+/* Per-wave segment record in LDS. */
+struct SegShared {
+	se3_segment sg;
+	double dist, inv_dist, vd;
+	int exact; /* every partial sum k*vd of the step sequence is exact */
+	int early;
+};
+
+template <bool WIDE>
+__device__ __forceinline__ bool seg_state_collides(const DevScene& s, const SegShared* sh, double tau, uint32_t* smem)
+{
+	PairStack<WIDE> stk, leaves;
+	thread_stacks<WIDE>(s, smem, stk, leaves);
+	return collide_dfs<WIDE>(s, StateFromSegment{&sh->sg, tau}, stk, leaves);
+}
+
+/*
+ * True when the sequential sums vd, vd+vd, ... of transitStateToWithContact's
+ * stepping loop are all exact up to dist + vd, so the k-th step sits at
+ * exactly k*vd: vd's lowest set bit is 2^e and every sum stays below
+ * 2^(e+53).
+ */
+__device__ __forceinline__ int steps_exact(double vd, double dist)
+{
+	if (!(vd > 0.0) || !(dist < 1e300))
+		return 0;
+	const unsigned long long bits = (unsigned long long)__double_as_longlong(vd);
+	const int bexp = (int)((bits >> 52) & 0x7ff);
+	if (bexp == 0 || bexp == 0x7ff)
+		return 0;
+	const unsigned long long mant = (bits & 0xfffffffffffffULL) | (1ULL << 52);
+	const int e = bexp - 1075 + __ffsll((long long)mant) - 1;
+	if (e + 53 > 1000)
+		return 0;
+	return (dist + 2.0 * vd) < ldexp(1.0, e + 53);
+}
 
 /* Exact parallel form of `levels` bisection halvings: each round evaluates
  * the 31 midpoints of a 5-level probe tree (every midpoint computed by the
  * same 0.5*(lo+hi) sequence the sequential search would use), then walks the
  * tree with the ballot of results. */
 template <bool WIDE>
-__device__ void bisect_wave(const DevScene& s, const se3_segment& sg, double& lo, double& hi, int levels,
+__device__ void bisect_wave(const DevScene& s, const SegShared* sh, double& lo, double& hi, int levels,
                             uint32_t* smem)
 {
 	const int lane = threadIdx.x & 63;
@@ -380,16 +556,14 @@ __device__ void bisect_wave(const DevScene& s, const se3_segment& sg, double& lo
 			const int dh = 31 - __clz(h);
 			double l0 = lo, h0 = hi;
 			for (int bbit = dh - 1; bbit >= 0; bbit--) {
-				double m = 0.5 * (l0 + h0);
+				const double m = 0.5 * (l0 + h0);
 				if ((h >> bbit) & 1)
 					l0 = m;
 				else
 					h0 = m;
 			}
 			mid = 0.5 * (l0 + h0);
-			double st[7];
-			se3_segment_at(&sg, mid, st);
-			ok = !state_collides<WIDE>(s, st, smem);
+			ok = !seg_state_collides<WIDE>(s, sh, mid, smem);
 		}
 		const uint64_t vm = __ballot(act && ok);
 		int h = 1;
@@ -415,21 +589,43 @@ struct SegResult {
 	double last_tau, tau, lo, hi;
 };
 
-template <bool WIDE, int MODE>
-__device__ SegResult segment_wave(const DevScene& s, const double* __restrict__ from, const double* __restrict__ to,
-                                  double vd_in, int levels, se3_segment& sg, uint32_t* smem)
+/* Fill the wave's segment record (every lane computes it; lane 0 stores). */
+template <int MODE>
+__device__ __forceinline__ void seg_setup(SegShared* sh, const double* __restrict__ from,
+                                          const double* __restrict__ to, double vd_in)
 {
-	const int lane = threadIdx.x & 63;
-	SegResult r;
-	r.early = 0;
-	r.lo_is_from = 0;
-	const double dist = se3_distance(from, to);
+	double f[7], t[7];
+#pragma unroll
+	for (int k = 0; k < 7; k++) {
+		f[k] = from[k];
+		t[k] = to[k];
+	}
+	const double dist = se3_distance(f, t);
 	double vd = vd_in;
 	if (MODE == MODE_TRANSITION) {
 		const double half = dist / 2.0;
 		vd = (vd_in < half) ? vd_in : half; /* std::min(d / 2, initial_verify_delta) */
 	}
-	if (vd >= dist) {
+	se3_segment sg;
+	se3_segment_init(&sg, f, t);
+	if ((threadIdx.x & 63) == 0) {
+		sh->sg = sg;
+		sh->dist = dist;
+		sh->inv_dist = 1.0 / dist;
+		sh->vd = vd;
+		sh->early = vd >= dist;
+		sh->exact = steps_exact(vd, dist);
+	}
+}
+
+template <bool WIDE, int MODE>
+__device__ SegResult segment_walk(const DevScene& s, const SegShared* sh, int levels, uint32_t* smem)
+{
+	const int lane = threadIdx.x & 63;
+	SegResult r;
+	r.early = 0;
+	r.lo_is_from = 0;
+	if (sh->early) {
 		r.done = 0;
 		r.early = 1;
 		r.lo_is_from = 1;
@@ -440,29 +636,36 @@ __device__ SegResult segment_wave(const DevScene& s, const double* __restrict__ 
 		r.hi = 1.0;
 		return r;
 	}
-	const double inv_dist = 1.0 / dist;
-	se3_segment_init(&sg, from, to);
+	const double dist = sh->dist, inv_dist = sh->inv_dist, vd = sh->vd;
+	const int exact = sh->exact;
 	double dbase = vd;
 	double prev_tau = 0.0;
 	int64_t base = 0;
 	for (;;) {
-		double d = dbase;
-		for (int k = 0; k < lane; k++) {
-			if (d < dist) {
-				d += vd;
-				d = (d < dist) ? d : dist;
-			} else {
-				d += vd;
+		double d;
+		bool active;
+		if (exact) {
+			/* k-th step of the chunk at exactly dbase + k*vd, clamped to dist */
+			d = dbase + (double)lane * vd;
+			const double dprev = dbase + (double)(lane - 1) * vd;
+			active = (lane == 0) ? (dbase <= dist) : (dprev < dist);
+			d = (d < dist) ? d : dist;
+		} else {
+			d = dbase;
+			for (int k = 0; k < lane; k++) {
+				if (d < dist) {
+					d += vd;
+					d = (d < dist) ? d : dist;
+				} else {
+					d += vd;
+				}
 			}
+			active = d <= dist;
 		}
-		const bool active = d <= dist;
 		const double tau = d * inv_dist;
 		bool ok = true;
-		if (active) {
-			double st[7];
-			se3_segment_at(&sg, tau, st);
-			ok = !state_collides<WIDE>(s, st, smem);
-		}
+		if (active)
+			ok = !seg_state_collides<WIDE>(s, sh, tau, smem);
 		const uint64_t cm = __ballot(active && !ok);
 		const uint64_t am = __ballot(active);
 		if (cm) {
@@ -477,7 +680,7 @@ __device__ SegResult segment_wave(const DevScene& s, const double* __restrict__ 
 			r.lo = r.last_tau;
 			r.hi = thi;
 			if (MODE == MODE_TOUCH) {
-				bisect_wave<WIDE>(s, sg, r.lo, r.hi, levels, smem);
+				bisect_wave<WIDE>(s, sh, r.lo, r.hi, levels, smem);
 				r.steps += levels;
 			}
 			return r;
@@ -508,23 +711,22 @@ __device__ SegResult segment_wave(const DevScene& s, const double* __restrict__ 
 }
 
 template <bool WIDE, int MODE>
-__global__ void __launch_bounds__(256) k_segments(DevScene s, const double* __restrict__ from_all,
+__global__ void __launch_bounds__(256, SE3CQ_MIN_WAVES) k_segments(DevScene s, const double* __restrict__ from_all,
                                                   const double* __restrict__ to_all, int64_t n, double vd,
                                                   int levels, SegOut o)
 {
 	extern __shared__ uint32_t smem[];
-	const int64_t seg = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+	__shared__ SegShared segsh[4];
+	const int wave = threadIdx.x >> 6;
+	const int lane = threadIdx.x & 63;
+	const int64_t seg = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave;
+	SegShared* sh = &segsh[wave];
+	const int64_t sv = seg < n ? seg : n - 1;
+	seg_setup<MODE>(sh, from_all + 7 * sv, to_all + 7 * sv, vd);
+	__syncthreads();
 	if (seg >= n)
 		return;
-	const int lane = threadIdx.x & 63;
-	double from[7], to[7];
-#pragma unroll
-	for (int k = 0; k < 7; k++) {
-		from[k] = from_all[7 * seg + k];
-		to[k] = to_all[7 * seg + k];
-	}
-	se3_segment sg;
-	const SegResult r = segment_wave<WIDE, MODE>(s, from, to, vd, levels, sg, smem);
+	const SegResult r = segment_walk<WIDE, MODE>(s, sh, levels, smem);
 	if (lane == 0) {
 		o.done[seg] = (uint8_t)r.done;
 		if (o.last_tau) o.last_tau[seg] = r.last_tau;
@@ -536,18 +738,20 @@ __global__ void __launch_bounds__(256) k_segments(DevScene s, const double* __re
 	if (o.last_free && lane < 7) {
 		/* reconstruct the returned states; interpolation is deterministic so
 		 * these equal the states the walk evaluated */
+		const double* from = from_all + 7 * seg;
+		const double* to = to_all + 7 * seg;
 		double a[7], b[7];
 		if (r.lo_is_from) {
 			for (int k = 0; k < 7; k++) a[k] = from[k];
 		} else {
-			se3_segment_at(&sg, r.last_tau, a);
+			se3_segment_at(&sh->sg, r.last_tau, a);
 		}
 		if (r.early) {
 			for (int k = 0; k < 7; k++) b[k] = to[k];
 		} else if (r.done) {
 			for (int k = 0; k < 7; k++) b[k] = a[k];
 		} else {
-			se3_segment_at(&sg, r.tau, b);
+			se3_segment_at(&sh->sg, r.tau, b);
 		}
 		double va = a[0], vb = b[0];
 		for (int k = 1; k < 7; k++)
@@ -578,36 +782,34 @@ __device__ __forceinline__ void tri_pair(int64_t p, int64_t n, int64_t& fi, int6
  * bits: bit-packed row-major n x n (upper and lower triangles), either may be
  * null. */
 template <bool WIDE, int VIS>
-__global__ void __launch_bounds__(256) k_visibility(DevScene s, const double* __restrict__ qs0,
+__global__ void __launch_bounds__(256, SE3CQ_MIN_WAVES) k_visibility(DevScene s, const double* __restrict__ qs0,
                                                     const double* __restrict__ qs1, int64_t m, int64_t n,
                                                     int64_t npairs, double ivd, int8_t* __restrict__ out8,
                                                     uint32_t* __restrict__ bits, int32_t* __restrict__ steps)
 {
 	extern __shared__ uint32_t smem[];
-	const int64_t p = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-	if (p >= npairs)
-		return;
+	__shared__ SegShared segsh[4];
+	const int wave = threadIdx.x >> 6;
+	const int lane = threadIdx.x & 63;
+	const int64_t p = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave;
+	const int64_t pv = p < npairs ? p : npairs - 1;
 	int64_t fi, ti;
 	if (VIS == VIS_MATRIX) {
-		tri_pair(p, n, fi, ti);
+		tri_pair(pv, n, fi, ti);
 	} else if (VIS == VIS_MATRIX2) {
-		fi = p / n;
-		ti = p % n;
+		fi = pv / n;
+		ti = pv % n;
 	} else {
-		fi = p;
-		ti = p;
+		fi = pv;
+		ti = pv;
 	}
-	double from[7], to[7];
-#pragma unroll
-	for (int k = 0; k < 7; k++) {
-		from[k] = qs0[7 * fi + k];
-		to[k] = qs1[7 * ti + k];
-	}
-	se3_segment sg;
-	const SegResult r = segment_wave<WIDE, MODE_TRANSITION>(s, from, to, ivd, 0, sg, smem);
+	SegShared* sh = &segsh[wave];
+	seg_setup<MODE_TRANSITION>(sh, qs0 + 7 * fi, qs1 + 7 * ti, ivd);
+	__syncthreads();
+	if (p >= npairs)
+		return;
+	const SegResult r = segment_walk<WIDE, MODE_TRANSITION>(s, sh, 0, smem);
 	const uint8_t done = (uint8_t)r.done;
-	const int32_t st = r.steps;
-	const int lane = threadIdx.x & 63;
 	if (lane == 0) {
 		if (VIS == VIS_MATRIX) {
 			if (out8) {
@@ -615,27 +817,27 @@ __global__ void __launch_bounds__(256) k_visibility(DevScene s, const double* __
 				out8[ti * n + fi] = (int8_t)done;
 			}
 			if (bits && done) {
-				int64_t b1 = fi * n + ti, b2 = ti * n + fi;
+				const int64_t b1 = fi * n + ti, b2 = ti * n + fi;
 				atomicOr(bits + (b1 >> 5), 1u << (b1 & 31));
 				atomicOr(bits + (b2 >> 5), 1u << (b2 & 31));
 			}
 		} else if (VIS == VIS_MATRIX2) {
 			if (out8) out8[fi * n + ti] = (int8_t)done;
 			if (bits && done) {
-				int64_t b1 = fi * n + ti;
+				const int64_t b1 = fi * n + ti;
 				atomicOr(bits + (b1 >> 5), 1u << (b1 & 31));
 			}
 		} else {
 			if (out8) out8[p] = (int8_t)done;
 		}
-		if (steps) steps[p] = st;
+		if (steps) steps[p] = r.steps;
 	}
 }
 
 /* ---- launch wrappers (host) ---- */
 static inline size_t stack_bytes_per_thread(const DevScene& s)
 {
-	return (size_t)s.stack_depth * (s.wide ? 2 : 1) * sizeof(uint32_t);
+	return (size_t)lds_words_per_thread(s) * sizeof(uint32_t);
 }
 
 /* Largest block (multiple of 64, at most 256) whose traversal stacks fit in

@@ -513,14 +513,26 @@ DevScene make_scene(se3cq_world* w)
 		dm.ntris = (int32_t)(w->hm[g].tris.size() / 9);
 		dm.depth = w->hm[g].depth;
 	}
-	double t[3];
-	se3_quat_to_rotmat(w->pert[3], w->pert[4], w->pert[5], w->pert[6], s.Re);
+	const double* p = w->pert;
+	se3_quat_to_rotmat(p[3], p[4], p[5], p[6], s.Re);
 	for (int i = 0; i < 3; i++)
-		s.te[i] = w->pert[i];
-	double qn[4] = {w->pert[3], w->pert[4], w->pert[5], w->pert[6]};
+		s.te[i] = p[i];
+	double qn[4] = {p[3], p[4], p[5], p[6]};
 	se3_quat_normalize(qn);
-	se3_quat_to_rotmat(qn[0], qn[1], qn[2], qn[3], s.ReHat);
-	(void)t;
+	double Rh[9];
+	se3_quat_to_rotmat(qn[0], qn[1], qn[2], qn[3], Rh);
+	double fr = 0.0, fd = 0.0;
+	for (int i = 0; i < 3; i++)
+		for (int j = 0; j < 3; j++) {
+			s.PeT[3 * i + j] = (float)Rh[3 * j + i];
+			fr += s.Re[3 * i + j] * s.Re[3 * i + j];
+			const double d = s.Re[3 * i + j] - Rh[3 * i + j];
+			fd += d * d;
+		}
+	s.fro_Re = round_up_f(std::sqrt(fr) * (1.0 + 1e-9));
+	s.fro_dRe = round_up_f(std::sqrt(3.0 * fd) * (1.0 + 1e-9) + (fd > 0 ? 1e-15 : 0.0));
+	s.pert_identity = (p[0] == 0.0 && p[1] == 0.0 && p[2] == 0.0 && p[3] == 1.0 && p[4] == 0.0 && p[5] == 0.0 &&
+	                   p[6] == 0.0) ? 1 : 0;
 	s.rob_reach = w->hm[1].reach;
 	s.scene_mag = w->hm[0].mag + w->hm[1].mag;
 	s.stack_depth = w->hm[0].depth + w->hm[1].depth + 2;
