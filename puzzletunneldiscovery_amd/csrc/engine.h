@@ -47,17 +47,23 @@ struct DevModel {
 	int32_t pad;
 };
 
+/* Perturbation (environment transform) data, read only when the
+ * perturbation is not the identity. */
+struct PertData {
+	double Re[9];  /* rotation of the raw perturbation quaternion, row-major */
+	double te[3];  /* perturbation translation */
+	float PeT[9];  /* binary32 transpose of the normalised perturbation rotation */
+	float fro_Re;  /* ||Re||_F, rounded up */
+	float fro_dRe; /* sqrt(3) * ||Re - R(normalised)||_F, rounded up */
+};
+
 /* Everything a kernel needs about the scene, passed by value. */
 struct DevScene {
 	DevModel env;
 	DevModel rob;
-	double Re[9];    /* perturbation rotation (raw quaternion), row-major */
-	double te[3];    /* perturbation translation */
-	float PeT[9];    /* binary32 transpose of the normalised perturbation rotation */
-	float fro_Re;    /* ||Re||_F, rounded up */
-	float fro_dRe;   /* sqrt(3) * ||Re - R(normalised)||_F, rounded up */
-	float rob_reach; /* max over robot nodes of |centre| + |extents|, rounded up */
-	float scene_mag; /* max |centre|_inf + max |extent| over both models */
+	const PertData* pert;  /* device copy */
+	float rob_reach;       /* max over robot nodes of |centre| + |extents|, rounded up */
+	float scene_mag;       /* max |centre|_inf + max |extent| over both models */
 	int32_t pert_identity; /* perturbation is exactly [0,0,0,1,0,0,0] */
 	int32_t stack_depth;   /* per-thread traversal stack entries */
 	int32_t wide;          /* 1 if node ids need more than 16 bits */

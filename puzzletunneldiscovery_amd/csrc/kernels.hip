@@ -79,6 +79,7 @@ __device__ __forceinline__ void make_qcull(const DevScene& s, const double* q, Q
 			Rq[i] = (i % 4 == 0) ? 1.0f : 0.0f;
 	}
 	float dl1 = 0.0f;
+	float fro_Re = 1.7320509f, fro_dRe = 0.0f; /* sqrt(3) rounded up, 0 */
 	if (s.pert_identity) {
 #pragma unroll
 		for (int i = 0; i < 9; i++)
@@ -87,25 +88,28 @@ __device__ __forceinline__ void make_qcull(const DevScene& s, const double* q, Q
 		for (int i = 0; i < 3; i++)
 			c.T[i] = (float)q[i];
 	} else {
+		const PertData* P = s.pert;
+		fro_Re = P->fro_Re;
+		fro_dRe = P->fro_dRe;
 		float d[3];
 #pragma unroll
 		for (int i = 0; i < 3; i++) {
-			d[i] = (float)(q[i] - s.te[i]);
+			d[i] = (float)(q[i] - P->te[i]);
 			dl1 += fabsf(d[i]);
 		}
 #pragma unroll
 		for (int i = 0; i < 3; i++) {
 #pragma unroll
 			for (int j = 0; j < 3; j++)
-				c.O[3 * i + j] = fmaf(s.PeT[3 * i], Rq[j], fmaf(s.PeT[3 * i + 1], Rq[3 + j], s.PeT[3 * i + 2] * Rq[6 + j]));
-			c.T[i] = fmaf(s.PeT[3 * i], d[0], fmaf(s.PeT[3 * i + 1], d[1], s.PeT[3 * i + 2] * d[2]));
+				c.O[3 * i + j] = fmaf(P->PeT[3 * i], Rq[j], fmaf(P->PeT[3 * i + 1], Rq[3 + j], P->PeT[3 * i + 2] * Rq[6 + j]));
+			c.T[i] = fmaf(P->PeT[3 * i], d[0], fmaf(P->PeT[3 * i + 1], d[1], P->PeT[3 * i + 2] * d[2]));
 		}
 	}
 	const float dev = fabsf(1.0f - n2) + 1e-6f * n2;
-	const float delta = fmaf(s.fro_Re * 2.83f, dev, s.fro_dRe + 3e-5f);
+	const float delta = fmaf(fro_Re * 2.83f, dev, fro_dRe + 3e-5f);
 	c.infl = delta * s.rob_reach;
 	const float tm = fmaxf(fabsf(c.T[0]), fmaxf(fabsf(c.T[1]), fabsf(c.T[2])));
-	c.eta = fmaf(1e-5f, tm + s.scene_mag, s.fro_dRe * dl1);
+	c.eta = fmaf(1e-5f, tm + s.scene_mag, fro_dRe * dl1);
 }
 
 /* Robot box in the env frame, cached across consecutive tests that share the
@@ -304,13 +308,25 @@ __device__ __forceinline__ void make_exact(const DevScene& s, const double* q, E
 {
 	double Rr[9];
 	se3_quat_to_rotmat(q[3], q[4], q[5], q[6], Rr);
-	const double d[3] = {q[0] - s.te[0], q[1] - s.te[1], q[2] - s.te[2]};
+	if (s.pert_identity) {
+		/* Re = I, te = 0: Re^T Rr and Re^T t equal Rr and t up to the sign
+		 * of zero entries, which no comparison in the SAT can observe. */
+#pragma unroll
+		for (int i = 0; i < 9; i++)
+			x.R[i] = Rr[i];
+#pragma unroll
+		for (int i = 0; i < 3; i++)
+			x.T[i] = q[i];
+		return;
+	}
+	const PertData* P = s.pert;
+	const double d[3] = {q[0] - P->te[0], q[1] - P->te[1], q[2] - P->te[2]};
 #pragma unroll
 	for (int i = 0; i < 3; i++) {
 #pragma unroll
 		for (int j = 0; j < 3; j++)
-			x.R[3 * i + j] = (s.Re[i] * Rr[j] + s.Re[3 + i] * Rr[3 + j]) + s.Re[6 + i] * Rr[6 + j];
-		x.T[i] = (s.Re[i] * d[0] + s.Re[3 + i] * d[1]) + s.Re[6 + i] * d[2];
+			x.R[3 * i + j] = (P->Re[i] * Rr[j] + P->Re[3 + i] * Rr[3 + j]) + P->Re[6 + i] * Rr[6 + j];
+		x.T[i] = (P->Re[i] * d[0] + P->Re[3 + i] * d[1]) + P->Re[6 + i] * d[2];
 	}
 }
 
@@ -496,6 +512,158 @@ __global__ void __launch_bounds__(256, SE3CQ_MIN_WAVES) k_valid_states(DevScene 
 	out[i] = collide_dfs<WIDE>(s, StateFromGlobal{qp}, stk, leaves) ? 0 : 1;
 }
 
+/* ---- packet traversal: one wavefront, one pose per lane ---- */
+/*
+ * Segment steps, bisection probes and visibility pairs hand a wavefront 64
+ * nearly identical poses.  The wave walks a single BVH x BVH traversal; each
+ * stack entry carries the 64-bit mask of lanes whose boxes still overlap, a
+ * pair is expanded if any lane overlaps it, and each lane tests only its own
+ * pose.  Control flow and node addresses are wave-uniform, so nodes and
+ * triangles are read with scalar loads (constant address space) and the
+ * stack lives in VGPR lanes (entry k in lane k, writelane/readlane).
+ */
+__device__ __forceinline__ GNode load_node_uniform(const GNode* nodes, int i)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+	const __attribute__((address_space(4))) float4* p =
+	    (const __attribute__((address_space(4))) float4*)(nodes + i);
+	GNode n;
+	n.c_ex = p[0];
+	n.a0_ey = p[1];
+	n.a1_ez = p[2];
+	n.a2_child = p[3];
+	return n;
+#else
+	return nodes[i];
+#endif
+}
+
+__device__ __forceinline__ int rdl(int v, int k) { return __builtin_amdgcn_readlane(v, k); }
+/* llvm.amdgcn.writelane: put a wave-uniform value into lane k of a VGPR. */
+__device__ int se3cq_amdgcn_writelane(int val, int lane, int old) __asm("llvm.amdgcn.writelane.i32");
+__device__ __forceinline__ int wrl(int val, int k, int old) { return se3cq_amdgcn_writelane(val, k, old); }
+
+/* Exact leaf test with uniform triangle ids and per-lane exact transform. */
+__device__ __forceinline__ bool leaf_pair_x(const DevScene& s, const ExactTf& x, int te, int tr)
+{
+	const double* P = s.env.tris + 9 * (int64_t)te;
+	const double* qv = s.rob.tris + 9 * (int64_t)tr;
+	const double P1[3] = {P[0], P[1], P[2]};
+	const double p1[3] = {0.0, 0.0, 0.0};
+	double p2[3], p3[3], q1[3], q2[3], q3[3];
+#pragma unroll
+	for (int k = 0; k < 3; k++) {
+		p2[k] = P[3 + k] - P1[k];
+		p3[k] = P[6 + k] - P1[k];
+	}
+	double* qq[3] = {q1, q2, q3};
+#pragma unroll
+	for (int v = 0; v < 3; v++) {
+		const double a = qv[3 * v], b = qv[3 * v + 1], c = qv[3 * v + 2];
+#pragma unroll
+		for (int r = 0; r < 3; r++)
+			qq[v][r] = (((x.R[3 * r] * a + x.R[3 * r + 1] * b) + x.R[3 * r + 2] * c) + x.T[r]) - P1[r];
+	}
+	return sat17(p1, p2, p3, q1, q2, q3);
+}
+
+/*
+ * Returns the mask of lanes in `live` whose pose (exact binary64 state q)
+ * puts the robot in collision.  Traversal phases cull in binary32 and queue
+ * the (env triangle, robot triangle, lane mask) leaf candidates in VGPR
+ * lanes; leaf phases run the exact binary64 tests on the queue.  The culling
+ * transform is rebuilt at the start of every traversal phase so it is dead
+ * while the SATs run.
+ */
+enum { PACKET_LEAVES = 32 };
+
+__device__ uint64_t packet_collide(const DevScene& s, const double* q, uint64_t live)
+{
+	const int lane = threadIdx.x & 63;
+	int va = 0, vb = 0, vml = 0, vmh = 0;
+	va = wrl(0, 0, va);
+	vb = wrl(0, 0, vb);
+	vml = wrl((int)(uint32_t)live, 0, vml);
+	vmh = wrl((int)(uint32_t)(live >> 32), 0, vmh);
+	int sp = 1;
+	uint64_t hit = 0;
+	for (;;) {
+		int lte = 0, ltr = 0, lml = 0, lmh = 0;
+		int nl = 0;
+		{
+			QCull c;
+			make_qcull(s, q, c);
+			while (sp > 0 && nl < PACKET_LEAVES) {
+				sp--;
+				const int a = rdl(va, sp), b = rdl(vb, sp);
+				uint64_t m = ((uint64_t)(uint32_t)rdl(vmh, sp) << 32) | (uint64_t)(uint32_t)rdl(vml, sp);
+				m &= ~hit;
+				if (m == 0)
+					continue;
+				const GNode A = load_node_uniform(s.env.nodes, a);
+				const GNode Bn = load_node_uniform(s.rob.nodes, b);
+				bool ov;
+				{
+					RobBox B;
+					load_rob_box(Bn, c, B);
+					ov = ((m >> lane) & 1ull) && !obb_disjoint(A, B, c.eta);
+				}
+				const uint64_t mo = __ballot(ov);
+				if (mo == 0)
+					continue;
+				const int ca = __float_as_int(A.a2_child.w);
+				const int cb = __float_as_int(Bn.a2_child.w);
+				const bool la = ca < 0, lb = cb < 0;
+				const int ml = (int)(uint32_t)mo, mh = (int)(uint32_t)(mo >> 32);
+				if (la && lb) {
+					lte = wrl(~ca, nl, lte);
+					ltr = wrl(~cb, nl, ltr);
+					lml = wrl(ml, nl, lml);
+					lmh = wrl(mh, nl, lmh);
+					nl++;
+					continue;
+				}
+				if (sp + 2 > s.stack_depth || sp + 2 > 64) {
+					if (lane == 0)
+						atomicOr(s.error_flag, 1u);
+					return live;
+				}
+				/* FCL's rule: split the env node unless it is a leaf or the
+				 * robot box is the larger one (squared extent norms). */
+				const float sa = fmaf(A.c_ex.w, A.c_ex.w, fmaf(A.a0_ey.w, A.a0_ey.w, A.a1_ez.w * A.a1_ez.w));
+				const float sb = fmaf(Bn.c_ex.w, Bn.c_ex.w, fmaf(Bn.a0_ey.w, Bn.a0_ey.w, Bn.a1_ez.w * Bn.a1_ez.w));
+				int a1, b1, a2, b2;
+				if (!la && (lb || sa > sb)) {
+					a1 = ca + 1; b1 = b; a2 = ca; b2 = b;
+				} else {
+					a1 = a; b1 = cb + 1; a2 = a; b2 = cb;
+				}
+				va = wrl(a1, sp, va); vb = wrl(b1, sp, vb); vml = wrl(ml, sp, vml); vmh = wrl(mh, sp, vmh);
+				sp++;
+				va = wrl(a2, sp, va); vb = wrl(b2, sp, vb); vml = wrl(ml, sp, vml); vmh = wrl(mh, sp, vmh);
+				sp++;
+			}
+		}
+		if (nl > 0) {
+			ExactTf x;
+			make_exact(s, q, x);
+			for (int i = 0; i < nl; i++) {
+				uint64_t m = ((uint64_t)(uint32_t)rdl(lmh, i) << 32) | (uint64_t)(uint32_t)rdl(lml, i);
+				m &= ~hit;
+				if (m == 0)
+					continue;
+				const int te = rdl(lte, i), tr = rdl(ltr, i);
+				const bool h = ((m >> lane) & 1ull) && leaf_pair_x(s, x, te, tr);
+				hit |= __ballot(h);
+			}
+			if ((hit & live) == live)
+				return live;
+		}
+		if (sp == 0)
+			return hit & live;
+	}
+}
+
 /* ---- segments: one wavefront per segment ---- */
 __device__ __forceinline__ double shfl_d(double v, int src) { return __shfl(v, src, 64); }
 
@@ -507,12 +675,13 @@ struct SegShared {
 	int early;
 };
 
-template <bool WIDE>
-__device__ __forceinline__ bool seg_state_collides(const DevScene& s, const SegShared* sh, double tau, uint32_t* smem)
+/* Wave-level check of one pose per lane (lanes outside `live` are ignored):
+ * returns the mask of colliding lanes. */
+__device__ __forceinline__ uint64_t seg_poses_collide(const DevScene& s, const SegShared* sh, double tau, uint64_t live)
 {
-	PairStack<WIDE> stk, leaves;
-	thread_stacks<WIDE>(s, smem, stk, leaves);
-	return collide_dfs<WIDE>(s, StateFromSegment{&sh->sg, tau}, stk, leaves);
+	double q[7];
+	se3_segment_at(&sh->sg, tau, q);
+	return packet_collide(s, q, live);
 }
 
 /*
@@ -563,8 +732,10 @@ __device__ void bisect_wave(const DevScene& s, const SegShared* sh, double& lo, 
 					h0 = m;
 			}
 			mid = 0.5 * (l0 + h0);
-			ok = !seg_state_collides<WIDE>(s, sh, mid, smem);
 		}
+		const uint64_t live = __ballot(act);
+		const uint64_t hitm = seg_poses_collide(s, sh, act ? mid : lo, live);
+		ok = !((hitm >> lane) & 1ull);
 		const uint64_t vm = __ballot(act && ok);
 		int h = 1;
 		for (int d = 0; d < k; d++) {
@@ -663,11 +834,8 @@ __device__ SegResult segment_walk(const DevScene& s, const SegShared* sh, int le
 			active = d <= dist;
 		}
 		const double tau = d * inv_dist;
-		bool ok = true;
-		if (active)
-			ok = !seg_state_collides<WIDE>(s, sh, tau, smem);
-		const uint64_t cm = __ballot(active && !ok);
 		const uint64_t am = __ballot(active);
+		const uint64_t cm = am ? seg_poses_collide(s, sh, active ? tau : sh->vd * inv_dist, am) : 0ull;
 		if (cm) {
 			const int f = __ffsll((long long)cm) - 1;
 			const double thi = shfl_d(tau, f);

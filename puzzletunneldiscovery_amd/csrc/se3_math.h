@@ -56,7 +56,9 @@
 #define SE3_RED_P2 0x1.0b4611a600000p-34
 #define SE3_RED_P3 0x1.3198a2e037073p-69
 #define SE3_TAN_PI_8 0x1.a827999fcef32p-2
-/* Largest |x| handled by the deterministic argument reduction. */
+/* Beyond |x| = 2^20 the three-part reduction loses accuracy (k*P1 is no
+ * longer exact); it stays deterministic, which is what the state arithmetic
+ * needs, and slerp/apply never produce such arguments. */
 #define SE3_RED_MAX 0x1p20
 
 /* sin(r) on |r| <= pi/4 (+ a little): r + r^3 * P(r^2), Taylor to r^21. */
@@ -92,47 +94,40 @@ SE3_HD static inline double se3_cos_kernel(double r)
 	return 1.0 + r2 * (-0.5 + r2 * p);
 }
 
-/* Reduce x to r in [-pi/4, pi/4] (approximately) and a quadrant q = k mod 4. */
+/* Reduce x to r in [-pi/4, pi/4] (approximately) and a quadrant q = k mod 4;
+ * |x| <= pi/4 is left as is (k = 0). */
 SE3_HD static inline double se3_reduce_pio2(double x, int* quadrant)
 {
 	double k = floor(x * SE3_TWO_OVER_PI + 0.5);
+	if (fabs(x) <= SE3_PIO4_HI)
+		k = 0.0;
 	double r = ((x - k * SE3_RED_P1) - k * SE3_RED_P2) - k * SE3_RED_P3;
-	*quadrant = ((int)k) & 3;
+	*quadrant = (int)(k - 4.0 * floor(k * 0.25)); /* k mod 4, also for huge k */
 	return r;
 }
 
+/* Both kernels are evaluated and the quadrant selects: straight-line code on
+ * the GPU (no divergent branches), the same values as a switch. */
 SE3_HD static inline double se3_sin(double x)
 {
-	double ax = fabs(x);
-	if (ax <= SE3_PIO4_HI)
-		return se3_sin_kernel(x);
-	if (!(ax <= SE3_RED_MAX))
-		return (x != x) ? x + x : sin(x); /* NaN, or outside the state domain */
+	if (x != x || fabs(x) == INFINITY)
+		return x - x; /* NaN */
 	int q;
-	double r = se3_reduce_pio2(x, &q);
-	switch (q) {
-	case 0: return se3_sin_kernel(r);
-	case 1: return se3_cos_kernel(r);
-	case 2: return -se3_sin_kernel(r);
-	default: return -se3_cos_kernel(r);
-	}
+	const double r = se3_reduce_pio2(x, &q);
+	const double sv = se3_sin_kernel(r), cv = se3_cos_kernel(r);
+	const double v = (q & 1) ? cv : sv;
+	return (q & 2) ? -v : v;
 }
 
 SE3_HD static inline double se3_cos(double x)
 {
-	double ax = fabs(x);
-	if (ax <= SE3_PIO4_HI)
-		return se3_cos_kernel(x);
-	if (!(ax <= SE3_RED_MAX))
-		return (x != x) ? x + x : cos(x);
+	if (x != x || fabs(x) == INFINITY)
+		return x - x; /* NaN */
 	int q;
-	double r = se3_reduce_pio2(x, &q);
-	switch (q) {
-	case 0: return se3_cos_kernel(r);
-	case 1: return -se3_sin_kernel(r);
-	case 2: return -se3_cos_kernel(r);
-	default: return se3_sin_kernel(r);
-	}
+	const double r = se3_reduce_pio2(x, &q);
+	const double sv = se3_sin_kernel(r), cv = se3_cos_kernel(r);
+	const double v = (q & 1) ? sv : cv;
+	return ((q + 1) & 2) ? -v : v;
 }
 
 /* asin(x) for |x| <= 0.5: x + x^3 * A(x^2), Maclaurin series to x^49. */

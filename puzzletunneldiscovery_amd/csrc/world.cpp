@@ -13,8 +13,10 @@
  *   - angleModel(0,0): env matrix diag(s); robot matrix scale(s)*translate(-c)
  *     built in binary32 (glm), then CDModel::addVF applies it in binary64:
  *     (m0*v0 + m1*v1) + (m2*v2 + m3) (lib/osr/cdmodel.cc:120-160).
- * The BVH is this engine's own (median split, PCA boxes, binary32 storage
- * made conservative against the binary64 vertices); only the exact leaf test
+ * The BVH is this engine's own (PCA-fitted boxes, median split along the
+ * box axis that minimises the children's summed surface area, binary32
+ * storage made conservative against the binary64 vertices); only the exact
+ * leaf test
  * decides a boolean, so the tree shape never changes an answer.
  */
 #include <hip/hip_runtime.h>
@@ -104,6 +106,8 @@ struct se3cq_world {
 	hipStream_t stream = nullptr;
 	DevBuf d_nodes[2], d_tris[2];
 	DevBuf d_err;
+	DevBuf d_pert;
+	bool pert_uploaded = false;
 	DevBuf ws[12];
 	std::mutex mu;
 
@@ -121,6 +125,8 @@ struct se3cq_world {
 			d_tris[g].release();
 		}
 		d_err.release();
+		d_pert.release();
+		pert_uploaded = false;
 		for (auto& b : ws)
 			b.release();
 		if (stream)
@@ -475,19 +481,31 @@ void build_model(HostModel& m, const Mesh& me, const float* M)
 			set_child(g, ~perm[it.lo]);
 			continue;
 		}
-		double blo[3] = {INFINITY, INFINITY, INFINITY}, bhi[3] = {-INFINITY, -INFINITY, -INFINITY};
-		for (int i = it.lo; i < it.hi; i++)
-			for (int r = 0; r < 3; r++) {
-				blo[r] = std::min(blo[r], cen[3 * perm[i] + r]);
-				bhi[r] = std::max(bhi[r], cen[3 * perm[i] + r]);
-			}
-		int axis = 0;
-		for (int r = 1; r < 3; r++)
-			if (bhi[r] - blo[r] > bhi[axis] - blo[axis])
-				axis = r;
+		/* Median split along whichever of the node box's three axes gives
+		 * the smallest summed surface area of the two child boxes: the
+		 * expected number of box tests in a BVH x BVH traversal grows with
+		 * child box size. */
 		const int mid = (it.lo + it.hi) / 2;
-		std::nth_element(perm.begin() + it.lo, perm.begin() + mid, perm.begin() + it.hi,
-		                 [&](int a, int b) { return cen[3 * a + axis] < cen[3 * b + axis]; });
+		const float* axes[3] = {&g.a0_ey.x, &g.a1_ez.x, &g.a2_child.x};
+		double best = INFINITY;
+		std::vector<int> best_perm;
+		for (int ax = 0; ax < 3; ax++) {
+			const double a0 = axes[ax][0], a1 = axes[ax][1], a2 = axes[ax][2];
+			auto proj = [&](int t) { return a0 * cen[3 * t] + a1 * cen[3 * t + 1] + a2 * cen[3 * t + 2]; };
+			std::nth_element(perm.begin() + it.lo, perm.begin() + mid, perm.begin() + it.hi,
+			                 [&](int a, int b) { return proj(a) < proj(b); });
+			GNode gl, gr;
+			double cl[3], el[3], cr[3], er[3];
+			fit_node(m, me.F.data(), perm, it.lo, mid, gl, cl, el);
+			fit_node(m, me.F.data(), perm, mid, it.hi, gr, cr, er);
+			const double cost = (el[0] * el[1] + el[1] * el[2] + el[0] * el[2]) +
+			                    (er[0] * er[1] + er[1] * er[2] + er[0] * er[2]);
+			if (cost < best) {
+				best = cost;
+				best_perm.assign(perm.begin() + it.lo, perm.begin() + it.hi);
+			}
+		}
+		std::copy(best_perm.begin(), best_perm.end(), perm.begin() + it.lo);
 		const int first = next;
 		next += 2;
 		set_child(g, first);
@@ -514,23 +532,7 @@ DevScene make_scene(se3cq_world* w)
 		dm.depth = w->hm[g].depth;
 	}
 	const double* p = w->pert;
-	se3_quat_to_rotmat(p[3], p[4], p[5], p[6], s.Re);
-	for (int i = 0; i < 3; i++)
-		s.te[i] = p[i];
-	double qn[4] = {p[3], p[4], p[5], p[6]};
-	se3_quat_normalize(qn);
-	double Rh[9];
-	se3_quat_to_rotmat(qn[0], qn[1], qn[2], qn[3], Rh);
-	double fr = 0.0, fd = 0.0;
-	for (int i = 0; i < 3; i++)
-		for (int j = 0; j < 3; j++) {
-			s.PeT[3 * i + j] = (float)Rh[3 * j + i];
-			fr += s.Re[3 * i + j] * s.Re[3 * i + j];
-			const double d = s.Re[3 * i + j] - Rh[3 * i + j];
-			fd += d * d;
-		}
-	s.fro_Re = round_up_f(std::sqrt(fr) * (1.0 + 1e-9));
-	s.fro_dRe = round_up_f(std::sqrt(3.0 * fd) * (1.0 + 1e-9) + (fd > 0 ? 1e-15 : 0.0));
+	s.pert = (const PertData*)w->d_pert.p;
 	s.pert_identity = (p[0] == 0.0 && p[1] == 0.0 && p[2] == 0.0 && p[3] == 1.0 && p[4] == 0.0 && p[5] == 0.0 &&
 	                   p[6] == 0.0) ? 1 : 0;
 	s.rob_reach = w->hm[1].reach;
@@ -539,6 +541,40 @@ DevScene make_scene(se3cq_world* w)
 	s.wide = (w->hm[0].nodes.size() > 65536 || w->hm[1].nodes.size() > 65536) ? 1 : 0;
 	s.error_flag = (uint32_t*)w->d_err.p;
 	return s;
+}
+
+/* Perturbation data for the device (see PertData). */
+PertData make_pert(const double* p)
+{
+	PertData d;
+	std::memset(&d, 0, sizeof(d));
+	se3_quat_to_rotmat(p[3], p[4], p[5], p[6], d.Re);
+	for (int i = 0; i < 3; i++)
+		d.te[i] = p[i];
+	double qn[4] = {p[3], p[4], p[5], p[6]};
+	se3_quat_normalize(qn);
+	double Rh[9];
+	se3_quat_to_rotmat(qn[0], qn[1], qn[2], qn[3], Rh);
+	double fr = 0.0, fd = 0.0;
+	for (int i = 0; i < 3; i++)
+		for (int j = 0; j < 3; j++) {
+			d.PeT[3 * i + j] = (float)Rh[3 * j + i];
+			fr += d.Re[3 * i + j] * d.Re[3 * i + j];
+			const double e = d.Re[3 * i + j] - Rh[3 * i + j];
+			fd += e * e;
+		}
+	d.fro_Re = round_up_f(std::sqrt(fr) * (1.0 + 1e-9));
+	d.fro_dRe = round_up_f(std::sqrt(3.0 * fd) * (1.0 + 1e-9) + (fd > 0 ? 1e-15 : 0.0));
+	return d;
+}
+
+int upload_pert(se3cq_world* w)
+{
+	const PertData d = make_pert(w->pert);
+	if (hipMemcpy(w->d_pert.p, &d, sizeof(d), hipMemcpyHostToDevice) != hipSuccess)
+		return fail(SE3CQ_EDEVICE, "device upload failed");
+	w->pert_uploaded = true;
+	return SE3CQ_OK;
 }
 
 int upload(se3cq_world* w)
@@ -558,6 +594,13 @@ int upload(se3cq_world* w)
 	}
 	if (!w->d_err.reserve(sizeof(uint32_t)) || hipMemset(w->d_err.p, 0, sizeof(uint32_t)) != hipSuccess)
 		return fail(SE3CQ_EDEVICE, "device allocation failed");
+	if (!w->d_pert.reserve(sizeof(PertData)))
+		return fail(SE3CQ_EDEVICE, "device allocation failed");
+	{
+		const int rc = upload_pert(w);
+		if (rc)
+			return rc;
+	}
 	const int need = (int)(w->hm[0].depth + w->hm[1].depth + 2) * 2 * 4 * 64;
 	if (need > 48 * 1024)
 		return fail(SE3CQ_EINVAL, "BVH too deep for the traversal stack");
@@ -867,6 +910,12 @@ int se3cq_set_perturbation(se3cq_world* w, const double* q7)
 	CHECK_WORLD(w);
 	std::lock_guard<std::mutex> lk(w->mu);
 	std::memcpy(w->pert, q7, 7 * sizeof(double));
+	if (w->uploaded) {
+		/* queries already issued on the world's stream see the old values */
+		if (hipStreamSynchronize(w->stream) != hipSuccess)
+			return fail(SE3CQ_EDEVICE, "stream synchronisation failed");
+		return upload_pert(w);
+	}
 	return SE3CQ_OK;
 }
 
