@@ -298,20 +298,50 @@ int or_load_arrays(OrWorld* w, int geo, const float* V, int64_t nv, const uint32
 
 /* OBJ ingest, restating what assimp's importer with Triangulate |
  * JoinIdenticalVertices hands Mesh::Mesh (lib/osr/scene.cc:99-105,
- * lib/osr/mesh.cc:24-63): one vertex per distinct (position, uv, normal)
- * value tuple in first-use order over the face corners, polygons split into
- * fans.  Coordinates are parsed with strtof (correctly rounded). */
+ * lib/osr/mesh.cc:24-63).  Polygons are split into fans over their face
+ * corners.  assimp (external, JoinVerticesProcess) then walks the corners in
+ * order and folds a corner into the first already-kept vertex at the same
+ * position whose normal and uv each differ from it by a squared length of at
+ * most (1e-5f)^2 in binary32; otherwise the corner becomes a new vertex.
+ * Coordinates are parsed with strtof (correctly rounded). */
 typedef struct { float a[8]; } Corner; /* pos3, uv2, nrm3 */
 
 static uint64_t corner_hash(const Corner* c)
 {
 	uint64_t h = 1469598103934665603ULL;
-	const unsigned char* p = (const unsigned char*)c->a;
-	for (size_t i = 0; i < sizeof(c->a); i++) {
+	float pos[3];
+	for (int k = 0; k < 3; k++)
+		pos[k] = c->a[k] + 0.0f; /* -0 and +0 are the same position */
+	const unsigned char* p = (const unsigned char*)pos;
+	for (size_t i = 0; i < sizeof(pos); i++) {
 		h ^= p[i];
 		h *= 1099511628211ULL;
 	}
 	return h;
+}
+
+static float sqdiff(const float* a, const float* b, int n)
+{
+	float s = 0.0f;
+	for (int k = 0; k < n; k++) {
+		const float d = a[k] - b[k];
+		s = s + d * d;
+	}
+	return s;
+}
+
+/* assimp's areVerticesEqual for the attributes an OBJ file carries. */
+static int corners_join(const Corner* kept, const Corner* c)
+{
+	const float eps = 1e-5f;
+	const float eps2 = eps * eps;
+	if (!(kept->a[0] == c->a[0] && kept->a[1] == c->a[1] && kept->a[2] == c->a[2]))
+		return 0;
+	if (sqdiff(kept->a + 5, c->a + 5, 3) > eps2)
+		return 0;
+	if (sqdiff(kept->a + 3, c->a + 3, 2) > eps2)
+		return 0;
+	return 1;
 }
 
 int or_load_obj(OrWorld* w, int geo, const char* path)
@@ -396,7 +426,9 @@ int or_load_obj(OrWorld* w, int geo, const char* path)
 		}
 	}
 	fclose(fp);
-	/* Join identical corners (first use wins) with an open-addressing table. */
+	/* Join corners (first kept vertex wins); the open-addressing table is
+	 * keyed by position, and a probe chain visits the kept vertices of one
+	 * position in the order they were kept. */
 	size_t tsz = 1;
 	while (tsz < 2 * nc + 2)
 		tsz <<= 1;
@@ -418,7 +450,7 @@ int or_load_obj(OrWorld* w, int geo, const char* path)
 				remap[i] = nuniq++;
 				break;
 			}
-			if (memcmp(corners[uniq_src[u]].a, corners[i].a, sizeof(corners[i].a)) == 0) {
+			if (corners_join(&corners[uniq_src[u]], &corners[i])) {
 				remap[i] = u;
 				break;
 			}
@@ -1231,3 +1263,11 @@ void or_visibility_pair(const OrWorld* w, const double* qs0, const double* qs1, 
 	free(u0);
 	free(u1);
 }
+
+/* ------------------------------------------------------------------ */
+/* The portable transcendental functions, exported for their tests.   */
+/* ------------------------------------------------------------------ */
+double or_se3_sin(double x) { return se3_sin(x); }
+double or_se3_cos(double x) { return se3_cos(x); }
+double or_se3_acos(double x) { return se3_acos(x); }
+double or_se3_atan2(double y, double x) { return se3_atan2(y, x); }

@@ -107,6 +107,12 @@ void or_visibility_matrix2(const OrWorld* w, const double* qs0, int64_t m, int u
 void or_visibility_pair(const OrWorld* w, const double* qs0, const double* qs1, int64_t n,
                         int unit0, int unit1, double vd, int8_t* out, int nthreads);
 
+/* The library's portable transcendental functions (se3_math.h). */
+double or_se3_sin(double x);
+double or_se3_cos(double x);
+double or_se3_acos(double x);
+double or_se3_atan2(double y, double x);
+
 #ifdef __cplusplus
 }
 #endif

@@ -79,6 +79,10 @@ def lib():
                                              ctypes.c_int64, ctypes.c_int, ctypes.c_double, _c_i8_p, ctypes.c_int]),
             "or_visibility_pair": (None, [vp, _c_double_p, _c_double_p, ctypes.c_int64, ctypes.c_int, ctypes.c_int,
                                           ctypes.c_double, _c_i8_p, ctypes.c_int]),
+            "or_se3_sin": (ctypes.c_double, [ctypes.c_double]),
+            "or_se3_cos": (ctypes.c_double, [ctypes.c_double]),
+            "or_se3_acos": (ctypes.c_double, [ctypes.c_double]),
+            "or_se3_atan2": (ctypes.c_double, [ctypes.c_double, ctypes.c_double]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -110,8 +114,9 @@ class OracleWorld:
         self._w = lib().or_world_new()
 
     def __del__(self):
-        if getattr(self, "_w", None):
-            lib().or_world_free(self._w)
+        w = getattr(self, "_w", None)
+        if w and _lib is not None:
+            _lib.or_world_free(w)
             self._w = None
 
     # ---- ingest -------------------------------------------------------

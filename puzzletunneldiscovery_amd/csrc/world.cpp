@@ -176,6 +176,11 @@ int mesh_from_arrays(Mesh& m, const float* V, int64_t nv, const uint32_t* F, int
 	for (int64_t i = 0; i < 3 * nf; i++)
 		if ((int64_t)F[i] >= nv)
 			return fail(SE3CQ_EINVAL, "load_mesh: face index out of range");
+	/* The exact leaf test relies on P1 - P1 being the zero vector and on
+	 * products staying finite, so coordinates must be finite and moderate. */
+	for (int64_t i = 0; i < 3 * nv; i++)
+		if (!(std::fabs(V[i]) <= 1e30f))
+			return fail(SE3CQ_EINVAL, "load_mesh: non-finite or out-of-range vertex coordinate");
 	m = Mesh();
 	m.V.assign(V, V + 3 * nv);
 	m.F.assign(F, F + 3 * nf);
@@ -184,30 +189,55 @@ int mesh_from_arrays(Mesh& m, const float* V, int64_t nv, const uint32_t* F, int
 }
 
 /* Wavefront OBJ as assimp (Triangulate | JoinIdenticalVertices) presents it
- * to Mesh::Mesh (lib/osr/mesh.cc:24-63): polygons become triangle fans and
- * face corners with identical (position, uv, normal) values share one
- * vertex, numbered in order of first use. */
+ * to Mesh::Mesh (lib/osr/mesh.cc:24-63): polygons become triangle fans, and
+ * each face corner, in file order, joins the first already-kept vertex at the
+ * same position whose normal and uv are each within a squared binary32
+ * distance of (1e-5f)^2 of its own (assimp's JoinVerticesProcess, external);
+ * a corner that joins none becomes a new vertex. */
+namespace {
+struct ObjCorner {
+	float a[8]; // pos3, uv2, nrm3
+};
+struct PosKey {
+	float p[3];
+	bool operator==(const PosKey& o) const { return p[0] == o.p[0] && p[1] == o.p[1] && p[2] == o.p[2]; }
+};
+struct PosKeyHash {
+	size_t operator()(const PosKey& k) const
+	{
+		float q[3] = {k.p[0] + 0.0f, k.p[1] + 0.0f, k.p[2] + 0.0f}; // -0 == +0
+		uint64_t h = 1469598103934665603ULL;
+		const unsigned char* b = reinterpret_cast<const unsigned char*>(q);
+		for (size_t i = 0; i < sizeof(q); i++)
+			h = (h ^ b[i]) * 1099511628211ULL;
+		return (size_t)h;
+	}
+};
+float sq_gap(const float* a, const float* b, int n)
+{
+	float s = 0.0f;
+	for (int k = 0; k < n; k++) {
+		const float d = a[k] - b[k];
+		s = s + d * d;
+	}
+	return s;
+}
+bool corners_join(const ObjCorner& kept, const ObjCorner& c)
+{
+	const float eps = 1e-5f;
+	const float eps2 = eps * eps;
+	return !(sq_gap(kept.a + 5, c.a + 5, 3) > eps2) && !(sq_gap(kept.a + 3, c.a + 3, 2) > eps2);
+}
+} // namespace
+
 int mesh_from_obj(Mesh& m, const char* path)
 {
 	FILE* fp = std::fopen(path, "r");
 	if (!fp)
 		return fail(SE3CQ_EIO, std::string("cannot open ") + path);
 	std::vector<float> pv, pt, pn;
-	struct Key {
-		float a[8];
-		bool operator==(const Key& o) const { return std::memcmp(a, o.a, sizeof(a)) == 0; }
-	};
-	struct KeyHash {
-		size_t operator()(const Key& k) const
-		{
-			uint64_t h = 1469598103934665603ULL;
-			const unsigned char* p = reinterpret_cast<const unsigned char*>(k.a);
-			for (size_t i = 0; i < sizeof(k.a); i++)
-				h = (h ^ p[i]) * 1099511628211ULL;
-			return (size_t)h;
-		}
-	};
-	std::unordered_map<Key, uint32_t, KeyHash> uniq;
+	std::unordered_map<PosKey, std::vector<uint32_t>, PosKeyHash> at_pos;
+	std::vector<ObjCorner> kept;
 	std::vector<float> V;
 	std::vector<uint32_t> F;
 	char line[8192];
@@ -251,22 +281,26 @@ int mesh_from_obj(Mesh& m, const char* path)
 				if (vi < 0) vi = nvv + 1 + vi;
 				if (ti < 0) ti = ntt + 1 + ti;
 				if (ni < 0) ni = nnn + 1 + ni;
-				Key key;
-				std::memset(key.a, 0, sizeof(key.a));
+				ObjCorner c;
+				std::memset(c.a, 0, sizeof(c.a));
 				if (vi >= 1 && vi <= nvv)
-					std::memcpy(key.a, &pv[3 * (vi - 1)], 3 * sizeof(float));
+					std::memcpy(c.a, &pv[3 * (vi - 1)], 3 * sizeof(float));
 				if (ti >= 1 && ti <= ntt)
-					std::memcpy(key.a + 3, &pt[2 * (ti - 1)], 2 * sizeof(float));
+					std::memcpy(c.a + 3, &pt[2 * (ti - 1)], 2 * sizeof(float));
 				if (ni >= 1 && ni <= nnn)
-					std::memcpy(key.a + 5, &pn[3 * (ni - 1)], 3 * sizeof(float));
-				auto it = uniq.find(key);
-				uint32_t id;
-				if (it == uniq.end()) {
-					id = (uint32_t)(V.size() / 3);
-					uniq.emplace(key, id);
-					V.insert(V.end(), key.a, key.a + 3);
-				} else {
-					id = it->second;
+					std::memcpy(c.a + 5, &pn[3 * (ni - 1)], 3 * sizeof(float));
+				std::vector<uint32_t>& here = at_pos[PosKey{{c.a[0], c.a[1], c.a[2]}}];
+				uint32_t id = UINT32_MAX;
+				for (uint32_t u : here)
+					if (corners_join(kept[u], c)) {
+						id = u;
+						break;
+					}
+				if (id == UINT32_MAX) {
+					id = (uint32_t)kept.size();
+					kept.push_back(c);
+					here.push_back(id);
+					V.insert(V.end(), c.a, c.a + 3);
 				}
 				ids.push_back(id);
 			}

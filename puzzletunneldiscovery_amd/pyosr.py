@@ -116,8 +116,8 @@ class UnitWorld:
 
     def __del__(self):
         h = getattr(self, "_w", None)
-        if h:
-            N.lib().se3cq_world_destroy(h)
+        if h and N is not None and N._lib is not None:
+            N._lib.se3cq_world_destroy(h)
             self._w = None
 
     # ---- setup (module.cc:129-137) ----------------------------------

@@ -1,0 +1,179 @@
+"""Pins the oracle's UnitWorld restatement: BVH traversal against brute force
+over all triangle pairs, the stepping semantics of transitStateToWithContact
+and isValidTransition (lib/osr/unit_world.cc:405-486) against closed forms,
+bisection invariants, and the reference's own fixtures."""
+import os
+
+import numpy as np
+import pytest
+
+from puzzletunneldiscovery_amd import workloads as W
+
+REF = os.path.join(os.path.dirname(__file__), "golden", "ref")
+
+
+@pytest.fixture(scope="module")
+def small_world(oracle_mod):
+    env = W.torus(0.3, 0.08, 14, 8, center=(0.5, 0.5, 0.5), seed=3, noise=0.05)
+    rob = W.torus(0.1, 0.03, 8, 6, seed=4, noise=0.05)
+    return oracle_mod.OracleWorld().setup(*env, *rob)
+
+
+def test_bvh_equals_bruteforce(small_world):
+    qs = W.random_states(4000, seed=21)
+    rng = np.random.default_rng(22)
+    qs[:800, 3:] *= rng.uniform(0.6, 1.4, size=(800, 1))  # non-unit quaternions
+    v = small_world.valid_states(qs)
+    bf = np.array([small_world.is_valid_bruteforce(q) for q in qs])
+    assert 0.1 < (~bf).mean() < 0.9
+    assert np.array_equal(v.astype(bool), bf)
+
+
+def test_bvh_equals_bruteforce_with_perturbation(oracle_mod):
+    env = W.torus(0.3, 0.08, 14, 8, center=(0.5, 0.5, 0.5), seed=5, noise=0.05)
+    rob = W.torus(0.1, 0.03, 8, 6, seed=6, noise=0.05)
+    ow = oracle_mod.OracleWorld().setup(*env, *rob)
+    ow.set_perturbation([0.02, -0.03, 0.01, 0.95, 0.1, -0.2, 0.05])
+    qs = W.random_states(1500, seed=23)
+    v = ow.valid_states(qs).astype(bool)
+    bf = np.array([ow.is_valid_bruteforce(q) for q in qs])
+    assert np.array_equal(v, bf)
+
+
+def test_unloaded_world_is_valid(oracle_mod):
+    ow = oracle_mod.OracleWorld()
+    assert ow.is_valid([0.5, 0.5, 0.5, 1, 0, 0, 0])  # unit_world.cc:218-219
+
+
+def test_unit_world_normalisation(oracle_mod, torus_meshes):
+    """scaleToUnit: float(1/max(span)) with origin-seeded boxes; robot centred
+    at the binary32 mean of its vertices (unit_world.cc:123-161)."""
+    (eV, eF), (rV, rF) = torus_meshes
+    ow = oracle_mod.OracleWorld().setup(eV, eF, rV, rF)
+    def span(V):
+        lo = np.minimum(V.min(0), 0)
+        hi = np.maximum(V.max(0), 0)
+        return np.float32((hi - lo).max())
+    s = np.float32(1.0 / float(max(span(eV), span(rV))))
+    assert ow.scene_scale == s
+    assert np.array_equal(ow.cd_vertices(0), eV.astype(np.float64) * np.float64(s))
+    c = ow.center(1)
+    assert np.allclose(c, rV.astype(np.float64).mean(0), atol=1e-6)
+    t = (s * -c).astype(np.float32)
+    assert np.array_equal(ow.cd_vertices(1), rV.astype(np.float64) * np.float64(s) + t.astype(np.float64))
+
+
+def test_transit_closed_forms(torus_oracle):
+    qs = W.random_states(2000, seed=31)
+    free = qs[torus_oracle.valid_states(qs).astype(bool)][:200]
+    ends = W.segment_ends(free, seed=32, lmin=0.02, lmax=0.4)
+    vd = 1.0 / 1024  # sums of a power of two are exact: step k is at k * vd
+    for a, b in zip(free, ends):
+        from oracle import oracle as O
+        d = O.distance(a, b)
+        lf, st, done, lt, t, ns = torus_oracle.transit(a, b, vd)
+        if done:
+            assert ns == int(np.ceil(d / vd))
+            assert t == 0.0  # the shadowed outer tau
+            assert lt == d * (1.0 / d)  # the last sample is at delta = dist
+            assert np.array_equal(lf, st)
+        else:
+            assert 1 <= ns <= int(np.ceil(d / vd))
+            assert t == min(ns * vd, d) * (1.0 / d)
+            assert lt == ((ns - 1) * vd) * (1.0 / d)
+            assert not torus_oracle.is_valid(st)
+            if ns == 1:
+                assert np.array_equal(lf, a)
+            else:
+                assert torus_oracle.is_valid(lf)
+    # distance 0 and verify_delta >= distance: (from, to, false, 0, 1) without checks
+    a = free[0]
+    lf, st, done, lt, t, ns = torus_oracle.transit(a, a, vd)
+    assert (done, lt, t, ns) == (False, 0.0, 1.0, 0)
+    assert not torus_oracle.is_valid_transition(a, a, vd)
+    lf, st, done, lt, t, ns = torus_oracle.transit(a, ends[0], 10.0)
+    assert (done, lt, t, ns) == (False, 0.0, 1.0, 0) and np.array_equal(st, ends[0])
+
+
+def test_bisection_invariants(torus_oracle):
+    from oracle import oracle as O
+    qs = W.random_states(2000, seed=41)
+    free = qs[torus_oracle.valid_states(qs).astype(bool)][:300]
+    ends = W.segment_ends(free, seed=42)
+    r = torus_oracle.touch_batch(free, ends, 1.0 / 1024, 20)
+    hit = np.nonzero(~r["done"].astype(bool))[0]
+    assert len(hit) > 10
+    for i in hit[:40]:
+        lo, hi = r["lo"][i], r["hi"][i]
+        assert r["last_tau"][i] <= lo < hi <= r["tau"][i]
+        # each halving rounds 0.5 * (lo + hi) once
+        assert abs((hi - lo) - (r["tau"][i] - r["last_tau"][i]) / 2 ** 20) <= 64 * np.finfo(float).eps
+        if lo > 0:
+            assert torus_oracle.is_valid(O.interpolate(free[i], ends[i], lo))
+        assert not torus_oracle.is_valid(O.interpolate(free[i], ends[i], hi))
+        assert r["queries"][i] == torus_oracle.transit(free[i], ends[i], 1.0 / 1024)[5] + 20
+
+
+def test_visibility_semantics(torus_oracle):
+    qs = W.random_states(400, seed=51)
+    ms = qs[torus_oracle.valid_states(qs).astype(bool)][:24]
+    M = torus_oracle.visibility_matrix(ms, True, 1.0 / 256)
+    assert np.array_equal(M, M.T) and not np.diag(M).any()
+    for i in range(0, 24, 5):
+        for j in range(i + 1, 24, 3):
+            assert M[i, j] == torus_oracle.is_valid_transition(ms[i], ms[j], 1.0 / 256)
+    M2 = torus_oracle.visibility_matrix2(ms[:10], True, ms[10:], True, 1.0 / 256)
+    assert np.array_equal(M2, M[:10, 10:])
+    P = torus_oracle.visibility_pair(ms[:10], True, ms[10:20], True, 1.0 / 256)
+    assert np.array_equal(P, np.diag(M[:10, 10:20]))
+
+
+# ---- the reference's own fixtures ------------------------------------------------
+def _alpha_world(oracle_mod, env, rob):
+    ow = oracle_mod.OracleWorld()
+    ow.load_obj(0, os.path.join(REF, env))
+    ow.load_obj(1, os.path.join(REF, rob))
+    ow.scale_to_unit()
+    ow.angle_model()
+    return ow
+
+
+def test_alpha_expecting_true(oracle_mod):
+    """src/RL/test_collide.py:40-43: both states print "Expecting True"."""
+    ow = _alpha_world(oracle_mod, "env-1.2.obj", "robot.obj")
+    for x in (0.04, 0.7):
+        q = np.array([x, 0.0, 0.0, 0.5, 0.5, 0.5, 0.5], np.float32).astype(np.float64)
+        assert ow.is_valid(q)
+
+
+def _ompl_to_unit(ow, q):
+    """translateOMPLStateToUnitState (unit_world.cc:689-711) with the robot
+    centre equal to the OMPL centre: w-last -> w-first, translateToUnitState,
+    then compose (which normalises the quaternion again)."""
+    from oracle import oracle as O
+    w = np.array([q[0], q[1], q[2], q[6], q[3], q[4], q[5]])
+    u = ow.translate_to_unit_state(w)
+    qq = u[3:] / np.sqrt((u[4] * u[4] + u[6] * u[6]) + (u[5] * u[5] + u[3] * u[3]))
+    return np.concatenate([u[:3], qq])
+
+
+def test_alpha_solution_path_states_valid(oracle_mod):
+    """res/alpha/alpha-1.2.org.path is an OMPL solution path of the alpha
+    puzzle (geometry per src/GP/aniconf12.py:6-16): every state on it is
+    collision free."""
+    ow = _alpha_world(oracle_mod, "alpha_env-1.2.org.obj", "alpha-1.2.org.obj")
+    path = np.loadtxt(os.path.join(REF, "alpha-1.2.org.path"))
+    assert path.shape == (81, 7)
+    unit = np.array([_ompl_to_unit(ow, q) for q in path])
+    v = ow.valid_states(unit).astype(bool)
+    assert v.all(), np.nonzero(~v)[0]
+
+
+def test_alpha_robot_center_matches_recorded_ompl_center(oracle_mod):
+    """src/GP/aniconf12.py:16 records the robot's OMPL centre, the binary32
+    mean of the vertices assimp's importer produced for alpha-1.2.org.obj;
+    it pins the corner-joining rule (normals within (1e-5)^2 fold together)."""
+    ow = oracle_mod.OracleWorld()
+    ow.load_obj(1, os.path.join(REF, "alpha-1.2.org.obj"))
+    want = np.array([16.973146438598633, 1.2278236150741577, 10.204807281494141])
+    assert np.array_equal(ow.center(1).astype(np.float64), want)
