@@ -104,6 +104,9 @@ def main():
     ap.add_argument("--segments", type=int, default=N_SEG)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="first copy a buffer of this many MiB three times (a known-size "
+                         "kernel that calibrates the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -116,6 +119,13 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
+    if args.copy_mib:
+        src = torch.ones(args.copy_mib << 18, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     env, rob = W.torus_scene(0)
     uw = pyosr.create_unit_world_from_arrays(env, rob)
     starts, ends = build_workload(uw, W, args.segments)

@@ -710,8 +710,7 @@ __device__ __forceinline__ int steps_exact(double vd, double dist)
  * same 0.5*(lo+hi) sequence the sequential search would use), then walks the
  * tree with the ballot of results. */
 template <bool WIDE>
-__device__ void bisect_wave(const DevScene& s, const SegShared* sh, double& lo, double& hi, int levels,
-                            uint32_t* smem)
+__device__ void bisect_wave(const DevScene& s, const SegShared* sh, double& lo, double& hi, int levels)
 {
 	const int lane = threadIdx.x & 63;
 	for (int done = 0; done < levels;) {
@@ -749,6 +748,266 @@ __device__ void bisect_wave(const DevScene& s, const SegShared* sh, double& lo, 
 		}
 		done += k;
 	}
+}
+
+/* ---- touch search: bisection over a swept candidate list ---- */
+/*
+ * Every probe of a contact bisection lies on the segment between the bracket
+ * ends [lo, hi].  One traversal with the robot boxes grown by the largest
+ * displacement any robot point undergoes over that interval lists the leaf
+ * pairs (env leaf, robot leaf) that can touch at any probe; each round then
+ * tests only those pairs: a wave-uniform loop of binary32 leaf-box tests with
+ * one probe per lane, and the exact binary64 triangle tests packed 64
+ * (probe, pair) combinations per pass.  The outcome per probe is the same
+ * "some triangle pair intersects" the full traversal decides.
+ */
+enum { SWEEP_QUEUE = 256, SWEEP_CANDS = 128 };
+
+struct BisectShared {
+	double xf[32][12];                           /* exact transform per probe: R row-major, T */
+	uint32_t qa[SWEEP_QUEUE], qb[SWEEP_QUEUE];   /* breadth-first queue of node pairs (ring) */
+	uint32_t ca[SWEEP_CANDS], cb[SWEEP_CANDS];   /* candidate leaf node pairs */
+	uint32_t combo[64];                          /* (candidate << 8) | probe, one pass */
+};
+
+/* Make this wave's LDS stores visible to its own later LDS loads. */
+__device__ __forceinline__ void wave_lds_sync()
+{
+	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+	__builtin_amdgcn_wave_barrier();
+	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+/* Number of set bits of m below this lane. */
+__device__ __forceinline__ int lane_rank(uint64_t m)
+{
+	return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+/*
+ * Candidate leaf pairs for all poses se3_segment_at(sg, t), t in [t0, t1]:
+ * breadth first, one node pair per lane.  Poses are placed at the middle
+ * parameter tm; a slerp moves the rotation by at most 2 * theta * |t - tm|
+ * radians and the translation by |p1 - p0| * |t - tm|, so growing the robot
+ * boxes by that rotation times the robot's reach plus that translation (with
+ * a relative 1e-3 to spare) keeps every probe's boxes inside.  Returns the
+ * number of candidates, or -1 if the queue or the list would overflow.
+ */
+__device__ int sweep_candidates(const DevScene& s, const se3_segment* sg, double t0, double t1, BisectShared* bs)
+{
+	const int lane = threadIdx.x & 63;
+	QCull c;
+	{
+		double q[7];
+		se3_segment_at(sg, 0.5 * (t0 + t1), q);
+		make_qcull(s, q, c);
+	}
+	const double half = 0.5 * (t1 - t0);
+	/* normalised lerp between quaternions closer than acos(1 - 2^-52) */
+	const double th = sg->lerp ? 4.5e-8 : sg->theta;
+	double d2 = 0.0;
+#pragma unroll
+	for (int k = 0; k < 3; k++) {
+		const double d = sg->p1[k] - sg->p0[k];
+		d2 += d * d;
+	}
+	const double grow = (2.0 * th * half * (double)s.rob_reach + sqrt(d2) * half) * 1.001 + 1e-12;
+	c.infl += (float)grow * 1.0001f;
+	int head = 0, tail = 1, nc = 0;
+	if (lane == 0) {
+		bs->qa[0] = 0u;
+		bs->qb[0] = 0u;
+	}
+	wave_lds_sync();
+	while (head < tail) {
+		const int nproc = (tail - head < 64) ? (tail - head) : 64;
+		const bool act = lane < nproc;
+		int a = 0, b = 0;
+		if (act) {
+			const int slot = (head + lane) & (SWEEP_QUEUE - 1);
+			a = (int)bs->qa[slot];
+			b = (int)bs->qb[slot];
+		}
+		head += nproc;
+		bool leaf = false, inner = false;
+		int a1 = 0, b1 = 0, a2 = 0, b2 = 0;
+		if (act) {
+			const GNode A = load_node(s.env.nodes, a);
+			const GNode Bn = load_node(s.rob.nodes, b);
+			bool ov;
+			{
+				RobBox B;
+				load_rob_box(Bn, c, B);
+				ov = !obb_disjoint(A, B, c.eta);
+			}
+			const int ca = __float_as_int(A.a2_child.w);
+			const int cb = __float_as_int(Bn.a2_child.w);
+			const bool la = ca < 0, lb = cb < 0;
+			leaf = ov && la && lb;
+			inner = ov && !(la && lb);
+			const float sa = fmaf(A.c_ex.w, A.c_ex.w, fmaf(A.a0_ey.w, A.a0_ey.w, A.a1_ez.w * A.a1_ez.w));
+			const float sb = fmaf(Bn.c_ex.w, Bn.c_ex.w, fmaf(Bn.a0_ey.w, Bn.a0_ey.w, Bn.a1_ez.w * Bn.a1_ez.w));
+			if (!la && (lb || sa > sb)) {
+				a1 = ca; b1 = b; a2 = ca + 1; b2 = b;
+			} else {
+				a1 = a; b1 = cb; a2 = a; b2 = cb + 1;
+			}
+		}
+		const uint64_t ml = __ballot(leaf), mi = __ballot(inner);
+		const int nl = __popcll(ml), ni = __popcll(mi);
+		if (nc + nl > SWEEP_CANDS || (tail - head) + 2 * ni > SWEEP_QUEUE)
+			return -1;
+		if (leaf) {
+			const int r = nc + lane_rank(ml);
+			bs->ca[r] = (uint32_t)a;
+			bs->cb[r] = (uint32_t)b;
+		}
+		if (inner) {
+			const int r = tail + 2 * lane_rank(mi);
+			bs->qa[r & (SWEEP_QUEUE - 1)] = (uint32_t)a1;
+			bs->qb[r & (SWEEP_QUEUE - 1)] = (uint32_t)b1;
+			bs->qa[(r + 1) & (SWEEP_QUEUE - 1)] = (uint32_t)a2;
+			bs->qb[(r + 1) & (SWEEP_QUEUE - 1)] = (uint32_t)b2;
+		}
+		nc += nl;
+		tail += 2 * ni;
+		wave_lds_sync();
+	}
+	return nc;
+}
+
+/* Exact tests of the first nb (candidate, probe) combinations in bs->combo,
+ * one per lane; returns the mask of probes found colliding. */
+__device__ uint64_t sat_pass(const DevScene& s, const BisectShared* bs, int nb)
+{
+	const int lane = threadIdx.x & 63;
+	bool h = false;
+	int p = 0;
+	if (lane < nb) {
+		const uint32_t cmb = bs->combo[lane];
+		const int i = (int)(cmb >> 8);
+		p = (int)(cmb & 0xffu);
+		ExactTf x;
+#pragma unroll
+		for (int k = 0; k < 9; k++)
+			x.R[k] = bs->xf[p][k];
+#pragma unroll
+		for (int k = 0; k < 3; k++)
+			x.T[k] = bs->xf[p][9 + k];
+		const int te = ~__float_as_int(s.env.nodes[bs->ca[i]].a2_child.w);
+		const int tr = ~__float_as_int(s.rob.nodes[bs->cb[i]].a2_child.w);
+		h = leaf_pair_x(s, x, te, tr);
+	}
+	uint64_t hb = __ballot(h);
+	uint64_t out = 0;
+	while (hb) {
+		const int j = __ffsll((long long)hb) - 1;
+		hb &= hb - 1;
+		out |= 1ull << rdl(p, j);
+	}
+	return out;
+}
+
+/* `levels` exact bisection halvings (the probe-tree rounds of bisect_wave)
+ * evaluated on the nc candidate pairs in bs. */
+__device__ void bisect_cands(const DevScene& s, const SegShared* sh, double& lo, double& hi, int levels,
+                             BisectShared* bs, int nc)
+{
+	const int lane = threadIdx.x & 63;
+	for (int done = 0; done < levels;) {
+		const int k = (levels - done < 5) ? (levels - done) : 5;
+		const int nodes = (1 << k) - 1;
+		const bool act = lane < nodes;
+		double mid = 0.0;
+		if (act) {
+			const int h = lane + 1;
+			const int dh = 31 - __clz(h);
+			double l0 = lo, h0 = hi;
+			for (int bbit = dh - 1; bbit >= 0; bbit--) {
+				const double m = 0.5 * (l0 + h0);
+				if ((h >> bbit) & 1)
+					l0 = m;
+				else
+					h0 = m;
+			}
+			mid = 0.5 * (l0 + h0);
+		}
+		QCull c;
+		{
+			double q[7];
+			se3_segment_at(&sh->sg, act ? mid : lo, q);
+			make_qcull(s, q, c);
+			ExactTf x;
+			make_exact(s, q, x);
+			if (act) {
+#pragma unroll
+				for (int t = 0; t < 9; t++)
+					bs->xf[lane][t] = x.R[t];
+#pragma unroll
+				for (int t = 0; t < 3; t++)
+					bs->xf[lane][9 + t] = x.T[t];
+			}
+		}
+		wave_lds_sync();
+		uint64_t hitm = 0;
+		int nb = 0;
+		for (int i = 0; i < nc; i++) {
+			const int a = __builtin_amdgcn_readfirstlane((int)bs->ca[i]);
+			const int b = __builtin_amdgcn_readfirstlane((int)bs->cb[i]);
+			const GNode A = load_node_uniform(s.env.nodes, a);
+			const GNode Bn = load_node_uniform(s.rob.nodes, b);
+			bool ov;
+			{
+				RobBox B;
+				load_rob_box(Bn, c, B);
+				ov = act && !((hitm >> lane) & 1ull) && !obb_disjoint(A, B, c.eta);
+			}
+			const uint64_t m = __ballot(ov);
+			const int cnt = __popcll(m);
+			if (cnt == 0)
+				continue;
+			if (nb + cnt > 64) {
+				hitm |= sat_pass(s, bs, nb);
+				nb = 0;
+				wave_lds_sync();
+			}
+			if (ov)
+				bs->combo[nb + lane_rank(m)] = ((uint32_t)i << 8) | (uint32_t)lane;
+			nb += cnt;
+			wave_lds_sync();
+		}
+		if (nb > 0)
+			hitm |= sat_pass(s, bs, nb);
+		wave_lds_sync();
+		const bool ok = !((hitm >> lane) & 1ull);
+		const uint64_t vm = __ballot(act && ok);
+		int h = 1;
+		for (int d = 0; d < k; d++) {
+			const double m = shfl_d(mid, h - 1);
+			const int bit = (int)((vm >> (h - 1)) & 1ull);
+			if (bit)
+				lo = m;
+			else
+				hi = m;
+			h = 2 * h + bit;
+		}
+		done += k;
+	}
+}
+
+/* Contact refinement of a touch search: candidate-list bisection, or the
+ * packet traversal per probe round when the candidate list overflows. */
+template <bool WIDE>
+__device__ void bisect_touch(const DevScene& s, const SegShared* sh, double& lo, double& hi, int levels,
+                             BisectShared* bs)
+{
+	if (levels <= 0)
+		return;
+	const int nc = sweep_candidates(s, &sh->sg, lo, hi, bs);
+	if (nc >= 0)
+		bisect_cands(s, sh, lo, hi, levels, bs, nc);
+	else
+		bisect_wave<WIDE>(s, sh, lo, hi, levels);
 }
 
 /* Wave-uniform outcome of one segment walk. */
@@ -790,7 +1049,7 @@ __device__ __forceinline__ void seg_setup(SegShared* sh, const double* __restric
 }
 
 template <bool WIDE, int MODE>
-__device__ SegResult segment_walk(const DevScene& s, const SegShared* sh, int levels, uint32_t* smem)
+__device__ SegResult segment_walk(const DevScene& s, const SegShared* sh, int levels, BisectShared* bs)
 {
 	const int lane = threadIdx.x & 63;
 	SegResult r;
@@ -848,7 +1107,7 @@ __device__ SegResult segment_walk(const DevScene& s, const SegShared* sh, int le
 			r.lo = r.last_tau;
 			r.hi = thi;
 			if (MODE == MODE_TOUCH) {
-				bisect_wave<WIDE>(s, sh, r.lo, r.hi, levels, smem);
+				bisect_touch<WIDE>(s, sh, r.lo, r.hi, levels, bs);
 				r.steps += levels;
 			}
 			return r;
@@ -883,7 +1142,7 @@ __global__ void __launch_bounds__(256, SE3CQ_MIN_WAVES) k_segments(DevScene s, c
                                                   const double* __restrict__ to_all, int64_t n, double vd,
                                                   int levels, SegOut o)
 {
-	extern __shared__ uint32_t smem[];
+	extern __shared__ double seg_lds[];
 	__shared__ SegShared segsh[4];
 	const int wave = threadIdx.x >> 6;
 	const int lane = threadIdx.x & 63;
@@ -894,7 +1153,8 @@ __global__ void __launch_bounds__(256, SE3CQ_MIN_WAVES) k_segments(DevScene s, c
 	__syncthreads();
 	if (seg >= n)
 		return;
-	const SegResult r = segment_walk<WIDE, MODE>(s, sh, levels, smem);
+	BisectShared* bs = (MODE == MODE_TOUCH) ? reinterpret_cast<BisectShared*>(seg_lds) + wave : nullptr;
+	const SegResult r = segment_walk<WIDE, MODE>(s, sh, levels, bs);
 	if (lane == 0) {
 		o.done[seg] = (uint8_t)r.done;
 		if (o.last_tau) o.last_tau[seg] = r.last_tau;
@@ -955,7 +1215,6 @@ __global__ void __launch_bounds__(256, SE3CQ_MIN_WAVES) k_visibility(DevScene s,
                                                     int64_t npairs, double ivd, int8_t* __restrict__ out8,
                                                     uint32_t* __restrict__ bits, int32_t* __restrict__ steps)
 {
-	extern __shared__ uint32_t smem[];
 	__shared__ SegShared segsh[4];
 	const int wave = threadIdx.x >> 6;
 	const int lane = threadIdx.x & 63;
@@ -976,7 +1235,7 @@ __global__ void __launch_bounds__(256, SE3CQ_MIN_WAVES) k_visibility(DevScene s,
 	__syncthreads();
 	if (p >= npairs)
 		return;
-	const SegResult r = segment_walk<WIDE, MODE_TRANSITION>(s, sh, 0, smem);
+	const SegResult r = segment_walk<WIDE, MODE_TRANSITION>(s, sh, 0, nullptr);
 	const uint8_t done = (uint8_t)r.done;
 	if (lane == 0) {
 		if (VIS == VIS_MATRIX) {
@@ -1068,10 +1327,12 @@ template <int MODE>
 static hipError_t launch_seg_mode(const DevScene& s, const double* from, const double* to, int64_t n, double vd,
                                   int levels, const SegOut& o, hipStream_t st)
 {
-	const int block = pick_block(s);
+	/* one wavefront per segment, four per block; the touch search keeps a
+	 * BisectShared record per wavefront in LDS */
+	const int block = 256;
 	const int per_block = block / 64;
 	const int64_t grid = (n + per_block - 1) / per_block;
-	const size_t lds = stack_bytes(s, block);
+	const size_t lds = (MODE == MODE_TOUCH) ? per_block * sizeof(BisectShared) : 0;
 	if (s.wide)
 		hipLaunchKernelGGL((k_segments<true, MODE>), dim3((unsigned)grid), dim3(block), lds, st, s, from, to, n, vd,
 		                   levels, o);
@@ -1099,10 +1360,10 @@ static hipError_t launch_vis_mode(const DevScene& s, const double* qs0, const do
                                   int64_t npairs, double ivd, int8_t* out8, uint32_t* bits, int32_t* steps,
                                   hipStream_t st)
 {
-	const int block = pick_block(s);
+	const int block = 256;
 	const int per_block = block / 64;
 	const int64_t grid = (npairs + per_block - 1) / per_block;
-	const size_t lds = stack_bytes(s, block);
+	const size_t lds = 0;
 	if (s.wide)
 		hipLaunchKernelGGL((k_visibility<true, VIS>), dim3((unsigned)grid), dim3(block), lds, st, s, qs0, qs1, m, n,
 		                   npairs, ivd, out8, bits, steps);
