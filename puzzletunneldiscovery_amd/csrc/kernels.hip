@@ -569,7 +569,9 @@ __device__ __forceinline__ bool leaf_pair_x(const DevScene& s, const ExactTf& x,
 
 /*
  * Returns the mask of lanes in `live` whose pose (exact binary64 state q)
- * puts the robot in collision.  Traversal phases cull in binary32 and queue
+ * puts the robot in collision.  With FIRST, only the lowest colliding lane is
+ * asked for: once a hit is known, lanes above it are dropped, and the mask
+ * returned is exact up to and including its lowest set bit.  Traversal phases cull in binary32 and queue
  * the (env triangle, robot triangle, lane mask) leaf candidates in VGPR
  * lanes; leaf phases run the exact binary64 tests on the queue.  The culling
  * transform is rebuilt at the start of every traversal phase so it is dead
@@ -577,6 +579,7 @@ __device__ __forceinline__ bool leaf_pair_x(const DevScene& s, const ExactTf& x,
  */
 enum { PACKET_LEAVES = 32 };
 
+template <bool FIRST>
 __device__ uint64_t packet_collide(const DevScene& s, const double* q, uint64_t live)
 {
 	const int lane = threadIdx.x & 63;
@@ -598,6 +601,8 @@ __device__ uint64_t packet_collide(const DevScene& s, const double* q, uint64_t 
 				const int a = rdl(va, sp), b = rdl(vb, sp);
 				uint64_t m = ((uint64_t)(uint32_t)rdl(vmh, sp) << 32) | (uint64_t)(uint32_t)rdl(vml, sp);
 				m &= ~hit;
+				if (FIRST && hit)
+					m &= (hit & (~hit + 1ull)) - 1ull;
 				if (m == 0)
 					continue;
 				const GNode A = load_node_uniform(s.env.nodes, a);
@@ -650,14 +655,16 @@ __device__ uint64_t packet_collide(const DevScene& s, const double* q, uint64_t 
 			for (int i = 0; i < nl; i++) {
 				uint64_t m = ((uint64_t)(uint32_t)rdl(lmh, i) << 32) | (uint64_t)(uint32_t)rdl(lml, i);
 				m &= ~hit;
+				if (FIRST && hit)
+					m &= (hit & (~hit + 1ull)) - 1ull;
 				if (m == 0)
 					continue;
 				const int te = rdl(lte, i), tr = rdl(ltr, i);
 				const bool h = ((m >> lane) & 1ull) && leaf_pair_x(s, x, te, tr);
 				hit |= __ballot(h);
 			}
-			if ((hit & live) == live)
-				return live;
+			if ((hit & live) == live || (FIRST && (hit & live & (~live + 1ull))))
+				return hit & live;
 		}
 		if (sp == 0)
 			return hit & live;
@@ -677,11 +684,12 @@ struct SegShared {
 
 /* Wave-level check of one pose per lane (lanes outside `live` are ignored):
  * returns the mask of colliding lanes. */
+template <bool FIRST>
 __device__ __forceinline__ uint64_t seg_poses_collide(const DevScene& s, const SegShared* sh, double tau, uint64_t live)
 {
 	double q[7];
 	se3_segment_at(&sh->sg, tau, q);
-	return packet_collide(s, q, live);
+	return packet_collide<FIRST>(s, q, live);
 }
 
 /*
@@ -733,7 +741,7 @@ __device__ void bisect_wave(const DevScene& s, const SegShared* sh, double& lo, 
 			mid = 0.5 * (l0 + h0);
 		}
 		const uint64_t live = __ballot(act);
-		const uint64_t hitm = seg_poses_collide(s, sh, act ? mid : lo, live);
+		const uint64_t hitm = seg_poses_collide<false>(s, sh, act ? mid : lo, live);
 		ok = !((hitm >> lane) & 1ull);
 		const uint64_t vm = __ballot(act && ok);
 		int h = 1;
@@ -908,6 +916,66 @@ __device__ uint64_t sat_pass(const DevScene& s, const BisectShared* bs, int nb)
 	return out;
 }
 
+/*
+ * Mask of the lanes in `live` (all below lane 32) whose pose
+ * se3_segment_at(sg, tau) collides, decided on the nc candidate pairs in bs,
+ * which must cover every such pose.
+ */
+__device__ uint64_t cands_collide(const DevScene& s, const se3_segment* sg, BisectShared* bs, int nc, double tau,
+                                  uint64_t live)
+{
+	const int lane = threadIdx.x & 63;
+	const bool act = (live >> lane) & 1ull;
+	QCull c;
+	{
+		double q[7];
+		se3_segment_at(sg, tau, q);
+		make_qcull(s, q, c);
+		ExactTf x;
+		make_exact(s, q, x);
+		if (act) {
+#pragma unroll
+			for (int t = 0; t < 9; t++)
+				bs->xf[lane][t] = x.R[t];
+#pragma unroll
+			for (int t = 0; t < 3; t++)
+				bs->xf[lane][9 + t] = x.T[t];
+		}
+	}
+	wave_lds_sync();
+	uint64_t hitm = 0;
+	int nb = 0;
+	for (int i = 0; i < nc; i++) {
+		const int a = __builtin_amdgcn_readfirstlane((int)bs->ca[i]);
+		const int b = __builtin_amdgcn_readfirstlane((int)bs->cb[i]);
+		const GNode A = load_node_uniform(s.env.nodes, a);
+		const GNode Bn = load_node_uniform(s.rob.nodes, b);
+		bool ov;
+		{
+			RobBox B;
+			load_rob_box(Bn, c, B);
+			ov = act && !((hitm >> lane) & 1ull) && !obb_disjoint(A, B, c.eta);
+		}
+		const uint64_t m = __ballot(ov);
+		const int cnt = __popcll(m);
+		if (cnt == 0)
+			continue;
+		if (nb + cnt > 64) {
+			hitm |= sat_pass(s, bs, nb);
+			nb = 0;
+			wave_lds_sync();
+		}
+		if (ov)
+			bs->combo[nb + lane_rank(m)] = ((uint32_t)i << 8) | (uint32_t)lane;
+		nb += cnt;
+		wave_lds_sync();
+	}
+	if (nb > 0)
+		hitm |= sat_pass(s, bs, nb);
+	wave_lds_sync();
+	return hitm & live;
+}
+
 /* `levels` exact bisection halvings (the probe-tree rounds of bisect_wave)
  * evaluated on the nc candidate pairs in bs. */
 __device__ void bisect_cands(const DevScene& s, const SegShared* sh, double& lo, double& hi, int levels,
@@ -932,53 +1000,7 @@ __device__ void bisect_cands(const DevScene& s, const SegShared* sh, double& lo,
 			}
 			mid = 0.5 * (l0 + h0);
 		}
-		QCull c;
-		{
-			double q[7];
-			se3_segment_at(&sh->sg, act ? mid : lo, q);
-			make_qcull(s, q, c);
-			ExactTf x;
-			make_exact(s, q, x);
-			if (act) {
-#pragma unroll
-				for (int t = 0; t < 9; t++)
-					bs->xf[lane][t] = x.R[t];
-#pragma unroll
-				for (int t = 0; t < 3; t++)
-					bs->xf[lane][9 + t] = x.T[t];
-			}
-		}
-		wave_lds_sync();
-		uint64_t hitm = 0;
-		int nb = 0;
-		for (int i = 0; i < nc; i++) {
-			const int a = __builtin_amdgcn_readfirstlane((int)bs->ca[i]);
-			const int b = __builtin_amdgcn_readfirstlane((int)bs->cb[i]);
-			const GNode A = load_node_uniform(s.env.nodes, a);
-			const GNode Bn = load_node_uniform(s.rob.nodes, b);
-			bool ov;
-			{
-				RobBox B;
-				load_rob_box(Bn, c, B);
-				ov = act && !((hitm >> lane) & 1ull) && !obb_disjoint(A, B, c.eta);
-			}
-			const uint64_t m = __ballot(ov);
-			const int cnt = __popcll(m);
-			if (cnt == 0)
-				continue;
-			if (nb + cnt > 64) {
-				hitm |= sat_pass(s, bs, nb);
-				nb = 0;
-				wave_lds_sync();
-			}
-			if (ov)
-				bs->combo[nb + lane_rank(m)] = ((uint32_t)i << 8) | (uint32_t)lane;
-			nb += cnt;
-			wave_lds_sync();
-		}
-		if (nb > 0)
-			hitm |= sat_pass(s, bs, nb);
-		wave_lds_sync();
+		const uint64_t hitm = cands_collide(s, &sh->sg, bs, nc, act ? mid : lo, __ballot(act));
 		const bool ok = !((hitm >> lane) & 1ull);
 		const uint64_t vm = __ballot(act && ok);
 		int h = 1;
@@ -1094,7 +1116,7 @@ __device__ SegResult segment_walk(const DevScene& s, const SegShared* sh, int le
 		}
 		const double tau = d * inv_dist;
 		const uint64_t am = __ballot(active);
-		const uint64_t cm = am ? seg_poses_collide(s, sh, active ? tau : sh->vd * inv_dist, am) : 0ull;
+		const uint64_t cm = am ? seg_poses_collide<true>(s, sh, active ? tau : sh->vd * inv_dist, am) : 0ull;
 		if (cm) {
 			const int f = __ffsll((long long)cm) - 1;
 			const double thi = shfl_d(tau, f);
