@@ -34,6 +34,15 @@ LEVELS = 20
 C_BV = 286
 C_TRI = 753
 PEAK_FP32_TFLOPS = 157.3
+# HBM bytes per launch of the touch kernel on this workload, from rocprofv3
+# --pmc FETCH_SIZE and WRITE_SIZE (KiB per dispatch), each scaled by what the
+# same counter reads for a 256 MiB device copy (FETCH_SIZE 131,084 KiB, i.e.
+# x2.000; WRITE_SIZE 262,144 KiB, x1.000); see
+# profiles/k_segments_touch_torus65536.md.  Measured, not computed here.
+TRAFFIC_FETCH_KIB = 34443
+TRAFFIC_WRITE_KIB = 271760
+FETCH_SCALE = 262144 / 131084
+WRITE_SCALE = 1.0
 
 
 def build_workload(uw, W, n_seg, seed=0):
@@ -184,8 +193,12 @@ def main():
     roof = None
     if flops_q is not None:
         achieved = queries * flops_q / (kern_ms * 1e-3) / 1e12
-        roof = dict(bound="fp32-valu", achieved=round(achieved, 3), peak=PEAK_FP32_TFLOPS, unit="TFLOP/s",
-                    frac=round(achieved / PEAK_FP32_TFLOPS, 5), traffic=None,
+        traffic = int((TRAFFIC_FETCH_KIB * FETCH_SCALE + TRAFFIC_WRITE_KIB * WRITE_SCALE) * 1024)
+        roof = dict(bound="mfma", achieved=round(achieved, 3), peak=PEAK_FP32_TFLOPS, unit="TFLOP/s",
+                    frac=round(achieved / PEAK_FP32_TFLOPS, 5), traffic=traffic,
+                    note="compute-bound on vector ALUs (no matrix instructions); the compute roof is "
+                         "MI355X's fp32 peak; algorithmic flops = oracle node/triangle test counts x "
+                         "their straight-line binary64 op counts",
                     flops_per_query=round(flops_q, 1), oracle_node_tests_per_query=round(nbv, 2),
                     oracle_tri_tests_per_query=round(ntri, 3), kernel="k_segments<touch>",
                     kernel_ms=round(kern_ms, 4))
