@@ -101,9 +101,11 @@ def icosphere(level):
 
 
 def shell_with_holes(level=6, r_out=0.45, r_in=0.40, n_holes=64, hole_r=0.05, seed=0,
-                     noise=0.002, tube_seg=24):
+                     noise=0.002, tube_seg=64, tube_rings=9):
     """Noisy icosphere shell (outer + inner surface) with cylindrical holes
-    drilled radially through it; each hole is lined by a tube wall."""
+    drilled radially through it; each hole is lined by a tube wall of
+    tube_seg x (tube_rings - 1) quads.  The defaults give about 195k
+    triangles."""
     rng = np.random.default_rng(seed)
     S, SF = icosphere(level)
     dirs = _shoemake(rng, n_holes)[:, 1:]
@@ -135,14 +137,14 @@ def shell_with_holes(level=6, r_out=0.45, r_in=0.40, n_holes=64, hole_r=0.05, se
         u /= np.linalg.norm(u)
         v = np.cross(d, u)
         ring = np.cos(ang)[:, None] * u + np.sin(ang)[:, None] * v
-        rings = []
-        for rad in (r_in * 0.97, r_out * 1.03):
-            along = np.sqrt(max(rad ** 2 - hole_r ** 2, 0.0))
-            rings.append(along * d + hole_r * ring)
-        P = np.concatenate(rings)
+        a0 = np.sqrt(max((r_in * 0.97) ** 2 - hole_r ** 2, 0.0))
+        a1 = np.sqrt(max((r_out * 1.03) ** 2 - hole_r ** 2, 0.0))
+        P = np.concatenate([along * d + hole_r * ring for along in np.linspace(a0, a1, tube_rings)])
         a = np.arange(tube_seg)
         b = (a + 1) % tube_seg
-        F = np.concatenate([np.stack([a, b, b + tube_seg], 1), np.stack([a, b + tube_seg, a + tube_seg], 1)])
+        quads = [np.concatenate([np.stack([a, b, b + tube_seg], 1), np.stack([a, b + tube_seg, a + tube_seg], 1)])
+                 + k * tube_seg for k in range(tube_rings - 1)]
+        F = np.concatenate(quads)
         Vs.append(P)
         Fs.append(F + off)
         off += len(P)
@@ -184,7 +186,8 @@ def bent_rod(n_along=250, n_around=40, radius=0.012, length=0.3, bend=1.2, seed=
 
 
 def shell_scene(seed=0):
-    """About 200k-triangle shell with 64 holes and a 20k-triangle bent rod."""
+    """About 200k-triangle (195,439) shell with 64 holes and a 20k-triangle
+    (20,080) bent rod."""
     env = shell_with_holes(level=6, seed=seed)
     rob = bent_rod(seed=seed + 1)
     return env, rob
@@ -199,16 +202,19 @@ def random_states(n, seed=0, lo=0.0, hi=1.0):
     return np.concatenate([t, q], axis=1).astype(np.float32).astype(np.float64)
 
 
-def segment_ends(starts, seed=0, lmin=0.05, lmax=0.5):
+def segment_ends(starts, seed=0, lmin=0.05, lmax=0.5, lengths=None):
     """End states for touch segments: a uniform direction on S^5 (translation and
     axis-angle parts) scaled so the two parts' magnitudes sum to a length
-    L ~ U[lmin, lmax]; the end state is the start moved by the translation and
-    rotated by the axis-angle (the apply() of lib/osr/osr_state.cc:243-259)."""
+    L ~ U[lmin, lmax] (or L = lengths when given); the end state is the start
+    moved by the translation and rotated by the axis-angle (the apply() of
+    lib/osr/osr_state.cc:243-259)."""
     rng = np.random.default_rng(seed)
     n = starts.shape[0]
     d = rng.normal(size=(n, 6))
     d /= np.linalg.norm(d, axis=1, keepdims=True)
     L = lmin + (lmax - lmin) * rng.random(n)
+    if lengths is not None:
+        L = np.asarray(lengths, dtype=np.float64).reshape(n)
     mag = np.linalg.norm(d[:, :3], axis=1) + np.linalg.norm(d[:, 3:], axis=1)
     d *= (L / mag)[:, None]
     tr, aa = d[:, :3], d[:, 3:]
