@@ -105,14 +105,196 @@ def oracle_flops_per_query(ow, starts, ends, n):
     return float(C_BV * nbv.mean() + C_TRI * ntri.mean()), float(nbv.mean()), float(ntri.mean())
 
 
+class _Timer:
+    """HIP events on the stream the engine's kernels run on."""
+
+    def __init__(self, torch, stream):
+        self.torch, self.stream = torch, stream
+
+    def run(self, fn, steps, warmup):
+        torch = self.torch
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        ev0.record(self.stream)
+        for _ in range(steps):
+            fn()
+        ev1.record(self.stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        return wall, ev0.elapsed_time(ev1) / steps
+
+
+def _line(workload, value, wall, args, extra):
+    line = {
+        "metric": METRIC, "value": round(value, 1), "unit": "queries/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 (exact leaf tests) + f32 (conservative culling)",
+        "data": "synthetic (seeded scenes, Shoemake states)",
+        "config": {"workload": workload},
+    }
+    line.update(extra)
+    return line
+
+
+def _sample_threads():
+    return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+
+
+def run_states(args, torch, pyosr, W):
+    """1M single-state validity queries, torus scene."""
+    env, rob = W.torus_scene(0)
+    uw = pyosr.create_unit_world_from_arrays(env, rob)
+    n = args.segments or 1_000_000
+    qs = W.random_states(n, seed=1)
+    stream = torch.cuda.Stream()
+    torch.cuda.set_stream(stream)
+    d_q = torch.from_numpy(qs).cuda()
+    d_out = torch.zeros(n, dtype=torch.uint8, device="cuda")
+    wall, kern_ms = _Timer(torch, stream).run(
+        lambda: uw.valid_states_device(d_q.data_ptr(), n, d_out.data_ptr(), stream.cuda_stream), args.steps, args.warmup)
+    uw.check_device()
+    out = d_out.cpu().numpy().astype(bool)
+    extra = {"mismatches": None, "cpu_baseline": None, "kernel": "k_valid_states", "kernel_ms": round(kern_ms, 4)}
+    if not args.no_cpu:
+        from oracle import oracle as O
+        O.set_libm(False)
+        ow = O.OracleWorld().setup(*env, *rob)
+        m = 200_000
+        t0 = time.perf_counter()
+        c, nbv, ntri = ow.valid_states(qs[:m], nthreads=_sample_threads(), counters=True)
+        dt = time.perf_counter() - t0
+        extra["mismatches"] = int((out[:m] != c.astype(bool)).sum())
+        extra["cpu_baseline"] = dict(value=m / dt, unit="queries/s", cores=_sample_threads(), kind="port",
+                                     sample=f"first {m} of the {n} states ({dt:.2f} s)")
+        fq = C_BV * nbv.mean() + C_TRI * ntri.mean()
+        extra["flops_per_query"] = round(float(fq), 1)
+        extra["achieved_tflops"] = round(n * fq / (kern_ms * 1e-3) / 1e12, 3)
+    line = _line(f"state validity: torus env 2,048 tris / robot 1,024 tris, {n} states", n * args.steps / wall,
+                 wall, args, extra)
+    line["config"]["states"] = n
+    return line
+
+
+def _free_states(uw, W, n, seed):
+    out, have, k = [], 0, 0
+    while have < n:
+        qs = W.random_states(2 * n, seed=seed + k)
+        v = uw.is_valid_states(qs)
+        out.append(qs[v])
+        have += int(v.sum())
+        k += 1
+    return np.ascontiguousarray(np.concatenate(out)[:n])
+
+
+def run_uneven(args, torch, pyosr, W):
+    """Advance-until-contact over uneven segments (lognormal step counts), shell scene."""
+    env, rob = W.shell_scene(0)
+    uw = pyosr.create_unit_world_from_arrays(env, rob)
+    n = args.segments or 4 * 1024 * 1024
+    vd = VD
+    starts = _free_states(uw, W, n, 5000)
+    steps_req = W.lognormal_steps(n, seed=5001)
+    ends = W.segment_ends(starts, seed=5002, lengths=steps_req * vd)
+    stream = torch.cuda.Stream()
+    torch.cuda.set_stream(stream)
+    d_f = torch.from_numpy(starts).cuda()
+    d_t = torch.from_numpy(ends).cuda()
+    d_done = torch.zeros(n, dtype=torch.uint8, device="cuda")
+    d_lt = torch.zeros(n, dtype=torch.float64, device="cuda")
+    d_ta = torch.zeros(n, dtype=torch.float64, device="cuda")
+    d_st = torch.zeros(n, dtype=torch.int32, device="cuda")
+    wall, kern_ms = _Timer(torch, stream).run(
+        lambda: uw.transit_device(d_f.data_ptr(), d_t.data_ptr(), n, vd, d_done.data_ptr(), d_lt.data_ptr(),
+                                  d_ta.data_ptr(), d_st.data_ptr(), stream=stream.cuda_stream),
+        args.steps, args.warmup)
+    uw.check_device()
+    steps = d_st.cpu().numpy().astype(np.int64)
+    done = d_done.cpu().numpy().astype(bool)
+    q = int(steps.sum())
+    extra = {"mismatches": None, "cpu_baseline": None, "kernel": "k_segments<transit>", "kernel_ms": round(kern_ms, 4)}
+    if not args.no_cpu:
+        from oracle import oracle as O
+        O.set_libm(False)
+        ow = O.OracleWorld().setup(*env, *rob)
+        m = 4096
+        t0 = time.perf_counter()
+        o = ow.touch_batch(starts[:m], ends[:m], vd, 0, nthreads=_sample_threads())
+        dt = time.perf_counter() - t0
+        lt = d_lt.cpu().numpy()
+        mism = int((o["done"].astype(bool) != done[:m]).sum() + (o["queries"] != steps[:m]).sum()
+                   + (o["last_tau"] != lt[:m]).sum())
+        extra["mismatches"] = mism
+        extra["cpu_baseline"] = dict(value=float(o["queries"].sum()) / dt, unit="queries/s", cores=_sample_threads(),
+                                     kind="port", sample=f"first {m} of the {n} segments "
+                                                         f"({int(o['queries'].sum())} queries, {dt:.2f} s)")
+    line = _line(f"uneven segments: shell env {len(env[1])} tris / bent rod {len(rob[1])} tris, {n} segments, "
+                 f"lognormal(5, 1.2) step counts in [8, 4096] at step 1/1024", q * args.steps / wall, wall, args, extra)
+    line["config"].update({"segments": n, "queries_per_step": q, "free_segments": int(done.sum())})
+    return line
+
+
+def run_visibility(args, torch, pyosr, W):
+    """Bit-packed visibility matrix of 4,096 free milestones, shell scene."""
+    env, rob = W.shell_scene(0)
+    uw = pyosr.create_unit_world_from_arrays(env, rob)
+    n = args.segments or 4096
+    ms = _free_states(uw, W, n, 6000)
+    stream = torch.cuda.Stream()
+    torch.cuda.set_stream(stream)
+    d_q = torch.from_numpy(ms).cuda()
+    words = (n * n + 31) // 32
+    d_bits = torch.zeros(words, dtype=torch.int32, device="cuda")
+    npairs = n * (n - 1) // 2
+    d_steps = torch.zeros(npairs, dtype=torch.int32, device="cuda")
+    wall, kern_ms = _Timer(torch, stream).run(
+        lambda: uw.visibility_bits_device(d_q.data_ptr(), n, VD, d_bits.data_ptr(), d_steps.data_ptr(),
+                                          stream.cuda_stream), args.steps, args.warmup)
+    uw.check_device()
+    steps = d_steps.cpu().numpy().astype(np.int64)
+    q = int(steps.sum())
+    bits = d_bits.cpu().numpy().view(np.uint32)
+    extra = {"mismatches": None, "cpu_baseline": None, "kernel": "k_visibility<matrix>", "kernel_ms": round(kern_ms, 4)}
+    if not args.no_cpu:
+        from oracle import oracle as O
+        O.set_libm(False)
+        ow = O.OracleWorld().setup(*env, *rob)
+        m = 64
+        t0 = time.perf_counter()
+        c = ow.visibility_matrix(ms[:m], True, VD, nthreads=_sample_threads())
+        dt = time.perf_counter() - t0
+        flat = np.unpackbits(bits.view(np.uint8), bitorder="little")[: n * n].reshape(n, n)
+        extra["mismatches"] = int((flat[:m, :m].astype(np.int8) != c).sum())
+        iu = np.triu_indices(m, 1)
+        row, col = iu
+        pidx = row * (2 * n - row - 1) // 2 + (col - row - 1)
+        qs_cpu = int(steps[pidx].sum())
+        extra["cpu_baseline"] = dict(value=qs_cpu / dt, unit="queries/s", cores=_sample_threads(), kind="port",
+                                     sample=f"{m} x {m} block of the {n} x {n} matrix ({qs_cpu} queries, {dt:.2f} s)")
+    line = _line(f"visibility matrix: shell env {len(env[1])} tris / bent rod {len(rob[1])} tris, {n} milestones, "
+                 f"{npairs} pairs at step 1/1024", q * args.steps / wall, wall, args, extra)
+    line["config"].update({"milestones": n, "pairs": npairs, "queries_per_step": q,
+                           "visible_pairs": int(np.unpackbits(bits.view(np.uint8)).sum()) // 2})
+    return line
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--segments", type=int, default=N_SEG)
+    ap.add_argument("--segments", type=int, default=None,
+                    help="queries per launch (default: the workload's own size)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=["touch", "states", "uneven", "visibility"], default="touch",
+                    help="touch (default, the headline): touch-configuration search on the torus scene; "
+                         "states: 1M state validity; uneven: 4M lognormal segments on the shell scene; "
+                         "visibility: 4,096-milestone matrix on the shell scene")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first copy a buffer of this many MiB three times (a known-size "
                          "kernel that calibrates the FETCH_SIZE / WRITE_SIZE counters)")
@@ -135,9 +317,13 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+    if args.workload != "touch":
+        fn = {"states": run_states, "uneven": run_uneven, "visibility": run_visibility}[args.workload]
+        print(json.dumps(fn(args, torch, pyosr, W)), flush=True)
+        return
     env, rob = W.torus_scene(0)
     uw = pyosr.create_unit_world_from_arrays(env, rob)
-    starts, ends = build_workload(uw, W, args.segments)
+    starts, ends = build_workload(uw, W, args.segments or N_SEG)
     n = starts.shape[0]
     dev = torch.device("cuda:0")
     d_from = torch.from_numpy(starts).to(dev)

@@ -369,6 +369,12 @@ class UnitWorld:
     def valid_states_device(self, d_q, n, d_out, stream=None):
         N.check(N.lib().se3cq_valid_states_device(self._w, d_q, int(n), d_out, stream))
 
+    def transit_device(self, d_from, d_to, n, verify_delta, d_done, d_last_tau, d_tau, d_steps,
+                       d_last_free=None, d_state=None, stream=None):
+        """Device-pointer form of transit_states_to_with_contact (asynchronous)."""
+        N.check(N.lib().se3cq_transit_device(self._w, d_from, d_to, int(n), float(verify_delta), d_last_free,
+                                             d_state, d_done, d_last_tau, d_tau, d_steps, stream))
+
     def touch_search_device(self, d_from, d_to, n, verify_delta, levels, d_done, d_last_tau, d_tau, d_lo, d_hi,
                             d_steps, stream=None):
         N.check(N.lib().se3cq_touch_search_device(self._w, d_from, d_to, int(n), float(verify_delta), int(levels),
