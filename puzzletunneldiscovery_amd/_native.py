@@ -71,7 +71,21 @@ SIGNATURES = {
     "se3cq_decompose_2": [c_double_p, c_double_p, c_double_p],
     "se3cq_compose_from_angleaxis": [c_double_p, cdbl, c_double_p, c_double_p],
 }
-_RESTYPE = {"se3cq_last_error": ctypes.c_char_p, "se3cq_version": ctypes.c_char_p}
+# include/se3mlp.h: the learned prescreen
+MLP_SIGNATURES = {
+    "se3mlp_last_error": [],
+    "se3mlp_create": [i64, ctypes.c_uint64, ctypes.POINTER(vp)],
+    "se3mlp_destroy": [vp],
+    "se3mlp_param_layout": [c_i64_p],
+    "se3mlp_get_params": [vp, cint, c_float_p, c_float_p],
+    "se3mlp_set_params": [vp, cint, c_float_p, c_float_p],
+    "se3mlp_forward_device": [vp, vp, i64, vp, vp, vp, vp],
+    "se3mlp_train_step_device": [vp, vp, vp, i64, ctypes.c_float, c_float_p, vp],
+    "se3mlp_gemm_device": [vp, i64, vp, i64, cint, cint, cint, vp, i64, vp],
+    "se3mlp_features_device": [vp, i64, vp, vp],
+}
+_RESTYPE = {"se3cq_last_error": ctypes.c_char_p, "se3cq_version": ctypes.c_char_p,
+            "se3mlp_last_error": ctypes.c_char_p}
 
 _lib = None
 
@@ -85,7 +99,7 @@ def lib():
                 f"libse3cq.so not found at {LIB_PATH}; build it first "
                 "(__graft_entry__.build()). There is no CPU fallback.")
         L = ctypes.CDLL(LIB_PATH)
-        for name, args in SIGNATURES.items():
+        for name, args in list(SIGNATURES.items()) + list(MLP_SIGNATURES.items()):
             f = getattr(L, name)
             f.argtypes = args
             f.restype = _RESTYPE.get(name, ctypes.c_int)
@@ -97,6 +111,13 @@ def check(rc):
     if rc != 0:
         msg = lib().se3cq_last_error()
         raise RuntimeError((msg.decode() if msg else "") or f"se3cq error {rc}")
+    return rc
+
+
+def check_mlp(rc):
+    if rc != 0:
+        msg = lib().se3mlp_last_error()
+        raise RuntimeError((msg.decode() if msg else "") or f"se3mlp error {rc}")
     return rc
 
 

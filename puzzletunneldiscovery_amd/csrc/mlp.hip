@@ -1,0 +1,577 @@
+/*
+ * mlp.hip -- the learned prescreen: a 12 -> 256 -> 256 -> 256 -> 256 -> 2
+ * multilayer perceptron over SE(3) states, trained and evaluated with
+ * gfx950 BF16 matrix cores (v_mfma_f32_32x32x16_bf16) and FP32 accumulation.
+ *
+ * Input features per state: the rotation matrix of the normalised quaternion
+ * (row-major, decompose_2 of lib/osr/osr_state.cc:94-99) and the translation
+ * minus 0.5, padded with zeros to 32.  Output: two logits, z_free and z_coll;
+ * P(free) = sigmoid(z_free - z_coll) and the decision margin is
+ * z_free - z_coll.  The surrogate only ranks and prescreens: every state it
+ * accepts is re-verified by the exact engine.
+ *
+ * Every product is one "NT" GEMM, C[m][n] = sum_k A[m][k] * Bt[n][k], with the
+ * reduction index contiguous in both operands: the forward pass uses W
+ * (N x K), the input gradient a transposed bf16 copy of W, and the weight
+ * gradient transposed copies of the activations and output gradients, which
+ * the producing kernels write alongside the row-major ones.
+ */
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/se3mlp.h"
+
+namespace se3mlp {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+enum { IN = 32, HID = 256, OUT = 32, LAYERS = 5, OUT_USED = 2 };
+
+__host__ __device__ inline uint16_t f2bf(float x)
+{
+	uint32_t u;
+	std::memcpy(&u, &x, 4);
+	if ((u & 0x7f800000u) == 0x7f800000u)
+		return (uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u)); /* inf stays inf, NaN stays NaN */
+	u += 0x7fffu + ((u >> 16) & 1u);                                /* round to nearest even */
+	return (uint16_t)(u >> 16);
+}
+
+__host__ __device__ inline float bf2f(uint16_t h)
+{
+	const uint32_t u = (uint32_t)h << 16;
+	float x;
+	std::memcpy(&x, &u, 4);
+	return x;
+}
+
+/* ---- GEMM ---------------------------------------------------------------- */
+struct Gemm {
+	const uint16_t* A;  /* A[m][k] = A[m * lda + k] */
+	const uint16_t* Bt; /* B[k][n] = Bt[n * ldb + k] */
+	int64_t lda, ldb;
+	int M, N, K;        /* M, N multiples of 32; K a multiple of 16 */
+	const float* bias;  /* [N] or null */
+	int relu;           /* max(0, .) after the bias */
+	const uint16_t* mask; int64_t ldm; /* if set: zero where mask[m][n] <= 0 (ReLU backward) */
+	float* C; int64_t ldc;             /* f32 out, optional */
+	uint16_t* Cb; int64_t ldcb;        /* bf16 out, optional */
+	uint16_t* Ct; int64_t ldct;        /* bf16 transposed out Ct[n][m], optional */
+};
+
+/*
+ * One wavefront per 32 x 32 output tile, four per block.  Lane l holds
+ * A[m0 + l%32][k + 8*(l/32) .. +7] and B[k + 8*(l/32) .. +7][n0 + l%32], and
+ * accumulator element i of lane l is C[m0 + 8*(i/4) + 4*(l/32) + i%4][n0 + l%32].
+ */
+__global__ void __launch_bounds__(256) k_gemm_nt(Gemm g)
+{
+	const int wave = threadIdx.x >> 6;
+	const int lane = threadIdx.x & 63;
+	const int m0 = (blockIdx.y * 2 + (wave >> 1)) * 32;
+	const int n0 = (blockIdx.x * 2 + (wave & 1)) * 32;
+	if (m0 >= g.M || n0 >= g.N)
+		return;
+	const int r = lane & 31, kq = (lane >> 5) * 8;
+	const uint16_t* pa = g.A + (int64_t)(m0 + r) * g.lda + kq;
+	const uint16_t* pb = g.Bt + (int64_t)(n0 + r) * g.ldb + kq;
+	f32x16 acc = {};
+	for (int k = 0; k < g.K; k += 16) {
+		const uint4 va = *reinterpret_cast<const uint4*>(pa + k);
+		const uint4 vb = *reinterpret_cast<const uint4*>(pb + k);
+		bf16x8 a, b;
+		std::memcpy(&a, &va, 16);
+		std::memcpy(&b, &vb, 16);
+		acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+	}
+	const int n = n0 + r;
+	const float bn = g.bias ? g.bias[n] : 0.0f;
+#pragma unroll
+	for (int i = 0; i < 16; i++) {
+		const int m = m0 + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
+		float v = acc[i] + bn;
+		if (g.relu)
+			v = v > 0.0f ? v : 0.0f;
+		if (g.mask && !(bf2f(g.mask[(int64_t)m * g.ldm + n]) > 0.0f))
+			v = 0.0f;
+		if (g.C)
+			g.C[(int64_t)m * g.ldc + n] = v;
+		const uint16_t h = f2bf(v);
+		if (g.Cb)
+			g.Cb[(int64_t)m * g.ldcb + n] = h;
+		if (g.Ct)
+			g.Ct[(int64_t)n * g.ldct + m] = h;
+	}
+}
+
+/* ---- features, loss, reductions, optimiser ---------------------------------- */
+/* Rotation of the normalised quaternion (Eigen toRotationMatrix on q/|q|) and
+ * translation - 0.5; rows >= n (padding up to the tile size) are zero. */
+__global__ void k_features(const double* __restrict__ q, int64_t n, int64_t rows, uint16_t* __restrict__ X,
+                           uint16_t* __restrict__ XT)
+{
+	const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= rows)
+		return;
+	float f[IN];
+#pragma unroll
+	for (int k = 0; k < IN; k++)
+		f[k] = 0.0f;
+	if (i < n) {
+		const double* s = q + 7 * i;
+		double w = s[3], x = s[4], y = s[5], z = s[6];
+		const double nn = sqrt((x * x + z * z) + (y * y + w * w));
+		if (nn > 0.0) {
+			w /= nn; x /= nn; y /= nn; z /= nn;
+		} else {
+			w = 1.0;
+		}
+		const double tx = 2 * x, ty = 2 * y, tz = 2 * z;
+		const double R[9] = {1 - (ty * y + tz * z), tx * y - tz * w, tx * z + ty * w,
+		                     tx * y + tz * w, 1 - (tx * x + tz * z), ty * z - tx * w,
+		                     tx * z - ty * w, ty * z + tx * w, 1 - (tx * x + ty * y)};
+#pragma unroll
+		for (int k = 0; k < 9; k++)
+			f[k] = (float)R[k];
+#pragma unroll
+		for (int k = 0; k < 3; k++)
+			f[9 + k] = (float)(s[k] - 0.5);
+	}
+#pragma unroll
+	for (int k = 0; k < IN; k++) {
+		const uint16_t h = f2bf(f[k]);
+		X[i * IN + k] = h;
+		if (XT)
+			XT[(int64_t)k * rows + i] = h;
+	}
+}
+
+/* Two-class softmax cross-entropy on columns 0 (free) and 1 (colliding) of
+ * the output logits; label 1 = free.  Writes the gradient with respect to the
+ * logits (scaled by 1/n) as bf16, row-major and transposed, and the per-row
+ * loss. */
+__global__ void k_xent(const float* __restrict__ Z, int64_t n, int64_t rows, const uint8_t* __restrict__ label,
+                       uint16_t* __restrict__ dZ, uint16_t* __restrict__ dZT, float* __restrict__ loss)
+{
+	const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= rows)
+		return;
+	float g0 = 0.0f, g1 = 0.0f, l = 0.0f;
+	if (i < n) {
+		const float z0 = Z[i * OUT], z1 = Z[i * OUT + 1];
+		const float d = z1 - z0; /* log-odds of "colliding" */
+		const float p0 = 1.0f / (1.0f + expf(d)); /* P(free) */
+		const float y0 = label[i] ? 1.0f : 0.0f;
+		const float inv = 1.0f / (float)n;
+		g0 = (p0 - y0) * inv;
+		g1 = -g0;
+		/* -log p(label), computed stably */
+		const float t = y0 > 0.0f ? d : -d;
+		l = t > 0.0f ? t + log1pf(expf(-t)) : log1pf(expf(t));
+	}
+	for (int k = 0; k < OUT; k++) {
+		const float v = k == 0 ? g0 : (k == 1 ? g1 : 0.0f);
+		const uint16_t h = f2bf(v);
+		dZ[i * OUT + k] = h;
+		dZT[(int64_t)k * rows + i] = h;
+	}
+	loss[i] = l;
+}
+
+/* out[j] = sum_i XT[j][i] over i < rows (bias gradient), one wave per j. */
+__global__ void k_rowsum_bf16(const uint16_t* __restrict__ XT, int64_t rows, int cols, float* __restrict__ out)
+{
+	const int j = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+	const int lane = threadIdx.x & 63;
+	if (j >= cols)
+		return;
+	float s = 0.0f;
+	for (int64_t i = lane; i < rows; i += 64)
+		s += bf2f(XT[(int64_t)j * rows + i]);
+	for (int off = 32; off > 0; off >>= 1)
+		s += __shfl_down(s, off, 64);
+	if (lane == 0)
+		out[j] = s;
+}
+
+__global__ void k_sum_f32(const float* __restrict__ x, int64_t n, float* __restrict__ out)
+{
+	__shared__ float part[256];
+	float s = 0.0f;
+	for (int64_t i = threadIdx.x; i < n; i += 256)
+		s += x[i];
+	part[threadIdx.x] = s;
+	__syncthreads();
+	for (int off = 128; off > 0; off >>= 1) {
+		if ((int)threadIdx.x < off)
+			part[threadIdx.x] += part[threadIdx.x + off];
+		__syncthreads();
+	}
+	if (threadIdx.x == 0)
+		out[0] = part[0];
+}
+
+/* Adam on the fp32 master copy; refreshes the bf16 W[rows x cols] and its
+ * transpose WT[cols x rows].  Entries outside `used_rows` stay zero. */
+__global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
+                       int rows, int cols, int used_rows, float lr, float b1, float b2, float eps, float c1, float c2,
+                       uint16_t* __restrict__ W, uint16_t* __restrict__ WT)
+{
+	const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= (int64_t)rows * cols)
+		return;
+	const int r = (int)(i / cols), c = (int)(i % cols);
+	if (r < used_rows) {
+		const float gi = g[i];
+		const float mi = b1 * m[i] + (1.0f - b1) * gi;
+		const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+		m[i] = mi;
+		v[i] = vi;
+		w[i] -= lr * (mi * c1) / (sqrtf(vi * c2) + eps);
+	}
+	if (W) {
+		const uint16_t h = f2bf(w[i]);
+		W[i] = h;
+		if (WT)
+			WT[(int64_t)c * rows + r] = h;
+	}
+}
+
+__global__ void k_cast(const float* __restrict__ w, int rows, int cols, uint16_t* __restrict__ W,
+                       uint16_t* __restrict__ WT)
+{
+	const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= (int64_t)rows * cols)
+		return;
+	const uint16_t h = f2bf(w[i]);
+	W[i] = h;
+	if (WT)
+		WT[(int64_t)(i % cols) * rows + i / cols] = h;
+}
+
+/* P(free) and margin from the output logits. */
+__global__ void k_head(const float* __restrict__ Z, int64_t n, float* __restrict__ pfree, float* __restrict__ margin)
+{
+	const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n)
+		return;
+	const float d = Z[i * OUT] - Z[i * OUT + 1];
+	if (pfree)
+		pfree[i] = 1.0f / (1.0f + expf(-d));
+	if (margin)
+		margin[i] = d;
+}
+
+/* ---- host side ------------------------------------------------------------ */
+struct Buf {
+	void* p = nullptr;
+	size_t n = 0;
+	~Buf() { release(); }
+	void release()
+	{
+		if (p)
+			(void)hipFree(p);
+		p = nullptr;
+		n = 0;
+	}
+	bool alloc(size_t bytes)
+	{
+		release();
+		if (hipMalloc(&p, bytes) != hipSuccess) {
+			p = nullptr;
+			return false;
+		}
+		n = bytes;
+		return hipMemset(p, 0, bytes) == hipSuccess;
+	}
+	template <class T> T* as() const { return (T*)p; }
+};
+
+static const int kIn[LAYERS] = {IN, HID, HID, HID, HID};
+static const int kOut[LAYERS] = {HID, HID, HID, HID, OUT};
+static const int kUsed[LAYERS] = {HID, HID, HID, HID, OUT_USED};
+static const int kFanIn[LAYERS] = {12, HID, HID, HID, HID};
+
+} // namespace se3mlp
+
+using namespace se3mlp;
+
+struct se3mlp_net {
+	int64_t rows = 0; /* batch capacity, a multiple of 64 */
+	int step = 0;
+	Buf w[LAYERS], b[LAYERS], mw[LAYERS], vw[LAYERS], mb[LAYERS], vb[LAYERS], gw[LAYERS], gb[LAYERS];
+	Buf W[LAYERS], WT[LAYERS];
+	Buf X, XT;                  /* features */
+	Buf H[LAYERS], HT[LAYERS];  /* bf16 activations after layer l (H[4] unused) */
+	Buf Z;                      /* f32 output logits */
+	Buf D[LAYERS], DT[LAYERS];  /* bf16 gradient w.r.t. the pre-activation of layer l */
+	Buf loss, lsum;
+	hipStream_t stream = nullptr;
+	std::mutex mu;
+};
+
+namespace {
+thread_local std::string g_err;
+int fail(int rc, const std::string& m)
+{
+	g_err = m;
+	return rc;
+}
+#define HCHK(x)                                                                 \
+	do {                                                                        \
+		hipError_t e_ = (x);                                                    \
+		if (e_ != hipSuccess)                                                   \
+			return fail(SE3MLP_EDEVICE, std::string(#x ": ") + hipGetErrorString(e_)); \
+	} while (0)
+
+uint64_t splitmix(uint64_t& s)
+{
+	uint64_t z = (s += 0x9e3779b97f4a7c15ULL);
+	z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+	z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+	return z ^ (z >> 31);
+}
+
+hipError_t gemm(hipStream_t st, const uint16_t* A, int64_t lda, const uint16_t* Bt, int64_t ldb, int M, int N, int K,
+                const float* bias, int relu, const uint16_t* mask, int64_t ldm, float* C, int64_t ldc, uint16_t* Cb,
+                int64_t ldcb, uint16_t* Ct, int64_t ldct)
+{
+	Gemm g{A, Bt, lda, ldb, M, N, K, bias, relu, mask, ldm, C, ldc, Cb, ldcb, Ct, ldct};
+	dim3 grid((unsigned)((N + 63) / 64), (unsigned)((M + 63) / 64));
+	hipLaunchKernelGGL(k_gemm_nt, grid, dim3(256), 0, st, g);
+	return hipGetLastError();
+}
+
+inline unsigned blocks(int64_t n, int t = 256) { return (unsigned)((n + t - 1) / t); }
+
+/* forward over rows (already featurised into X); fills H/HT and Z */
+int forward_rows(se3mlp_net* net, int64_t rows, bool keep_t, hipStream_t st)
+{
+	const uint16_t* in = net->X.as<uint16_t>();
+	int64_t ldin = IN;
+	for (int l = 0; l < LAYERS; l++) {
+		const bool last = l == LAYERS - 1;
+		HCHK(gemm(st, in, ldin, net->W[l].as<uint16_t>(), kIn[l], (int)rows, kOut[l], kIn[l], net->b[l].as<float>(),
+		          last ? 0 : 1, nullptr, 0, last ? net->Z.as<float>() : nullptr, OUT,
+		          last ? nullptr : net->H[l].as<uint16_t>(), HID,
+		          (last || !keep_t) ? nullptr : net->HT[l].as<uint16_t>(), rows));
+		in = net->H[l].as<uint16_t>();
+		ldin = HID;
+	}
+	return SE3MLP_OK;
+}
+} // namespace
+
+extern "C" {
+
+const char* se3mlp_last_error(void) { return g_err.c_str(); }
+
+int se3mlp_create(int64_t max_batch, uint64_t seed, se3mlp_net** out)
+{
+	if (!out || max_batch <= 0)
+		return fail(SE3MLP_EINVAL, "se3mlp_create: bad arguments");
+	int dev = 0;
+	if (hipGetDevice(&dev) != hipSuccess)
+		return fail(SE3MLP_EDEVICE, "no HIP device available (the surrogate has no CPU fallback)");
+	auto* net = new se3mlp_net();
+	net->rows = (max_batch + 63) / 64 * 64;
+	const int64_t R = net->rows;
+	bool ok = true;
+	uint64_t s = seed;
+	for (int l = 0; l < LAYERS && ok; l++) {
+		const size_t nw = (size_t)kOut[l] * kIn[l];
+		ok = net->w[l].alloc(nw * 4) && net->b[l].alloc(kOut[l] * 4) && net->mw[l].alloc(nw * 4) &&
+		     net->vw[l].alloc(nw * 4) && net->mb[l].alloc(kOut[l] * 4) && net->vb[l].alloc(kOut[l] * 4) &&
+		     net->gw[l].alloc(nw * 4) && net->gb[l].alloc(kOut[l] * 4) && net->W[l].alloc(nw * 2) &&
+		     net->WT[l].alloc(nw * 2) && net->H[l].alloc((size_t)R * HID * 2) &&
+		     net->HT[l].alloc((size_t)R * HID * 2) && net->D[l].alloc((size_t)R * HID * 2) &&
+		     net->DT[l].alloc((size_t)R * HID * 2);
+		if (!ok)
+			break;
+		/* Kaiming-uniform weights over the used fan-in; padding rows and
+		 * columns stay zero */
+		std::vector<float> hw(nw, 0.0f);
+		const float bound = std::sqrt(6.0f / (float)kFanIn[l]);
+		for (int r = 0; r < kUsed[l]; r++)
+			for (int c = 0; c < kFanIn[l]; c++) {
+				const double u = (double)(splitmix(s) >> 11) * (1.0 / 9007199254740992.0);
+				hw[(size_t)r * kIn[l] + c] = (float)((2.0 * u - 1.0) * bound);
+			}
+		ok = hipMemcpy(net->w[l].p, hw.data(), nw * 4, hipMemcpyHostToDevice) == hipSuccess;
+	}
+	ok = ok && net->X.alloc((size_t)R * IN * 2) && net->XT.alloc((size_t)R * IN * 2) &&
+	     net->Z.alloc((size_t)R * OUT * 4) && net->loss.alloc((size_t)R * 4) && net->lsum.alloc(4);
+	if (!ok) {
+		delete net;
+		return fail(SE3MLP_ENOMEM, "se3mlp_create: device allocation failed");
+	}
+	for (int l = 0; l < LAYERS; l++) {
+		hipLaunchKernelGGL(k_cast, dim3(blocks((int64_t)kOut[l] * kIn[l])), dim3(256), 0, nullptr, net->w[l].as<float>(),
+		                   kOut[l], kIn[l], net->W[l].as<uint16_t>(), net->WT[l].as<uint16_t>());
+	}
+	if (hipDeviceSynchronize() != hipSuccess) {
+		delete net;
+		return fail(SE3MLP_EDEVICE, "se3mlp_create: weight upload failed");
+	}
+	*out = net;
+	return SE3MLP_OK;
+}
+
+int se3mlp_destroy(se3mlp_net* net)
+{
+	delete net;
+	return SE3MLP_OK;
+}
+
+int se3mlp_param_layout(int64_t* sizes)
+{
+	/* per layer: weight rows, cols, used rows, used cols */
+	for (int l = 0; l < LAYERS; l++) {
+		sizes[4 * l] = kOut[l];
+		sizes[4 * l + 1] = kIn[l];
+		sizes[4 * l + 2] = kUsed[l];
+		sizes[4 * l + 3] = kFanIn[l];
+	}
+	return LAYERS;
+}
+
+int se3mlp_get_params(se3mlp_net* net, int layer, float* w, float* b)
+{
+	if (!net || layer < 0 || layer >= LAYERS)
+		return fail(SE3MLP_EINVAL, "se3mlp_get_params: bad layer");
+	std::lock_guard<std::mutex> lk(net->mu);
+	HCHK(hipDeviceSynchronize());
+	if (w)
+		HCHK(hipMemcpy(w, net->w[layer].p, (size_t)kOut[layer] * kIn[layer] * 4, hipMemcpyDeviceToHost));
+	if (b)
+		HCHK(hipMemcpy(b, net->b[layer].p, (size_t)kOut[layer] * 4, hipMemcpyDeviceToHost));
+	return SE3MLP_OK;
+}
+
+int se3mlp_set_params(se3mlp_net* net, int layer, const float* w, const float* b)
+{
+	if (!net || layer < 0 || layer >= LAYERS)
+		return fail(SE3MLP_EINVAL, "se3mlp_set_params: bad layer");
+	std::lock_guard<std::mutex> lk(net->mu);
+	HCHK(hipDeviceSynchronize());
+	if (w)
+		HCHK(hipMemcpy(net->w[layer].p, w, (size_t)kOut[layer] * kIn[layer] * 4, hipMemcpyHostToDevice));
+	if (b)
+		HCHK(hipMemcpy(net->b[layer].p, b, (size_t)kOut[layer] * 4, hipMemcpyHostToDevice));
+	hipLaunchKernelGGL(k_cast, dim3(blocks((int64_t)kOut[layer] * kIn[layer])), dim3(256), 0, nullptr,
+	                   net->w[layer].as<float>(), kOut[layer], kIn[layer], net->W[layer].as<uint16_t>(),
+	                   net->WT[layer].as<uint16_t>());
+	HCHK(hipDeviceSynchronize());
+	return SE3MLP_OK;
+}
+
+int se3mlp_forward_device(se3mlp_net* net, const double* d_states, int64_t n, float* d_pfree, float* d_margin,
+                          float* d_logits, void* stream)
+{
+	if (!net || n < 0 || (n > 0 && !d_states))
+		return fail(SE3MLP_EINVAL, "se3mlp_forward_device: bad arguments");
+	std::lock_guard<std::mutex> lk(net->mu);
+	hipStream_t st = (hipStream_t)stream;
+	for (int64_t off = 0; off < n; off += net->rows) {
+		const int64_t cnt = (n - off < net->rows) ? (n - off) : net->rows;
+		const int64_t rows = (cnt + 63) / 64 * 64;
+		hipLaunchKernelGGL(k_features, dim3(blocks(rows)), dim3(256), 0, st, d_states + 7 * off, cnt, rows,
+		                   net->X.as<uint16_t>(), (uint16_t*)nullptr);
+		HCHK(hipGetLastError());
+		const int rc = forward_rows(net, rows, false, st);
+		if (rc)
+			return rc;
+		hipLaunchKernelGGL(k_head, dim3(blocks(cnt)), dim3(256), 0, st, net->Z.as<float>(), cnt,
+		                   d_pfree ? d_pfree + off : nullptr, d_margin ? d_margin + off : nullptr);
+		HCHK(hipGetLastError());
+		if (d_logits)
+			HCHK(hipMemcpy2DAsync(d_logits + 2 * off, 2 * sizeof(float), net->Z.p, OUT * sizeof(float),
+			                      2 * sizeof(float), (size_t)cnt, hipMemcpyDeviceToDevice, st));
+	}
+	return SE3MLP_OK;
+}
+
+int se3mlp_train_step_device(se3mlp_net* net, const double* d_states, const uint8_t* d_labels, int64_t n, float lr,
+                             float* loss_out, void* stream)
+{
+	if (!net || n <= 0 || n > net->rows || !d_states || !d_labels)
+		return fail(SE3MLP_EINVAL, "se3mlp_train_step_device: batch must hold 1..max_batch states");
+	std::lock_guard<std::mutex> lk(net->mu);
+	hipStream_t st = (hipStream_t)stream;
+	const int64_t rows = (n + 63) / 64 * 64;
+	hipLaunchKernelGGL(k_features, dim3(blocks(rows)), dim3(256), 0, st, d_states, n, rows, net->X.as<uint16_t>(),
+	                   net->XT.as<uint16_t>());
+	HCHK(hipGetLastError());
+	int rc = forward_rows(net, rows, true, st);
+	if (rc)
+		return rc;
+	/* output gradient into D[4] / DT[4] (32 columns) */
+	hipLaunchKernelGGL(k_xent, dim3(blocks(rows)), dim3(256), 0, st, net->Z.as<float>(), n, rows, d_labels,
+	                   net->D[4].as<uint16_t>(), net->DT[4].as<uint16_t>(), net->loss.as<float>());
+	HCHK(hipGetLastError());
+	for (int l = LAYERS - 1; l >= 0; l--) {
+		const int no = kOut[l], ni = kIn[l];
+		const uint16_t* inT = l == 0 ? net->XT.as<uint16_t>() : net->HT[l - 1].as<uint16_t>();
+		/* weight gradient: gW[no x ni] = D_l^T (no x rows) . X_l (rows x ni) */
+		HCHK(gemm(st, net->DT[l].as<uint16_t>(), rows, inT, rows, no, ni, (int)rows, nullptr, 0, nullptr, 0,
+		          net->gw[l].as<float>(), ni, nullptr, 0, nullptr, 0));
+		hipLaunchKernelGGL(k_rowsum_bf16, dim3((unsigned)((no + 3) / 4)), dim3(256), 0, st, net->DT[l].as<uint16_t>(),
+		                   rows, no, net->gb[l].as<float>());
+		HCHK(hipGetLastError());
+		if (l > 0) {
+			/* input gradient through ReLU: D_{l-1} = (D_l . W_l) * [H_{l-1} > 0] */
+			HCHK(gemm(st, net->D[l].as<uint16_t>(), no, net->WT[l].as<uint16_t>(), no, (int)rows, ni, no, nullptr, 0,
+			          net->H[l - 1].as<uint16_t>(), HID, nullptr, 0, net->D[l - 1].as<uint16_t>(), HID,
+			          net->DT[l - 1].as<uint16_t>(), rows));
+		}
+	}
+	net->step++;
+	const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+	const float c1 = 1.0f / (1.0f - std::pow(b1, (float)net->step));
+	const float c2 = 1.0f / (1.0f - std::pow(b2, (float)net->step));
+	for (int l = 0; l < LAYERS; l++) {
+		hipLaunchKernelGGL(k_adam, dim3(blocks((int64_t)kOut[l] * kIn[l])), dim3(256), 0, st, net->w[l].as<float>(),
+		                   net->gw[l].as<float>(), net->mw[l].as<float>(), net->vw[l].as<float>(), kOut[l], kIn[l],
+		                   kUsed[l], lr, b1, b2, eps, c1, c2, net->W[l].as<uint16_t>(), net->WT[l].as<uint16_t>());
+		hipLaunchKernelGGL(k_adam, dim3(blocks(kOut[l])), dim3(256), 0, st, net->b[l].as<float>(),
+		                   net->gb[l].as<float>(), net->mb[l].as<float>(), net->vb[l].as<float>(), kOut[l], 1,
+		                   kUsed[l], lr, b1, b2, eps, c1, c2, (uint16_t*)nullptr, (uint16_t*)nullptr);
+		HCHK(hipGetLastError());
+	}
+	if (loss_out) {
+		hipLaunchKernelGGL(k_sum_f32, dim3(1), dim3(256), 0, st, net->loss.as<float>(), n, net->lsum.as<float>());
+		HCHK(hipGetLastError());
+		HCHK(hipMemcpyAsync(loss_out, net->lsum.p, 4, hipMemcpyDeviceToHost, st));
+		HCHK(hipStreamSynchronize(st));
+		*loss_out /= (float)n;
+	}
+	return SE3MLP_OK;
+}
+
+int se3mlp_gemm_device(const uint16_t* d_A, int64_t lda, const uint16_t* d_Bt, int64_t ldb, int M, int N, int K,
+                       float* d_C, int64_t ldc, void* stream)
+{
+	if (M <= 0 || N <= 0 || K <= 0 || M % 32 || N % 32 || K % 16 || lda % 8 || ldb % 8)
+		return fail(SE3MLP_EINVAL, "se3mlp_gemm_device: M, N must be multiples of 32, K of 16, lda/ldb of 8");
+	HCHK(gemm((hipStream_t)stream, d_A, lda, d_Bt, ldb, M, N, K, nullptr, 0, nullptr, 0, d_C, ldc, nullptr, 0, nullptr,
+	          0));
+	return SE3MLP_OK;
+}
+
+int se3mlp_features_device(const double* d_states, int64_t n, uint16_t* d_X, void* stream)
+{
+	if (n <= 0)
+		return n == 0 ? SE3MLP_OK : fail(SE3MLP_EINVAL, "negative count");
+	hipLaunchKernelGGL(k_features, dim3(blocks(n)), dim3(256), 0, (hipStream_t)stream, d_states, n, n, d_X,
+	                   (uint16_t*)nullptr);
+	HCHK(hipGetLastError());
+	return SE3MLP_OK;
+}
+
+} // extern "C"

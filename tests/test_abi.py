@@ -18,10 +18,10 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 REF = os.path.join(ROOT, "tests", "golden", "ref")
 
 
-def _declared():
-    text = open(os.path.join(ROOT, "include", "se3cq.h")).read()
+def _declared(header="se3cq.h", prefix="se3cq_"):
+    text = open(os.path.join(ROOT, "include", header)).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(se3cq_[a-z0-9_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(" + prefix + r"[a-z0-9_]+)\s*\(", text)))
 
 
 def _gpu():
@@ -41,6 +41,10 @@ def test_every_declared_symbol_is_exported():
         assert name in N.SIGNATURES, name
     assert set(N.SIGNATURES) == set(names)
     assert N.lib().se3cq_version().decode()
+    mlp = _declared("se3mlp.h", "se3mlp_")
+    assert len(mlp) >= 9 and set(N.MLP_SIGNATURES) == set(mlp)
+    for name in mlp:
+        assert hasattr(L, name), name
 
 
 def _pair(oracle_mod, env, rob, obj=False):
