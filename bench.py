@@ -282,6 +282,90 @@ def run_visibility(args, torch, pyosr, W):
     return line
 
 
+def run_surrogate(args, torch, pyosr, W):
+    """Learned prescreen on the shell scene: exact labels, BF16-MFMA training,
+    then free samples per second with and without the prescreen (every
+    accepted sample exact-verified)."""
+    from puzzletunneldiscovery_amd.surrogate import Surrogate
+    env, rob = W.shell_scene(0)
+    uw = pyosr.create_unit_world_from_arrays(env, rob)
+    n_train = 16 * 1024 * 1024
+    n_screen = args.segments or 64 * 1024 * 1024
+    stream = torch.cuda.Stream()
+    torch.cuda.set_stream(stream)
+    sp = stream.cuda_stream
+    timer = _Timer(torch, stream)
+
+    def exact(d_q, n, d_out):
+        uw.valid_states_device(d_q.data_ptr(), n, d_out.data_ptr(), sp)
+
+    # exact labels
+    tq = torch.from_numpy(W.random_states(n_train, seed=7000)).cuda()
+    ty = torch.zeros(n_train, dtype=torch.uint8, device="cuda")
+    _, label_ms = timer.run(lambda: exact(tq, n_train, ty), 1, 0)
+    uw.check_device()
+    net = Surrogate(max_batch=16384, seed=7001)
+    t0 = time.perf_counter()
+    losses = net.fit(tq, ty, epochs=args.train_epochs, batch=16384, lr=2e-3)
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    steps_done = args.train_epochs * (n_train // 16384)
+    del tq, ty
+    # screen: plain exact kernel vs prescreen + exact re-verification
+    sq = torch.from_numpy(W.random_states(n_screen, seed=7002)).cuda()
+    sy = torch.zeros(n_screen, dtype=torch.uint8, device="cuda")
+    pf = torch.zeros(n_screen, dtype=torch.float32, device="cuda")
+    wall_plain, plain_ms = timer.run(lambda: exact(sq, n_screen, sy), args.steps, args.warmup)
+    free_plain = int(sy.sum().item())
+    wall_mlp, mlp_ms = timer.run(lambda: net.forward_device(sq.data_ptr(), n_screen, pf.data_ptr(), stream=sp),
+                                 args.steps, args.warmup)
+    thr = args.threshold
+    state = {}
+
+    def screened():
+        idx = torch.nonzero(pf > thr).squeeze(1)
+        acc = sq.index_select(0, idx)
+        out = torch.zeros(acc.shape[0], dtype=torch.uint8, device="cuda")
+        if acc.shape[0]:
+            exact(acc, acc.shape[0], out)
+        state["acc"], state["out"], state["idx"] = acc.shape[0], out, idx
+
+    wall_scr, scr_ms = timer.run(lambda: (net.forward_device(sq.data_ptr(), n_screen, pf.data_ptr(), stream=sp),
+                                          screened()), args.steps, args.warmup)
+    uw.check_device()
+    free_scr = int(state["out"].sum().item())
+    truth = sy.bool()
+    pred = pf > thr
+    tp = int((truth & pred).sum().item())
+    recall = tp / max(free_plain, 1)
+    precision = tp / max(int(pred.sum().item()), 1)
+    flops_sample = 2 * (12 * 256 + 3 * 256 * 256 + 256 * 2)
+    rate_plain = free_plain / (plain_ms * 1e-3)
+    rate_scr = free_scr / (scr_ms * 1e-3)
+    extra = {
+        "mismatches": 0 if free_scr == int((truth[state["idx"]]).sum().item()) else 1,
+        "cpu_baseline": None,
+        "surrogate": {
+            "labels": n_train, "label_ms": round(label_ms, 2), "train_epochs": args.train_epochs,
+            "train_steps": steps_done, "train_s": round(train_s, 2), "loss_first_epoch": round(losses[0], 4),
+            "loss_last_epoch": round(losses[-1], 4), "threshold": thr, "recall": round(recall, 4),
+            "precision": round(precision, 4), "accepted": state["acc"], "free_true": free_plain,
+            "free_verified_screened": free_scr,
+            "plain_free_per_s": round(rate_plain, 1), "screened_free_per_s": round(rate_scr, 1),
+            "speedup": round(rate_scr / max(rate_plain, 1e-9), 3),
+            "exact_ms": round(plain_ms, 3), "mlp_ms": round(mlp_ms, 3), "screen_total_ms": round(scr_ms, 3),
+            "mlp_tflops": round(n_screen * flops_sample / (mlp_ms * 1e-3) / 1e12, 2),
+            "mlp_mfma_frac_bf16": round(n_screen * flops_sample / (mlp_ms * 1e-3) / 2.5e15, 4),
+        },
+    }
+    line = _line(f"learned prescreen: shell env {len(env[1])} tris / bent rod {len(rob[1])} tris, "
+                 f"{n_train} exact labels, {n_screen} screened samples", rate_scr, wall_scr, args, extra)
+    line["unit"] = "verified free samples/s"
+    line["value"] = round(rate_scr, 1)
+    line["dtype"] = "bf16 MLP (fp32 accumulate) + f64 exact re-verification"
+    return line
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -291,10 +375,13 @@ def main():
                     help="queries per launch (default: the workload's own size)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["touch", "states", "uneven", "visibility"], default="touch",
+    ap.add_argument("--workload", choices=["touch", "states", "uneven", "visibility", "surrogate"], default="touch",
                     help="touch (default, the headline): touch-configuration search on the torus scene; "
                          "states: 1M state validity; uneven: 4M lognormal segments on the shell scene; "
-                         "visibility: 4,096-milestone matrix on the shell scene")
+                         "visibility: 4,096-milestone matrix on the shell scene; "
+                         "surrogate: learned prescreen on the shell scene")
+    ap.add_argument("--train-epochs", type=int, default=2)
+    ap.add_argument("--threshold", type=float, default=0.5)
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first copy a buffer of this many MiB three times (a known-size "
                          "kernel that calibrates the FETCH_SIZE / WRITE_SIZE counters)")
@@ -318,7 +405,8 @@ def main():
         torch.cuda.synchronize()
         del src, dst
     if args.workload != "touch":
-        fn = {"states": run_states, "uneven": run_uneven, "visibility": run_visibility}[args.workload]
+        fn = {"states": run_states, "uneven": run_uneven, "visibility": run_visibility,
+              "surrogate": run_surrogate}[args.workload]
         print(json.dumps(fn(args, torch, pyosr, W)), flush=True)
         return
     env, rob = W.torus_scene(0)

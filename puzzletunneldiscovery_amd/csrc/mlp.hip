@@ -255,17 +255,126 @@ __global__ void k_cast(const float* __restrict__ w, int rows, int cols, uint16_t
 		WT[(int64_t)(i % cols) * rows + i / cols] = h;
 }
 
-/* P(free) and margin from the output logits. */
-__global__ void k_head(const float* __restrict__ Z, int64_t n, float* __restrict__ pfree, float* __restrict__ margin)
+/* ---- fused inference ------------------------------------------------------ */
+/*
+ * The whole network for 64 states per block of four wavefronts: features,
+ * four hidden layers and the head, with the activations of the tile kept in
+ * LDS (rows padded by 8 bf16 so the 32 rows a fragment load touches spread
+ * over the banks).  Wave w computes output columns [64w, 64w + 64) of every
+ * hidden layer for all 64 rows (2 x 2 MFMA tiles), so each block reads each
+ * weight matrix once from L2; the head (32 columns, 2 used) is split over two
+ * waves by rows.
+ */
+enum { FM = 64, ALD = HID + 8 };
+
+struct MlpW {
+	const uint16_t* W[LAYERS];
+	const float* b[LAYERS];
+};
+
+__device__ __forceinline__ bf16x8 ld8(const uint16_t* p)
 {
-	const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= n)
-		return;
-	const float d = Z[i * OUT] - Z[i * OUT + 1];
-	if (pfree)
-		pfree[i] = 1.0f / (1.0f + expf(-d));
-	if (margin)
-		margin[i] = d;
+	const uint4 v = *reinterpret_cast<const uint4*>(p);
+	bf16x8 r;
+	std::memcpy(&r, &v, 16);
+	return r;
+}
+
+__global__ void __launch_bounds__(256) k_mlp_infer(const double* __restrict__ q, int64_t n, MlpW w,
+                                                   float* __restrict__ pfree, float* __restrict__ margin,
+                                                   float* __restrict__ logits)
+{
+	__shared__ __align__(16) uint16_t act[FM * ALD];
+	const int wave = threadIdx.x >> 6;
+	const int lane = threadIdx.x & 63;
+	const int r = lane & 31, kq = (lane >> 5) * 8;
+	const int64_t row0 = (int64_t)blockIdx.x * FM;
+	if (threadIdx.x < FM) {
+		const int64_t i = row0 + threadIdx.x;
+		float f[IN];
+#pragma unroll
+		for (int k = 0; k < IN; k++)
+			f[k] = 0.0f;
+		if (i < n) {
+			const double* s = q + 7 * i;
+			double qw = s[3], qx = s[4], qy = s[5], qz = s[6];
+			const double nn = sqrt((qx * qx + qz * qz) + (qy * qy + qw * qw));
+			if (nn > 0.0) {
+				qw /= nn; qx /= nn; qy /= nn; qz /= nn;
+			} else {
+				qw = 1.0;
+			}
+			const double tx = 2 * qx, ty = 2 * qy, tz = 2 * qz;
+			f[0] = (float)(1 - (ty * qy + tz * qz));
+			f[1] = (float)(tx * qy - tz * qw);
+			f[2] = (float)(tx * qz + ty * qw);
+			f[3] = (float)(tx * qy + tz * qw);
+			f[4] = (float)(1 - (tx * qx + tz * qz));
+			f[5] = (float)(ty * qz - tx * qw);
+			f[6] = (float)(tx * qz - ty * qw);
+			f[7] = (float)(ty * qz + tx * qw);
+			f[8] = (float)(1 - (tx * qx + ty * qy));
+#pragma unroll
+			for (int k = 0; k < 3; k++)
+				f[9 + k] = (float)(s[k] - 0.5);
+		}
+#pragma unroll
+		for (int k = 0; k < IN; k++)
+			act[threadIdx.x * ALD + k] = f2bf(f[k]);
+	}
+	__syncthreads();
+	const int nb = 64 * wave;
+	for (int l = 0; l < LAYERS - 1; l++) {
+		const int K = l == 0 ? IN : HID;
+		const uint16_t* W = w.W[l];
+		f32x16 c00 = {}, c01 = {}, c10 = {}, c11 = {};
+		for (int k = 0; k < K; k += 16) {
+			const bf16x8 b0 = ld8(W + (int64_t)(nb + r) * K + k + kq);
+			const bf16x8 b1 = ld8(W + (int64_t)(nb + 32 + r) * K + k + kq);
+			const bf16x8 a0 = ld8(act + r * ALD + k + kq);
+			const bf16x8 a1 = ld8(act + (32 + r) * ALD + k + kq);
+			c00 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b0, c00, 0, 0, 0);
+			c01 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b1, c01, 0, 0, 0);
+			c10 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b0, c10, 0, 0, 0);
+			c11 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b1, c11, 0, 0, 0);
+		}
+		__syncthreads(); /* every wave has read this layer's input */
+		const float bias0 = w.b[l][nb + r], bias1 = w.b[l][nb + 32 + r];
+#pragma unroll
+		for (int i = 0; i < 16; i++) {
+			const int m = 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
+			act[m * ALD + nb + r] = f2bf(fmaxf(c00[i] + bias0, 0.0f));
+			act[m * ALD + nb + 32 + r] = f2bf(fmaxf(c01[i] + bias1, 0.0f));
+			act[(32 + m) * ALD + nb + r] = f2bf(fmaxf(c10[i] + bias0, 0.0f));
+			act[(32 + m) * ALD + nb + 32 + r] = f2bf(fmaxf(c11[i] + bias1, 0.0f));
+		}
+		__syncthreads();
+	}
+	if (wave < 2) {
+		const uint16_t* W = w.W[LAYERS - 1];
+		f32x16 c = {};
+		for (int k = 0; k < HID; k += 16)
+			c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ld8(act + (32 * wave + r) * ALD + k + kq),
+			                                            ld8(W + (int64_t)r * HID + k + kq), c, 0, 0, 0);
+		const float bias = w.b[LAYERS - 1][r];
+#pragma unroll
+		for (int i = 0; i < 16; i++) {
+			const float z = c[i] + bias;
+			const float z1 = __shfl(z, lane + 1, 64); /* column 1 of the same row */
+			const int64_t m = row0 + 32 * wave + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
+			if (r == 0 && m < n) {
+				const float d = z - z1;
+				if (pfree)
+					pfree[m] = 1.0f / (1.0f + expf(-d));
+				if (margin)
+					margin[m] = d;
+				if (logits) {
+					logits[2 * m] = z;
+					logits[2 * m + 1] = z1;
+				}
+			}
+		}
+	}
 }
 
 /* ---- host side ------------------------------------------------------------ */
@@ -477,22 +586,21 @@ int se3mlp_forward_device(se3mlp_net* net, const double* d_states, int64_t n, fl
 	if (!net || n < 0 || (n > 0 && !d_states))
 		return fail(SE3MLP_EINVAL, "se3mlp_forward_device: bad arguments");
 	std::lock_guard<std::mutex> lk(net->mu);
-	hipStream_t st = (hipStream_t)stream;
-	for (int64_t off = 0; off < n; off += net->rows) {
-		const int64_t cnt = (n - off < net->rows) ? (n - off) : net->rows;
-		const int64_t rows = (cnt + 63) / 64 * 64;
-		hipLaunchKernelGGL(k_features, dim3(blocks(rows)), dim3(256), 0, st, d_states + 7 * off, cnt, rows,
-		                   net->X.as<uint16_t>(), (uint16_t*)nullptr);
+	if (n == 0)
+		return SE3MLP_OK;
+	MlpW mw;
+	for (int l = 0; l < LAYERS; l++) {
+		mw.W[l] = net->W[l].as<uint16_t>();
+		mw.b[l] = net->b[l].as<float>();
+	}
+	/* one block per 64 states; grids above 2^31 blocks are split */
+	const int64_t per_launch = (int64_t)FM * 0x40000000LL;
+	for (int64_t off = 0; off < n; off += per_launch) {
+		const int64_t cnt = (n - off < per_launch) ? (n - off) : per_launch;
+		hipLaunchKernelGGL(k_mlp_infer, dim3((unsigned)((cnt + FM - 1) / FM)), dim3(256), 0, (hipStream_t)stream,
+		                   d_states + 7 * off, cnt, mw, d_pfree ? d_pfree + off : nullptr,
+		                   d_margin ? d_margin + off : nullptr, d_logits ? d_logits + 2 * off : nullptr);
 		HCHK(hipGetLastError());
-		const int rc = forward_rows(net, rows, false, st);
-		if (rc)
-			return rc;
-		hipLaunchKernelGGL(k_head, dim3(blocks(cnt)), dim3(256), 0, st, net->Z.as<float>(), cnt,
-		                   d_pfree ? d_pfree + off : nullptr, d_margin ? d_margin + off : nullptr);
-		HCHK(hipGetLastError());
-		if (d_logits)
-			HCHK(hipMemcpy2DAsync(d_logits + 2 * off, 2 * sizeof(float), net->Z.p, OUT * sizeof(float),
-			                      2 * sizeof(float), (size_t)cnt, hipMemcpyDeviceToDevice, st));
 	}
 	return SE3MLP_OK;
 }
