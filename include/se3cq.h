@@ -139,6 +139,13 @@ int se3cq_visibility_pair_host(se3cq_world* w, const double* qs0, int qs0_are_un
 int se3cq_visibility_bits_device(se3cq_world* w, const double* d_qs, int64_t n, double verify_magnitude,
                                  uint32_t* d_bits, int32_t* d_steps, void* stream);
 
+/* UnitWorld::isDisentangled (unit_world.cc:249-264 -> CDModel::collideBB,
+ * cdmodel.cc:180-198): out[i] = 1 when the robot's and the environment's
+ * bounding boxes (each model's AABB, placed by its transform) are disjoint at
+ * state i.  Both models must be loaded (the reference throws otherwise). */
+int se3cq_disentangled_host(se3cq_world* w, const double* q, int64_t n, uint8_t* out);
+int se3cq_disentangled_device(se3cq_world* w, const double* d_q, int64_t n, uint8_t* d_out, void* stream);
+
 /* Surface device-side failures of earlier _device calls (after sync). */
 int se3cq_check(se3cq_world* w);
 

@@ -62,6 +62,7 @@ typedef struct {
 	ONode* nodes;
 	int64_t nn;
 	int depth;
+	double bc[3], bh[3]; /* local AABB: centre (min+max)*0.5, half extents (max-min)*0.5 */
 } OrModel;
 
 struct OrWorld {
@@ -774,6 +775,21 @@ int or_angle_model(OrWorld* w, float latitude, float longitude)
 		for (int64_t i = 0; i < m->nv; i++)
 			xform_vertex(w->xf[g], me->V + 3 * i, m->V + 3 * i);
 		model_build(m);
+		/* FCL's BVHModel::computeLocalAABB: the AABB of all vertices
+		 * (cdmodel.cc:90-100 wraps it in an fcl::Box) */
+		double lo[3], hi[3];
+		for (int k = 0; k < 3; k++)
+			lo[k] = hi[k] = m->V[k];
+		for (int64_t i = 1; i < m->nv; i++)
+			for (int k = 0; k < 3; k++) {
+				const double v = m->V[3 * i + k];
+				if (v < lo[k]) lo[k] = v;
+				if (v > hi[k]) hi[k] = v;
+			}
+		for (int k = 0; k < 3; k++) {
+			m->bc[k] = (lo[k] + hi[k]) * 0.5;
+			m->bh[k] = (hi[k] - lo[k]) * 0.5;
+		}
 		w->has_cd[g] = 1;
 	}
 	return 0;
@@ -1018,6 +1034,46 @@ int or_is_valid_counted(const OrWorld* w, const double* q7, int64_t* n_bv, int64
 }
 
 int or_is_valid(const OrWorld* w, const double* q7) { return or_is_valid_counted(w, q7, NULL, NULL); }
+
+/* UnitWorld::isDisentangled (unit_world.cc:249-264) -> CDModel::collideBB
+ * (cdmodel.cc:180-198): FCL Box shapes of both models' local AABBs, each
+ * placed by its model transform times the translation to the AABB centre
+ * (the "uncentrializer", cdmodel.cc:93-96), robot box first, then FCL's
+ * box-box test.  1 = the boxes are disjoint; -1 = models not loaded. */
+int or_is_disentangled(const OrWorld* w, const double* q7)
+{
+	if (!w->has_cd[0] || !w->has_cd[1])
+		return -1;
+	double Re[9], te[3], Rr[9], tr[3], pe[3], pr[3];
+	or_state_to_transform(w->pert, Re, te);
+	or_state_to_transform(q7, Rr, tr);
+	se3_place(Re, te, w->cd[0].bc, pe);
+	se3_place(Rr, tr, w->cd[1].bc, pr);
+	return se3_boxes_disjoint(Rr, pr, w->cd[1].bh, Re, pe, w->cd[0].bh);
+}
+
+void or_disentangled_states(const OrWorld* w, const double* q, int64_t n, int8_t* out, int nthreads)
+{
+#ifdef _OPENMP
+	if (nthreads > 0)
+		omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(static)
+#endif
+	for (int64_t i = 0; i < n; i++)
+		out[i] = (int8_t)or_is_disentangled(w, q + 7 * i);
+}
+
+int or_boxes_disjoint(const double* R1, const double* p1, const double* A, const double* R2, const double* p2,
+                      const double* B)
+{
+	return se3_boxes_disjoint(R1, p1, A, R2, p2, B);
+}
+
+void or_model_box(const OrWorld* w, int geo, double* c3, double* h3)
+{
+	memcpy(c3, w->cd[geo].bc, sizeof(double) * 3);
+	memcpy(h3, w->cd[geo].bh, sizeof(double) * 3);
+}
 
 int or_is_valid_bruteforce(const OrWorld* w, const double* q7)
 {

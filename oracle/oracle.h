@@ -107,6 +107,16 @@ void or_visibility_matrix2(const OrWorld* w, const double* qs0, int64_t m, int u
 void or_visibility_pair(const OrWorld* w, const double* qs0, const double* qs1, int64_t n,
                         int unit0, int unit1, double vd, int8_t* out, int nthreads);
 
+/* isDisentangled: 1 if the two models' bounding boxes are disjoint at q,
+ * 0 if they intersect, -1 if a model is missing. */
+int or_is_disentangled(const OrWorld* w, const double* q7);
+void or_disentangled_states(const OrWorld* w, const double* q, int64_t n, int8_t* out, int nthreads);
+/* The box-box test alone (box i: row-major rotation, centre, half extents). */
+int or_boxes_disjoint(const double* R1, const double* p1, const double* A, const double* R2, const double* p2,
+                      const double* B);
+/* A model's local AABB centre and half extents. */
+void or_model_box(const OrWorld* w, int geo, double* c3, double* h3);
+
 /* The library's portable transcendental functions (se3_math.h). */
 double or_se3_sin(double x);
 double or_se3_cos(double x);

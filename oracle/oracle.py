@@ -79,6 +79,11 @@ def lib():
                                              ctypes.c_int64, ctypes.c_int, ctypes.c_double, _c_i8_p, ctypes.c_int]),
             "or_visibility_pair": (None, [vp, _c_double_p, _c_double_p, ctypes.c_int64, ctypes.c_int, ctypes.c_int,
                                           ctypes.c_double, _c_i8_p, ctypes.c_int]),
+            "or_is_disentangled": (ctypes.c_int, [vp, _c_double_p]),
+            "or_disentangled_states": (None, [vp, _c_double_p, ctypes.c_int64, _c_i8_p, ctypes.c_int]),
+            "or_boxes_disjoint": (ctypes.c_int, [_c_double_p, _c_double_p, _c_double_p, _c_double_p, _c_double_p,
+                                                 _c_double_p]),
+            "or_model_box": (None, [vp, ctypes.c_int, _c_double_p, _c_double_p]),
             "or_se3_sin": (ctypes.c_double, [ctypes.c_double]),
             "or_se3_cos": (ctypes.c_double, [ctypes.c_double]),
             "or_se3_acos": (ctypes.c_double, [ctypes.c_double]),
@@ -215,6 +220,27 @@ class OracleWorld:
             return out, nbv, ntri
         return out
 
+    def is_disentangled(self, q):
+        q = _f64(q, (7,))
+        r = lib().or_is_disentangled(self._w, _d(q))
+        if r < 0:
+            raise RuntimeError("models not loaded")
+        return bool(r)
+
+    def disentangled_states(self, qs, nthreads=0):
+        qs = _f64(qs).reshape(-1, 7)
+        out = np.zeros(qs.shape[0], np.int8)
+        lib().or_disentangled_states(self._w, _d(qs), qs.shape[0], out.ctypes.data_as(_c_i8_p), nthreads)
+        if (out < 0).any():
+            raise RuntimeError("models not loaded")
+        return out.astype(bool)
+
+    def model_box(self, geo):
+        c = np.zeros(3)
+        h = np.zeros(3)
+        lib().or_model_box(self._w, geo, _d(c), _d(h))
+        return c, h
+
     def transit(self, frm, to, vd):
         frm = _f64(frm, (7,))
         to = _f64(to, (7,))
@@ -316,6 +342,12 @@ def state_to_transform(q):
     t = np.zeros(3)
     lib().or_state_to_transform(_d(q), _d(R), _d(t))
     return R, t
+
+
+def boxes_disjoint(R1, p1, A, R2, p2, B):
+    """FCL's box-box test (boolean): 1 if the boxes are disjoint."""
+    a = [_f64(x).reshape(-1) for x in (R1, p1, A, R2, p2, B)]
+    return bool(lib().or_boxes_disjoint(*[_d(x) for x in a]))
 
 
 def tri_intersect(P, Q):

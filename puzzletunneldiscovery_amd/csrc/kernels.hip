@@ -1283,6 +1283,27 @@ __global__ void __launch_bounds__(256, SE3CQ_MIN_WAVES) k_visibility(DevScene s,
 	}
 }
 
+/* ---- isDisentangled: one thread per state ---- */
+/* The robot's and the environment's AABB boxes, placed by their model
+ * transforms (the robot's from the raw state, the environment's from the
+ * perturbation), tested with FCL's box-box test (UnitWorld::isDisentangled,
+ * lib/osr/unit_world.cc:249-264 -> CDModel::collideBB, cdmodel.cc:180-198).
+ * out = 1 when the boxes are disjoint. */
+__global__ void k_disentangled(BoxPair bp, const PertData* __restrict__ pert, const double* __restrict__ qs,
+                               int64_t n, uint8_t* __restrict__ out)
+{
+	const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n)
+		return;
+	const double* q = qs + 7 * i;
+	double Rr[9], pe[3], pr[3];
+	se3_quat_to_rotmat(q[3], q[4], q[5], q[6], Rr);
+	const double tr[3] = {q[0], q[1], q[2]};
+	se3_place(pert->Re, pert->te, bp.ce, pe);
+	se3_place(Rr, tr, bp.cr, pr);
+	out[i] = (uint8_t)se3_boxes_disjoint(Rr, pr, bp.hr, pert->Re, pe, bp.he);
+}
+
 /* ---- launch wrappers (host) ---- */
 static inline size_t stack_bytes_per_thread(const DevScene& s)
 {
@@ -1313,6 +1334,15 @@ __global__ void k_fill_u8(uint8_t* p, int64_t n, uint8_t v)
 	const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
 	if (i < n)
 		p[i] = v;
+}
+
+hipError_t launch_disentangled(const BoxPair& b, const PertData* pert, const double* q, int64_t n, uint8_t* out,
+                               hipStream_t st)
+{
+	if (n <= 0)
+		return hipSuccess;
+	hipLaunchKernelGGL(k_disentangled, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, b, pert, q, n, out);
+	return hipGetLastError();
 }
 
 hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t st)

@@ -30,6 +30,14 @@ hipError_t launch_segments(const DevScene& s, int mode, const double* from, cons
 hipError_t launch_visibility(const DevScene& s, int vis, const double* qs0, const double* qs1, int64_t m,
                              int64_t n, double ivd, int8_t* out8, uint32_t* bits, int32_t* steps, hipStream_t st);
 
+/* Local AABBs of the two models (centre, half extents), for isDisentangled. */
+struct BoxPair {
+	double ce[3], he[3]; /* environment */
+	double cr[3], hr[3]; /* robot */
+};
+hipError_t launch_disentangled(const BoxPair& b, const PertData* pert, const double* q, int64_t n, uint8_t* out,
+                               hipStream_t st);
+
 /* Fill kernels (used instead of hipMemsetAsync so callers may capture the
  * device entry points in a HIP graph). */
 hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t st);

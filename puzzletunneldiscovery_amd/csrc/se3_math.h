@@ -294,6 +294,63 @@ SE3_HD static inline void se3_quat_to_rotmat(double w, double x, double y, doubl
 	R[8] = 1.0 - (txx + tyy);
 }
 
+/*
+ * Box-box separating-axis test of FCL's box shapes (FCL boxBox2, itself ODE's
+ * dBoxBox; external), boolean part: box i has rotation Ri (row-major, column
+ * k = axis k), centre pi and half extents hi.  Returns 1 when one of the 15
+ * axes separates the boxes (|projected centre distance| exceeds the summed
+ * projected radii).  The nine edge-edge axes use |R| + 1e-6, the guard those
+ * implementations add against near-parallel edges.  Association orders follow
+ * the macro expansions of dBoxBox.
+ */
+SE3_HD static inline int se3_boxes_disjoint(const double* R1, const double* p1, const double* A, const double* R2,
+                                            const double* p2, const double* B)
+{
+	double p[3], pp[3], R[9], Q[9];
+	for (int k = 0; k < 3; k++)
+		p[k] = p2[k] - p1[k];
+	for (int i = 0; i < 3; i++) {
+		pp[i] = (R1[i] * p[0] + R1[3 + i] * p[1]) + R1[6 + i] * p[2];
+		for (int j = 0; j < 3; j++) {
+			R[3 * i + j] = (R1[i] * R2[j] + R1[3 + i] * R2[3 + j]) + R1[6 + i] * R2[6 + j];
+			Q[3 * i + j] = fabs(R[3 * i + j]);
+		}
+	}
+	/* axes of box 1 */
+	for (int i = 0; i < 3; i++)
+		if (fabs(pp[i]) - (((A[i] + B[0] * Q[3 * i]) + B[1] * Q[3 * i + 1]) + B[2] * Q[3 * i + 2]) > 0.0)
+			return 1;
+	/* axes of box 2 */
+	for (int j = 0; j < 3; j++) {
+		const double d = (R2[j] * p[0] + R2[3 + j] * p[1]) + R2[6 + j] * p[2];
+		if (fabs(d) - (((A[0] * Q[j] + A[1] * Q[3 + j]) + A[2] * Q[6 + j]) + B[j]) > 0.0)
+			return 1;
+	}
+	for (int k = 0; k < 9; k++)
+		Q[k] = Q[k] + 1e-6;
+	/* u_i x v_j: with (i1, i2) the other two axes of box 1 and (j1, j2) of box 2 */
+	for (int i = 0; i < 3; i++) {
+		const int i1 = (i + 1) % 3, i2 = (i + 2) % 3;
+		for (int j = 0; j < 3; j++) {
+			const int j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+			const double e = pp[i2] * R[3 * i1 + j] - pp[i1] * R[3 * i2 + j];
+			const double r = ((A[i1] * Q[3 * i2 + j] + A[i2] * Q[3 * i1 + j]) + B[j1] * Q[3 * i + j2]) +
+			                 B[j2] * Q[3 * i + j1];
+			if (fabs(e) - r > 0.0)
+				return 1;
+		}
+	}
+	return 0;
+}
+
+/* Centre of a model's box in the world: the model transform (rotation R,
+ * translation t) applied to the box's local centre c. */
+SE3_HD static inline void se3_place(const double* R, const double* t, const double* c, double* out)
+{
+	for (int i = 0; i < 3; i++)
+		out[i] = ((R[3 * i] * c[0] + R[3 * i + 1] * c[1]) + R[3 * i + 2] * c[2]) + t[i];
+}
+
 /* distance() of lib/osr/osr_state.cc:130-154: |t0-t1| + angularDistance. */
 SE3_HD static inline double se3_distance(const double* a, const double* b)
 {

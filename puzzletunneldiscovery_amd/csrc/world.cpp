@@ -64,6 +64,8 @@ struct HostModel {
 	int depth = 0;
 	float reach = 0.0f;           /* max |c| + |e| over nodes */
 	float mag = 0.0f;             /* max |c|_inf + max extent */
+	double bc[3] = {0, 0, 0};     /* AABB of Vcd: centre (min+max)*0.5 */
+	double bh[3] = {0, 0, 0};     /* and half extents (max-min)*0.5 */
 };
 
 struct DevBuf {
@@ -477,6 +479,25 @@ void build_model(HostModel& m, const Mesh& me, const float* M)
 			const double a = (double)M[0 * 4 + r] * (double)v[0] + (double)M[1 * 4 + r] * (double)v[1];
 			const double b = (double)M[2 * 4 + r] * (double)v[2] + (double)M[3 * 4 + r] * 1.0;
 			m.Vcd[3 * i + r] = a + b;
+		}
+	}
+	/* FCL's BVHModel::computeLocalAABB over the collision vertices, wrapped
+	 * in an fcl::Box for isDisentangled (cdmodel.cc:90-100) */
+	{
+		double lo[3], hi[3];
+		for (int k = 0; k < 3; k++)
+			lo[k] = hi[k] = nv ? m.Vcd[k] : 0.0;
+		for (size_t i = 1; i < nv; i++)
+			for (int k = 0; k < 3; k++) {
+				const double v = m.Vcd[3 * i + k];
+				if (v < lo[k])
+					lo[k] = v;
+				if (v > hi[k])
+					hi[k] = v;
+			}
+		for (int k = 0; k < 3; k++) {
+			m.bc[k] = (lo[k] + hi[k]) * 0.5;
+			m.bh[k] = (hi[k] - lo[k]) * 0.5;
 		}
 	}
 	m.tris.resize(9 * nf);
@@ -1027,6 +1048,64 @@ int se3cq_valid_states_host(se3cq_world* w, const double* q, int64_t n, uint8_t*
 		return fail(SE3CQ_ENOMEM, "device allocation failed");
 	HIPCHK(hipMemcpyAsync(dq, q, 7 * n * sizeof(double), hipMemcpyHostToDevice, w->stream));
 	const hipError_t le = launch_valid_states(make_scene(w), dq, n, dout, w->stream);
+	int rc = sync_and_check(w, w->stream, le);
+	if (rc)
+		return rc;
+	HIPCHK(hipMemcpy(out, dout, (size_t)n, hipMemcpyDeviceToHost));
+	return SE3CQ_OK;
+}
+
+static BoxPair box_pair(const se3cq_world* w)
+{
+	BoxPair b;
+	for (int k = 0; k < 3; k++) {
+		b.ce[k] = w->hm[0].bc[k];
+		b.he[k] = w->hm[0].bh[k];
+		b.cr[k] = w->hm[1].bc[k];
+		b.hr[k] = w->hm[1].bh[k];
+	}
+	return b;
+}
+
+int se3cq_disentangled_device(se3cq_world* w, const double* d_q, int64_t n, uint8_t* d_out, void* stream)
+{
+	CHECK_WORLD(w);
+	if (n < 0)
+		return fail(SE3CQ_EINVAL, "negative count");
+	if (!w->ready())
+		return fail(SE3CQ_ESTATE, "models not loaded");
+	{
+		std::lock_guard<std::mutex> lk0(w->mu);
+		const int rc0 = ensure_device(w);
+		if (rc0)
+			return rc0;
+	}
+	const hipError_t e =
+	    launch_disentangled(box_pair(w), (const PertData*)w->d_pert.p, d_q, n, d_out, pick_stream(w, stream));
+	if (e != hipSuccess)
+		return fail(SE3CQ_EDEVICE, std::string("launch failed: ") + hipGetErrorString(e));
+	return SE3CQ_OK;
+}
+
+int se3cq_disentangled_host(se3cq_world* w, const double* q, int64_t n, uint8_t* out)
+{
+	CHECK_WORLD(w);
+	if (n < 0 || (n > 0 && (!q || !out)))
+		return fail(SE3CQ_EINVAL, "bad arguments");
+	if (!w->ready())
+		return fail(SE3CQ_ESTATE, "models not loaded");
+	if (n == 0)
+		return SE3CQ_OK;
+	std::lock_guard<std::mutex> lk(w->mu);
+	const int rc0 = ensure_device(w);
+	if (rc0)
+		return rc0;
+	double* dq = ws_get<double>(w, 0, 7 * n);
+	uint8_t* dout = ws_get<uint8_t>(w, 1, n);
+	if (!dq || !dout)
+		return fail(SE3CQ_ENOMEM, "device allocation failed");
+	HIPCHK(hipMemcpyAsync(dq, q, 7 * n * sizeof(double), hipMemcpyHostToDevice, w->stream));
+	const hipError_t le = launch_disentangled(box_pair(w), (const PertData*)w->d_pert.p, dq, n, dout, w->stream);
 	int rc = sync_and_check(w, w->stream, le);
 	if (rc)
 		return rc;

@@ -236,8 +236,16 @@ class UnitWorld:
     def is_valid_state(self, state):
         return bool(self.is_valid_states(_state(state))[0])
 
+    def is_disentangled_states(self, qs):
+        """Batched UnitWorld::isDisentangled: True where the two models'
+        bounding boxes are disjoint (unit_world.cc:249-264)."""
+        qs = _states(qs)
+        out = np.zeros(qs.shape[0], np.uint8)
+        N.check(N.lib().se3cq_disentangled_host(self._w, N.dptr(qs), qs.shape[0], out.ctypes.data_as(N.c_u8_p)))
+        return out.astype(bool)
+
     def is_disentangled(self, state):
-        raise NotImplementedError("is_disentangled (UnitWorld::isDisentangled) is not provided by this engine yet")
+        return bool(self.is_disentangled_states(_state(state))[0])
 
     def transit_state(self, state, action, transit_magnitude, verify_delta):
         raise NotImplementedError("transit_state (discrete RL actions) is outside this engine's scope")
@@ -366,6 +374,9 @@ class UnitWorld:
         return out
 
     # ---- device-buffer entry points (pointers to HBM, e.g. torch .data_ptr()) --
+    def disentangled_device(self, d_q, n, d_out, stream=None):
+        N.check(N.lib().se3cq_disentangled_device(self._w, d_q, int(n), d_out, stream))
+
     def valid_states_device(self, d_q, n, d_out, stream=None):
         N.check(N.lib().se3cq_valid_states_device(self._w, d_q, int(n), d_out, stream))
 

@@ -174,6 +174,9 @@ def test_unported_members_raise():
     w = pyosr.UnitWorld()
     with pytest.raises(NotImplementedError):
         w.transit_state(pyosr.get_identity_state_vector(), 0, 0.1, 0.01)
+    # isDisentangled without models throws in the reference (unit_world.cc:251-253)
+    with pytest.raises(RuntimeError, match="not loaded"):
+        w.is_disentangled(pyosr.get_identity_state_vector())
 
 
 @pytest.mark.skipif(_gpu(), reason="checks the no-device error path")

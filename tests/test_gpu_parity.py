@@ -164,3 +164,25 @@ def test_visibility_bits(gpu_world, torus_oracle):
     flat = np.unpackbits(b.view(np.uint8), bitorder="little")[: n * n].reshape(n, n)
     c = torus_oracle.visibility_matrix(ms, True, 1.0 / 256)
     assert np.array_equal(flat.astype(np.int8), c)
+
+
+def test_disentangled(torus_meshes, oracle_mod):
+    """isDisentangled (unit_world.cc:249-264): bounding boxes of both models,
+    bit-identical to the oracle, also with non-unit quaternions and a
+    perturbation."""
+    env, rob = torus_meshes
+    uw = pyosr.create_unit_world_from_arrays(env, rob)
+    ow = oracle_mod.OracleWorld().setup(*env, *rob)
+    rng = np.random.default_rng(17)
+    qs = W.random_states(50_000, seed=18, lo=-0.2, hi=1.2)
+    qs[:5000, 3:] *= rng.uniform(0.5, 1.5, size=(5000, 1))
+    _flush("disentangled: 50k states")
+    g = uw.is_disentangled_states(qs)
+    c = ow.disentangled_states(qs)
+    assert 0.05 < c.mean() < 0.95
+    assert np.array_equal(g, c)
+    assert uw.is_disentangled(qs[0]) == bool(c[0])
+    pert = np.array([0.03, -0.01, 0.02, 0.97, 0.1, -0.2, 0.05])
+    uw.set_perturbation(pert)
+    ow.set_perturbation(pert)
+    assert np.array_equal(uw.is_disentangled_states(qs), ow.disentangled_states(qs))
