@@ -186,3 +186,41 @@ def test_disentangled(torus_meshes, oracle_mod):
     uw.set_perturbation(pert)
     ow.set_perturbation(pert)
     assert np.array_equal(uw.is_disentangled_states(qs), ow.disentangled_states(qs))
+
+
+def test_touch_search_in_hip_graph(gpu_world, torus_oracle):
+    """The device entry points allocate nothing after the first call and
+    clear with kernels, so a caller may capture them in a HIP graph: every
+    replay (with eager work and synchronisation in between) must reproduce
+    the eager result."""
+    import torch
+    starts = _free_starts(gpu_world, 512, 111)
+    ends = W.segment_ends(starts, seed=112)
+    n = len(starts)
+    dev = torch.device("cuda")
+    df, dt = torch.from_numpy(starts).to(dev), torch.from_numpy(ends).to(dev)
+    outs = [torch.zeros(n, dtype=t, device=dev) for t in (torch.uint8, torch.float64, torch.float64,
+                                                            torch.float64, torch.float64, torch.int32)]
+    s = torch.cuda.Stream()
+
+    def call():
+        gpu_world.touch_search_device(df.data_ptr(), dt.data_ptr(), n, 1.0 / 1024, 20,
+                                      *[o.data_ptr() for o in outs], stream=s.cuda_stream)
+
+    with torch.cuda.stream(s):
+        call()  # first call uploads the scene; after that nothing is allocated
+    torch.cuda.synchronize()
+    eager = [o.clone() for o in outs]
+    o = torus_oracle.touch_batch(starts, ends, 1.0 / 1024, 20)
+    assert np.array_equal(eager[3].cpu().numpy(), o["lo"]) and np.array_equal(eager[4].cpu().numpy(), o["hi"])
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        call()
+    for rep in range(3):
+        for x in outs:
+            x.zero_()
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        for a, b in zip(outs, eager):
+            assert torch.equal(a, b), rep
