@@ -146,6 +146,19 @@ int se3cq_visibility_bits_device(se3cq_world* w, const double* d_qs, int64_t n, 
 int se3cq_disentangled_host(se3cq_world* w, const double* q, int64_t n, uint8_t* out);
 int se3cq_disentangled_device(se3cq_world* w, const double* d_q, int64_t n, uint8_t* d_out, void* stream);
 
+/* UnitWorld::intersectingSegments (unit_world.cc:959-1012) for n states:
+ * every (environment triangle, robot triangle) pair that intersects
+ * (CDModel::collideForDetails, cdmodel.cc:201-229), and each pair's
+ * intersection segment in world coordinates (Moller's triangle-triangle test
+ * with end points, lib/tritri).  For state i, counts[i] is the number of
+ * intersecting pairs; the first min(counts[i], cap) of them, in (env, robot)
+ * order, are written to pairs[(i*cap + k)*2 + {0,1}], the segment end points
+ * to segs[(i*cap + k)*6 + {0..5}] and flags[i*cap + k] (bit 0: the segment
+ * test meets; bit 1: the triangles are coplanar, segment left at 0).  Which
+ * pairs are listed when counts[i] > cap depends on the traversal order. */
+int se3cq_contacts_host(se3cq_world* w, const double* q, int64_t n, int32_t cap, int32_t* pairs, double* segs,
+                        int8_t* flags, int32_t* counts);
+
 /* Surface device-side failures of earlier _device calls (after sync). */
 int se3cq_check(se3cq_world* w);
 

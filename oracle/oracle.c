@@ -1035,6 +1035,54 @@ int or_is_valid_counted(const OrWorld* w, const double* q7, int64_t* n_bv, int64
 
 int or_is_valid(const OrWorld* w, const double* q7) { return or_is_valid_counted(w, q7, NULL, NULL); }
 
+/* UnitWorld::intersectingSegments (unit_world.cc:959-1012): every
+ * intersecting (env triangle, robot triangle) pair by brute force over all
+ * pairs (the contacts CDModel::collideForDetails collects, cdmodel.cc:201-229),
+ * in (env, robot) order, with the intersection segment of each pair in world
+ * coordinates (Moller's test with end points, lib/tritri).  Writes the first
+ * `cap` pairs; returns their total number (-1 if a model is missing). */
+int or_contacts(const OrWorld* w, const double* q7, int cap, int32_t* pairs, double* segs, int8_t* flags)
+{
+	if (!w->has_cd[0] || !w->has_cd[1])
+		return -1;
+	double R[9], T[3], Re[9], te[3], Rr[9], tr[3];
+	relative_tf_raw(w->pert, q7, R, T);
+	or_state_to_transform(w->pert, Re, te);
+	or_state_to_transform(q7, Rr, tr);
+	const OrModel* env = &w->cd[0];
+	const OrModel* rob = &w->cd[1];
+	int found = 0;
+	for (int64_t i = 0; i < env->nf; i++)
+		for (int64_t j = 0; j < rob->nf; j++) {
+			if (!leaf_test(env, rob, (int)i, (int)j, R, T))
+				continue;
+			if (found < cap) {
+				double V[9], U[9], p1[3], p2[3];
+				int cop = 0;
+				for (int v = 0; v < 3; v++) {
+					se3_place(Re, te, env->V + 3 * env->F[3 * i + v], V + 3 * v);
+					se3_place(Rr, tr, rob->V + 3 * rob->F[3 * j + v], U + 3 * v);
+				}
+				const int hit = se3_tri_isect_segment(V, V + 3, V + 6, U, U + 3, U + 6, &cop, p1, p2);
+				pairs[2 * found] = (int32_t)i;
+				pairs[2 * found + 1] = (int32_t)j;
+				for (int k = 0; k < 3; k++) {
+					segs[6 * found + k] = p1[k];
+					segs[6 * found + 3 + k] = p2[k];
+				}
+				flags[found] = (int8_t)(hit | (cop << 1));
+			}
+			found++;
+		}
+	return found;
+}
+
+/* The triangle-triangle segment test alone. */
+int or_tri_isect_segment(const double* V9, const double* U9, int* coplanar, double* p1, double* p2)
+{
+	return se3_tri_isect_segment(V9, V9 + 3, V9 + 6, U9, U9 + 3, U9 + 6, coplanar, p1, p2);
+}
+
 /* UnitWorld::isDisentangled (unit_world.cc:249-264) -> CDModel::collideBB
  * (cdmodel.cc:180-198): FCL Box shapes of both models' local AABBs, each
  * placed by its model transform times the translation to the AABB centre

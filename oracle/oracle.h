@@ -107,6 +107,12 @@ void or_visibility_matrix2(const OrWorld* w, const double* qs0, int64_t m, int u
 void or_visibility_pair(const OrWorld* w, const double* qs0, const double* qs1, int64_t n,
                         int unit0, int unit1, double vd, int8_t* out, int nthreads);
 
+/* intersectingSegments: all intersecting triangle pairs at q by brute force,
+ * (env, robot) order, first `cap` written with their world-frame segments
+ * (flags bit 0: the segment test meets, bit 1: coplanar); returns the count. */
+int or_contacts(const OrWorld* w, const double* q7, int cap, int32_t* pairs, double* segs, int8_t* flags);
+int or_tri_isect_segment(const double* V9, const double* U9, int* coplanar, double* p1, double* p2);
+
 /* isDisentangled: 1 if the two models' bounding boxes are disjoint at q,
  * 0 if they intersect, -1 if a model is missing. */
 int or_is_disentangled(const OrWorld* w, const double* q7);

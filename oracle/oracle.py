@@ -79,6 +79,9 @@ def lib():
                                              ctypes.c_int64, ctypes.c_int, ctypes.c_double, _c_i8_p, ctypes.c_int]),
             "or_visibility_pair": (None, [vp, _c_double_p, _c_double_p, ctypes.c_int64, ctypes.c_int, ctypes.c_int,
                                           ctypes.c_double, _c_i8_p, ctypes.c_int]),
+            "or_contacts": (ctypes.c_int, [vp, _c_double_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int32),
+                                           _c_double_p, _c_i8_p]),
+            "or_tri_isect_segment": (ctypes.c_int, [_c_double_p, _c_double_p, _c_int_p, _c_double_p, _c_double_p]),
             "or_is_disentangled": (ctypes.c_int, [vp, _c_double_p]),
             "or_disentangled_states": (None, [vp, _c_double_p, ctypes.c_int64, _c_i8_p, ctypes.c_int]),
             "or_boxes_disjoint": (ctypes.c_int, [_c_double_p, _c_double_p, _c_double_p, _c_double_p, _c_double_p,
@@ -235,6 +238,20 @@ class OracleWorld:
             raise RuntimeError("models not loaded")
         return out.astype(bool)
 
+    def contacts(self, q, cap=4096):
+        """(pairs [m, 2], segs [m, 6], flags [m]) of all intersecting
+        triangle pairs at q, (env, robot) order."""
+        q = _f64(q, (7,))
+        pairs = np.zeros((cap, 2), np.int32)
+        segs = np.zeros((cap, 6))
+        flags = np.zeros(cap, np.int8)
+        m = lib().or_contacts(self._w, _d(q), cap, pairs.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                              _d(segs), flags.ctypes.data_as(_c_i8_p))
+        if m < 0:
+            raise RuntimeError("models not loaded")
+        k = min(m, cap)
+        return pairs[:k], segs[:k], flags[:k], m
+
     def model_box(self, geo):
         c = np.zeros(3)
         h = np.zeros(3)
@@ -342,6 +359,18 @@ def state_to_transform(q):
     t = np.zeros(3)
     lib().or_state_to_transform(_d(q), _d(R), _d(t))
     return R, t
+
+
+def tri_isect_segment(V, U):
+    """Moller's triangle-triangle test with the intersection segment:
+    (meets, coplanar, p1, p2)."""
+    V = _f64(V, (9,))
+    U = _f64(U, (9,))
+    cop = ctypes.c_int(0)
+    p1 = np.zeros(3)
+    p2 = np.zeros(3)
+    r = lib().or_tri_isect_segment(_d(V), _d(U), ctypes.byref(cop), _d(p1), _d(p2))
+    return bool(r), bool(cop.value), p1, p2
 
 
 def boxes_disjoint(R1, p1, A, R2, p2, B):

@@ -61,6 +61,7 @@ SIGNATURES = {
     "se3cq_visibility_matrix2_host": [vp, c_double_p, i64, cint, c_double_p, i64, cint, cdbl, c_i8_p],
     "se3cq_visibility_pair_host": [vp, c_double_p, cint, c_double_p, cint, i64, cdbl, c_i8_p],
     "se3cq_visibility_bits_device": [vp, vp, i64, cdbl, vp, vp, vp],
+    "se3cq_contacts_host": [vp, c_double_p, i64, ctypes.c_int32, c_i32_p, c_double_p, c_i8_p, c_i32_p],
     "se3cq_disentangled_host": [vp, c_double_p, i64, c_u8_p],
     "se3cq_disentangled_device": [vp, vp, i64, vp, vp],
     "se3cq_check": [vp],

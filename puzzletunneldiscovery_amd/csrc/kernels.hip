@@ -1283,6 +1283,126 @@ __global__ void __launch_bounds__(256, SE3CQ_MIN_WAVES) k_visibility(DevScene s,
 	}
 }
 
+/* ---- contact enumeration: one thread per state ---- */
+/*
+ * Every (env triangle, robot triangle) pair whose exact test intersects at
+ * the state (CDModel::collideForDetails, lib/osr/cdmodel.cc:201-229, asks FCL
+ * for all contacts), found by the depth-first traversal of k_valid_states
+ * without its early exit.  The first `cap` pairs found are written; count
+ * holds how many there are.
+ */
+template <bool WIDE>
+__global__ void __launch_bounds__(256, SE3CQ_MIN_WAVES) k_contacts(DevScene s, const double* __restrict__ qs, int64_t n,
+                                                      int cap, int32_t* __restrict__ pairs,
+                                                      int32_t* __restrict__ count)
+{
+	extern __shared__ uint32_t smem[];
+	const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n)
+		return;
+	const double* qp = qs + 7 * i;
+	PairStack<WIDE> stk, leaves;
+	thread_stacks<WIDE>(s, smem, stk, leaves);
+	int found = 0;
+	int sp = 0;
+	stk.put(sp++, 0, 0);
+	const int depth_cap = s.stack_depth;
+	while (sp > 0) {
+		int nleaf = 0;
+		{
+			QCull c;
+			{
+				double q[7];
+				for (int k = 0; k < 7; k++)
+					q[k] = qp[k];
+				make_qcull(s, q, c);
+			}
+			while (sp > 0 && nleaf < LEAF_BUF) {
+				int a, b;
+				stk.get(--sp, a, b);
+				const GNode A = load_node(s.env.nodes, a);
+				const GNode Bn = load_node(s.rob.nodes, b);
+				RobBox B;
+				load_rob_box(Bn, c, B);
+				if (obb_disjoint(A, B, c.eta))
+					continue;
+				const int ca = __float_as_int(A.a2_child.w);
+				const bool la = ca < 0, lb = B.child < 0;
+				if (la && lb) {
+					leaves.put(nleaf++, ~ca, ~B.child);
+					continue;
+				}
+				if (sp + 2 > depth_cap) {
+					atomicOr(s.error_flag, 1u);
+					count[i] = -1;
+					return;
+				}
+				const float sa = fmaf(A.c_ex.w, A.c_ex.w, fmaf(A.a0_ey.w, A.a0_ey.w, A.a1_ez.w * A.a1_ez.w));
+				const float sb = fmaf(Bn.c_ex.w, Bn.c_ex.w, fmaf(Bn.a0_ey.w, Bn.a0_ey.w, Bn.a1_ez.w * Bn.a1_ez.w));
+				if (!la && (lb || sa > sb)) {
+					stk.put(sp++, ca + 1, b);
+					stk.put(sp++, ca, b);
+				} else {
+					stk.put(sp++, a, B.child + 1);
+					stk.put(sp++, a, B.child);
+				}
+			}
+		}
+		for (int j = 0; j < nleaf; j++) {
+			int te, tr;
+			leaves.get(j, te, tr);
+			double q[7];
+			for (int k = 0; k < 7; k++)
+				q[k] = qp[k];
+			if (leaf_pair(s, q, te, tr)) {
+				if (found < cap) {
+					pairs[(i * cap + found) * 2] = te;
+					pairs[(i * cap + found) * 2 + 1] = tr;
+				}
+				found++;
+			}
+		}
+	}
+	count[i] = found;
+}
+
+/*
+ * Intersection segment of each listed pair, in world coordinates as
+ * UnitWorld::intersectingSegments computes it (unit_world.cc:959-1012): the
+ * environment triangle moved by the perturbation transform and the robot
+ * triangle by the state's transform.  One thread per (state, slot).
+ */
+__global__ void k_contact_segments(DevScene s, const double* __restrict__ qs, int64_t n, int cap,
+                                   const int32_t* __restrict__ pairs, const int32_t* __restrict__ count,
+                                   double* __restrict__ segs, int8_t* __restrict__ flags)
+{
+	const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= n * (int64_t)cap)
+		return;
+	const int64_t i = t / cap;
+	const int slot = (int)(t % cap);
+	if (slot >= count[i])
+		return;
+	const double* q = qs + 7 * i;
+	double Rr[9];
+	se3_quat_to_rotmat(q[3], q[4], q[5], q[6], Rr);
+	const double* E = s.env.tris + 9 * (int64_t)pairs[2 * t];
+	const double* Q = s.rob.tris + 9 * (int64_t)pairs[2 * t + 1];
+	double V[9], U[9];
+	for (int v = 0; v < 3; v++) {
+		se3_place(s.pert->Re, s.pert->te, E + 3 * v, V + 3 * v);
+		se3_place(Rr, q, Q + 3 * v, U + 3 * v);
+	}
+	int cop = 0;
+	double p1[3], p2[3];
+	const int hit = se3_tri_isect_segment(V, V + 3, V + 6, U, U + 3, U + 6, &cop, p1, p2);
+	for (int k = 0; k < 3; k++) {
+		segs[6 * t + k] = p1[k];
+		segs[6 * t + 3 + k] = p2[k];
+	}
+	flags[t] = (int8_t)(hit | (cop << 1));
+}
+
 /* ---- isDisentangled: one thread per state ---- */
 /* The robot's and the environment's AABB boxes, placed by their model
  * transforms (the robot's from the raw state, the environment's from the
@@ -1334,6 +1454,27 @@ __global__ void k_fill_u8(uint8_t* p, int64_t n, uint8_t v)
 	const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
 	if (i < n)
 		p[i] = v;
+}
+
+hipError_t launch_contacts(const DevScene& s, const double* q, int64_t n, int cap, int32_t* pairs, int32_t* count,
+                           double* segs, int8_t* flags, hipStream_t st)
+{
+	if (n <= 0)
+		return hipSuccess;
+	const int block = pick_block(s);
+	const int64_t grid = (n + block - 1) / block;
+	const size_t lds = stack_bytes(s, block);
+	if (s.wide)
+		hipLaunchKernelGGL(k_contacts<true>, dim3((unsigned)grid), dim3(block), lds, st, s, q, n, cap, pairs, count);
+	else
+		hipLaunchKernelGGL(k_contacts<false>, dim3((unsigned)grid), dim3(block), lds, st, s, q, n, cap, pairs, count);
+	hipError_t e = hipGetLastError();
+	if (e != hipSuccess)
+		return e;
+	const int64_t slots = n * (int64_t)cap;
+	hipLaunchKernelGGL(k_contact_segments, dim3((unsigned)((slots + 255) / 256)), dim3(256), 0, st, s, q, n, cap,
+	                   pairs, count, segs, flags);
+	return hipGetLastError();
 }
 
 hipError_t launch_disentangled(const BoxPair& b, const PertData* pert, const double* q, int64_t n, uint8_t* out,

@@ -30,6 +30,12 @@ hipError_t launch_segments(const DevScene& s, int mode, const double* from, cons
 hipError_t launch_visibility(const DevScene& s, int vis, const double* qs0, const double* qs1, int64_t m,
                              int64_t n, double ivd, int8_t* out8, uint32_t* bits, int32_t* steps, hipStream_t st);
 
+/* All intersecting triangle pairs per state (up to cap; count[i] = how many,
+ * -1 after a traversal overflow) and each listed pair's intersection segment
+ * (segs [n*cap*6]) and flags (bit 0: the segment test meets, bit 1: coplanar). */
+hipError_t launch_contacts(const DevScene& s, const double* q, int64_t n, int cap, int32_t* pairs, int32_t* count,
+                           double* segs, int8_t* flags, hipStream_t st);
+
 /* Local AABBs of the two models (centre, half extents), for isDisentangled. */
 struct BoxPair {
 	double ce[3], he[3]; /* environment */

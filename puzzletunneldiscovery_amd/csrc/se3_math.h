@@ -343,6 +343,146 @@ SE3_HD static inline int se3_boxes_disjoint(const double* R1, const double* p1, 
 	return 0;
 }
 
+/*
+ * Intersection segment of two triangles, after Moller, "A Fast
+ * Triangle-Triangle Intersection Test" (JGT 1997), interval variant with the
+ * segment end points, as lib/tritri wraps it for UnitWorld::intersectingSegments
+ * (lib/osr/unit_world.cc:959-1012): plane distances below 1e-6 count as
+ * zero; the triangles' intervals on the line where their planes meet are
+ * compared, and the overlap's end points are returned in p1, p2.  Returns 1
+ * when the triangles meet; coplanar triangles return 1 with *coplanar set and
+ * p1 = p2 = 0 (the segment is undefined there).
+ */
+SE3_HD static inline void se3_mol_isect2(const double* v0, const double* v1, const double* v2, double vv0, double vv1,
+                                         double vv2, double d0, double d1, double d2, double* i0, double* i1,
+                                         double* p0, double* p1)
+{
+	double t = d0 / (d0 - d1);
+	*i0 = vv0 + (vv1 - vv0) * t;
+	for (int k = 0; k < 3; k++)
+		p0[k] = (v1[k] - v0[k]) * t + v0[k];
+	t = d0 / (d0 - d2);
+	*i1 = vv0 + (vv2 - vv0) * t;
+	for (int k = 0; k < 3; k++)
+		p1[k] = v0[k] + (v2[k] - v0[k]) * t;
+}
+
+/* Interval of triangle (v0, v1, v2) on the common line; returns 1 if the
+ * triangle lies in the other's plane. */
+SE3_HD static inline int se3_mol_interval(const double* v0, const double* v1, const double* v2, double vv0, double vv1,
+                                          double vv2, double d0, double d1, double d2, double d0d1, double d0d2,
+                                          double* i0, double* i1, double* p0, double* p1)
+{
+	if (d0d1 > 0.0)
+		se3_mol_isect2(v2, v0, v1, vv2, vv0, vv1, d2, d0, d1, i0, i1, p0, p1);
+	else if (d0d2 > 0.0)
+		se3_mol_isect2(v1, v0, v2, vv1, vv0, vv2, d1, d0, d2, i0, i1, p0, p1);
+	else if (d1 * d2 > 0.0 || d0 != 0.0)
+		se3_mol_isect2(v0, v1, v2, vv0, vv1, vv2, d0, d1, d2, i0, i1, p0, p1);
+	else if (d1 != 0.0)
+		se3_mol_isect2(v1, v0, v2, vv1, vv0, vv2, d1, d0, d2, i0, i1, p0, p1);
+	else if (d2 != 0.0)
+		se3_mol_isect2(v2, v0, v1, vv2, vv0, vv1, d2, d0, d1, i0, i1, p0, p1);
+	else
+		return 1;
+	return 0;
+}
+
+SE3_HD static inline double se3_mol_dot(const double* a, const double* b)
+{
+	return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2];
+}
+
+SE3_HD static inline void se3_mol_cross(const double* a, const double* b, double* r)
+{
+	r[0] = a[1] * b[2] - a[2] * b[1];
+	r[1] = a[2] * b[0] - a[0] * b[2];
+	r[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+SE3_HD static inline int se3_tri_isect_segment(const double* V0, const double* V1, const double* V2, const double* U0,
+                                               const double* U1, const double* U2, int* coplanar, double* p1,
+                                               double* p2)
+{
+	const double eps = 0.000001;
+	double e1[3], e2[3], n1[3], n2[3], dd[3];
+	*coplanar = 0;
+	for (int k = 0; k < 3; k++) {
+		p1[k] = 0.0;
+		p2[k] = 0.0;
+		e1[k] = V1[k] - V0[k];
+		e2[k] = V2[k] - V0[k];
+	}
+	se3_mol_cross(e1, e2, n1);
+	const double d1 = -se3_mol_dot(n1, V0);
+	double du0 = se3_mol_dot(n1, U0) + d1, du1 = se3_mol_dot(n1, U1) + d1, du2 = se3_mol_dot(n1, U2) + d1;
+	if (fabs(du0) < eps) du0 = 0.0;
+	if (fabs(du1) < eps) du1 = 0.0;
+	if (fabs(du2) < eps) du2 = 0.0;
+	const double du0du1 = du0 * du1, du0du2 = du0 * du2;
+	if (du0du1 > 0.0 && du0du2 > 0.0)
+		return 0;
+	for (int k = 0; k < 3; k++) {
+		e1[k] = U1[k] - U0[k];
+		e2[k] = U2[k] - U0[k];
+	}
+	se3_mol_cross(e1, e2, n2);
+	const double d2 = -se3_mol_dot(n2, U0);
+	double dv0 = se3_mol_dot(n2, V0) + d2, dv1 = se3_mol_dot(n2, V1) + d2, dv2 = se3_mol_dot(n2, V2) + d2;
+	if (fabs(dv0) < eps) dv0 = 0.0;
+	if (fabs(dv1) < eps) dv1 = 0.0;
+	if (fabs(dv2) < eps) dv2 = 0.0;
+	const double dv0dv1 = dv0 * dv1, dv0dv2 = dv0 * dv2;
+	if (dv0dv1 > 0.0 && dv0dv2 > 0.0)
+		return 0;
+	se3_mol_cross(n1, n2, dd);
+	double mx = fabs(dd[0]);
+	int idx = 0;
+	if (fabs(dd[1]) > mx) {
+		mx = fabs(dd[1]);
+		idx = 1;
+	}
+	if (fabs(dd[2]) > mx)
+		idx = 2;
+	double i1a, i1b, i2a, i2b, a1[3], a2[3], b1[3], b2[3];
+	if (se3_mol_interval(V0, V1, V2, V0[idx], V1[idx], V2[idx], dv0, dv1, dv2, dv0dv1, dv0dv2, &i1a, &i1b, a1, a2)) {
+		*coplanar = 1;
+		return 1;
+	}
+	se3_mol_interval(U0, U1, U2, U0[idx], U1[idx], U2[idx], du0, du1, du2, du0du1, du0du2, &i2a, &i2b, b1, b2);
+	int s1 = 0, s2 = 0;
+	if (i1a > i1b) {
+		const double t = i1a; i1a = i1b; i1b = t;
+		s1 = 1;
+	}
+	if (i2a > i2b) {
+		const double t = i2a; i2a = i2b; i2b = t;
+		s2 = 1;
+	}
+	if (i1b < i2a || i2b < i1a)
+		return 0;
+	const double* q1;
+	const double* q2;
+	if (i2a < i1a) {
+		q1 = s1 == 0 ? a1 : a2;
+		if (i2b < i1b)
+			q2 = s2 == 0 ? b2 : b1;
+		else
+			q2 = s1 == 0 ? a2 : a1;
+	} else {
+		q1 = s2 == 0 ? b1 : b2;
+		if (i2b > i1b)
+			q2 = s1 == 0 ? a2 : a1;
+		else
+			q2 = s2 == 0 ? b2 : b1;
+	}
+	for (int k = 0; k < 3; k++) {
+		p1[k] = q1[k];
+		p2[k] = q2[k];
+	}
+	return 1;
+}
+
 /* Centre of a model's box in the world: the model transform (rotation R,
  * translation t) applied to the box's local centre c. */
 SE3_HD static inline void se3_place(const double* R, const double* t, const double* c, double* out)

@@ -1055,6 +1055,65 @@ int se3cq_valid_states_host(se3cq_world* w, const double* q, int64_t n, uint8_t*
 	return SE3CQ_OK;
 }
 
+int se3cq_contacts_host(se3cq_world* w, const double* q, int64_t n, int32_t cap, int32_t* pairs, double* segs,
+                        int8_t* flags, int32_t* counts)
+{
+	CHECK_WORLD(w);
+	if (n < 0 || cap < 0 || (n > 0 && (!q || !counts)) || (n > 0 && cap > 0 && (!pairs || !segs || !flags)))
+		return fail(SE3CQ_EINVAL, "bad arguments");
+	if (!w->ready())
+		return fail(SE3CQ_ESTATE, "models not loaded");
+	if (n == 0)
+		return SE3CQ_OK;
+	std::lock_guard<std::mutex> lk(w->mu);
+	const int rc0 = ensure_device(w);
+	if (rc0)
+		return rc0;
+	const size_t slots = (size_t)n * (size_t)(cap > 0 ? cap : 1);
+	double* dq = ws_get<double>(w, 0, 7 * n);
+	int32_t* dpairs = ws_get<int32_t>(w, 2, 2 * slots);
+	int32_t* dcount = ws_get<int32_t>(w, 3, (size_t)n);
+	double* dsegs = ws_get<double>(w, 4, 6 * slots);
+	int8_t* dflags = ws_get<int8_t>(w, 5, slots);
+	if (!dq || !dpairs || !dcount || !dsegs || !dflags)
+		return fail(SE3CQ_ENOMEM, "device allocation failed");
+	HIPCHK(hipMemcpyAsync(dq, q, 7 * n * sizeof(double), hipMemcpyHostToDevice, w->stream));
+	const hipError_t le = launch_contacts(make_scene(w), dq, n, cap, dpairs, dcount, dsegs, dflags, w->stream);
+	int rc = sync_and_check(w, w->stream, le);
+	if (rc)
+		return rc;
+	HIPCHK(hipMemcpy(counts, dcount, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost));
+	if (cap == 0)
+		return SE3CQ_OK;
+	std::vector<int32_t> hp(2 * slots);
+	std::vector<double> hs(6 * slots);
+	std::vector<int8_t> hf(slots);
+	HIPCHK(hipMemcpy(hp.data(), dpairs, hp.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+	HIPCHK(hipMemcpy(hs.data(), dsegs, hs.size() * sizeof(double), hipMemcpyDeviceToHost));
+	HIPCHK(hipMemcpy(hf.data(), dflags, hf.size(), hipMemcpyDeviceToHost));
+	/* report the listed pairs of each state in (env, robot) order */
+	std::vector<int> ord;
+	for (int64_t i = 0; i < n; i++) {
+		const int m = counts[i] < cap ? counts[i] : cap;
+		ord.resize((size_t)m);
+		for (int k = 0; k < m; k++)
+			ord[(size_t)k] = k;
+		const int32_t* P = &hp[(size_t)i * cap * 2];
+		std::sort(ord.begin(), ord.end(), [P](int x, int y) {
+			return P[2 * x] != P[2 * y] ? P[2 * x] < P[2 * y] : P[2 * x + 1] < P[2 * y + 1];
+		});
+		for (int k = 0; k < m; k++) {
+			const size_t src = (size_t)i * cap + (size_t)ord[(size_t)k], dst = (size_t)i * cap + (size_t)k;
+			pairs[2 * dst] = hp[2 * src];
+			pairs[2 * dst + 1] = hp[2 * src + 1];
+			for (int c = 0; c < 6; c++)
+				segs[6 * dst + c] = hs[6 * src + c];
+			flags[dst] = hf[src];
+		}
+	}
+	return SE3CQ_OK;
+}
+
 static BoxPair box_pair(const se3cq_world* w)
 {
 	BoxPair b;

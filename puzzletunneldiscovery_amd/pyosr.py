@@ -236,6 +236,36 @@ class UnitWorld:
     def is_valid_state(self, state):
         return bool(self.is_valid_states(_state(state))[0])
 
+    def contacts(self, qs, cap=1024):
+        """All intersecting (env triangle, robot triangle) pairs per state with
+        their intersection segments.  Returns (counts int32[n], pairs
+        int32[n, cap, 2], segs float64[n, cap, 6], flags int8[n, cap])."""
+        qs = _states(qs)
+        n = qs.shape[0]
+        counts = np.zeros(n, np.int32)
+        pairs = np.zeros((n, cap, 2), np.int32)
+        segs = np.zeros((n, cap, 6))
+        flags = np.zeros((n, cap), np.int8)
+        N.check(N.lib().se3cq_contacts_host(self._w, N.dptr(qs), n, int(cap), pairs.ctypes.data_as(N.c_i32_p),
+                                            N.dptr(segs), flags.ctypes.data_as(N.c_i8_p),
+                                            counts.ctypes.data_as(N.c_i32_p)))
+        return counts, pairs, segs, flags
+
+    def intersecting_segments(self, unitq):
+        """UnitWorld::intersectingSegments (unit_world.cc:959-1012): segment
+        start points [m, 3], end points [m, 3], lengths [m] and the
+        (env face, robot face) pairs [m, 2] of the intersecting triangles."""
+        cap = 1024
+        while True:
+            counts, pairs, segs, _ = self.contacts(_state(unitq), cap)
+            m = int(counts[0])
+            if m <= cap:
+                break
+            cap = m
+        p0, p1 = segs[0, :m, :3].copy(), segs[0, :m, 3:].copy()
+        mag = np.sqrt(((p1 - p0) ** 2).sum(axis=1))
+        return p0, p1, mag, pairs[0, :m].astype(np.int32)
+
     def is_disentangled_states(self, qs):
         """Batched UnitWorld::isDisentangled: True where the two models'
         bounding boxes are disjoint (unit_world.cc:249-264)."""

@@ -224,3 +224,26 @@ def test_touch_search_in_hip_graph(gpu_world, torus_oracle):
         torch.cuda.synchronize()
         for a, b in zip(outs, eager):
             assert torch.equal(a, b), rep
+
+
+def test_contacts(gpu_world, torus_oracle):
+    """intersectingSegments (unit_world.cc:959-1012): the GPU lists the same
+    intersecting triangle pairs as the oracle's brute force, with bit-equal
+    segments and flags."""
+    qs = W.random_states(300, seed=121)
+    valid = torus_oracle.valid_states(qs).astype(bool)
+    qs = qs[~valid][:60]
+    _flush("contacts: 60 colliding states")
+    counts, pairs, segs, flags = gpu_world.contacts(qs, cap=512)
+    for i, q in enumerate(qs):
+        p, sg, fl, m = torus_oracle.contacts(q)
+        assert counts[i] == m and m > 0
+        k = min(m, 512)
+        assert np.array_equal(pairs[i, :k], p[:k])
+        assert np.array_equal(segs[i, :k], sg[:k])
+        assert np.array_equal(flags[i, :k], fl[:k])
+    p0, p1, mag, fp = gpu_world.intersecting_segments(qs[0])
+    assert fp.shape == (counts[0], 2) and p0.shape == (counts[0], 3)
+    assert np.allclose(mag, np.linalg.norm(p1 - p0, axis=1))
+    free = W.random_states(300, seed=121)[valid][:5]
+    assert not gpu_world.contacts(free)[0].any()
