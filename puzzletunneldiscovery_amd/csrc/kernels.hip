@@ -596,6 +596,10 @@ __device__ uint64_t packet_collide(const DevScene& s, const double* q, uint64_t 
 		{
 			QCull c;
 			make_qcull(s, q, c);
+			/* the robot box of the last robot node, reused while the env
+			 * side is split */
+			int cached_b = -1;
+			RobBox B;
 			while (sp > 0 && nl < PACKET_LEAVES) {
 				sp--;
 				const int a = rdl(va, sp), b = rdl(vb, sp);
@@ -606,18 +610,16 @@ __device__ uint64_t packet_collide(const DevScene& s, const double* q, uint64_t 
 				if (m == 0)
 					continue;
 				const GNode A = load_node_uniform(s.env.nodes, a);
-				const GNode Bn = load_node_uniform(s.rob.nodes, b);
-				bool ov;
-				{
-					RobBox B;
-					load_rob_box(Bn, c, B);
-					ov = ((m >> lane) & 1ull) && !obb_disjoint(A, B, c.eta);
+				if (b != cached_b) {
+					load_rob_box(load_node_uniform(s.rob.nodes, b), c, B);
+					cached_b = b;
 				}
+				const bool ov = ((m >> lane) & 1ull) && !obb_disjoint(A, B, c.eta);
 				const uint64_t mo = __ballot(ov);
 				if (mo == 0)
 					continue;
 				const int ca = __float_as_int(A.a2_child.w);
-				const int cb = __float_as_int(Bn.a2_child.w);
+				const int cb = B.child;
 				const bool la = ca < 0, lb = cb < 0;
 				const int ml = (int)(uint32_t)mo, mh = (int)(uint32_t)(mo >> 32);
 				if (la && lb) {
@@ -636,7 +638,8 @@ __device__ uint64_t packet_collide(const DevScene& s, const double* q, uint64_t 
 				/* FCL's rule: split the env node unless it is a leaf or the
 				 * robot box is the larger one (squared extent norms). */
 				const float sa = fmaf(A.c_ex.w, A.c_ex.w, fmaf(A.a0_ey.w, A.a0_ey.w, A.a1_ez.w * A.a1_ez.w));
-				const float sb = fmaf(Bn.c_ex.w, Bn.c_ex.w, fmaf(Bn.a0_ey.w, Bn.a0_ey.w, Bn.a1_ez.w * Bn.a1_ez.w));
+				const float eb0 = B.e[0] - c.infl, eb1 = B.e[1] - c.infl, eb2 = B.e[2] - c.infl;
+				const float sb = fmaf(eb0, eb0, fmaf(eb1, eb1, eb2 * eb2));
 				int a1, b1, a2, b2;
 				if (!la && (lb || sa > sb)) {
 					a1 = ca + 1; b1 = b; a2 = ca; b2 = b;
