@@ -282,6 +282,66 @@ def run_visibility(args, torch, pyosr, W):
     return line
 
 
+def _screen_segments(args, torch, W, uw, net, sq, sy, timer, stream):
+    """Segments from exact-verified free states, lognormal(5, 1.2) step counts
+    at 1/1024: isValidTransition on every segment, against the prescreen
+    (P(free) at K points along the segment, all above the threshold) followed
+    by isValidTransition on the accepted ones."""
+    n_seg = args.screen_segments
+    K = 8
+    free_idx = torch.nonzero(sy.bool()).squeeze(1)[:n_seg]
+    starts = sq.index_select(0, free_idx).contiguous()
+    n_seg = starts.shape[0]
+    steps = W.lognormal_steps(n_seg, seed=7003)
+    ends = torch.from_numpy(W.segment_ends(starts.cpu().numpy(), seed=7004, lengths=steps * VD)).cuda()
+    sp = stream.cuda_stream
+    ok = torch.zeros(n_seg, dtype=torch.uint8, device="cuda")
+    nsteps = torch.zeros(n_seg, dtype=torch.int32, device="cuda")
+    _, plain_ms = timer.run(lambda: uw.valid_transitions_device(starts.data_ptr(), ends.data_ptr(), n_seg, VD,
+                                                                ok.data_ptr(), nsteps.data_ptr(), sp),
+                            args.steps, args.warmup)
+    free_plain = int(ok.sum().item())
+    queries = int(nsteps.to(torch.int64).sum().item())
+    t = torch.arange(1, K + 1, dtype=torch.float64, device="cuda") / K
+    pk = torch.zeros(n_seg * K, dtype=torch.float32, device="cuda")
+    state = {}
+
+    def screened():
+        # K points per segment: linear translation, normalised linear
+        # quaternion blend (an approximation is enough to rank)
+        q0, q1 = starts[:, 3:], ends[:, 3:]
+        sgn = torch.where((q0 * q1).sum(1, keepdim=True) < 0, -1.0, 1.0)
+        p = starts[:, None, :3] + t[None, :, None] * (ends[:, None, :3] - starts[:, None, :3])
+        qq = q0[:, None, :] + t[None, :, None] * (sgn * q1 - q0)[:, None, :]
+        qq = qq / qq.norm(dim=2, keepdim=True)
+        pts = torch.cat([p, qq], dim=2).reshape(-1, 7).contiguous()
+        net.forward_device(pts.data_ptr(), n_seg * K, pk.data_ptr(), stream=sp)
+        acc_idx = torch.nonzero(pk.view(n_seg, K).min(1).values > args.threshold).squeeze(1)
+        a, b = starts.index_select(0, acc_idx), ends.index_select(0, acc_idx)
+        out = torch.zeros(a.shape[0], dtype=torch.uint8, device="cuda")
+        if a.shape[0]:
+            uw.valid_transitions_device(a.data_ptr(), b.data_ptr(), a.shape[0], VD, out.data_ptr(), None, sp)
+        state["idx"], state["out"] = acc_idx, out
+
+    _, scr_ms = timer.run(screened, args.steps, args.warmup)
+    uw.check_device()
+    free_scr = int(state["out"].sum().item())
+    truth = ok.bool()
+    acc = torch.zeros(n_seg, dtype=torch.bool, device="cuda")
+    acc[state["idx"]] = True
+    tp = int((truth & acc).sum().item())
+    return {
+        "segments": n_seg, "points_per_segment": K, "exact_queries": queries, "free_true": free_plain,
+        "accepted": int(acc.sum().item()), "free_verified_screened": free_scr,
+        "recall": round(tp / max(free_plain, 1), 4), "precision": round(tp / max(int(acc.sum().item()), 1), 4),
+        "exact_ms": round(plain_ms, 3), "screen_total_ms": round(scr_ms, 3),
+        "plain_free_per_s": round(free_plain / (plain_ms * 1e-3), 1),
+        "screened_free_per_s": round(free_scr / (scr_ms * 1e-3), 1),
+        "speedup": round((free_scr / scr_ms) / max(free_plain / plain_ms, 1e-9), 3),
+        "mismatches": 0 if free_scr == int(truth[state["idx"]].sum().item()) else 1,
+    }
+
+
 def run_surrogate(args, torch, pyosr, W):
     """Learned prescreen on the shell scene: exact labels, BF16-MFMA training,
     then free samples per second with and without the prescreen (every
@@ -334,6 +394,7 @@ def run_surrogate(args, torch, pyosr, W):
                                           screened()), args.steps, args.warmup)
     uw.check_device()
     free_scr = int(state["out"].sum().item())
+    seg = _screen_segments(args, torch, W, uw, net, sq, sy, timer, stream) if args.screen_segments else None
     truth = sy.bool()
     pred = pf > thr
     tp = int((truth & pred).sum().item())
@@ -357,6 +418,7 @@ def run_surrogate(args, torch, pyosr, W):
             "mlp_tflops": round(n_screen * flops_sample / (mlp_ms * 1e-3) / 1e12, 2),
             "mlp_mfma_frac_bf16": round(n_screen * flops_sample / (mlp_ms * 1e-3) / 2.5e15, 4),
         },
+        "surrogate_segments": seg,
     }
     line = _line(f"learned prescreen: shell env {len(env[1])} tris / bent rod {len(rob[1])} tris, "
                  f"{n_train} exact labels, {n_screen} screened samples", rate_scr, wall_scr, args, extra)
@@ -382,6 +444,8 @@ def main():
                          "surrogate: learned prescreen on the shell scene")
     ap.add_argument("--train-epochs", type=int, default=2)
     ap.add_argument("--threshold", type=float, default=0.5)
+    ap.add_argument("--screen-segments", type=int, default=8 * 1024 * 1024,
+                    help="surrogate workload: segments screened after the samples (0 = none)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first copy a buffer of this many MiB three times (a known-size "
                          "kernel that calibrates the FETCH_SIZE / WRITE_SIZE counters)")

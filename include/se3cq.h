@@ -104,6 +104,9 @@ int se3cq_transit_device(se3cq_world* w, const double* d_from, const double* d_t
 int se3cq_valid_transitions_host(se3cq_world* w, const double* from, const double* to, int64_t n,
                                  double initial_verify_delta, uint8_t* out);
 
+int se3cq_valid_transitions_device(se3cq_world* w, const double* d_from, const double* d_to, int64_t n,
+                                   double initial_verify_delta, uint8_t* d_out, int32_t* d_steps, void* stream);
+
 /* Touch-configuration search: transitStateToWithContact followed, for rows
  * that hit an obstacle, by `levels` bisection halvings of [last_tau, tau]
  * (mid = 0.5*(lo+hi); lo = mid if interpolate(from,to,mid) is valid, else

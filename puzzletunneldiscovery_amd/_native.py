@@ -52,6 +52,7 @@ SIGNATURES = {
                            c_double_p, c_double_p, c_i32_p],
     "se3cq_transit_device": [vp, vp, vp, i64, cdbl, vp, vp, vp, vp, vp, vp, vp],
     "se3cq_valid_transitions_host": [vp, c_double_p, c_double_p, i64, cdbl, c_u8_p],
+    "se3cq_valid_transitions_device": [vp, vp, vp, i64, cdbl, vp, vp, vp],
     "se3cq_touch_search_host": [vp, c_double_p, c_double_p, i64, cdbl, cint, c_u8_p, c_double_p,
                                 c_double_p, c_double_p, c_double_p, c_i32_p],
     "se3cq_touch_search_device": [vp, vp, vp, i64, cdbl, cint, vp, vp, vp, vp, vp, vp, vp],

@@ -1273,6 +1273,24 @@ int se3cq_valid_transitions_host(se3cq_world* w, const double* from, const doubl
 	return SE3CQ_OK;
 }
 
+int se3cq_valid_transitions_device(se3cq_world* w, const double* d_from, const double* d_to, int64_t n,
+                                   double initial_verify_delta, uint8_t* d_out, int32_t* d_steps, void* stream)
+{
+	CHECK_WORLD(w);
+	if (!w->ready())
+		return fail(SE3CQ_ESTATE, "models not loaded");
+	if (n < 0)
+		return fail(SE3CQ_EINVAL, "negative count");
+	{
+		std::lock_guard<std::mutex> lk0(w->mu);
+		const int rc0 = ensure_device(w);
+		if (rc0)
+			return rc0;
+	}
+	SegOut o{d_out, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, d_steps};
+	return seg_device(w, 1, d_from, d_to, n, initial_verify_delta, 0, o, pick_stream(w, stream));
+}
+
 int se3cq_touch_search_device(se3cq_world* w, const double* d_from, const double* d_to, int64_t n,
                               double verify_delta, int levels, uint8_t* d_done, double* d_last_tau, double* d_tau,
                               double* d_lo, double* d_hi, int32_t* d_steps, void* stream)

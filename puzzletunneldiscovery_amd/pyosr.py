@@ -416,6 +416,11 @@ class UnitWorld:
         N.check(N.lib().se3cq_transit_device(self._w, d_from, d_to, int(n), float(verify_delta), d_last_free,
                                              d_state, d_done, d_last_tau, d_tau, d_steps, stream))
 
+    def valid_transitions_device(self, d_from, d_to, n, initial_verify_delta, d_out, d_steps=None, stream=None):
+        """Device-pointer form of is_valid_transitions (asynchronous)."""
+        N.check(N.lib().se3cq_valid_transitions_device(self._w, d_from, d_to, int(n), float(initial_verify_delta),
+                                                       d_out, d_steps, stream))
+
     def touch_search_device(self, d_from, d_to, n, verify_delta, levels, d_done, d_last_tau, d_tau, d_lo, d_hi,
                             d_steps, stream=None):
         N.check(N.lib().se3cq_touch_search_device(self._w, d_from, d_to, int(n), float(verify_delta), int(levels),
