@@ -169,6 +169,22 @@ int se3cq_check(se3cq_world* w);
 /* translateToUnitState (unit_world.cc:526-539) / translateFromUnitState (541-555). */
 int se3cq_translate_to_unit(const se3cq_world* w, const double* q, int64_t n, double* out);
 int se3cq_translate_from_unit(const se3cq_world* w, const double* q, int64_t n, double* out);
+/* OMPL and "vanilla" state conventions (module.cc:159-167).  OMPL states
+ * are [t, qx, qy, qz, qw] (w last) and place the robot by its OMPL centre
+ * (the mean of its mesh vertices, before enforceRobotCenter).
+ *   translateOMPLStateToUnitState (unit_world.cc:689-711)
+ *   translateUnitStateToOMPLState (558-605); to_angle_axis writes
+ *     [t, angle, axis] instead of a quaternion
+ *   translateVanillaStateToOMPLState / translateOMPLStateToVanillaState (606-660)
+ *   translateVanillaStateToUnitState (662-666)
+ *   translateVanillaPointsToUnitPoints (668-687): pts [n, 3] */
+int se3cq_translate_ompl_to_unit(const se3cq_world* w, const double* q, int64_t n, double* out);
+int se3cq_translate_unit_to_ompl(const se3cq_world* w, const double* q, int64_t n, int to_angle_axis, double* out);
+int se3cq_translate_vanilla_to_ompl(const se3cq_world* w, const double* q, int64_t n, double* out);
+int se3cq_translate_ompl_to_vanilla(const se3cq_world* w, const double* q, int64_t n, double* out);
+int se3cq_translate_vanilla_to_unit(const se3cq_world* w, const double* q, int64_t n, double* out);
+int se3cq_translate_vanilla_pts_to_unit(const se3cq_world* w, int geo, const double* pts, int64_t n, double* out);
+
 /* distance (lib/osr/osr_state.cc:130-154) and multi_distance (156-166). */
 int se3cq_distance(const double* a7, const double* b7, double* out);
 int se3cq_multi_distance(const double* origin7, const double* targets, int64_t n, double* out);

@@ -68,6 +68,12 @@ SIGNATURES = {
     "se3cq_check": [vp],
     "se3cq_translate_to_unit": [vp, c_double_p, i64, c_double_p],
     "se3cq_translate_from_unit": [vp, c_double_p, i64, c_double_p],
+    "se3cq_translate_ompl_to_unit": [vp, c_double_p, i64, c_double_p],
+    "se3cq_translate_unit_to_ompl": [vp, c_double_p, i64, cint, c_double_p],
+    "se3cq_translate_vanilla_to_ompl": [vp, c_double_p, i64, c_double_p],
+    "se3cq_translate_ompl_to_vanilla": [vp, c_double_p, i64, c_double_p],
+    "se3cq_translate_vanilla_to_unit": [vp, c_double_p, i64, c_double_p],
+    "se3cq_translate_vanilla_pts_to_unit": [vp, cint, c_double_p, i64, c_double_p],
     "se3cq_distance": [c_double_p, c_double_p, c_double_p],
     "se3cq_multi_distance": [c_double_p, c_double_p, i64, c_double_p],
     "se3cq_interpolate": [c_double_p, c_double_p, cdbl, c_double_p],

@@ -1539,6 +1539,178 @@ int se3cq_translate_from_unit(const se3cq_world* w, const double* q, int64_t n, 
 	return SE3CQ_OK;
 }
 
+namespace {
+/* Eigen's Quaternion * Vector3 (_transformVector) for a unit quaternion
+ * (w, x, y, z): uv = 2 (vec x v); v + w uv + vec x uv. */
+void quat_apply(const double* q, const double* v, double* out)
+{
+	const double u[3] = {q[1], q[2], q[3]};
+	double uv[3], c[3];
+	uv[0] = u[1] * v[2] - u[2] * v[1];
+	uv[1] = u[2] * v[0] - u[0] * v[2];
+	uv[2] = u[0] * v[1] - u[1] * v[0];
+	for (int k = 0; k < 3; k++)
+		uv[k] = uv[k] + uv[k];
+	c[0] = u[1] * uv[2] - u[2] * uv[1];
+	c[1] = u[2] * uv[0] - u[0] * uv[2];
+	c[2] = u[0] * uv[1] - u[1] * uv[0];
+	for (int k = 0; k < 3; k++)
+		out[k] = (v[k] + q[0] * uv[k]) + c[k];
+}
+
+/* (OMPL centre - centre) * scene scale: the robot-frame offset between the
+ * OMPL convention and the unit world when enforceRobotCenter moved the
+ * centre (unit_world.cc:564-565, 702-703); glm float difference, then double. */
+void ompl_delta(const se3cq_world* w, double* d)
+{
+	for (int k = 0; k < 3; k++)
+		d[k] = (double)(w->mesh[1].ompl_center[k] - w->mesh[1].center[k]) * (double)w->scale;
+}
+} // namespace
+
+int se3cq_translate_ompl_to_unit(const se3cq_world* w, const double* q, int64_t n, double* out)
+{
+	/* translateOMPLStateToUnitState (unit_world.cc:689-711) */
+	CHECK_WORLD(w);
+	if (!w->mesh[1].loaded)
+		return fail(SE3CQ_ESTATE, "robot not loaded");
+	double d[3];
+	ompl_delta(w, d);
+	for (int64_t i = 0; i < n; i++) {
+		const double* s = q + 7 * i;
+		const double wf[7] = {s[0], s[1], s[2], s[6], s[3], s[4], s[5]};
+		double u[7], rot[4], rd[3];
+		unit_row(w, wf, u);
+		for (int k = 0; k < 4; k++)
+			rot[k] = u[3 + k];
+		se3_quat_normalize(rot);
+		quat_apply(rot, d, rd);
+		double* o = out + 7 * i;
+		for (int k = 0; k < 3; k++)
+			o[k] = u[k] - rd[k];
+		se3_quat_normalize(rot);
+		for (int k = 0; k < 4; k++)
+			o[3 + k] = rot[k];
+	}
+	return SE3CQ_OK;
+}
+
+int se3cq_translate_unit_to_ompl(const se3cq_world* w, const double* q, int64_t n, int to_angle_axis, double* out)
+{
+	/* translateUnitStateToOMPLState (unit_world.cc:558-605) */
+	CHECK_WORLD(w);
+	if (!w->mesh[1].loaded)
+		return fail(SE3CQ_ESTATE, "robot not loaded");
+	double d[3];
+	ompl_delta(w, d);
+	for (int64_t i = 0; i < n; i++) {
+		const double* s = q + 7 * i;
+		double rot[4] = {s[3], s[4], s[5], s[6]}, rd[3], c[7], f[7];
+		se3_quat_normalize(rot);
+		quat_apply(rot, d, rd);
+		for (int k = 0; k < 3; k++)
+			c[k] = s[k] + rd[k];
+		se3_quat_normalize(rot);
+		for (int k = 0; k < 4; k++)
+			c[3 + k] = rot[k];
+		se3cq_translate_from_unit(w, c, 1, f);
+		double* o = out + 7 * i;
+		for (int k = 0; k < 3; k++)
+			o[k] = f[k];
+		if (to_angle_axis) {
+			/* Eigen (3.3) AngleAxis from a unit quaternion: angle
+			 * 2 atan2(|v|, |w|), axis v / |v| with the sign of w */
+			const double vx = f[4], vy = f[5], vz = f[6];
+			double nv = std::sqrt((vx * vx + vy * vy) + vz * vz);
+			if (nv != 0.0) {
+				o[3] = 2.0 * std::atan2(nv, std::fabs(f[3]));
+				if (f[3] < 0.0)
+					nv = -nv;
+				o[4] = vx / nv;
+				o[5] = vy / nv;
+				o[6] = vz / nv;
+			} else {
+				o[3] = 0.0;
+				o[4] = 1.0;
+				o[5] = 0.0;
+				o[6] = 0.0;
+			}
+		} else {
+			/* w first -> w last */
+			o[3] = f[4];
+			o[4] = f[5];
+			o[5] = f[6];
+			o[6] = f[3];
+		}
+	}
+	return SE3CQ_OK;
+}
+
+static int vanilla_ompl(const se3cq_world* w, const double* q, int64_t n, double sign, double* out)
+{
+	if (!w->mesh[1].loaded)
+		return fail(SE3CQ_ESTATE, "robot not loaded");
+	const double oc[3] = {w->mesh[1].ompl_center[0], w->mesh[1].ompl_center[1], w->mesh[1].ompl_center[2]};
+	for (int64_t i = 0; i < n; i++) {
+		const double* s = q + 7 * i;
+		double rot[4] = {s[6], s[3], s[4], s[5]}, roo[3];
+		se3_quat_normalize(rot);
+		quat_apply(rot, oc, roo);
+		double* o = out + 7 * i;
+		for (int k = 0; k < 3; k++)
+			o[k] = sign > 0 ? s[k] + roo[k] : s[k] - roo[k];
+		for (int k = 3; k < 7; k++)
+			o[k] = s[k];
+	}
+	return SE3CQ_OK;
+}
+
+int se3cq_translate_vanilla_to_ompl(const se3cq_world* w, const double* q, int64_t n, double* out)
+{
+	/* translateVanillaStateToOMPLState (unit_world.cc:606-629) */
+	CHECK_WORLD(w);
+	return vanilla_ompl(w, q, n, 1.0, out);
+}
+
+int se3cq_translate_ompl_to_vanilla(const se3cq_world* w, const double* q, int64_t n, double* out)
+{
+	/* translateOMPLStateToVanillaState (unit_world.cc:639-660) */
+	CHECK_WORLD(w);
+	return vanilla_ompl(w, q, n, -1.0, out);
+}
+
+int se3cq_translate_vanilla_to_unit(const se3cq_world* w, const double* q, int64_t n, double* out)
+{
+	/* translateVanillaStateToUnitState (unit_world.cc:662-666) */
+	CHECK_WORLD(w);
+	std::vector<double> o((size_t)std::max<int64_t>(n, 1) * 7);
+	const int rc = vanilla_ompl(w, q, n, 1.0, o.data());
+	if (rc)
+		return rc;
+	return se3cq_translate_ompl_to_unit(w, o.data(), n, out);
+}
+
+int se3cq_translate_vanilla_pts_to_unit(const se3cq_world* w, int geo, const double* pts, int64_t n, double* out)
+{
+	/* translateVanillaPointsToUnitPoints (unit_world.cc:668-687): robot points
+	 * relative to the OMPL centre, then the calibration matrix */
+	CHECK_WORLD(w);
+	if (geo < 0 || geo > 1 || !w->mesh[geo].loaded)
+		return fail(SE3CQ_ESTATE, "mesh not loaded");
+	double M[16];
+	for (int i = 0; i < 4; i++)
+		for (int j = 0; j < 4; j++)
+			M[4 * i + j] = (double)w->xf[0][4 * j + i];
+	for (int64_t i = 0; i < n; i++) {
+		double p[3];
+		for (int k = 0; k < 3; k++)
+			p[k] = geo == 0 ? pts[3 * i + k] : pts[3 * i + k] - (double)w->mesh[1].ompl_center[k];
+		for (int r = 0; r < 3; r++)
+			out[3 * i + r] = ((M[4 * r] * p[0] + M[4 * r + 1] * p[1]) + M[4 * r + 2] * p[2]) + M[4 * r + 3];
+	}
+	return SE3CQ_OK;
+}
+
 int se3cq_distance(const double* a7, const double* b7, double* out)
 {
 	*out = se3_distance(a7, b7);

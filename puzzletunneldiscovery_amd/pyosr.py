@@ -368,6 +368,35 @@ class UnitWorld:
         N.check(N.lib().se3cq_translate_to_unit(self._w, N.dptr(q), 1, N.dptr(out)))
         return out
 
+    def _translate(self, fn, Q, *extra):
+        q = _states(Q)
+        out = np.zeros_like(q)
+        N.check(getattr(N.lib(), fn)(self._w, N.dptr(q), q.shape[0], *extra, N.dptr(out)))
+        return out
+
+    def translate_unit_to_ompl(self, Q, to_angle_axis=False):
+        """translateUnitStateToOMPLState (unit_world.cc:558-605)."""
+        return self._translate("se3cq_translate_unit_to_ompl", Q, int(bool(to_angle_axis)))
+
+    def translate_ompl_to_unit(self, Q):
+        """translateOMPLStateToUnitState (unit_world.cc:689-711)."""
+        return self._translate("se3cq_translate_ompl_to_unit", Q)
+
+    def translate_vanilla_to_ompl(self, Q):
+        return self._translate("se3cq_translate_vanilla_to_ompl", Q)
+
+    def translate_ompl_to_vanilla(self, Q):
+        return self._translate("se3cq_translate_ompl_to_vanilla", Q)
+
+    def translate_vanilla_to_unit(self, Q):
+        return self._translate("se3cq_translate_vanilla_to_unit", Q)
+
+    def translate_vanilla_pts_to_unit(self, geo, pts):
+        p = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3)
+        out = np.zeros_like(p)
+        N.check(N.lib().se3cq_translate_vanilla_pts_to_unit(self._w, int(geo), N.dptr(p), p.shape[0], N.dptr(out)))
+        return out
+
     def translate_from_unit_state(self, state):
         q = _state(state)
         out = np.zeros(7)
