@@ -185,6 +185,27 @@ int se3cq_translate_ompl_to_vanilla(const se3cq_world* w, const double* q, int64
 int se3cq_translate_vanilla_to_unit(const se3cq_world* w, const double* q, int64_t n, double* out);
 int se3cq_translate_vanilla_pts_to_unit(const se3cq_world* w, int geo, const double* pts, int64_t n, double* out);
 
+/* OMPL-side validity, the two queries of pyse3ompl's OmplDriver that do not
+ * plan (lib/pyse3ompl/pyse3ompl.cc:77-81), over this world's geometry.
+ * States are OMPL states (x, y, z, qx, qy, qz, qw) with unit quaternions.
+ *   se3cq_ompl_longest_valid_segment: the bounding box diagonal plus pi/2,
+ *     times the collision resolution (OmplDriver::set_bb / set_cdres,
+ *     ompldriver.cc:676-685); the resolution must lie in (0, 1).
+ *   validate_states (ompldriver.cc:566-589): out[i] = 1 if state i is free.
+ *   validate_motions (ompldriver.cc:592-620): OMPL's discrete motion
+ *     validator on (q0[i], q1[i]); out[i] = 1 if the end state and every
+ *     interpolate(k / nd), 0 < k < nd, nd = ceil(distance / lvs), are free.
+ *     The start state is not tested.  samples[i] (may be NULL) = max(nd, 1).
+ *     The device form writes -1 for a motion whose nd is not finite below
+ *     2^31; the host form fails with SE3CQ_EINVAL. */
+int se3cq_ompl_longest_valid_segment(const double* lo, const double* hi, double resolution, double* out);
+int se3cq_ompl_validate_states_host(se3cq_world* w, const double* q, int64_t n, int32_t* out);
+int se3cq_ompl_validate_states_device(se3cq_world* w, const double* d_q, int64_t n, int32_t* d_out, void* stream);
+int se3cq_ompl_validate_motions_host(se3cq_world* w, const double* q0, const double* q1, int64_t n, double lvs,
+                                     int32_t* out, int32_t* samples);
+int se3cq_ompl_validate_motions_device(se3cq_world* w, const double* d_q0, const double* d_q1, int64_t n, double lvs,
+                                       int32_t* d_out, int32_t* d_samples, void* stream);
+
 /* distance (lib/osr/osr_state.cc:130-154) and multi_distance (156-166). */
 int se3cq_distance(const double* a7, const double* b7, double* out);
 int se3cq_multi_distance(const double* origin7, const double* targets, int64_t n, double* out);

@@ -14,6 +14,7 @@
 #include "oracle.h"
 #include "../puzzletunneldiscovery_amd/csrc/se3_math.h"
 
+#include <float.h>
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -42,6 +43,7 @@ typedef struct {
 	uint32_t* F;
 	int64_t nf;
 	float center[3];   /* Scene::center_ (OMPL centre unless overridden) */
+	float ompl_center[3]; /* mean of the loaded vertices (getOMPLCenter) */
 	float bbox[6];     /* left, right, bottom, top, back, front (origin-seeded) */
 	int loaded;
 } OrMesh;
@@ -274,6 +276,7 @@ static void mesh_finish(OrMesh* m)
 	m->center[0] = sx / n;
 	m->center[1] = sy / n;
 	m->center[2] = sz / n;
+	memcpy(m->ompl_center, m->center, sizeof(m->center));
 	m->loaded = 1;
 }
 
@@ -1299,6 +1302,191 @@ void or_translate_to_unit_state(const OrWorld* w, const double* q, double* out)
 	out[2] = tret[2];
 	for (int i = 0; i < 4; i++)
 		out[3 + i] = qret[i];
+}
+
+/* Eigen's Quaternion * Vector3 (Quaternion::_transformVector) for a unit
+ * quaternion: uv = vec x v; uv += uv; v + w uv + vec x uv. */
+static void q_rotate(const double* q, const double* v, double* out)
+{
+	double uv[3], c[3];
+	cross3(q + 1, v, uv);
+	for (int k = 0; k < 3; k++)
+		uv[k] += uv[k];
+	cross3(q + 1, uv, c);
+	for (int k = 0; k < 3; k++)
+		out[k] = v[k] + q[0] * uv[k] + c[k];
+}
+
+void or_ompl_center(const OrWorld* w, float* c3) { memcpy(c3, w->mesh[1].ompl_center, 3 * sizeof(float)); }
+
+/* translateOMPLStateToUnitState (lib/osr/unit_world.cc:689-711): w last ->
+ * w first, translateToUnitState, then t - R * delta_center with
+ * delta_center = (OMPL centre - centre) (glm binary32) * scene_scale_. */
+void or_translate_ompl_to_unit(const OrWorld* w, const double* q, double* out)
+{
+	const double wf[7] = {q[0], q[1], q[2], q[6], q[3], q[4], q[5]};
+	double u[7], delta[3], rd[3];
+	for (int k = 0; k < 3; k++) {
+		float dc = w->mesh[1].ompl_center[k] - w->mesh[1].center[k];
+		delta[k] = (double)dc;
+		delta[k] *= (double)w->scale;
+	}
+	or_translate_to_unit_state(w, wf, u);
+	double rot[4] = {u[3], u[4], u[5], u[6]};
+	q_normalize(rot);
+	q_rotate(rot, delta, rd);
+	for (int k = 0; k < 3; k++)
+		out[k] = u[k] - rd[k];
+	q_normalize(rot);
+	memcpy(out + 3, rot, 4 * sizeof(double));
+}
+
+/* ------------------------------------------------------------------ */
+/* OMPL-side validity (pyse3ompl OmplDriver::validateStates and       */
+/* validateMotionPairs, lib/pyse3ompl/ompldriver.cc:566-620).  OMPL is */
+/* not part of the reference's tree (third-party/ompl.app is an empty  */
+/* submodule; the fork builds on OMPL 1.4.1); this follows OMPL        */
+/* 1.4.1's published sources:                                          */
+/*  - SE3StateSpace: RealVectorStateSpace(3) + SO3StateSpace, weights 1 */
+/*  - getMaximumExtent: bounds diagonal + pi/2; the longest valid      */
+/*    segment is that times the resolution                             */
+/*  - validSegmentCount: (unsigned) ceil(distance / longest segment)   */
+/*  - DiscreteMotionValidator::checkMotion: end state first, then the  */
+/*    midpoints of a queue of index intervals                          */
+/* States are OMPL states (x, y, z, qx, qy, qz, qw).                   */
+/* ------------------------------------------------------------------ */
+static double ompl_arc(const double* a, const double* b)
+{
+	/* SO3StateSpace.cpp arcLength; MAX_QUATERNION_NORM_ERROR = 1e-9 */
+	double dq = fabs(a[3] * b[3] + a[4] * b[4] + a[5] * b[5] + a[6] * b[6]);
+	if (dq > 1.0 - 1e-9)
+		return 0.0;
+	return o_acos(dq);
+}
+
+double or_ompl_distance(const double* a, const double* b)
+{
+	double dist = 0.0, r = 0.0;
+	for (int i = 0; i < 3; i++) {
+		double diff = a[i] - b[i];
+		r += diff * diff;
+	}
+	dist += 1.0 * sqrt(r);
+	dist += 1.0 * ompl_arc(a, b);
+	return dist;
+}
+
+double or_ompl_longest_segment(const double* lo, const double* hi, double resolution)
+{
+	double e = 0.0, ext = 0.0;
+	for (int i = 0; i < 3; i++) {
+		double d = hi[i] - lo[i];
+		e += d * d;
+	}
+	ext += 1.0 * sqrt(e);
+	ext += 1.0 * (0.5 * 3.14159265358979323846);
+	return ext * resolution;
+}
+
+void or_ompl_interpolate(const double* from, const double* to, double t, double* out)
+{
+	for (int i = 0; i < 3; i++)
+		out[i] = from[i] + (to[i] - from[i]) * t;
+	double theta = ompl_arc(from, to);
+	if (theta > DBL_EPSILON) {
+		double d = 1.0 / o_sin(theta);
+		double s0 = o_sin((1.0 - t) * theta);
+		double s1 = o_sin(t * theta);
+		double dq = from[3] * to[3] + from[4] * to[4] + from[5] * to[5] + from[6] * to[6];
+		if (dq < 0)
+			s1 = -s1;
+		for (int k = 3; k < 7; k++)
+			out[k] = (from[k] * s0 + to[k] * s1) * d;
+	} else {
+		for (int k = 3; k < 7; k++)
+			out[k] = from[k];
+	}
+}
+
+int or_ompl_is_valid(const OrWorld* w, const double* q)
+{
+	double u[7];
+	or_translate_ompl_to_unit(w, q, u);
+	return or_is_valid(w, u);
+}
+
+/* 1 valid, 0 invalid, -1 when the segment count is not finite below 2^31;
+ * *nd_out = the segment count. */
+int or_ompl_check_motion(const OrWorld* w, const double* s1, const double* s2, double lvs, int64_t* nd_out)
+{
+	double c = ceil(or_ompl_distance(s1, s2) / lvs);
+	if (!(c >= 0.0 && c < 2147483648.0)) {
+		*nd_out = -1;
+		return -1;
+	}
+	int64_t nd = (int64_t)c;
+	*nd_out = nd;
+	if (!or_ompl_is_valid(w, s2))
+		return 0;
+	if (nd < 2)
+		return 1;
+	/* FIFO of index intervals [first, second] */
+	int64_t* qa = (int64_t*)malloc(sizeof(int64_t) * 2 * (size_t)nd);
+	int64_t head = 0, tail = 0;
+	int result = 1;
+	qa[0] = 1;
+	qa[1] = nd - 1;
+	tail = 1;
+	double test[7];
+	while (head < tail) {
+		int64_t first = qa[2 * (head % nd)], second = qa[2 * (head % nd) + 1];
+		int64_t mid = (first + second) / 2;
+		or_ompl_interpolate(s1, s2, (double)mid / (double)nd, test);
+		if (!or_ompl_is_valid(w, test)) {
+			result = 0;
+			break;
+		}
+		head++;
+		if (first < mid) {
+			qa[2 * (tail % nd)] = first;
+			qa[2 * (tail % nd) + 1] = mid - 1;
+			tail++;
+		}
+		if (second > mid) {
+			qa[2 * (tail % nd)] = mid + 1;
+			qa[2 * (tail % nd) + 1] = second;
+			tail++;
+		}
+	}
+	free(qa);
+	return result;
+}
+
+void or_ompl_validate_states(const OrWorld* w, const double* q, int64_t n, int32_t* out, int nthreads)
+{
+#ifdef _OPENMP
+	if (nthreads > 0)
+		omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 64)
+#endif
+	for (int64_t i = 0; i < n; i++)
+		out[i] = or_ompl_is_valid(w, q + 7 * i);
+}
+
+void or_ompl_validate_motions(const OrWorld* w, const double* q0, const double* q1, int64_t n, double lvs,
+                              int32_t* out, int32_t* samples, int nthreads)
+{
+#ifdef _OPENMP
+	if (nthreads > 0)
+		omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1)
+#endif
+	for (int64_t i = 0; i < n; i++) {
+		int64_t nd = 0;
+		out[i] = or_ompl_check_motion(w, q0 + 7 * i, q1 + 7 * i, lvs, &nd);
+		if (samples)
+			samples[i] = nd < 0 ? 0 : (int32_t)(nd > 1 ? nd : 1);
+	}
 }
 
 static double* to_unit_rows(const OrWorld* w, const double* qs, int64_t n, int is_unit)

@@ -69,6 +69,20 @@ double or_distance(const double* a7, const double* b7);
 void or_interpolate(const double* a7, const double* b7, double tau, double* out7);
 void or_apply(const double* from7, const double* tr3, const double* aa3, double* out7);
 void or_translate_to_unit_state(const OrWorld* w, const double* q7, double* out7);
+/* translateOMPLStateToUnitState (lib/osr/unit_world.cc:689-711). */
+void or_translate_ompl_to_unit(const OrWorld* w, const double* q7, double* out7);
+void or_ompl_center(const OrWorld* w, float* c3);
+
+/* OMPL-side validity (OMPL 1.4.1's SE3StateSpace and discrete motion
+ * validator, restated; states (x, y, z, qx, qy, qz, qw)). */
+double or_ompl_distance(const double* a7, const double* b7);
+double or_ompl_longest_segment(const double* lo3, const double* hi3, double resolution);
+void or_ompl_interpolate(const double* from7, const double* to7, double t, double* out7);
+int or_ompl_is_valid(const OrWorld* w, const double* q7);
+int or_ompl_check_motion(const OrWorld* w, const double* s1, const double* s2, double lvs, int64_t* nd_out);
+void or_ompl_validate_states(const OrWorld* w, const double* q, int64_t n, int32_t* out, int nthreads);
+void or_ompl_validate_motions(const OrWorld* w, const double* q0, const double* q1, int64_t n, double lvs,
+                              int32_t* out, int32_t* samples, int nthreads);
 
 /* Triangle-triangle separating-axis test (FCL/PQP, 17 axes), binary64.
  * P: 3 env vertices; Q: 3 robot vertices already in the env frame. */

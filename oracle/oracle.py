@@ -60,6 +60,18 @@ def lib():
             "or_interpolate": (None, [_c_double_p, _c_double_p, ctypes.c_double, _c_double_p]),
             "or_apply": (None, [_c_double_p, _c_double_p, _c_double_p, _c_double_p]),
             "or_translate_to_unit_state": (None, [vp, _c_double_p, _c_double_p]),
+            "or_translate_ompl_to_unit": (None, [vp, _c_double_p, _c_double_p]),
+            "or_ompl_center": (None, [vp, _c_float_p]),
+            "or_ompl_distance": (ctypes.c_double, [_c_double_p, _c_double_p]),
+            "or_ompl_longest_segment": (ctypes.c_double, [_c_double_p, _c_double_p, ctypes.c_double]),
+            "or_ompl_interpolate": (None, [_c_double_p, _c_double_p, ctypes.c_double, _c_double_p]),
+            "or_ompl_is_valid": (ctypes.c_int, [vp, _c_double_p]),
+            "or_ompl_check_motion": (ctypes.c_int, [vp, _c_double_p, _c_double_p, ctypes.c_double, _c_i64_p]),
+            "or_ompl_validate_states": (None, [vp, _c_double_p, ctypes.c_int64, ctypes.POINTER(ctypes.c_int32),
+                                               ctypes.c_int]),
+            "or_ompl_validate_motions": (None, [vp, _c_double_p, _c_double_p, ctypes.c_int64, ctypes.c_double,
+                                                ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
+                                                ctypes.c_int]),
             "or_tri_intersect": (ctypes.c_int, [_c_double_p, _c_double_p]),
             "or_is_valid": (ctypes.c_int, [vp, _c_double_p]),
             "or_is_valid_counted": (ctypes.c_int, [vp, _c_double_p, _c_i64_p, _c_i64_p]),
@@ -304,6 +316,46 @@ class OracleWorld:
         lib().or_translate_to_unit_state(self._w, _d(q), _d(out))
         return out
 
+    def translate_ompl_to_unit(self, q):
+        q = _f64(q, (7,))
+        out = np.zeros(7)
+        lib().or_translate_ompl_to_unit(self._w, _d(q), _d(out))
+        return out
+
+    def ompl_center(self):
+        c = np.zeros(3, np.float32)
+        lib().or_ompl_center(self._w, c.ctypes.data_as(_c_float_p))
+        return c
+
+    def ompl_is_valid(self, q):
+        return bool(lib().or_ompl_is_valid(self._w, _d(_f64(q, (7,)))))
+
+    def ompl_check_motion(self, s1, s2, lvs):
+        """(result, nd): DiscreteMotionValidator::checkMotion as OMPL runs it."""
+        nd = ctypes.c_int64(0)
+        r = lib().or_ompl_check_motion(self._w, _d(_f64(s1, (7,))), _d(_f64(s2, (7,))), float(lvs),
+                                       ctypes.byref(nd))
+        return r, nd.value
+
+    def ompl_validate_states(self, qs, nthreads=0):
+        qs = _f64(qs).reshape(-1, 7)
+        out = np.zeros(len(qs), np.int32)
+        lib().or_ompl_validate_states(self._w, _d(qs), len(qs),
+                                      out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), nthreads)
+        return out
+
+    def ompl_validate_motions(self, q0, q1, lvs, nthreads=0):
+        """(out, samples) for the motions q0[i] -> q1[i]."""
+        q0 = _f64(q0).reshape(-1, 7)
+        q1 = _f64(q1).reshape(-1, 7)
+        n = min(len(q0), len(q1))
+        out = np.zeros(n, np.int32)
+        sam = np.zeros(n, np.int32)
+        lib().or_ompl_validate_motions(self._w, _d(q0), _d(q1), n, float(lvs),
+                                       out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                       sam.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), nthreads)
+        return out, sam
+
     def visibility_matrix(self, qs, is_unit, vd, nthreads=0):
         qs = _f64(qs).reshape(-1, 7)
         n = qs.shape[0]
@@ -359,6 +411,20 @@ def state_to_transform(q):
     t = np.zeros(3)
     lib().or_state_to_transform(_d(q), _d(R), _d(t))
     return R, t
+
+
+def ompl_distance(a, b):
+    return lib().or_ompl_distance(_d(_f64(a, (7,))), _d(_f64(b, (7,))))
+
+
+def ompl_longest_segment(lo, hi, resolution):
+    return lib().or_ompl_longest_segment(_d(_f64(lo, (3,))), _d(_f64(hi, (3,))), float(resolution))
+
+
+def ompl_interpolate(a, b, t):
+    out = np.zeros(7)
+    lib().or_ompl_interpolate(_d(_f64(a, (7,))), _d(_f64(b, (7,))), float(t), _d(out))
+    return out
 
 
 def tri_isect_segment(V, U):

@@ -74,6 +74,11 @@ SIGNATURES = {
     "se3cq_translate_ompl_to_vanilla": [vp, c_double_p, i64, c_double_p],
     "se3cq_translate_vanilla_to_unit": [vp, c_double_p, i64, c_double_p],
     "se3cq_translate_vanilla_pts_to_unit": [vp, cint, c_double_p, i64, c_double_p],
+    "se3cq_ompl_longest_valid_segment": [c_double_p, c_double_p, ctypes.c_double, c_double_p],
+    "se3cq_ompl_validate_states_host": [vp, c_double_p, i64, c_i32_p],
+    "se3cq_ompl_validate_states_device": [vp, vp, i64, vp, vp],
+    "se3cq_ompl_validate_motions_host": [vp, c_double_p, c_double_p, i64, ctypes.c_double, c_i32_p, c_i32_p],
+    "se3cq_ompl_validate_motions_device": [vp, vp, vp, i64, ctypes.c_double, vp, vp, vp],
     "se3cq_distance": [c_double_p, c_double_p, c_double_p],
     "se3cq_multi_distance": [c_double_p, c_double_p, i64, c_double_p],
     "se3cq_interpolate": [c_double_p, c_double_p, cdbl, c_double_p],

@@ -57,6 +57,13 @@ struct PertData {
 	float fro_dRe; /* sqrt(3) * ||Re - R(normalised)||_F, rounded up */
 };
 
+/* OMPL state -> unit-world state (se3_ompl_to_unit), passed by value. */
+struct OmplXf {
+	double M[16];    /* calibration matrix, row-major */
+	double pert[7];  /* perturbation (t, w, x, y, z) */
+	double delta[3]; /* (OMPL centre - robot centre) * scene scale */
+};
+
 /* Everything a kernel needs about the scene, passed by value. */
 struct DevScene {
 	DevModel env;

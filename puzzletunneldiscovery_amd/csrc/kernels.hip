@@ -404,6 +404,12 @@ struct StateFromGlobal {
 	}
 };
 
+struct StateFromOmpl {
+	const double* q; /* OMPL order (x, y, z, qx, qy, qz, qw) */
+	const OmplXf* x;
+	__device__ __forceinline__ void get(double* o) const { se3_ompl_to_unit(x->M, x->pert, x->delta, q, o); }
+};
+
 struct StateFromSegment {
 	const se3_segment* sg; /* LDS */
 	double tau;
@@ -510,6 +516,22 @@ __global__ void __launch_bounds__(256, SE3CQ_MIN_WAVES) k_valid_states(DevScene 
 	PairStack<WIDE> stk, leaves;
 	thread_stacks<WIDE>(s, smem, stk, leaves);
 	out[i] = collide_dfs<WIDE>(s, StateFromGlobal{qp}, stk, leaves) ? 0 : 1;
+}
+
+/* OMPL-convention states (OmplDriver::validateStates,
+ * lib/pyse3ompl/ompldriver.cc:566-589): converted to the unit world on load,
+ * then the per-thread traversal.  out: 1 = valid. */
+template <bool WIDE>
+__global__ void __launch_bounds__(256, SE3CQ_MIN_WAVES) k_ompl_states(DevScene s, OmplXf x, const double* __restrict__ qs,
+                                                     int64_t n, int32_t* __restrict__ out)
+{
+	extern __shared__ uint32_t smem[];
+	const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n)
+		return;
+	PairStack<WIDE> stk, leaves;
+	thread_stacks<WIDE>(s, smem, stk, leaves);
+	out[i] = collide_dfs<WIDE>(s, StateFromOmpl{qs + 7 * i, &x}, stk, leaves) ? 0 : 1;
 }
 
 /* ---- packet traversal: one wavefront, one pose per lane ---- */
@@ -1286,6 +1308,82 @@ __global__ void __launch_bounds__(256, SE3CQ_MIN_WAVES) k_visibility(DevScene s,
 	}
 }
 
+/* ---- OMPL discrete motions: one wavefront per motion ---- */
+/*
+ * OmplDriver::validateMotionPairs (lib/pyse3ompl/ompldriver.cc:592-620) runs
+ * DiscreteMotionValidator::checkMotion on each pair: the motion is valid when
+ * the end state and every interior sample interpolate(k / nd), 0 < k < nd,
+ * are valid.  The answer does not depend on the order the samples are
+ * tested in, so the wave tests them 64 at a time in a strided order (chunk
+ * c, lane j -> sample 1 + c + C j with C chunks): the first chunk already
+ * spans the whole motion, the way the validator's midpoint-first order
+ * does, so colliding motions usually end after one packet traversal.
+ * out: 1 valid, 0 invalid, -1 the sample count is not a finite number below
+ * 2^31.  samples (optional): max(nd, 1), the states a valid motion costs.
+ */
+template <bool WIDE>
+__global__ void __launch_bounds__(256, SE3CQ_MIN_WAVES) k_ompl_motions(DevScene s, OmplXf x,
+                                                      const double* __restrict__ q0,
+                                                      const double* __restrict__ q1, int64_t n, double lvs,
+                                                      int32_t* __restrict__ out, int32_t* __restrict__ samples)
+{
+	/* the motion record and the transform sit in LDS (broadcast reads), so
+	 * the packet traversal keeps its registers */
+	__shared__ se3_ompl_motion msh[4];
+	__shared__ OmplXf xsh;
+	const int wave = threadIdx.x >> 6;
+	const int lane = threadIdx.x & 63;
+	const int64_t p = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave;
+	const int64_t pv = p < n ? p : n - 1;
+	if (threadIdx.x < 16)
+		xsh.M[threadIdx.x] = x.M[threadIdx.x];
+	else if (threadIdx.x < 23)
+		xsh.pert[threadIdx.x - 16] = x.pert[threadIdx.x - 16];
+	else if (threadIdx.x < 26)
+		xsh.delta[threadIdx.x - 23] = x.delta[threadIdx.x - 23];
+	if (lane == 0) {
+		double a[7], b[7];
+		for (int k = 0; k < 7; k++) {
+			a[k] = q0[7 * pv + k];
+			b[k] = q1[7 * pv + k];
+		}
+		se3_ompl_motion_init(&msh[wave], a, b, lvs);
+	}
+	__syncthreads();
+	if (p >= n)
+		return; /* wave-uniform */
+	const se3_ompl_motion* m = &msh[wave];
+	const int64_t nd = m->nd;
+	if (nd < 0) {
+		if (lane == 0) {
+			out[p] = -1;
+			if (samples)
+				samples[p] = 0;
+		}
+		return;
+	}
+	const int64_t K = nd > 1 ? nd : 1;
+	const int64_t C = (K + 63) / 64;
+	int valid = 1;
+	for (int64_t c = 0; c < C; c++) {
+		const int64_t k = 1 + c + C * lane;
+		const bool active = k <= K;
+		const uint64_t am = __ballot(active);
+		double st[7], u[7];
+		se3_ompl_motion_sample(m, active ? k : K, st);
+		se3_ompl_to_unit(xsh.M, xsh.pert, xsh.delta, st, u);
+		if (packet_collide<true>(s, u, am)) {
+			valid = 0;
+			break;
+		}
+	}
+	if (lane == 0) {
+		out[p] = valid;
+		if (samples)
+			samples[p] = (int32_t)K;
+	}
+}
+
 /* ---- contact enumeration: one thread per state ---- */
 /*
  * Every (env triangle, robot triangle) pair whose exact test intersects at
@@ -1516,6 +1614,37 @@ hipError_t launch_valid_states(const DevScene& s, const double* q, int64_t n, ui
 		hipLaunchKernelGGL(k_valid_states<true>, dim3((unsigned)grid), dim3(block), lds, st, s, q, n, out);
 	else
 		hipLaunchKernelGGL(k_valid_states<false>, dim3((unsigned)grid), dim3(block), lds, st, s, q, n, out);
+	return hipGetLastError();
+}
+
+hipError_t launch_ompl_states(const DevScene& s, const OmplXf& x, const double* q, int64_t n, int32_t* out,
+                              hipStream_t st)
+{
+	if (n <= 0)
+		return hipSuccess;
+	const int block = pick_block(s);
+	const int64_t grid = (n + block - 1) / block;
+	const size_t lds = stack_bytes(s, block);
+	if (s.wide)
+		hipLaunchKernelGGL(k_ompl_states<true>, dim3((unsigned)grid), dim3(block), lds, st, s, x, q, n, out);
+	else
+		hipLaunchKernelGGL(k_ompl_states<false>, dim3((unsigned)grid), dim3(block), lds, st, s, x, q, n, out);
+	return hipGetLastError();
+}
+
+hipError_t launch_ompl_motions(const DevScene& s, const OmplXf& x, const double* q0, const double* q1, int64_t n,
+                               double lvs, int32_t* out, int32_t* samples, hipStream_t st)
+{
+	if (n <= 0)
+		return hipSuccess;
+	const int block = 256;
+	const int64_t grid = (n + 3) / 4;
+	if (s.wide)
+		hipLaunchKernelGGL(k_ompl_motions<true>, dim3((unsigned)grid), dim3(block), 0, st, s, x, q0, q1, n, lvs, out,
+		                   samples);
+	else
+		hipLaunchKernelGGL(k_ompl_motions<false>, dim3((unsigned)grid), dim3(block), 0, st, s, x, q0, q1, n, lvs,
+		                   out, samples);
 	return hipGetLastError();
 }
 

@@ -36,6 +36,14 @@ hipError_t launch_visibility(const DevScene& s, int vis, const double* qs0, cons
 hipError_t launch_contacts(const DevScene& s, const double* q, int64_t n, int cap, int32_t* pairs, int32_t* count,
                            double* segs, int8_t* flags, hipStream_t st);
 
+/* OMPL-convention validity (OmplDriver::validateStates / validateMotionPairs):
+ * out 1 valid, 0 invalid; motions: -1 when the sample count is not finite
+ * below 2^31, samples = max(nd, 1) (may be null). */
+hipError_t launch_ompl_states(const DevScene& s, const OmplXf& x, const double* q, int64_t n, int32_t* out,
+                              hipStream_t st);
+hipError_t launch_ompl_motions(const DevScene& s, const OmplXf& x, const double* q0, const double* q1, int64_t n,
+                               double lvs, int32_t* out, int32_t* samples, hipStream_t st);
+
 /* Local AABBs of the two models (centre, half extents), for isDisentangled. */
 struct BoxPair {
 	double ce[3], he[3]; /* environment */

@@ -29,6 +29,7 @@
 #define SE3_MATH_H
 
 #include <math.h>
+#include <stdint.h>
 
 #if defined(__HIPCC__) || defined(__HIP__)
 #define SE3_HD __host__ __device__
@@ -489,6 +490,183 @@ SE3_HD static inline void se3_place(const double* R, const double* t, const doub
 {
 	for (int i = 0; i < 3; i++)
 		out[i] = ((R[3 * i] * c[0] + R[3 * i + 1] * c[1]) + R[3 * i + 2] * c[2]) + t[i];
+}
+
+/* Eigen's Quaternion * Vector3 (_transformVector) for a unit quaternion
+ * q = (w, x, y, z): uv = 2 (vec x v); v + w uv + vec x uv. */
+SE3_HD static inline void se3_quat_apply(const double* q, const double* v, double* out)
+{
+	const double u[3] = {q[1], q[2], q[3]};
+	double uv[3], c[3];
+	uv[0] = u[1] * v[2] - u[2] * v[1];
+	uv[1] = u[2] * v[0] - u[0] * v[2];
+	uv[2] = u[0] * v[1] - u[1] * v[0];
+	for (int k = 0; k < 3; k++)
+		uv[k] = uv[k] + uv[k];
+	c[0] = u[1] * uv[2] - u[2] * uv[1];
+	c[1] = u[2] * uv[0] - u[0] * uv[2];
+	c[2] = u[0] * uv[1] - u[1] * uv[0];
+	for (int k = 0; k < 3; k++)
+		out[k] = (v[k] + q[0] * uv[k]) + c[k];
+}
+
+/* translateToUnitState (lib/osr/unit_world.cc:526-539) then applyPertubation
+ * (713-724) of a w-first state: the translation through the calibration
+ * matrix M (row-major 4x4, the binary32 matrix widened), then the
+ * perturbation pert = (t, w, x, y, z) applied on the left. */
+SE3_HD static inline void se3_to_unit(const double* M, const double* pert, const double* q, double* out)
+{
+	const double t4[4] = {q[0], q[1], q[2], 1.0};
+	double nt[3];
+	for (int r = 0; r < 3; r++)
+		nt[r] = ((M[4 * r] * t4[0] + M[4 * r + 1] * t4[1]) + M[4 * r + 2] * t4[2]) + M[4 * r + 3] * t4[3];
+	double rot[4] = {q[3], q[4], q[5], q[6]};
+	se3_quat_normalize(rot);
+	double prot[4] = {pert[3], pert[4], pert[5], pert[6]};
+	se3_quat_normalize(prot);
+	double qret[4];
+	se3_quat_mul(prot, rot, qret);
+	double PR[9];
+	se3_quat_to_rotmat(prot[0], prot[1], prot[2], prot[3], PR);
+	for (int r = 0; r < 3; r++)
+		out[r] = pert[r] + ((PR[3 * r] * nt[0] + PR[3 * r + 1] * nt[1]) + PR[3 * r + 2] * nt[2]);
+	se3_quat_normalize(qret);
+	for (int i = 0; i < 4; i++)
+		out[3 + i] = qret[i];
+}
+
+/* translateOMPLStateToUnitState (lib/osr/unit_world.cc:689-711) of one OMPL
+ * state (x, y, z, qx, qy, qz, qw): reorder to w first, translate to the unit
+ * world, then move the translation by -R delta, where delta = (OMPL centre -
+ * robot centre) * scene scale is non-zero after enforceRobotCenter. */
+SE3_HD static inline void se3_ompl_to_unit(const double* M, const double* pert, const double* delta, const double* q,
+                                           double* out)
+{
+	const double wf[7] = {q[0], q[1], q[2], q[6], q[3], q[4], q[5]};
+	double u[7];
+	se3_to_unit(M, pert, wf, u);
+	double rot[4] = {u[3], u[4], u[5], u[6]};
+	se3_quat_normalize(rot); /* decompose */
+	double rd[3];
+	se3_quat_apply(rot, delta, rd);
+	for (int k = 0; k < 3; k++)
+		out[k] = u[k] - rd[k];
+	se3_quat_normalize(rot); /* compose */
+	for (int k = 0; k < 4; k++)
+		out[3 + k] = rot[k];
+}
+
+/* ---------------------------------------------------------------------- */
+/* OMPL's SE(3) state space and discrete motion validator (OMPL 1.4.1, the */
+/* version the reference's ompl.app fork builds on; restated from OMPL's    */
+/* published sources): SE3StateSpace is the compound of a 3-D real vector   */
+/* space and SO3StateSpace with weights 1.  States are in OMPL's order      */
+/* (x, y, z, qx, qy, qz, qw) with unit quaternions.                         */
+/* ---------------------------------------------------------------------- */
+#define SE3_OMPL_QUAT_EPS 1e-9 /* MAX_QUATERNION_NORM_ERROR */
+#define SE3_DBL_EPS 0x1p-52
+
+/* qs1->x * qs2->x + qs1->y * qs2->y + qs1->z * qs2->z + qs1->w * qs2->w */
+SE3_HD static inline double se3_ompl_qdot(const double* a, const double* b)
+{
+	return ((a[3] * b[3] + a[4] * b[4]) + a[5] * b[5]) + a[6] * b[6];
+}
+
+/* SO3StateSpace arcLength: 0 when |dot| > 1 - 1e-9, else acos(|dot|). */
+SE3_HD static inline double se3_ompl_arc(const double* a, const double* b)
+{
+	const double dq = fabs(se3_ompl_qdot(a, b));
+	if (dq > 1.0 - SE3_OMPL_QUAT_EPS)
+		return 0.0;
+	return se3_acos(dq);
+}
+
+/* CompoundStateSpace::distance: RealVectorStateSpace's Euclidean distance
+ * (squares summed in index order) plus the SO(3) arc length. */
+SE3_HD static inline double se3_ompl_distance(const double* a, const double* b)
+{
+	double e = 0.0;
+	for (int i = 0; i < 3; i++) {
+		const double d = a[i] - b[i];
+		e += d * d;
+	}
+	return sqrt(e) + se3_ompl_arc(a, b);
+}
+
+/* getMaximumExtent() * resolution, the longest valid segment after
+ * setStateValidityCheckingResolution(resolution): the bounding box diagonal
+ * (RealVectorStateSpace) plus pi / 2 (SO3StateSpace). */
+SE3_HD static inline double se3_ompl_longest_segment(const double* lo, const double* hi, double resolution)
+{
+	double e = 0.0;
+	for (int i = 0; i < 3; i++) {
+		const double d = hi[i] - lo[i];
+		e += d * d;
+	}
+	double ext = 0.0;
+	ext += 1.0 * sqrt(e);
+	ext += 1.0 * (0.5 * 3.141592653589793);
+	return ext * resolution;
+}
+
+/* One motion of DiscreteMotionValidator::checkMotion: nd =
+ * validSegmentCount = ceil(distance / longest valid segment); the validator
+ * tests the end state and, when nd >= 2, interpolate(from, to, k / nd) for
+ * k = 1 .. nd-1.  The start state is not tested. */
+typedef struct se3_ompl_motion {
+	double a[7], b[7];
+	double theta, inv_sin; /* slerp angle and 1 / sin(theta) */
+	int slerp;             /* theta > DBL_EPSILON */
+	int neg;               /* signed dot < 0: the long-arc case */
+	int64_t nd;            /* -1: not a finite count below 2^31 */
+} se3_ompl_motion;
+
+SE3_HD static inline void se3_ompl_motion_init(se3_ompl_motion* m, const double* from, const double* to, double lvs)
+{
+	for (int k = 0; k < 7; k++) {
+		m->a[k] = from[k];
+		m->b[k] = to[k];
+	}
+	const double c = ceil(se3_ompl_distance(from, to) / lvs);
+	m->nd = (c >= 0.0 && c < 2147483648.0) ? (int64_t)c : -1;
+	m->theta = se3_ompl_arc(from, to);
+	m->slerp = m->theta > SE3_DBL_EPS;
+	m->inv_sin = m->slerp ? 1.0 / se3_sin(m->theta) : 0.0;
+	m->neg = se3_ompl_qdot(from, to) < 0.0;
+}
+
+/* CompoundStateSpace::interpolate: from + (to - from) t for the
+ * translation; SO3StateSpace's slerp (q1 sin((1-t) theta) + q2 sin(t theta))
+ * / sin(theta), with the second weight negated on the long arc, or the
+ * `from` rotation when theta <= DBL_EPSILON. */
+SE3_HD static inline void se3_ompl_interpolate(const se3_ompl_motion* m, double t, double* out)
+{
+	for (int i = 0; i < 3; i++)
+		out[i] = m->a[i] + (m->b[i] - m->a[i]) * t;
+	if (m->slerp) {
+		const double s0 = se3_sin((1.0 - t) * m->theta);
+		double s1 = se3_sin(t * m->theta);
+		if (m->neg)
+			s1 = -s1;
+		for (int k = 3; k < 7; k++)
+			out[k] = (m->a[k] * s0 + m->b[k] * s1) * m->inv_sin;
+	} else {
+		for (int k = 3; k < 7; k++)
+			out[k] = m->a[k];
+	}
+}
+
+/* The k-th state the validator tests (1 <= k <= max(nd, 1)): the end state
+ * itself for the last, interpolate(k / nd) for the others. */
+SE3_HD static inline void se3_ompl_motion_sample(const se3_ompl_motion* m, int64_t k, double* out)
+{
+	const int64_t last = m->nd > 1 ? m->nd : 1;
+	if (k >= last) {
+		for (int i = 0; i < 7; i++)
+			out[i] = m->b[i];
+	} else {
+		se3_ompl_interpolate(m, (double)k / (double)m->nd, out);
+	}
 }
 
 /* distance() of lib/osr/osr_state.cc:130-154: |t0-t1| + angularDistance. */

@@ -719,30 +719,19 @@ int sync_and_check(se3cq_world* w, hipStream_t st, hipError_t le)
 			return fail(SE3CQ_EDEVICE, std::string(#x ": ") + hipGetErrorString(e_)); \
 	} while (0)
 
+void calib_rowmajor(const se3cq_world* w, double* M)
+{
+	for (int i = 0; i < 4; i++)
+		for (int j = 0; j < 4; j++)
+			M[4 * i + j] = (double)w->xf[0][4 * j + i];
+}
+
 void unit_row(const se3cq_world* w, const double* q, double* out)
 {
 	/* translateToUnitState (unit_world.cc:526-539) + applyPertubation (713-724) */
 	double M[16];
-	for (int i = 0; i < 4; i++)
-		for (int j = 0; j < 4; j++)
-			M[4 * i + j] = (double)w->xf[0][4 * j + i];
-	const double t4[4] = {q[0], q[1], q[2], 1.0};
-	double nt[3];
-	for (int r = 0; r < 3; r++)
-		nt[r] = ((M[4 * r] * t4[0] + M[4 * r + 1] * t4[1]) + M[4 * r + 2] * t4[2]) + M[4 * r + 3] * t4[3];
-	double rot[4] = {q[3], q[4], q[5], q[6]};
-	se3_quat_normalize(rot);
-	double prot[4] = {w->pert[3], w->pert[4], w->pert[5], w->pert[6]};
-	se3_quat_normalize(prot);
-	double qret[4];
-	se3_quat_mul(prot, rot, qret);
-	double PR[9];
-	se3_quat_to_rotmat(prot[0], prot[1], prot[2], prot[3], PR);
-	for (int r = 0; r < 3; r++)
-		out[r] = w->pert[r] + ((PR[3 * r] * nt[0] + PR[3 * r + 1] * nt[1]) + PR[3 * r + 2] * nt[2]);
-	se3_quat_normalize(qret);
-	for (int i = 0; i < 4; i++)
-		out[3 + i] = qret[i];
+	calib_rowmajor(w, M);
+	se3_to_unit(M, w->pert, q, out);
 }
 
 std::vector<double> unit_rows(const se3cq_world* w, const double* qs, int64_t n, int is_unit)
@@ -1540,24 +1529,6 @@ int se3cq_translate_from_unit(const se3cq_world* w, const double* q, int64_t n, 
 }
 
 namespace {
-/* Eigen's Quaternion * Vector3 (_transformVector) for a unit quaternion
- * (w, x, y, z): uv = 2 (vec x v); v + w uv + vec x uv. */
-void quat_apply(const double* q, const double* v, double* out)
-{
-	const double u[3] = {q[1], q[2], q[3]};
-	double uv[3], c[3];
-	uv[0] = u[1] * v[2] - u[2] * v[1];
-	uv[1] = u[2] * v[0] - u[0] * v[2];
-	uv[2] = u[0] * v[1] - u[1] * v[0];
-	for (int k = 0; k < 3; k++)
-		uv[k] = uv[k] + uv[k];
-	c[0] = u[1] * uv[2] - u[2] * uv[1];
-	c[1] = u[2] * uv[0] - u[0] * uv[2];
-	c[2] = u[0] * uv[1] - u[1] * uv[0];
-	for (int k = 0; k < 3; k++)
-		out[k] = (v[k] + q[0] * uv[k]) + c[k];
-}
-
 /* (OMPL centre - centre) * scene scale: the robot-frame offset between the
  * OMPL convention and the unit world when enforceRobotCenter moved the
  * centre (unit_world.cc:564-565, 702-703); glm float difference, then double. */
@@ -1574,24 +1545,11 @@ int se3cq_translate_ompl_to_unit(const se3cq_world* w, const double* q, int64_t 
 	CHECK_WORLD(w);
 	if (!w->mesh[1].loaded)
 		return fail(SE3CQ_ESTATE, "robot not loaded");
-	double d[3];
+	double d[3], M[16];
 	ompl_delta(w, d);
-	for (int64_t i = 0; i < n; i++) {
-		const double* s = q + 7 * i;
-		const double wf[7] = {s[0], s[1], s[2], s[6], s[3], s[4], s[5]};
-		double u[7], rot[4], rd[3];
-		unit_row(w, wf, u);
-		for (int k = 0; k < 4; k++)
-			rot[k] = u[3 + k];
-		se3_quat_normalize(rot);
-		quat_apply(rot, d, rd);
-		double* o = out + 7 * i;
-		for (int k = 0; k < 3; k++)
-			o[k] = u[k] - rd[k];
-		se3_quat_normalize(rot);
-		for (int k = 0; k < 4; k++)
-			o[3 + k] = rot[k];
-	}
+	calib_rowmajor(w, M);
+	for (int64_t i = 0; i < n; i++)
+		se3_ompl_to_unit(M, w->pert, d, q + 7 * i, out + 7 * i);
 	return SE3CQ_OK;
 }
 
@@ -1607,7 +1565,7 @@ int se3cq_translate_unit_to_ompl(const se3cq_world* w, const double* q, int64_t 
 		const double* s = q + 7 * i;
 		double rot[4] = {s[3], s[4], s[5], s[6]}, rd[3], c[7], f[7];
 		se3_quat_normalize(rot);
-		quat_apply(rot, d, rd);
+		se3_quat_apply(rot, d, rd);
 		for (int k = 0; k < 3; k++)
 			c[k] = s[k] + rd[k];
 		se3_quat_normalize(rot);
@@ -1655,7 +1613,7 @@ static int vanilla_ompl(const se3cq_world* w, const double* q, int64_t n, double
 		const double* s = q + 7 * i;
 		double rot[4] = {s[6], s[3], s[4], s[5]}, roo[3];
 		se3_quat_normalize(rot);
-		quat_apply(rot, oc, roo);
+		se3_quat_apply(rot, oc, roo);
 		double* o = out + 7 * i;
 		for (int k = 0; k < 3; k++)
 			o[k] = sign > 0 ? s[k] + roo[k] : s[k] - roo[k];
@@ -1708,6 +1666,134 @@ int se3cq_translate_vanilla_pts_to_unit(const se3cq_world* w, int geo, const dou
 		for (int r = 0; r < 3; r++)
 			out[3 * i + r] = ((M[4 * r] * p[0] + M[4 * r + 1] * p[1]) + M[4 * r + 2] * p[2]) + M[4 * r + 3];
 	}
+	return SE3CQ_OK;
+}
+
+/* ---- OMPL-side validity (pyse3ompl) ---- */
+static void ompl_xf(const se3cq_world* w, OmplXf* x)
+{
+	calib_rowmajor(w, x->M);
+	for (int k = 0; k < 7; k++)
+		x->pert[k] = w->pert[k];
+	ompl_delta(w, x->delta);
+}
+
+int se3cq_ompl_longest_valid_segment(const double* lo, const double* hi, double resolution, double* out)
+{
+	/* SpaceInformation::setStateValidityCheckingResolution and
+	 * StateSpace::setup (OMPL 1.4.1), as OmplDriver configures them
+	 * (lib/pyse3ompl/ompldriver.cc:676-685) */
+	if (!lo || !hi || !out)
+		return fail(SE3CQ_EINVAL, "bad arguments");
+	const double eps = 0x1p-52;
+	if (!(resolution >= eps && resolution <= 1.0 - eps))
+		return fail(SE3CQ_EINVAL, "The fraction of the extent must be larger than 0 and less than 1");
+	for (int i = 0; i < 3; i++)
+		if (!(hi[i] >= lo[i]))
+			return fail(SE3CQ_EINVAL, "Lower bound cannot exceed upper bound");
+	const double lvs = se3_ompl_longest_segment(lo, hi, resolution);
+	if (!(lvs >= eps) || !(lvs < 1e300))
+		return fail(SE3CQ_EINVAL, "The longest valid segment must be positive");
+	*out = lvs;
+	return SE3CQ_OK;
+}
+
+static int ompl_prepare(se3cq_world* w)
+{
+	if (!w->ready())
+		return fail(SE3CQ_ESTATE, "models not loaded");
+	std::lock_guard<std::mutex> lk0(w->mu);
+	return ensure_device(w);
+}
+
+int se3cq_ompl_validate_states_device(se3cq_world* w, const double* d_q, int64_t n, int32_t* d_out, void* stream)
+{
+	CHECK_WORLD(w);
+	if (n < 0)
+		return fail(SE3CQ_EINVAL, "negative count");
+	const int rc0 = ompl_prepare(w);
+	if (rc0)
+		return rc0;
+	OmplXf x;
+	ompl_xf(w, &x);
+	const hipError_t e = launch_ompl_states(make_scene(w), x, d_q, n, d_out, pick_stream(w, stream));
+	if (e != hipSuccess)
+		return fail(SE3CQ_EDEVICE, std::string("launch failed: ") + hipGetErrorString(e));
+	return SE3CQ_OK;
+}
+
+int se3cq_ompl_validate_states_host(se3cq_world* w, const double* q, int64_t n, int32_t* out)
+{
+	CHECK_WORLD(w);
+	if (n < 0 || (n > 0 && (!q || !out)))
+		return fail(SE3CQ_EINVAL, "bad arguments");
+	const int rc0 = ompl_prepare(w);
+	if (rc0 || n == 0)
+		return rc0;
+	std::lock_guard<std::mutex> lk(w->mu);
+	double* dq = ws_get<double>(w, 0, 7 * n);
+	int32_t* dout = ws_get<int32_t>(w, 3, n);
+	if (!dq || !dout)
+		return fail(SE3CQ_ENOMEM, "device allocation failed");
+	OmplXf x;
+	ompl_xf(w, &x);
+	HIPCHK(hipMemcpyAsync(dq, q, 7 * n * sizeof(double), hipMemcpyHostToDevice, w->stream));
+	const int rc = sync_and_check(w, w->stream, launch_ompl_states(make_scene(w), x, dq, n, dout, w->stream));
+	if (rc)
+		return rc;
+	HIPCHK(hipMemcpy(out, dout, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost));
+	return SE3CQ_OK;
+}
+
+int se3cq_ompl_validate_motions_device(se3cq_world* w, const double* d_q0, const double* d_q1, int64_t n, double lvs,
+                                       int32_t* d_out, int32_t* d_samples, void* stream)
+{
+	CHECK_WORLD(w);
+	if (n < 0 || !(lvs > 0.0))
+		return fail(SE3CQ_EINVAL, "bad arguments");
+	const int rc0 = ompl_prepare(w);
+	if (rc0)
+		return rc0;
+	OmplXf x;
+	ompl_xf(w, &x);
+	const hipError_t e =
+	    launch_ompl_motions(make_scene(w), x, d_q0, d_q1, n, lvs, d_out, d_samples, pick_stream(w, stream));
+	if (e != hipSuccess)
+		return fail(SE3CQ_EDEVICE, std::string("launch failed: ") + hipGetErrorString(e));
+	return SE3CQ_OK;
+}
+
+int se3cq_ompl_validate_motions_host(se3cq_world* w, const double* q0, const double* q1, int64_t n, double lvs,
+                                     int32_t* out, int32_t* samples)
+{
+	CHECK_WORLD(w);
+	if (n < 0 || !(lvs > 0.0) || (n > 0 && (!q0 || !q1 || !out)))
+		return fail(SE3CQ_EINVAL, "bad arguments");
+	const int rc0 = ompl_prepare(w);
+	if (rc0 || n == 0)
+		return rc0;
+	std::lock_guard<std::mutex> lk(w->mu);
+	double* d0 = ws_get<double>(w, 0, 7 * n);
+	double* d1 = ws_get<double>(w, 6, 7 * n);
+	int32_t* dout = ws_get<int32_t>(w, 3, n);
+	int32_t* dsam = ws_get<int32_t>(w, 2, n);
+	if (!d0 || !d1 || !dout || !dsam)
+		return fail(SE3CQ_ENOMEM, "device allocation failed");
+	OmplXf x;
+	ompl_xf(w, &x);
+	HIPCHK(hipMemcpyAsync(d0, q0, 7 * n * sizeof(double), hipMemcpyHostToDevice, w->stream));
+	HIPCHK(hipMemcpyAsync(d1, q1, 7 * n * sizeof(double), hipMemcpyHostToDevice, w->stream));
+	const int rc =
+	    sync_and_check(w, w->stream, launch_ompl_motions(make_scene(w), x, d0, d1, n, lvs, dout, dsam, w->stream));
+	if (rc)
+		return rc;
+	HIPCHK(hipMemcpy(out, dout, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost));
+	if (samples)
+		HIPCHK(hipMemcpy(samples, dsam, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost));
+	for (int64_t i = 0; i < n; i++)
+		if (out[i] < 0)
+			return fail(SE3CQ_EINVAL, "motion " + std::to_string(i) +
+			                              " needs 2^31 or more segments (or has a non-finite state)");
 	return SE3CQ_OK;
 }
 
