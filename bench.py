@@ -282,6 +282,83 @@ def run_visibility(args, torch, pyosr, W):
     return line
 
 
+ALPHA_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden", "ref")
+# the alpha environment's and solution path's bounding boxes, widened by 10
+ALPHA_LO = np.array([-11.892796, -13.170522, -15.239033])
+ALPHA_HI = np.array([68.73492, 75.37971186, 56.18796844])
+
+
+def run_ompl(args, torch, pyosr, W):
+    """Key-configuration screening (src/GP/pipeline/solve2.py:219): OMPL's
+    discrete motion validator on every pair of np.tril_indices(N) of N free
+    key configurations of the alpha puzzle, at the pipeline's default
+    resolution 1e-4."""
+    from puzzletunneldiscovery_amd import pyse3ompl
+    drv = pyse3ompl.OmplDriver()
+    drv.set_model_file(pyse3ompl.MODEL_PART_ENV, os.path.join(ALPHA_DIR, "alpha_env-1.2.org.obj"))
+    drv.set_model_file(pyse3ompl.MODEL_PART_ROB, os.path.join(ALPHA_DIR, "alpha-1.2.org.obj"))
+    drv.set_bb(ALPHA_LO, ALPHA_HI)
+    drv.set_cdres(1e-4)
+    n = args.segments or 1024
+    path = np.loadtxt(os.path.join(ALPHA_DIR, "alpha-1.2.org.path"))
+    rng = np.random.default_rng(7000)
+    keys = [path]
+    have = len(path)
+    while have < n:
+        q = np.zeros((4 * n, 7))
+        q[:, :3] = rng.uniform(ALPHA_LO + 10, ALPHA_HI - 10, (4 * n, 3))
+        r = rng.standard_normal((4 * n, 4))
+        q[:, 3:] = r / np.linalg.norm(r, axis=1, keepdims=True)
+        q = q[drv.validate_states(q) == 1]
+        keys.append(q)
+        have += len(q)
+    keys = np.ascontiguousarray(np.vstack(keys)[:n])
+    fi, ti = np.tril_indices(n)
+    npairs = len(fi)
+    stream = torch.cuda.Stream()
+    torch.cuda.set_stream(stream)
+    d0 = torch.from_numpy(np.ascontiguousarray(keys[fi])).cuda()
+    d1 = torch.from_numpy(np.ascontiguousarray(keys[ti])).cuda()
+    d_out = torch.zeros(npairs, dtype=torch.int32, device="cuda")
+    d_sam = torch.zeros(npairs, dtype=torch.int32, device="cuda")
+    wall, kern_ms = _Timer(torch, stream).run(
+        lambda: drv.validate_motion_pairs_device(d0.data_ptr(), d1.data_ptr(), npairs, d_out.data_ptr(),
+                                                 d_sam.data_ptr(), stream.cuda_stream), args.steps, args.warmup)
+    drv.unit_world.check_device()
+    out = d_out.cpu().numpy()
+    sam = d_sam.cpu().numpy().astype(np.int64)
+    extra = {"mismatches": None, "cpu_baseline": None, "kernel": "k_ompl_motions", "kernel_ms": round(kern_ms, 4)}
+    if not args.no_cpu:
+        from oracle import oracle as O
+        O.set_libm(False)
+        ow = O.OracleWorld()
+        ow.load_obj(0, os.path.join(ALPHA_DIR, "alpha_env-1.2.org.obj"))
+        ow.load_obj(1, os.path.join(ALPHA_DIR, "alpha-1.2.org.obj"))
+        ow.scale_to_unit()
+        ow.angle_model()
+        perm = np.random.default_rng(7001).permutation(npairs)
+        done, mism, dt = 0, 0, 0.0
+        while dt < args.cpu_seconds and done < npairs:
+            idx = perm[done:done + 2048]
+            t0 = time.perf_counter()
+            c, cs = ow.ompl_validate_motions(keys[fi[idx]], keys[ti[idx]], drv.longest_valid_segment,
+                                             nthreads=_sample_threads())
+            dt += time.perf_counter() - t0
+            mism += int((c != out[idx]).sum() + (cs != sam[idx]).sum())
+            done += len(idx)
+        extra["mismatches"] = mism
+        extra["cpu_baseline"] = dict(value=done / dt, unit="motions/s", cores=_sample_threads(), kind="port",
+                                     sample=f"{done} random pairs of the {npairs} ({dt:.2f} s)")
+    line = _line(f"OMPL key screening: alpha puzzle (reference meshes), {n} free keys, {npairs} motion pairs "
+                 f"(np.tril_indices), resolution 1e-4", npairs * args.steps / wall, wall, args, extra)
+    line["unit"] = "motions/s"
+    line["metric"] = "OMPL discrete motion checks per second"
+    line["config"].update({"keys": n, "pairs": npairs, "valid_pairs": int(out.sum()),
+                           "samples_if_valid": int(sam.sum()),
+                           "longest_valid_segment": drv.longest_valid_segment})
+    return line
+
+
 def _screen_segments(args, torch, W, uw, net, sq, sy, timer, stream):
     """Segments from exact-verified free states, lognormal(5, 1.2) step counts
     at 1/1024: isValidTransition on every segment, against the prescreen
@@ -437,11 +514,13 @@ def main():
                     help="queries per launch (default: the workload's own size)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["touch", "states", "uneven", "visibility", "surrogate"], default="touch",
+    ap.add_argument("--workload", choices=["touch", "states", "uneven", "visibility", "surrogate", "ompl"],
+                    default="touch",
                     help="touch (default, the headline): touch-configuration search on the torus scene; "
                          "states: 1M state validity; uneven: 4M lognormal segments on the shell scene; "
                          "visibility: 4,096-milestone matrix on the shell scene; "
-                         "surrogate: learned prescreen on the shell scene")
+                         "surrogate: learned prescreen on the shell scene; "
+                         "ompl: OMPL key screening (all pairs of 1,024 keys) on the alpha puzzle")
     ap.add_argument("--train-epochs", type=int, default=2)
     ap.add_argument("--threshold", type=float, default=0.5)
     ap.add_argument("--screen-segments", type=int, default=8 * 1024 * 1024,
@@ -470,7 +549,7 @@ def main():
         del src, dst
     if args.workload != "touch":
         fn = {"states": run_states, "uneven": run_uneven, "visibility": run_visibility,
-              "surrogate": run_surrogate}[args.workload]
+              "surrogate": run_surrogate, "ompl": run_ompl}[args.workload]
         print(json.dumps(fn(args, torch, pyosr, W)), flush=True)
         return
     env, rob = W.torus_scene(0)

@@ -1314,10 +1314,13 @@ __global__ void __launch_bounds__(256, SE3CQ_MIN_WAVES) k_visibility(DevScene s,
  * DiscreteMotionValidator::checkMotion on each pair: the motion is valid when
  * the end state and every interior sample interpolate(k / nd), 0 < k < nd,
  * are valid.  The answer does not depend on the order the samples are
- * tested in, so the wave tests them 64 at a time in a strided order (chunk
- * c, lane j -> sample 1 + c + C j with C chunks): the first chunk already
- * spans the whole motion, the way the validator's midpoint-first order
- * does, so colliding motions usually end after one packet traversal.
+ * tested in.  The wave first tests 64 samples spread over the whole motion
+ * (1, 1 + C, 1 + 2C, ... with C = ceil(K / 64)), the way the validator's
+ * midpoint-first order starts, so most colliding motions end after one
+ * packet traversal; then it tests all K samples 64 neighbours at a time,
+ * whose nearly equal poses keep the packet traversal narrow.  (Measured on
+ * the alpha puzzle's key screening: 2.5x faster than a strided order for
+ * every pass, 1.1x faster than neighbours only.)
  * out: 1 valid, 0 invalid, -1 the sample count is not a finite number below
  * 2^31.  samples (optional): max(nd, 1), the states a valid motion costs.
  */
@@ -1365,17 +1368,27 @@ __global__ void __launch_bounds__(256, SE3CQ_MIN_WAVES) k_ompl_motions(DevScene 
 	const int64_t K = nd > 1 ? nd : 1;
 	const int64_t C = (K + 63) / 64;
 	int valid = 1;
-	for (int64_t c = 0; c < C; c++) {
-		const int64_t k = 1 + c + C * lane;
+	if (K > 64) {
+		/* coarse pass: samples 1, 1 + C, 1 + 2C, ... span the motion */
+		const int64_t k = 1 + C * lane;
 		const bool active = k <= K;
 		const uint64_t am = __ballot(active);
 		double st[7], u[7];
 		se3_ompl_motion_sample(m, active ? k : K, st);
 		se3_ompl_to_unit(xsh.M, xsh.pert, xsh.delta, st, u);
-		if (packet_collide<true>(s, u, am)) {
+		if (packet_collide<true>(s, u, am))
 			valid = 0;
-			break;
-		}
+	}
+	/* then every sample, 64 neighbours per pass */
+	for (int64_t c = 0; valid && c < C; c++) {
+		const int64_t k = 1 + 64 * c + lane;
+		const bool active = k <= K;
+		const uint64_t am = __ballot(active);
+		double st[7], u[7];
+		se3_ompl_motion_sample(m, active ? k : K, st);
+		se3_ompl_to_unit(xsh.M, xsh.pert, xsh.delta, st, u);
+		if (packet_collide<true>(s, u, am))
+			valid = 0;
 	}
 	if (lane == 0) {
 		out[p] = valid;
