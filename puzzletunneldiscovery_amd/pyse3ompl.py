@@ -15,8 +15,10 @@ coordinates, k / nd for 0 < k < nd, plus the end state).
 Planning (solve, optimize, presample, graph merging) is not part of this
 engine: those members raise NotImplementedError.
 """
+import configparser
 import ctypes
 import math
+import os
 
 import numpy as np
 
@@ -203,3 +205,55 @@ class OmplDriver:
     get_compact_graph = _no_planning
     get_graph_istate_indices = _no_planning
     get_graph_gstate_indices = _no_planning
+
+
+# ---- OMPL puzzle files -------------------------------------------------------------------
+def _read_xyz(config, prefix):
+    return np.array([config.getfloat("problem", prefix + "." + c) for c in "xyz"], np.float64)
+
+
+def read_puzzle(fn):
+    """An OMPL puzzle .cfg, read as src/GP/pipeline/parse_ompl.py:11-53 does:
+    [problem] world and robot (paths relative to the file), start and goal
+    (x, y, z, axis.x/y/z, theta), volume.min / volume.max, and
+    collision_resolution when present."""
+    config = configparser.ConfigParser()
+    if not config.read([fn]):
+        raise RuntimeError(f"cannot read puzzle file {fn}")
+    d = os.path.dirname(fn)
+    out = {
+        "env_fn": os.path.join(d, config.get("problem", "world")),
+        "rob_fn": os.path.join(d, config.get("problem", "robot")),
+        "volume_min": _read_xyz(config, "volume.min"),
+        "volume_max": _read_xyz(config, "volume.max"),
+        "collision_resolution": config.getfloat("problem", "collision_resolution", fallback=None),
+    }
+    for key in ("start", "goal"):
+        out[key] = (_read_xyz(config, key), _read_xyz(config, key + ".axis"),
+                    config.getfloat("problem", key + ".theta"))
+    return out
+
+
+def create_driver(puzzle, planner_id=PLANNER_RDT, sampler_id=VSTATE_SAMPLER_UNIFORM, saminj="", rdt_k=1,
+                  cdres=None, bvresize=None):
+    """An OmplDriver configured from a puzzle file, as
+    src/GP/pipeline/se3solver.py:35-63 configures the reference's."""
+    cfg = read_puzzle(puzzle)
+    drv = OmplDriver()
+    drv.set_planner(planner_id, sampler_id, saminj, rdt_k)
+    drv.set_model_file(MODEL_PART_ENV, cfg["env_fn"])
+    drv.set_model_file(MODEL_PART_ROB, cfg["rob_fn"])
+    for i, key in ((INIT_STATE, "start"), (GOAL_STATE, "goal")):
+        tr, axis, theta = cfg[key]
+        drv.set_state(i, tr, axis, theta)
+    lo, hi = cfg["volume_min"].copy(), cfg["volume_max"].copy()
+    if bvresize is not None:
+        lo -= bvresize
+        hi += bvresize
+    drv.set_bb(lo, hi)
+    if cdres is None:
+        cdres = cfg["collision_resolution"]
+        if cdres is None:
+            raise RuntimeError(f"{puzzle}: [problem] has no collision_resolution")
+    drv.set_cdres(cdres)
+    return drv

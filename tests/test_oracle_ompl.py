@@ -220,3 +220,38 @@ def test_driver_host_rules(tmp_path):
     with pytest.raises(ValueError):
         d.validate_states(np.zeros((2, 6)))
     assert pyse3ompl.PLANNER_RDT == pyse3ompl.PLANNER_ReRRT == 15
+
+
+def write_alpha_cfg(tmp_path, with_res=True):
+    import shutil
+    for fn in (ENV, ROB):
+        shutil.copy(fn, tmp_path)
+    lines = ["[problem]", "name=Alpha-1.2", f"world={os.path.basename(ENV)}", f"robot={os.path.basename(ROB)}",
+             "start.x=17.97", "start.y=7.23", "start.z=10.2", "start.axis.x=0", "start.axis.y=0", "start.axis.z=1",
+             "start.theta=0", "goal.x=57.3", "goal.y=-2.5", "goal.z=18.4", "goal.axis.x=0.2",
+             "goal.axis.y=0.4", "goal.axis.z=0.8", "goal.theta=1.5"]
+    for k, lo, hi in zip("xyz", LO, HI):
+        lines += [f"volume.min.{k}={float(lo)!r}", f"volume.max.{k}={float(hi)!r}"]
+    if with_res:
+        lines.append("collision_resolution=0.0001")
+    fn = tmp_path / "alpha.cfg"
+    fn.write_text("\n".join(lines) + "\n")
+    return str(fn)
+
+
+def test_puzzle_file(tmp_path):
+    fn = write_alpha_cfg(tmp_path)
+    cfg = pyse3ompl.read_puzzle(fn)
+    assert cfg["env_fn"] == os.path.join(str(tmp_path), os.path.basename(ENV))
+    assert np.array_equal(cfg["volume_min"], LO) and np.array_equal(cfg["volume_max"], HI)
+    assert cfg["collision_resolution"] == 1e-4
+    tr, axis, theta = cfg["goal"]
+    assert np.array_equal(tr, [57.3, -2.5, 18.4]) and np.array_equal(axis, [0.2, 0.4, 0.8]) and theta == 1.5
+    d = pyse3ompl.create_driver(fn)
+    assert d.longest_valid_segment == pyse3ompl.longest_valid_segment(LO, HI, 1e-4)
+    d = pyse3ompl.create_driver(fn, cdres=0.01, bvresize=5.0)
+    assert d.longest_valid_segment == pyse3ompl.longest_valid_segment(LO - 5, HI + 5, 0.01)
+    with pytest.raises(RuntimeError):
+        pyse3ompl.create_driver(write_alpha_cfg(tmp_path, with_res=False))
+    with pytest.raises(RuntimeError):
+        pyse3ompl.read_puzzle(str(tmp_path / "missing.cfg"))
