@@ -185,6 +185,38 @@ int se3cq_translate_ompl_to_vanilla(const se3cq_world* w, const double* q, int64
 int se3cq_translate_vanilla_to_unit(const se3cq_world* w, const double* q, int64_t n, double* out);
 int se3cq_translate_vanilla_pts_to_unit(const se3cq_world* w, int geo, const double* pts, int64_t n, double* out);
 
+/* Mixed batches in one launch (no single reference counterpart: they
+ * replace the samplers' and touch search's loops of isValid and
+ * transitStateToWithContact calls, e.g. src/GP/uw_random.py:46-52 and
+ * src/GP/touchq_util.py:8-27, with one device call).  Segments follow
+ * se3cq_transit_* (unit_world.cc:405-474; outputs as there, last_free/state
+ * optional, last_tau/tau/steps optional), states follow
+ * se3cq_valid_states_* (valid[i] = 1 if free).  Persistent wavefronts take
+ * segments and chunks of states from a work queue; the device form is one
+ * fill kernel plus one kernel on the stream, so it can be captured in a HIP
+ * graph.  Calls that overlap on one world race on its queue word.
+ * se3cq_batch_tune times every launch shape on the given batch, keeps the
+ * fastest for the world's later batch calls and leaves the batch's results
+ * in the outputs; *ms = its time per launch. */
+typedef struct se3cq_batch_config {
+	int32_t block_waves;     /* waves per block (1-4; 0 = as many as the LDS allows) */
+	int32_t blocks_per_cu;   /* resident blocks per CU to launch (0 = as many as fit) */
+	int32_t states_per_lane; /* states one lane checks per state work item (>= 1) */
+} se3cq_batch_config;
+int se3cq_batch_device(se3cq_world* w, const double* d_seg_from, const double* d_seg_to, int64_t n_seg,
+                       double verify_delta, double* d_last_free, double* d_state, uint8_t* d_done, double* d_last_tau,
+                       double* d_tau, int32_t* d_steps, const double* d_states, int64_t n_states, uint8_t* d_valid,
+                       void* stream);
+int se3cq_batch_host(se3cq_world* w, const double* seg_from, const double* seg_to, int64_t n_seg, double verify_delta,
+                     double* last_free, double* state, uint8_t* done, double* last_tau, double* tau, int32_t* steps,
+                     const double* states, int64_t n_states, uint8_t* valid);
+int se3cq_batch_tune(se3cq_world* w, const double* d_seg_from, const double* d_seg_to, int64_t n_seg,
+                     double verify_delta, double* d_last_free, double* d_state, uint8_t* d_done, double* d_last_tau,
+                     double* d_tau, int32_t* d_steps, const double* d_states, int64_t n_states, uint8_t* d_valid,
+                     int32_t repeats, se3cq_batch_config* chosen, float* ms);
+int se3cq_batch_get_config(const se3cq_world* w, se3cq_batch_config* cfg);
+int se3cq_batch_set_config(se3cq_world* w, const se3cq_batch_config* cfg);
+
 /* OMPL-side validity, the two queries of pyse3ompl's OmplDriver that do not
  * plan (lib/pyse3ompl/pyse3ompl.cc:77-81), over this world's geometry.
  * States are OMPL states (x, y, z, qx, qy, qz, qw) with unit quaternions.

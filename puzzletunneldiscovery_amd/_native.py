@@ -18,6 +18,12 @@ c_i8_p = ctypes.POINTER(ctypes.c_int8)
 c_i32_p = ctypes.POINTER(ctypes.c_int32)
 c_i64_p = ctypes.POINTER(ctypes.c_int64)
 vp = ctypes.c_void_p
+
+
+class BatchConfig(ctypes.Structure):
+    """se3cq_batch_config."""
+    _fields_ = [("block_waves", ctypes.c_int32), ("blocks_per_cu", ctypes.c_int32),
+                ("states_per_lane", ctypes.c_int32)]
 i64 = ctypes.c_int64
 cint = ctypes.c_int
 cdbl = ctypes.c_double
@@ -74,6 +80,14 @@ SIGNATURES = {
     "se3cq_translate_ompl_to_vanilla": [vp, c_double_p, i64, c_double_p],
     "se3cq_translate_vanilla_to_unit": [vp, c_double_p, i64, c_double_p],
     "se3cq_translate_vanilla_pts_to_unit": [vp, cint, c_double_p, i64, c_double_p],
+    "se3cq_batch_device": [vp, vp, vp, ctypes.c_int64, ctypes.c_double, vp, vp, vp, vp, vp, vp, vp, ctypes.c_int64,
+                           vp, vp],
+    "se3cq_batch_host": [vp, c_double_p, c_double_p, ctypes.c_int64, ctypes.c_double, c_double_p, c_double_p, c_u8_p,
+                         c_double_p, c_double_p, c_i32_p, c_double_p, ctypes.c_int64, c_u8_p],
+    "se3cq_batch_tune": [vp, vp, vp, ctypes.c_int64, ctypes.c_double, vp, vp, vp, vp, vp, vp, vp, ctypes.c_int64, vp,
+                         ctypes.c_int32, ctypes.POINTER(BatchConfig), ctypes.POINTER(ctypes.c_float)],
+    "se3cq_batch_get_config": [vp, ctypes.POINTER(BatchConfig)],
+    "se3cq_batch_set_config": [vp, ctypes.POINTER(BatchConfig)],
     "se3cq_ompl_longest_valid_segment": [c_double_p, c_double_p, ctypes.c_double, c_double_p],
     "se3cq_ompl_validate_states_host": [vp, c_double_p, i64, c_i32_p],
     "se3cq_ompl_validate_states_device": [vp, vp, i64, vp, vp],

@@ -74,7 +74,7 @@ struct DevScene {
 	int32_t pert_identity; /* perturbation is exactly [0,0,0,1,0,0,0] */
 	int32_t stack_depth;   /* per-thread traversal stack entries */
 	int32_t wide;          /* 1 if node ids need more than 16 bits */
-	uint32_t* error_flag;  /* device word, non-zero after a traversal overflow */
+	uint32_t* error_flag;  /* device word: bit 0 traversal overflow, bit 1 a segment with 2^31+ steps */
 };
 
 } // namespace se3cq

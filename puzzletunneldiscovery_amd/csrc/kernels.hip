@@ -705,6 +705,7 @@ struct SegShared {
 	double dist, inv_dist, vd;
 	int exact; /* every partial sum k*vd of the step sequence is exact */
 	int early;
+	int bad;   /* not an early return, yet 2^31 or more steps (or none) */
 };
 
 /* Wave-level check of one pose per lane (lanes outside `live` are ignored):
@@ -1092,6 +1093,10 @@ __device__ __forceinline__ void seg_setup(SegShared* sh, const double* __restric
 		sh->vd = vd;
 		sh->early = vd >= dist;
 		sh->exact = steps_exact(vd, dist);
+		/* a NaN distance never enters the stepping loop (the reference
+		 * returns "free"); an infinite one, a step that is not positive,
+		 * or 2^31 or more steps would never end */
+		sh->bad = (dist == dist) && !(vd >= dist) && !(dist / vd < 2147483648.0 && vd > 0.0);
 	}
 }
 
@@ -1102,6 +1107,20 @@ __device__ SegResult segment_walk(const DevScene& s, const SegShared* sh, int le
 	SegResult r;
 	r.early = 0;
 	r.lo_is_from = 0;
+	if (sh->bad) {
+		/* the walk would not end: flag it and report a collision at the
+		 * start, which the host turns into an error */
+		if (lane == 0)
+			atomicOr(s.error_flag, 2u);
+		r.done = 0;
+		r.lo_is_from = 1;
+		r.steps = 0;
+		r.last_tau = 0.0;
+		r.tau = 0.0;
+		r.lo = 0.0;
+		r.hi = 0.0;
+		return r;
+	}
 	if (sh->early) {
 		r.done = 0;
 		r.early = 1;
@@ -1184,24 +1203,13 @@ __device__ SegResult segment_walk(const DevScene& s, const SegShared* sh, int le
 	}
 }
 
-template <bool WIDE, int MODE>
-__global__ void __launch_bounds__(256, SE3CQ_MIN_WAVES) k_segments(DevScene s, const double* __restrict__ from_all,
-                                                  const double* __restrict__ to_all, int64_t n, double vd,
-                                                  int levels, SegOut o)
+/* One segment's outputs: lane 0 stores the scalars, lanes 0-6 the two
+ * returned states, rebuilt from the segment record (the interpolation is
+ * deterministic, so they equal the states the walk evaluated). */
+__device__ __forceinline__ void seg_store(const SegOut& o, int64_t seg, const SegResult& r, const SegShared* sh,
+                                          const double* from, const double* to)
 {
-	extern __shared__ double seg_lds[];
-	__shared__ SegShared segsh[4];
-	const int wave = threadIdx.x >> 6;
 	const int lane = threadIdx.x & 63;
-	const int64_t seg = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave;
-	SegShared* sh = &segsh[wave];
-	const int64_t sv = seg < n ? seg : n - 1;
-	seg_setup<MODE>(sh, from_all + 7 * sv, to_all + 7 * sv, vd);
-	__syncthreads();
-	if (seg >= n)
-		return;
-	BisectShared* bs = (MODE == MODE_TOUCH) ? reinterpret_cast<BisectShared*>(seg_lds) + wave : nullptr;
-	const SegResult r = segment_walk<WIDE, MODE>(s, sh, levels, bs);
 	if (lane == 0) {
 		o.done[seg] = (uint8_t)r.done;
 		if (o.last_tau) o.last_tau[seg] = r.last_tau;
@@ -1211,10 +1219,6 @@ __global__ void __launch_bounds__(256, SE3CQ_MIN_WAVES) k_segments(DevScene s, c
 		if (o.steps) o.steps[seg] = r.steps;
 	}
 	if (o.last_free && lane < 7) {
-		/* reconstruct the returned states; interpolation is deterministic so
-		 * these equal the states the walk evaluated */
-		const double* from = from_all + 7 * seg;
-		const double* to = to_all + 7 * seg;
 		double a[7], b[7];
 		if (r.lo_is_from) {
 			for (int k = 0; k < 7; k++) a[k] = from[k];
@@ -1236,6 +1240,82 @@ __global__ void __launch_bounds__(256, SE3CQ_MIN_WAVES) k_segments(DevScene s, c
 			}
 		o.last_free[7 * seg + lane] = va;
 		o.state[7 * seg + lane] = vb;
+	}
+}
+
+template <bool WIDE, int MODE>
+__global__ void __launch_bounds__(256, SE3CQ_MIN_WAVES) k_segments(DevScene s, const double* __restrict__ from_all,
+                                                  const double* __restrict__ to_all, int64_t n, double vd,
+                                                  int levels, SegOut o)
+{
+	extern __shared__ double seg_lds[];
+	__shared__ SegShared segsh[4];
+	const int wave = threadIdx.x >> 6;
+	const int lane = threadIdx.x & 63;
+	const int64_t seg = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave;
+	SegShared* sh = &segsh[wave];
+	const int64_t sv = seg < n ? seg : n - 1;
+	seg_setup<MODE>(sh, from_all + 7 * sv, to_all + 7 * sv, vd);
+	__syncthreads();
+	if (seg >= n)
+		return;
+	BisectShared* bs = (MODE == MODE_TOUCH) ? reinterpret_cast<BisectShared*>(seg_lds) + wave : nullptr;
+	const SegResult r = segment_walk<WIDE, MODE>(s, sh, levels, bs);
+	seg_store(o, seg, r, sh, from_all + 7 * seg, to_all + 7 * seg);
+}
+
+/* ---- mixed batches: persistent wavefronts over a work queue ---- */
+/*
+ * One launch answers a batch of uneven segments (transitStateToWithContact,
+ * lib/osr/unit_world.cc:405-474, at a fixed verify step) and of single
+ * states (isValid, 215-247).  The grid holds only as many wavefronts as the
+ * device keeps resident; each takes work items from a 64-bit counter
+ * (zeroed by a fill kernel on the same stream, so the pair of launches can
+ * be captured in a HIP graph) until the queue is empty.  Items
+ * [0, n_seg) are segments, one per wavefront, walked as k_segments walks
+ * them; the rest are chunks of 64 * states_per_lane states, one per lane
+ * with the per-thread traversal of k_valid_states.  A long segment occupies
+ * one wavefront while the others drain the queue, so uneven lengths do not
+ * idle whole blocks.
+ */
+template <bool WIDE>
+__global__ void __launch_bounds__(256, SE3CQ_MIN_WAVES) k_batch(DevScene s, BatchArgs a)
+{
+	extern __shared__ uint32_t smem[];
+	__shared__ SegShared segsh[4];
+	const int wave = threadIdx.x >> 6;
+	const int lane = threadIdx.x & 63;
+	SegShared* sh = &segsh[wave];
+	PairStack<WIDE> stk, leaves;
+	thread_stacks<WIDE>(s, smem, stk, leaves);
+	const int64_t per_item = 64 * (int64_t)a.states_per_lane;
+	const int64_t total = a.n_seg + (a.n_states + per_item - 1) / per_item;
+	for (;;) {
+		unsigned long long got = 0;
+		if (lane == 0)
+			got = atomicAdd(a.queue, 1ull);
+		const int64_t item = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(got >> 32)) << 32) |
+		                               (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)got));
+		if (item >= total)
+			break;
+		if (item < a.n_seg) {
+			const double* from = a.seg_from + 7 * item;
+			const double* to = a.seg_to + 7 * item;
+			seg_setup<MODE_TRANSIT>(sh, from, to, a.vd);
+			wave_lds_sync();
+			const SegResult r = segment_walk<WIDE, MODE_TRANSIT>(s, sh, 0, nullptr);
+			seg_store(a.seg, item, r, sh, from, to);
+			/* every lane is done with the record before the next item
+			 * overwrites it */
+			wave_lds_sync();
+		} else {
+			const int64_t base = (item - a.n_seg) * per_item;
+			for (int j = 0; j < a.states_per_lane; j++) {
+				const int64_t i = base + 64 * j + lane;
+				if (i < a.n_states)
+					a.valid[i] = collide_dfs<WIDE>(s, StateFromGlobal{a.states + 7 * i}, stk, leaves) ? 0 : 1;
+			}
+		}
 	}
 }
 
@@ -1659,6 +1739,35 @@ hipError_t launch_ompl_motions(const DevScene& s, const OmplXf& x, const double*
 		hipLaunchKernelGGL(k_ompl_motions<false>, dim3((unsigned)grid), dim3(block), 0, st, s, x, q0, q1, n, lvs,
 		                   out, samples);
 	return hipGetLastError();
+}
+
+hipError_t launch_batch(const DevScene& s, const BatchArgs& a, int block, int grid, hipStream_t st)
+{
+	if (a.n_seg <= 0 && a.n_states <= 0)
+		return hipSuccess;
+	if (block < 64 || block > 256 || block % 64 != 0 || grid <= 0 || a.states_per_lane < 1)
+		return hipErrorInvalidValue;
+	hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, reinterpret_cast<uint32_t*>(a.queue), (int64_t)2, 0u);
+	hipError_t e = hipGetLastError();
+	if (e != hipSuccess)
+		return e;
+	const size_t lds = stack_bytes(s, block);
+	if (s.wide)
+		hipLaunchKernelGGL(k_batch<true>, dim3((unsigned)grid), dim3(block), lds, st, s, a);
+	else
+		hipLaunchKernelGGL(k_batch<false>, dim3((unsigned)grid), dim3(block), lds, st, s, a);
+	return hipGetLastError();
+}
+
+int batch_default_block(const DevScene& s) { return pick_block(s); }
+
+int batch_blocks_per_cu(const DevScene& s, int block)
+{
+	int nb = 0;
+	const size_t lds = stack_bytes(s, block);
+	const hipError_t e = s.wide ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch<true>, block, lds)
+	                            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch<false>, block, lds);
+	return (e == hipSuccess && nb > 0) ? nb : 1;
 }
 
 template <int MODE>

@@ -36,6 +36,27 @@ hipError_t launch_visibility(const DevScene& s, int vis, const double* qs0, cons
 hipError_t launch_contacts(const DevScene& s, const double* q, int64_t n, int cap, int32_t* pairs, int32_t* count,
                            double* segs, int8_t* flags, hipStream_t st);
 
+/* A mixed batch for k_batch: segments under transitStateToWithContact
+ * semantics (outputs as SegOut, lo/hi unused) and single states (valid: 1 =
+ * free).  queue: a device word pair the launch zeroes first. */
+struct BatchArgs {
+	const double* seg_from;
+	const double* seg_to;
+	int64_t n_seg;
+	double vd;
+	SegOut seg;
+	const double* states;
+	int64_t n_states;
+	uint8_t* valid;
+	unsigned long long* queue;
+	int states_per_lane;
+};
+hipError_t launch_batch(const DevScene& s, const BatchArgs& a, int block, int grid, hipStream_t st);
+/* Largest block whose traversal stacks fit in LDS, and how many such blocks
+ * one CU keeps resident. */
+int batch_default_block(const DevScene& s);
+int batch_blocks_per_cu(const DevScene& s, int block);
+
 /* OMPL-convention validity (OmplDriver::validateStates / validateMotionPairs):
  * out 1 valid, 0 invalid; motions: -1 when the sample count is not finite
  * below 2^31, samples = max(nd, 1) (may be null). */

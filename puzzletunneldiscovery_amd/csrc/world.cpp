@@ -109,6 +109,9 @@ struct se3cq_world {
 	DevBuf d_nodes[2], d_tris[2];
 	DevBuf d_err;
 	DevBuf d_pert;
+	DevBuf d_queue; /* work counter of k_batch */
+	int num_cu = 0;
+	se3cq_batch_config batch_cfg = {0, 0, 1};
 	bool pert_uploaded = false;
 	DevBuf ws[12];
 	std::mutex mu;
@@ -128,6 +131,7 @@ struct se3cq_world {
 		}
 		d_err.release();
 		d_pert.release();
+		d_queue.release();
 		pert_uploaded = false;
 		for (auto& b : ws)
 			b.release();
@@ -649,8 +653,11 @@ int upload(se3cq_world* w)
 	}
 	if (!w->d_err.reserve(sizeof(uint32_t)) || hipMemset(w->d_err.p, 0, sizeof(uint32_t)) != hipSuccess)
 		return fail(SE3CQ_EDEVICE, "device allocation failed");
-	if (!w->d_pert.reserve(sizeof(PertData)))
+	if (!w->d_pert.reserve(sizeof(PertData)) || !w->d_queue.reserve(2 * sizeof(unsigned long long)))
 		return fail(SE3CQ_EDEVICE, "device allocation failed");
+	if (hipDeviceGetAttribute(&w->num_cu, hipDeviceAttributeMultiprocessorCount, w->device) != hipSuccess ||
+	    w->num_cu <= 0)
+		return fail(SE3CQ_EDEVICE, "cannot read the device's CU count");
 	{
 		const int rc = upload_pert(w);
 		if (rc)
@@ -681,12 +688,20 @@ int check_device(se3cq_world* w)
 	uint32_t flag = 0;
 	if (hipMemcpy(&flag, w->d_err.p, sizeof(flag), hipMemcpyDeviceToHost) != hipSuccess)
 		return fail(SE3CQ_EDEVICE, "device error check failed");
-	if (flag)
+	if (flag & 1u)
 		return fail(SE3CQ_EDEVICE, "traversal stack overflow on the device");
+	if (flag & 2u)
+		return fail(SE3CQ_EDEVICE, "a segment needs 2^31 or more verify steps (distance / verify step)");
+	if (flag)
+		return fail(SE3CQ_EDEVICE, "device error flag set");
 	return SE3CQ_OK;
 }
 
 hipStream_t pick_stream(se3cq_world* w, void* s) { return s ? (hipStream_t)s : w->stream; }
+
+/* A verify step that is not positive and finite never advances the walk
+ * (the reference loops forever on it); such calls are rejected. */
+bool check_step(double vd) { return vd > 0.0 && std::isfinite(vd); }
 
 template <typename T>
 T* ws_get(se3cq_world* w, int slot, size_t count)
@@ -1175,6 +1190,8 @@ int se3cq_transit_device(se3cq_world* w, const double* d_from, const double* d_t
                          int32_t* d_steps, void* stream)
 {
 	CHECK_WORLD(w);
+	if (!check_step(verify_delta))
+		return fail(SE3CQ_EINVAL, "the verify step must be positive and finite");
 	if (!w->ready())
 		return fail(SE3CQ_ESTATE, "models not loaded");
 	{
@@ -1194,6 +1211,8 @@ int se3cq_transit_host(se3cq_world* w, const double* from, const double* to, int
                        int32_t* steps)
 {
 	CHECK_WORLD(w);
+	if (!check_step(verify_delta))
+		return fail(SE3CQ_EINVAL, "the verify step must be positive and finite");
 	if (!w->ready())
 		return fail(SE3CQ_ESTATE, "models not loaded");
 	{
@@ -1235,6 +1254,8 @@ int se3cq_valid_transitions_host(se3cq_world* w, const double* from, const doubl
                                  double initial_verify_delta, uint8_t* out)
 {
 	CHECK_WORLD(w);
+	if (!check_step(initial_verify_delta))
+		return fail(SE3CQ_EINVAL, "the verify step must be positive and finite");
 	if (!w->ready())
 		return fail(SE3CQ_ESTATE, "models not loaded");
 	{
@@ -1266,6 +1287,8 @@ int se3cq_valid_transitions_device(se3cq_world* w, const double* d_from, const d
                                    double initial_verify_delta, uint8_t* d_out, int32_t* d_steps, void* stream)
 {
 	CHECK_WORLD(w);
+	if (!check_step(initial_verify_delta))
+		return fail(SE3CQ_EINVAL, "the verify step must be positive and finite");
 	if (!w->ready())
 		return fail(SE3CQ_ESTATE, "models not loaded");
 	if (n < 0)
@@ -1285,6 +1308,8 @@ int se3cq_touch_search_device(se3cq_world* w, const double* d_from, const double
                               double* d_lo, double* d_hi, int32_t* d_steps, void* stream)
 {
 	CHECK_WORLD(w);
+	if (!check_step(verify_delta))
+		return fail(SE3CQ_EINVAL, "the verify step must be positive and finite");
 	if (!w->ready())
 		return fail(SE3CQ_ESTATE, "models not loaded");
 	{
@@ -1306,6 +1331,8 @@ int se3cq_touch_search_host(se3cq_world* w, const double* from, const double* to
                             int32_t* steps)
 {
 	CHECK_WORLD(w);
+	if (!check_step(verify_delta))
+		return fail(SE3CQ_EINVAL, "the verify step must be positive and finite");
 	if (!w->ready())
 		return fail(SE3CQ_ESTATE, "models not loaded");
 	{
@@ -1409,6 +1436,8 @@ int se3cq_visibility_matrix_host(se3cq_world* w, const double* qs, int64_t n, in
                                  double verify_magnitude, int8_t* out)
 {
 	CHECK_WORLD(w);
+	if (!check_step(verify_magnitude))
+		return fail(SE3CQ_EINVAL, "the verify step must be positive and finite");
 	if (!w->ready())
 		return fail(SE3CQ_ESTATE, "models not loaded");
 	{
@@ -1427,6 +1456,8 @@ int se3cq_visibility_matrix2_host(se3cq_world* w, const double* qs0, int64_t m, 
                                   int64_t n, int qs1_are_unit, double verify_magnitude, int8_t* out)
 {
 	CHECK_WORLD(w);
+	if (!check_step(verify_magnitude))
+		return fail(SE3CQ_EINVAL, "the verify step must be positive and finite");
 	if (!w->ready())
 		return fail(SE3CQ_ESTATE, "models not loaded");
 	{
@@ -1448,6 +1479,8 @@ int se3cq_visibility_pair_host(se3cq_world* w, const double* qs0, int qs0_are_un
                                int qs1_are_unit, int64_t n, double verify_magnitude, int8_t* out)
 {
 	CHECK_WORLD(w);
+	if (!check_step(verify_magnitude))
+		return fail(SE3CQ_EINVAL, "the verify step must be positive and finite");
 	if (!w->ready())
 		return fail(SE3CQ_ESTATE, "models not loaded");
 	{
@@ -1467,6 +1500,8 @@ int se3cq_visibility_bits_device(se3cq_world* w, const double* d_qs, int64_t n, 
                                  uint32_t* d_bits, int32_t* d_steps, void* stream)
 {
 	CHECK_WORLD(w);
+	if (!check_step(verify_magnitude))
+		return fail(SE3CQ_EINVAL, "the verify step must be positive and finite");
 	if (!w->ready())
 		return fail(SE3CQ_ESTATE, "models not loaded");
 	{
@@ -1666,6 +1701,229 @@ int se3cq_translate_vanilla_pts_to_unit(const se3cq_world* w, int geo, const dou
 		for (int r = 0; r < 3; r++)
 			out[3 * i + r] = ((M[4 * r] * p[0] + M[4 * r + 1] * p[1]) + M[4 * r + 2] * p[2]) + M[4 * r + 3];
 	}
+	return SE3CQ_OK;
+}
+
+/* ---- mixed batches (persistent kernel) ---- */
+static int batch_shape(se3cq_world* w, const DevScene& s, const se3cq_batch_config& c, int* block, int* grid)
+{
+	const int bmax = batch_default_block(s);
+	int b = c.block_waves > 0 ? 64 * c.block_waves : bmax;
+	if (b > bmax)
+		b = bmax;
+	const int per_cu_max = batch_blocks_per_cu(s, b);
+	const int per_cu = (c.blocks_per_cu > 0 && c.blocks_per_cu < per_cu_max) ? c.blocks_per_cu : per_cu_max;
+	*block = b;
+	*grid = w->num_cu * per_cu;
+	return SE3CQ_OK;
+}
+
+static int batch_check(se3cq_world* w, int64_t n_seg, double vd, const double* d_last_free, const double* d_state,
+                       const uint8_t* d_done, int64_t n_states, const uint8_t* d_valid)
+{
+	if (n_seg < 0 || n_states < 0)
+		return fail(SE3CQ_EINVAL, "negative count");
+	if (n_seg > 0 && !check_step(vd))
+		return fail(SE3CQ_EINVAL, "the verify step must be positive and finite");
+	if ((d_last_free == nullptr) != (d_state == nullptr))
+		return fail(SE3CQ_EINVAL, "last_free and state must both be given or both be null");
+	if ((n_seg > 0 && !d_done) || (n_states > 0 && !d_valid))
+		return fail(SE3CQ_EINVAL, "missing output");
+	if (!w->ready())
+		return fail(SE3CQ_ESTATE, "models not loaded");
+	std::lock_guard<std::mutex> lk0(w->mu);
+	return ensure_device(w);
+}
+
+static BatchArgs batch_args(se3cq_world* w, const double* d_seg_from, const double* d_seg_to, int64_t n_seg,
+                            double vd, double* d_last_free, double* d_state, uint8_t* d_done, double* d_last_tau,
+                            double* d_tau, int32_t* d_steps, const double* d_states, int64_t n_states,
+                            uint8_t* d_valid, int states_per_lane)
+{
+	BatchArgs a;
+	a.seg_from = d_seg_from;
+	a.seg_to = d_seg_to;
+	a.n_seg = n_seg;
+	a.vd = vd;
+	a.seg = SegOut{d_done, d_last_tau, d_tau, d_last_free, d_state, nullptr, nullptr, d_steps};
+	a.states = d_states;
+	a.n_states = n_states;
+	a.valid = d_valid;
+	a.queue = (unsigned long long*)w->d_queue.p;
+	a.states_per_lane = states_per_lane > 0 ? states_per_lane : 1;
+	return a;
+}
+
+static int batch_launch(se3cq_world* w, const BatchArgs& a, const se3cq_batch_config& c, hipStream_t st)
+{
+	const DevScene s = make_scene(w);
+	int block = 0, grid = 0;
+	batch_shape(w, s, c, &block, &grid);
+	const hipError_t e = launch_batch(s, a, block, grid, st);
+	if (e != hipSuccess)
+		return fail(SE3CQ_EDEVICE, std::string("launch failed: ") + hipGetErrorString(e));
+	return SE3CQ_OK;
+}
+
+int se3cq_batch_device(se3cq_world* w, const double* d_seg_from, const double* d_seg_to, int64_t n_seg,
+                       double verify_delta, double* d_last_free, double* d_state, uint8_t* d_done, double* d_last_tau,
+                       double* d_tau, int32_t* d_steps, const double* d_states, int64_t n_states, uint8_t* d_valid,
+                       void* stream)
+{
+	CHECK_WORLD(w);
+	const int rc = batch_check(w, n_seg, verify_delta, d_last_free, d_state, d_done, n_states, d_valid);
+	if (rc)
+		return rc;
+	const BatchArgs a = batch_args(w, d_seg_from, d_seg_to, n_seg, verify_delta, d_last_free, d_state, d_done,
+	                               d_last_tau, d_tau, d_steps, d_states, n_states, d_valid, w->batch_cfg.states_per_lane);
+	return batch_launch(w, a, w->batch_cfg, pick_stream(w, stream));
+}
+
+int se3cq_batch_host(se3cq_world* w, const double* seg_from, const double* seg_to, int64_t n_seg, double verify_delta,
+                     double* last_free, double* state, uint8_t* done, double* last_tau, double* tau, int32_t* steps,
+                     const double* states, int64_t n_states, uint8_t* valid)
+{
+	CHECK_WORLD(w);
+	if ((n_seg > 0 && (!seg_from || !seg_to)) || (n_states > 0 && !states))
+		return fail(SE3CQ_EINVAL, "missing input");
+	const int rc0 = batch_check(w, n_seg, verify_delta, last_free, state, done, n_states, valid);
+	if (rc0 || (n_seg == 0 && n_states == 0))
+		return rc0;
+	std::lock_guard<std::mutex> lk(w->mu);
+	const size_t ns = (size_t)n_seg, nq = (size_t)n_states;
+	double* df = ws_get<double>(w, 0, 7 * ns);
+	double* dt = ws_get<double>(w, 2, 7 * ns);
+	double* dlf = last_free ? ws_get<double>(w, 3, 7 * ns) : nullptr;
+	double* dst = last_free ? ws_get<double>(w, 4, 7 * ns) : nullptr;
+	uint8_t* dd = ws_get<uint8_t>(w, 1, ns);
+	double* dlt = ws_get<double>(w, 5, ns);
+	double* dta = ws_get<double>(w, 6, ns);
+	int32_t* dsp = ws_get<int32_t>(w, 7, ns);
+	double* dq = ws_get<double>(w, 8, 7 * nq);
+	uint8_t* dv = ws_get<uint8_t>(w, 9, nq);
+	if (!df || !dt || (last_free && (!dlf || !dst)) || !dd || !dlt || !dta || !dsp || !dq || !dv)
+		return fail(SE3CQ_ENOMEM, "device allocation failed");
+	if (ns) {
+		HIPCHK(hipMemcpyAsync(df, seg_from, 7 * ns * sizeof(double), hipMemcpyHostToDevice, w->stream));
+		HIPCHK(hipMemcpyAsync(dt, seg_to, 7 * ns * sizeof(double), hipMemcpyHostToDevice, w->stream));
+	}
+	if (nq)
+		HIPCHK(hipMemcpyAsync(dq, states, 7 * nq * sizeof(double), hipMemcpyHostToDevice, w->stream));
+	const BatchArgs a = batch_args(w, df, dt, n_seg, verify_delta, dlf, dst, dd, dlt, dta, dsp, dq, n_states, dv,
+	                               w->batch_cfg.states_per_lane);
+	const DevScene s = make_scene(w);
+	int block = 0, grid = 0;
+	batch_shape(w, s, w->batch_cfg, &block, &grid);
+	const int rc = sync_and_check(w, w->stream, launch_batch(s, a, block, grid, w->stream));
+	if (rc)
+		return rc;
+	if (ns) {
+		HIPCHK(hipMemcpy(done, dd, ns, hipMemcpyDeviceToHost));
+		if (last_tau) HIPCHK(hipMemcpy(last_tau, dlt, ns * sizeof(double), hipMemcpyDeviceToHost));
+		if (tau) HIPCHK(hipMemcpy(tau, dta, ns * sizeof(double), hipMemcpyDeviceToHost));
+		if (steps) HIPCHK(hipMemcpy(steps, dsp, ns * sizeof(int32_t), hipMemcpyDeviceToHost));
+		if (last_free) {
+			HIPCHK(hipMemcpy(last_free, dlf, 7 * ns * sizeof(double), hipMemcpyDeviceToHost));
+			HIPCHK(hipMemcpy(state, dst, 7 * ns * sizeof(double), hipMemcpyDeviceToHost));
+		}
+	}
+	if (nq)
+		HIPCHK(hipMemcpy(valid, dv, nq, hipMemcpyDeviceToHost));
+	return SE3CQ_OK;
+}
+
+int se3cq_batch_get_config(const se3cq_world* w, se3cq_batch_config* cfg)
+{
+	CHECK_WORLD(w);
+	if (!cfg)
+		return fail(SE3CQ_EINVAL, "bad arguments");
+	*cfg = w->batch_cfg;
+	return SE3CQ_OK;
+}
+
+int se3cq_batch_set_config(se3cq_world* w, const se3cq_batch_config* cfg)
+{
+	CHECK_WORLD(w);
+	if (!cfg || cfg->block_waves < 0 || cfg->block_waves > 4 || cfg->blocks_per_cu < 0 || cfg->states_per_lane < 1 ||
+	    cfg->states_per_lane > 64)
+		return fail(SE3CQ_EINVAL, "bad batch configuration");
+	w->batch_cfg = *cfg;
+	return SE3CQ_OK;
+}
+
+int se3cq_batch_tune(se3cq_world* w, const double* d_seg_from, const double* d_seg_to, int64_t n_seg,
+                     double verify_delta, double* d_last_free, double* d_state, uint8_t* d_done, double* d_last_tau,
+                     double* d_tau, int32_t* d_steps, const double* d_states, int64_t n_states, uint8_t* d_valid,
+                     int32_t repeats, se3cq_batch_config* chosen, float* ms)
+{
+	/* Times every candidate launch shape on this batch (on the world's own
+	 * stream, synchronously) and keeps the fastest: waves per block (1, 2,
+	 * 4, as the LDS of the state stacks allows), resident blocks per CU (as
+	 * many as fit, or half), and states per lane of a state work item (1,
+	 * 2, 4; only when the batch has states).  The outputs hold the batch's
+	 * results afterwards, since every shape computes the same answers. */
+	CHECK_WORLD(w);
+	const int rc0 = batch_check(w, n_seg, verify_delta, d_last_free, d_state, d_done, n_states, d_valid);
+	if (rc0)
+		return rc0;
+	if (repeats < 1 || repeats > 100)
+		return fail(SE3CQ_EINVAL, "repeats must be in [1, 100]");
+	if (n_seg == 0 && n_states == 0)
+		return fail(SE3CQ_EINVAL, "an empty batch cannot be timed");
+	const DevScene s = make_scene(w);
+	const int bmax = batch_default_block(s) / 64;
+	std::vector<se3cq_batch_config> cands;
+	for (int bw = 1; bw <= bmax; bw *= 2) {
+		const int full = batch_blocks_per_cu(s, 64 * bw);
+		const int halves[2] = {full, (full + 1) / 2};
+		for (int h = 0; h < (full > 1 ? 2 : 1); h++)
+			for (int spl = 1; spl <= (n_states > 0 ? 4 : 1); spl *= 2)
+				cands.push_back(se3cq_batch_config{bw, halves[h], spl});
+	}
+	hipStream_t st = w->stream;
+	hipEvent_t e0, e1;
+	HIPCHK(hipEventCreate(&e0));
+	HIPCHK(hipEventCreate(&e1));
+	float best = -1.0f;
+	se3cq_batch_config best_cfg = cands[0];
+	int rc = SE3CQ_OK;
+	for (size_t i = 0; i <= cands.size() && rc == SE3CQ_OK; i++) {
+		/* a first untimed run warms caches and code; then each candidate */
+		const se3cq_batch_config c = cands[i == 0 ? 0 : i - 1];
+		const BatchArgs a = batch_args(w, d_seg_from, d_seg_to, n_seg, verify_delta, d_last_free, d_state, d_done,
+		                               d_last_tau, d_tau, d_steps, d_states, n_states, d_valid, c.states_per_lane);
+		if (hipEventRecord(e0, st) != hipSuccess) {
+			rc = fail(SE3CQ_EDEVICE, "hipEventRecord failed");
+			break;
+		}
+		for (int r = 0; r < (i == 0 ? 1 : repeats) && rc == SE3CQ_OK; r++)
+			rc = batch_launch(w, a, c, st);
+		if (rc)
+			break;
+		float t = 0.0f;
+		if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+		    hipEventElapsedTime(&t, e0, e1) != hipSuccess) {
+			rc = fail(SE3CQ_EDEVICE, "timing failed");
+			break;
+		}
+		if (i > 0 && (best < 0.0f || t < best)) {
+			best = t;
+			best_cfg = c;
+		}
+	}
+	(void)hipEventDestroy(e0);
+	(void)hipEventDestroy(e1);
+	if (rc)
+		return rc;
+	rc = check_device(w);
+	if (rc)
+		return rc;
+	/* leave the outputs as the chosen shape computes them (identical) */
+	w->batch_cfg = best_cfg;
+	if (chosen)
+		*chosen = best_cfg;
+	if (ms)
+		*ms = best / (float)repeats;
 	return SE3CQ_OK;
 }
 

@@ -459,6 +459,62 @@ class UnitWorld:
         N.check(N.lib().se3cq_visibility_bits_device(self._w, d_qs, int(n), float(verify_magnitude), d_bits, d_steps,
                                                      stream))
 
+    # ---- mixed batches (one persistent launch) -----------------------------------
+    def batch(self, seg_from, seg_to, verify_delta, states, return_states=False):
+        """Segments under transit_state_to_with_contact semantics and single
+        states in one launch.  Returns a dict: done, last_tau, tau, steps (per
+        segment), valid (per state), and last_free / state when asked."""
+        f = _states(seg_from) if len(seg_from) else np.zeros((0, 7))
+        t = _states(seg_to) if len(seg_to) else np.zeros((0, 7))
+        q = _states(states) if len(states) else np.zeros((0, 7))
+        if len(f) != len(t):
+            raise ValueError("seg_from and seg_to must have the same number of rows")
+        ns, nq = len(f), len(q)
+        out = {"done": np.zeros(ns, np.uint8), "last_tau": np.zeros(ns), "tau": np.zeros(ns),
+               "steps": np.zeros(ns, np.int32), "valid": np.zeros(nq, np.uint8)}
+        lf = st = None
+        if return_states:
+            out["last_free"] = np.zeros((ns, 7))
+            out["state"] = np.zeros((ns, 7))
+            lf, st = N.dptr(out["last_free"]), N.dptr(out["state"])
+        N.check(N.lib().se3cq_batch_host(self._w, N.dptr(f), N.dptr(t), ns, float(verify_delta), lf, st,
+                                         out["done"].ctypes.data_as(N.c_u8_p), N.dptr(out["last_tau"]),
+                                         N.dptr(out["tau"]), out["steps"].ctypes.data_as(N.c_i32_p), N.dptr(q), nq,
+                                         out["valid"].ctypes.data_as(N.c_u8_p)))
+        out["done"] = out["done"].astype(bool)
+        out["valid"] = out["valid"].astype(bool)
+        return out
+
+    def batch_device(self, d_from, d_to, n_seg, verify_delta, d_done, d_last_tau, d_tau, d_steps, d_states,
+                     n_states, d_valid, d_last_free=None, d_state=None, stream=None):
+        """Device-pointer form of batch (asynchronous on the stream)."""
+        N.check(N.lib().se3cq_batch_device(self._w, d_from, d_to, int(n_seg), float(verify_delta), d_last_free,
+                                           d_state, d_done, d_last_tau, d_tau, d_steps, d_states, int(n_states),
+                                           d_valid, stream))
+
+    def batch_tune(self, d_from, d_to, n_seg, verify_delta, d_done, d_last_tau, d_tau, d_steps, d_states, n_states,
+                   d_valid, d_last_free=None, d_state=None, repeats=3):
+        """Time every launch shape on this batch and keep the fastest for later
+        batch calls.  Returns (config dict, ms per launch)."""
+        cfg = N.BatchConfig()
+        ms = ctypes.c_float(0.0)
+        N.check(N.lib().se3cq_batch_tune(self._w, d_from, d_to, int(n_seg), float(verify_delta), d_last_free,
+                                         d_state, d_done, d_last_tau, d_tau, d_steps, d_states, int(n_states),
+                                         d_valid, int(repeats), ctypes.byref(cfg), ctypes.byref(ms)))
+        return {k: getattr(cfg, k) for k, _ in N.BatchConfig._fields_}, ms.value
+
+    @property
+    def batch_config(self):
+        cfg = N.BatchConfig()
+        N.check(N.lib().se3cq_batch_get_config(self._w, ctypes.byref(cfg)))
+        return {k: getattr(cfg, k) for k, _ in N.BatchConfig._fields_}
+
+    @batch_config.setter
+    def batch_config(self, c):
+        cfg = N.BatchConfig(int(c.get("block_waves", 0)), int(c.get("blocks_per_cu", 0)),
+                            int(c.get("states_per_lane", 1)))
+        N.check(N.lib().se3cq_batch_set_config(self._w, ctypes.byref(cfg)))
+
     def check_device(self):
         N.check(N.lib().se3cq_check(self._w))
 

@@ -185,3 +185,25 @@ def test_queries_fail_loudly_without_a_device(torus_meshes):
     w = pyosr.create_unit_world_from_arrays(*torus_meshes)
     with pytest.raises(RuntimeError, match="HIP device"):
         w.is_valid_states(W.random_states(8, seed=1))
+
+
+@pytest.mark.parametrize("vd", [0.0, -1.0 / 1024, float("nan"), float("inf")])
+def test_verify_step_must_be_positive(torus_meshes, vd):
+    """A verify step that is not positive and finite never advances the
+    reference's stepping loop (unit_world.cc:448-470); every segment entry
+    point rejects it before any device work."""
+    w = pyosr.create_unit_world_from_arrays(*torus_meshes)
+    a = W.random_states(4, seed=2)
+    b = W.random_states(4, seed=3)
+    calls = [
+        lambda: w.transit_states_to_with_contact(a, b, vd),
+        lambda: w.is_valid_transitions(a, b, vd),
+        lambda: w.touch_search(a, b, vd, 20),
+        lambda: w.calculate_visibility_matrix(a, True, vd),
+        lambda: w.calculate_visibility_matrix2(a, True, b, True, vd),
+        lambda: w.calculate_visibility_pair(a, True, b, True, vd),
+        lambda: w.batch(a, b, vd, a),
+    ]
+    for call in calls:
+        with pytest.raises(RuntimeError, match="verify step"):
+            call()

@@ -1,0 +1,161 @@
+"""Mixed batches (se3cq_batch_*): uneven segments under
+transitStateToWithContact semantics (lib/osr/unit_world.cc:405-474) and
+single states (isValid, 215-247) in one persistent launch.  Checked against
+the oracle and against the engine's separate entry points, for every launch
+shape the tuner may pick, under HIP graph replay, and at the edges (empty
+halves, early returns, zero-length segments, state counts off the chunk
+size)."""
+import numpy as np
+import pytest
+
+from puzzletunneldiscovery_amd import pyosr, workloads as W
+
+pytestmark = pytest.mark.gpu
+VD = 1.0 / 1024
+
+
+def _flush(msg):
+    print(msg, flush=True)
+
+
+def _free_starts(uw, n, seed):
+    qs = W.random_states(4 * n, seed=seed)
+    return qs[uw.is_valid_states(qs)][:n]
+
+
+def _uneven(uw, n, seed):
+    starts = _free_starts(uw, n, seed)
+    steps = W.lognormal_steps(len(starts), seed=seed + 1, hi=1024)
+    ends = W.segment_ends(starts, seed=seed + 2, lengths=steps * VD)
+    ends[:3] = starts[:3]          # zero length: early return
+    ends[3:6] = W.segment_ends(starts[3:6], seed=seed + 3, lengths=np.full(3, 0.5 * VD))  # shorter than a step
+    return starts, ends
+
+
+def test_batch_matches_oracle_and_separate_calls(gpu_world, torus_oracle):
+    _flush("batch: 1,500 uneven segments + 10,000 states")
+    starts, ends = _uneven(gpu_world, 1500, 301)
+    qs = W.random_states(10_000, seed=304)
+    b = gpu_world.batch(starts, ends, VD, qs, return_states=True)
+    o = torus_oracle.touch_batch(starts, ends, VD, 0)
+    assert np.array_equal(b["done"], o["done"].astype(bool))
+    assert np.array_equal(b["last_tau"], o["last_tau"]) and np.array_equal(b["tau"], o["tau"])
+    assert np.array_equal(b["steps"], o["queries"])
+    assert 0.05 < b["done"].mean() < 0.95
+    v = torus_oracle.valid_states(qs).astype(bool)
+    assert np.array_equal(b["valid"], v)
+    lf, st, done, lt, ta, steps = gpu_world.transit_states_to_with_contact(starts, ends, VD)
+    assert np.array_equal(b["last_free"], lf) and np.array_equal(b["state"], st)
+    assert np.array_equal(b["done"], done) and np.array_equal(b["steps"], steps)
+
+
+def test_batch_halves_and_empty(gpu_world, torus_oracle):
+    starts, ends = _uneven(gpu_world, 200, 311)
+    qs = W.random_states(1000 + 37, seed=312)  # not a multiple of 64
+    only_seg = gpu_world.batch(starts, ends, VD, np.zeros((0, 7)))
+    assert only_seg["valid"].shape == (0,)
+    o = torus_oracle.touch_batch(starts, ends, VD, 0)
+    assert np.array_equal(only_seg["tau"], o["tau"])
+    only_q = gpu_world.batch(np.zeros((0, 7)), np.zeros((0, 7)), VD, qs)
+    assert only_q["done"].shape == (0,)
+    assert np.array_equal(only_q["valid"], torus_oracle.valid_states(qs).astype(bool))
+    none = gpu_world.batch(np.zeros((0, 7)), np.zeros((0, 7)), VD, np.zeros((0, 7)))
+    assert none["done"].shape == (0,) and none["valid"].shape == (0,)
+
+
+def test_every_launch_shape_agrees(gpu_world):
+    import torch
+    starts, ends = _uneven(gpu_world, 700, 321)
+    qs = W.random_states(3000, seed=322)
+    ref = gpu_world.batch(starts, ends, VD, qs)
+    keep = gpu_world.batch_config
+    try:
+        for bw in (1, 2, 4):
+            for per_cu in (0, 1):
+                for spl in (1, 3):
+                    gpu_world.batch_config = {"block_waves": bw, "blocks_per_cu": per_cu, "states_per_lane": spl}
+                    got = gpu_world.batch(starts, ends, VD, qs)
+                    for k in ("done", "last_tau", "tau", "steps", "valid"):
+                        assert np.array_equal(got[k], ref[k]), (bw, per_cu, spl, k)
+    finally:
+        gpu_world.batch_config = keep
+    with pytest.raises(RuntimeError):
+        gpu_world.batch_config = {"block_waves": 5, "blocks_per_cu": 0, "states_per_lane": 1}
+    with pytest.raises(RuntimeError):
+        gpu_world.batch_config = {"block_waves": 1, "blocks_per_cu": 0, "states_per_lane": 0}
+
+
+def _device_batch(torch, starts, ends, qs):
+    dev = torch.device("cuda")
+    n, m = len(starts), len(qs)
+    ins = [torch.from_numpy(x).to(dev) for x in (starts, ends, qs)]
+    outs = [torch.zeros(n, dtype=torch.uint8, device=dev), torch.zeros(n, dtype=torch.float64, device=dev),
+            torch.zeros(n, dtype=torch.float64, device=dev), torch.zeros(n, dtype=torch.int32, device=dev),
+            torch.zeros(m, dtype=torch.uint8, device=dev)]
+    return ins, outs
+
+
+def test_batch_tune_keeps_results(gpu_world):
+    import torch
+    starts, ends = _uneven(gpu_world, 2000, 331)
+    qs = W.random_states(20_000, seed=332)
+    ref = gpu_world.batch(starts, ends, VD, qs)
+    ins, outs = _device_batch(torch, starts, ends, qs)
+    keep = gpu_world.batch_config
+    try:
+        cfg, ms = gpu_world.batch_tune(ins[0].data_ptr(), ins[1].data_ptr(), len(starts), VD,
+                                       *[o.data_ptr() for o in outs[:4]], ins[2].data_ptr(), len(qs),
+                                       outs[4].data_ptr(), repeats=2)
+        assert ms > 0 and cfg == gpu_world.batch_config
+        assert cfg["block_waves"] in (1, 2, 4) and cfg["states_per_lane"] in (1, 2, 4)
+        torch.cuda.synchronize()
+        for o, k in zip(outs, ("done", "last_tau", "tau", "steps", "valid")):
+            assert np.array_equal(o.cpu().numpy().astype(ref[k].dtype), ref[k]), k
+    finally:
+        gpu_world.batch_config = keep
+
+
+def test_batch_in_hip_graph(gpu_world):
+    """The device form is a fill kernel plus the batch kernel on the caller's
+    stream: captured once, every replay reproduces the eager result."""
+    import torch
+    starts, ends = _uneven(gpu_world, 600, 341)
+    qs = W.random_states(5000, seed=342)
+    ins, outs = _device_batch(torch, starts, ends, qs)
+    s = torch.cuda.Stream()
+
+    def call():
+        gpu_world.batch_device(ins[0].data_ptr(), ins[1].data_ptr(), len(starts), VD,
+                               *[o.data_ptr() for o in outs[:4]], ins[2].data_ptr(), len(qs), outs[4].data_ptr(),
+                               stream=s.cuda_stream)
+
+    with torch.cuda.stream(s):
+        call()
+    torch.cuda.synchronize()
+    eager = [o.clone() for o in outs]
+    ref = gpu_world.batch(starts, ends, VD, qs)
+    assert np.array_equal(eager[0].cpu().numpy().astype(bool), ref["done"])
+    assert np.array_equal(eager[4].cpu().numpy().astype(bool), ref["valid"])
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        call()
+    for rep in range(3):
+        for x in outs:
+            x.zero_()
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        for a, b in zip(outs, eager):
+            assert torch.equal(a, b), rep
+
+
+def test_walk_that_would_not_end_is_an_error(torus_meshes):
+    """A verify step that leaves 2^31 or more steps passes the host's check
+    but not the kernel's: the walk stops at once and the call fails.  Its own
+    world, since the device error flag stays set."""
+    env, rob = torus_meshes
+    uw = pyosr.create_unit_world_from_arrays(env, rob)
+    starts = _free_starts(uw, 4, 351)
+    ends = W.segment_ends(starts, seed=352)
+    with pytest.raises(RuntimeError, match="2\\^31"):
+        uw.transit_states_to_with_contact(starts, ends, 1e-12)
