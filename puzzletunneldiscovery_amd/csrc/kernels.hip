@@ -812,6 +812,20 @@ __device__ __forceinline__ void wave_lds_sync()
 	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+/* Persistent wavefronts: the grid holds only as many wavefronts as stay
+ * resident, and each takes its next work item from the launch's counter
+ * (lane 0 adds, the wave reads the value in scalar registers) until the
+ * items run out.  A long item then holds one wavefront, not a whole block,
+ * and a finished wavefront starts its next item at once. */
+__device__ __forceinline__ int64_t next_item(unsigned long long* queue)
+{
+	unsigned long long got = 0;
+	if ((threadIdx.x & 63) == 0)
+		got = atomicAdd(queue, 1ull);
+	return (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(got >> 32)) << 32) |
+	                 (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)got));
+}
+
 /* Number of set bits of m below this lane. */
 __device__ __forceinline__ int lane_rank(uint64_t m)
 {
@@ -1251,17 +1265,18 @@ __global__ void __launch_bounds__(256, SE3CQ_MIN_WAVES) k_segments(DevScene s, c
 	extern __shared__ double seg_lds[];
 	__shared__ SegShared segsh[4];
 	const int wave = threadIdx.x >> 6;
-	const int lane = threadIdx.x & 63;
-	const int64_t seg = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave;
 	SegShared* sh = &segsh[wave];
-	const int64_t sv = seg < n ? seg : n - 1;
-	seg_setup<MODE>(sh, from_all + 7 * sv, to_all + 7 * sv, vd);
-	__syncthreads();
-	if (seg >= n)
-		return;
 	BisectShared* bs = (MODE == MODE_TOUCH) ? reinterpret_cast<BisectShared*>(seg_lds) + wave : nullptr;
-	const SegResult r = segment_walk<WIDE, MODE>(s, sh, levels, bs);
-	seg_store(o, seg, r, sh, from_all + 7 * seg, to_all + 7 * seg);
+	for (;;) {
+		const int64_t seg = next_item(s.queue);
+		if (seg >= n)
+			break;
+		seg_setup<MODE>(sh, from_all + 7 * seg, to_all + 7 * seg, vd);
+		wave_lds_sync();
+		const SegResult r = segment_walk<WIDE, MODE>(s, sh, levels, bs);
+		seg_store(o, seg, r, sh, from_all + 7 * seg, to_all + 7 * seg);
+		wave_lds_sync();
+	}
 }
 
 /* ---- mixed batches: persistent wavefronts over a work queue ---- */
@@ -1276,9 +1291,11 @@ __global__ void __launch_bounds__(256, SE3CQ_MIN_WAVES) k_segments(DevScene s, c
  * them; the rest are chunks of 64 * states_per_lane states, one per lane
  * with the per-thread traversal of k_valid_states.  A long segment occupies
  * one wavefront while the others drain the queue, so uneven lengths do not
- * idle whole blocks.
+ * idle whole blocks.  Only a batch with states (STATES) reserves the
+ * per-thread traversal stacks in LDS, which limit how many blocks a CU
+ * holds; a batch of segments alone runs at full occupancy.
  */
-template <bool WIDE>
+template <bool WIDE, bool STATES>
 __global__ void __launch_bounds__(256, SE3CQ_MIN_WAVES) k_batch(DevScene s, BatchArgs a)
 {
 	extern __shared__ uint32_t smem[];
@@ -1287,15 +1304,12 @@ __global__ void __launch_bounds__(256, SE3CQ_MIN_WAVES) k_batch(DevScene s, Batc
 	const int lane = threadIdx.x & 63;
 	SegShared* sh = &segsh[wave];
 	PairStack<WIDE> stk, leaves;
-	thread_stacks<WIDE>(s, smem, stk, leaves);
+	if (STATES)
+		thread_stacks<WIDE>(s, smem, stk, leaves);
 	const int64_t per_item = 64 * (int64_t)a.states_per_lane;
 	const int64_t total = a.n_seg + (a.n_states + per_item - 1) / per_item;
 	for (;;) {
-		unsigned long long got = 0;
-		if (lane == 0)
-			got = atomicAdd(a.queue, 1ull);
-		const int64_t item = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(got >> 32)) << 32) |
-		                               (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)got));
+		const int64_t item = next_item(s.queue);
 		if (item >= total)
 			break;
 		if (item < a.n_seg) {
@@ -1308,7 +1322,7 @@ __global__ void __launch_bounds__(256, SE3CQ_MIN_WAVES) k_batch(DevScene s, Batc
 			/* every lane is done with the record before the next item
 			 * overwrites it */
 			wave_lds_sync();
-		} else {
+		} else if (STATES) {
 			const int64_t base = (item - a.n_seg) * per_item;
 			for (int j = 0; j < a.states_per_lane; j++) {
 				const int64_t i = base + 64 * j + lane;
@@ -1345,46 +1359,48 @@ __global__ void __launch_bounds__(256, SE3CQ_MIN_WAVES) k_visibility(DevScene s,
 	__shared__ SegShared segsh[4];
 	const int wave = threadIdx.x >> 6;
 	const int lane = threadIdx.x & 63;
-	const int64_t p = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave;
-	const int64_t pv = p < npairs ? p : npairs - 1;
-	int64_t fi, ti;
-	if (VIS == VIS_MATRIX) {
-		tri_pair(pv, n, fi, ti);
-	} else if (VIS == VIS_MATRIX2) {
-		fi = pv / n;
-		ti = pv % n;
-	} else {
-		fi = pv;
-		ti = pv;
-	}
 	SegShared* sh = &segsh[wave];
-	seg_setup<MODE_TRANSITION>(sh, qs0 + 7 * fi, qs1 + 7 * ti, ivd);
-	__syncthreads();
-	if (p >= npairs)
-		return;
-	const SegResult r = segment_walk<WIDE, MODE_TRANSITION>(s, sh, 0, nullptr);
-	const uint8_t done = (uint8_t)r.done;
-	if (lane == 0) {
+	for (;;) {
+		const int64_t p = next_item(s.queue);
+		if (p >= npairs)
+			break;
+		int64_t fi, ti;
 		if (VIS == VIS_MATRIX) {
-			if (out8) {
-				out8[fi * n + ti] = (int8_t)done;
-				out8[ti * n + fi] = (int8_t)done;
-			}
-			if (bits && done) {
-				const int64_t b1 = fi * n + ti, b2 = ti * n + fi;
-				atomicOr(bits + (b1 >> 5), 1u << (b1 & 31));
-				atomicOr(bits + (b2 >> 5), 1u << (b2 & 31));
-			}
+			tri_pair(p, n, fi, ti);
 		} else if (VIS == VIS_MATRIX2) {
-			if (out8) out8[fi * n + ti] = (int8_t)done;
-			if (bits && done) {
-				const int64_t b1 = fi * n + ti;
-				atomicOr(bits + (b1 >> 5), 1u << (b1 & 31));
-			}
+			fi = p / n;
+			ti = p % n;
 		} else {
-			if (out8) out8[p] = (int8_t)done;
+			fi = p;
+			ti = p;
 		}
-		if (steps) steps[p] = r.steps;
+		seg_setup<MODE_TRANSITION>(sh, qs0 + 7 * fi, qs1 + 7 * ti, ivd);
+		wave_lds_sync();
+		const SegResult r = segment_walk<WIDE, MODE_TRANSITION>(s, sh, 0, nullptr);
+		const uint8_t done = (uint8_t)r.done;
+		if (lane == 0) {
+			if (VIS == VIS_MATRIX) {
+				if (out8) {
+					out8[fi * n + ti] = (int8_t)done;
+					out8[ti * n + fi] = (int8_t)done;
+				}
+				if (bits && done) {
+					const int64_t b1 = fi * n + ti, b2 = ti * n + fi;
+					atomicOr(bits + (b1 >> 5), 1u << (b1 & 31));
+					atomicOr(bits + (b2 >> 5), 1u << (b2 & 31));
+				}
+			} else if (VIS == VIS_MATRIX2) {
+				if (out8) out8[fi * n + ti] = (int8_t)done;
+				if (bits && done) {
+					const int64_t b1 = fi * n + ti;
+					atomicOr(bits + (b1 >> 5), 1u << (b1 & 31));
+				}
+			} else {
+				if (out8) out8[p] = (int8_t)done;
+			}
+			if (steps) steps[p] = r.steps;
+		}
+		wave_lds_sync();
 	}
 }
 
@@ -1416,64 +1432,64 @@ __global__ void __launch_bounds__(256, SE3CQ_MIN_WAVES) k_ompl_motions(DevScene 
 	__shared__ OmplXf xsh;
 	const int wave = threadIdx.x >> 6;
 	const int lane = threadIdx.x & 63;
-	const int64_t p = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave;
-	const int64_t pv = p < n ? p : n - 1;
 	if (threadIdx.x < 16)
 		xsh.M[threadIdx.x] = x.M[threadIdx.x];
 	else if (threadIdx.x < 23)
 		xsh.pert[threadIdx.x - 16] = x.pert[threadIdx.x - 16];
 	else if (threadIdx.x < 26)
 		xsh.delta[threadIdx.x - 23] = x.delta[threadIdx.x - 23];
-	if (lane == 0) {
-		double a[7], b[7];
-		for (int k = 0; k < 7; k++) {
-			a[k] = q0[7 * pv + k];
-			b[k] = q1[7 * pv + k];
-		}
-		se3_ompl_motion_init(&msh[wave], a, b, lvs);
-	}
 	__syncthreads();
-	if (p >= n)
-		return; /* wave-uniform */
-	const se3_ompl_motion* m = &msh[wave];
-	const int64_t nd = m->nd;
-	if (nd < 0) {
+	se3_ompl_motion* m = &msh[wave];
+	for (;;) {
+		const int64_t p = next_item(s.queue);
+		if (p >= n)
+			break;
 		if (lane == 0) {
-			out[p] = -1;
-			if (samples)
-				samples[p] = 0;
+			double qa[7], qb[7];
+			for (int k = 0; k < 7; k++) {
+				qa[k] = q0[7 * p + k];
+				qb[k] = q1[7 * p + k];
+			}
+			se3_ompl_motion_init(m, qa, qb, lvs);
 		}
-		return;
-	}
-	const int64_t K = nd > 1 ? nd : 1;
-	const int64_t C = (K + 63) / 64;
-	int valid = 1;
-	if (K > 64) {
-		/* coarse pass: samples 1, 1 + C, 1 + 2C, ... span the motion */
-		const int64_t k = 1 + C * lane;
-		const bool active = k <= K;
-		const uint64_t am = __ballot(active);
-		double st[7], u[7];
-		se3_ompl_motion_sample(m, active ? k : K, st);
-		se3_ompl_to_unit(xsh.M, xsh.pert, xsh.delta, st, u);
-		if (packet_collide<true>(s, u, am))
-			valid = 0;
-	}
-	/* then every sample, 64 neighbours per pass */
-	for (int64_t c = 0; valid && c < C; c++) {
-		const int64_t k = 1 + 64 * c + lane;
-		const bool active = k <= K;
-		const uint64_t am = __ballot(active);
-		double st[7], u[7];
-		se3_ompl_motion_sample(m, active ? k : K, st);
-		se3_ompl_to_unit(xsh.M, xsh.pert, xsh.delta, st, u);
-		if (packet_collide<true>(s, u, am))
-			valid = 0;
-	}
-	if (lane == 0) {
-		out[p] = valid;
-		if (samples)
-			samples[p] = (int32_t)K;
+		wave_lds_sync();
+		const int64_t nd = m->nd;
+		int valid = 1;
+		int64_t K = 0;
+		if (nd < 0) {
+			valid = -1;
+		} else {
+			K = nd > 1 ? nd : 1;
+			const int64_t C = (K + 63) / 64;
+			if (K > 64) {
+				/* coarse pass: samples 1, 1 + C, 1 + 2C, ... span the motion */
+				const int64_t k = 1 + C * lane;
+				const bool active = k <= K;
+				const uint64_t am = __ballot(active);
+				double st[7], u[7];
+				se3_ompl_motion_sample(m, active ? k : K, st);
+				se3_ompl_to_unit(xsh.M, xsh.pert, xsh.delta, st, u);
+				if (packet_collide<true>(s, u, am))
+					valid = 0;
+			}
+			/* then every sample, 64 neighbours per pass */
+			for (int64_t c = 0; valid == 1 && c < C; c++) {
+				const int64_t k = 1 + 64 * c + lane;
+				const bool active = k <= K;
+				const uint64_t am = __ballot(active);
+				double st[7], u[7];
+				se3_ompl_motion_sample(m, active ? k : K, st);
+				se3_ompl_to_unit(xsh.M, xsh.pert, xsh.delta, st, u);
+				if (packet_collide<true>(s, u, am))
+					valid = 0;
+			}
+		}
+		if (lane == 0) {
+			out[p] = valid;
+			if (samples)
+				samples[p] = (int32_t)K;
+		}
+		wave_lds_sync();
 	}
 }
 
@@ -1636,6 +1652,31 @@ static inline int pick_block(const DevScene& s)
 
 static inline size_t stack_bytes(const DevScene& s, int block) { return stack_bytes_per_thread(s) * (size_t)block; }
 
+__global__ void k_fill_u32(uint32_t* p, int64_t n, uint32_t v);
+
+/* Zero the launch's work counter with a kernel (not hipMemsetAsync, so the
+ * launch may be captured in a HIP graph). */
+static hipError_t zero_queue(const DevScene& s, hipStream_t st)
+{
+	hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, reinterpret_cast<uint32_t*>(s.queue), (int64_t)2, 0u);
+	return hipGetLastError();
+}
+
+/* Blocks of a persistent launch: as many as stay resident on every CU, but
+ * no more than the items need. */
+template <class K>
+static unsigned persistent_grid(const DevScene& s, K kernel, int block, size_t lds, int64_t items)
+{
+	int nb = 0;
+	if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, block, lds) != hipSuccess || nb < 1)
+		nb = 1;
+	const int64_t full = (int64_t)(s.num_cu > 0 ? s.num_cu : 1) * nb;
+	const int64_t waves = block / 64;
+	const int64_t need = (items + waves - 1) / waves;
+	return (unsigned)(need < full ? need : full);
+}
+
+
 __global__ void k_fill_u32(uint32_t* p, int64_t n, uint32_t v)
 {
 	const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1731,13 +1772,15 @@ hipError_t launch_ompl_motions(const DevScene& s, const OmplXf& x, const double*
 	if (n <= 0)
 		return hipSuccess;
 	const int block = 256;
-	const int64_t grid = (n + 3) / 4;
+	hipError_t e = zero_queue(s, st);
+	if (e != hipSuccess)
+		return e;
 	if (s.wide)
-		hipLaunchKernelGGL(k_ompl_motions<true>, dim3((unsigned)grid), dim3(block), 0, st, s, x, q0, q1, n, lvs, out,
-		                   samples);
+		hipLaunchKernelGGL(k_ompl_motions<true>, dim3(persistent_grid(s, k_ompl_motions<true>, block, 0, n)),
+		                   dim3(block), 0, st, s, x, q0, q1, n, lvs, out, samples);
 	else
-		hipLaunchKernelGGL(k_ompl_motions<false>, dim3((unsigned)grid), dim3(block), 0, st, s, x, q0, q1, n, lvs,
-		                   out, samples);
+		hipLaunchKernelGGL(k_ompl_motions<false>, dim3(persistent_grid(s, k_ompl_motions<false>, block, 0, n)),
+		                   dim3(block), 0, st, s, x, q0, q1, n, lvs, out, samples);
 	return hipGetLastError();
 }
 
@@ -1747,26 +1790,35 @@ hipError_t launch_batch(const DevScene& s, const BatchArgs& a, int block, int gr
 		return hipSuccess;
 	if (block < 64 || block > 256 || block % 64 != 0 || grid <= 0 || a.states_per_lane < 1)
 		return hipErrorInvalidValue;
-	hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, reinterpret_cast<uint32_t*>(a.queue), (int64_t)2, 0u);
-	hipError_t e = hipGetLastError();
+	hipError_t e = zero_queue(s, st);
 	if (e != hipSuccess)
 		return e;
-	const size_t lds = stack_bytes(s, block);
-	if (s.wide)
-		hipLaunchKernelGGL(k_batch<true>, dim3((unsigned)grid), dim3(block), lds, st, s, a);
+	const bool states = a.n_states > 0;
+	const size_t lds = states ? stack_bytes(s, block) : 0;
+	if (states && s.wide)
+		hipLaunchKernelGGL((k_batch<true, true>), dim3((unsigned)grid), dim3(block), lds, st, s, a);
+	else if (states)
+		hipLaunchKernelGGL((k_batch<false, true>), dim3((unsigned)grid), dim3(block), lds, st, s, a);
+	else if (s.wide)
+		hipLaunchKernelGGL((k_batch<true, false>), dim3((unsigned)grid), dim3(block), lds, st, s, a);
 	else
-		hipLaunchKernelGGL(k_batch<false>, dim3((unsigned)grid), dim3(block), lds, st, s, a);
+		hipLaunchKernelGGL((k_batch<false, false>), dim3((unsigned)grid), dim3(block), lds, st, s, a);
 	return hipGetLastError();
 }
 
-int batch_default_block(const DevScene& s) { return pick_block(s); }
+int batch_default_block(const DevScene& s, bool states) { return states ? pick_block(s) : 256; }
 
-int batch_blocks_per_cu(const DevScene& s, int block)
+int batch_blocks_per_cu(const DevScene& s, int block, bool states)
 {
 	int nb = 0;
-	const size_t lds = stack_bytes(s, block);
-	const hipError_t e = s.wide ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch<true>, block, lds)
-	                            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch<false>, block, lds);
+	const size_t lds = states ? stack_bytes(s, block) : 0;
+	hipError_t e;
+	if (states)
+		e = s.wide ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch<true, true>, block, lds)
+		           : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch<false, true>, block, lds);
+	else
+		e = s.wide ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch<true, false>, block, lds)
+		           : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch<false, false>, block, lds);
 	return (e == hipSuccess && nb > 0) ? nb : 1;
 }
 
@@ -1778,14 +1830,16 @@ static hipError_t launch_seg_mode(const DevScene& s, const double* from, const d
 	 * BisectShared record per wavefront in LDS */
 	const int block = 256;
 	const int per_block = block / 64;
-	const int64_t grid = (n + per_block - 1) / per_block;
 	const size_t lds = (MODE == MODE_TOUCH) ? per_block * sizeof(BisectShared) : 0;
+	hipError_t e = zero_queue(s, st);
+	if (e != hipSuccess)
+		return e;
 	if (s.wide)
-		hipLaunchKernelGGL((k_segments<true, MODE>), dim3((unsigned)grid), dim3(block), lds, st, s, from, to, n, vd,
-		                   levels, o);
+		hipLaunchKernelGGL((k_segments<true, MODE>), dim3(persistent_grid(s, k_segments<true, MODE>, block, lds, n)),
+		                   dim3(block), lds, st, s, from, to, n, vd, levels, o);
 	else
-		hipLaunchKernelGGL((k_segments<false, MODE>), dim3((unsigned)grid), dim3(block), lds, st, s, from, to, n, vd,
-		                   levels, o);
+		hipLaunchKernelGGL((k_segments<false, MODE>), dim3(persistent_grid(s, k_segments<false, MODE>, block, lds, n)),
+		                   dim3(block), lds, st, s, from, to, n, vd, levels, o);
 	return hipGetLastError();
 }
 
@@ -1808,15 +1862,18 @@ static hipError_t launch_vis_mode(const DevScene& s, const double* qs0, const do
                                   hipStream_t st)
 {
 	const int block = 256;
-	const int per_block = block / 64;
-	const int64_t grid = (npairs + per_block - 1) / per_block;
 	const size_t lds = 0;
+	hipError_t e = zero_queue(s, st);
+	if (e != hipSuccess)
+		return e;
 	if (s.wide)
-		hipLaunchKernelGGL((k_visibility<true, VIS>), dim3((unsigned)grid), dim3(block), lds, st, s, qs0, qs1, m, n,
-		                   npairs, ivd, out8, bits, steps);
+		hipLaunchKernelGGL((k_visibility<true, VIS>),
+		                   dim3(persistent_grid(s, k_visibility<true, VIS>, block, lds, npairs)), dim3(block), lds, st,
+		                   s, qs0, qs1, m, n, npairs, ivd, out8, bits, steps);
 	else
-		hipLaunchKernelGGL((k_visibility<false, VIS>), dim3((unsigned)grid), dim3(block), lds, st, s, qs0, qs1, m, n,
-		                   npairs, ivd, out8, bits, steps);
+		hipLaunchKernelGGL((k_visibility<false, VIS>),
+		                   dim3(persistent_grid(s, k_visibility<false, VIS>, block, lds, npairs)), dim3(block), lds,
+		                   st, s, qs0, qs1, m, n, npairs, ivd, out8, bits, steps);
 	return hipGetLastError();
 }
 

@@ -38,7 +38,8 @@ hipError_t launch_contacts(const DevScene& s, const double* q, int64_t n, int ca
 
 /* A mixed batch for k_batch: segments under transitStateToWithContact
  * semantics (outputs as SegOut, lo/hi unused) and single states (valid: 1 =
- * free).  queue: a device word pair the launch zeroes first. */
+ * free).  The work counter is the scene's queue word, which the launch
+ * zeroes first. */
 struct BatchArgs {
 	const double* seg_from;
 	const double* seg_to;
@@ -48,14 +49,13 @@ struct BatchArgs {
 	const double* states;
 	int64_t n_states;
 	uint8_t* valid;
-	unsigned long long* queue;
 	int states_per_lane;
 };
 hipError_t launch_batch(const DevScene& s, const BatchArgs& a, int block, int grid, hipStream_t st);
-/* Largest block whose traversal stacks fit in LDS, and how many such blocks
- * one CU keeps resident. */
-int batch_default_block(const DevScene& s);
-int batch_blocks_per_cu(const DevScene& s, int block);
+/* Largest block (whose traversal stacks fit in LDS when the batch has
+ * states), and how many such blocks one CU keeps resident. */
+int batch_default_block(const DevScene& s, bool states);
+int batch_blocks_per_cu(const DevScene& s, int block, bool states);
 
 /* OMPL-convention validity (OmplDriver::validateStates / validateMotionPairs):
  * out 1 valid, 0 invalid; motions: -1 when the sample count is not finite

@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -94,6 +95,9 @@ struct DevBuf {
 
 } // namespace
 
+/* Work counters of the persistent kernels per world. */
+constexpr int QUEUE_SLOTS = 64;
+
 struct se3cq_world {
 	Mesh mesh[2];
 	float scale = 1.0f;
@@ -109,7 +113,11 @@ struct se3cq_world {
 	DevBuf d_nodes[2], d_tris[2];
 	DevBuf d_err;
 	DevBuf d_pert;
-	DevBuf d_queue; /* work counter of k_batch */
+	/* work counters of the persistent kernels: QUEUE_SLOTS words 64 B apart;
+	 * each launch takes the next (make_scene), so launches in flight on
+	 * different streams do not share one */
+	DevBuf d_queue;
+	std::atomic<uint32_t> queue_slot{0};
 	int num_cu = 0;
 	se3cq_batch_config batch_cfg = {0, 0, 1};
 	bool pert_uploaded = false;
@@ -599,6 +607,8 @@ DevScene make_scene(se3cq_world* w)
 	s.stack_depth = w->hm[0].depth + w->hm[1].depth + 2;
 	s.wide = (w->hm[0].nodes.size() > 65536 || w->hm[1].nodes.size() > 65536) ? 1 : 0;
 	s.error_flag = (uint32_t*)w->d_err.p;
+	s.queue = (unsigned long long*)((char*)w->d_queue.p + 64 * (w->queue_slot.fetch_add(1) % QUEUE_SLOTS));
+	s.num_cu = w->num_cu;
 	return s;
 }
 
@@ -653,7 +663,7 @@ int upload(se3cq_world* w)
 	}
 	if (!w->d_err.reserve(sizeof(uint32_t)) || hipMemset(w->d_err.p, 0, sizeof(uint32_t)) != hipSuccess)
 		return fail(SE3CQ_EDEVICE, "device allocation failed");
-	if (!w->d_pert.reserve(sizeof(PertData)) || !w->d_queue.reserve(2 * sizeof(unsigned long long)))
+	if (!w->d_pert.reserve(sizeof(PertData)) || !w->d_queue.reserve(QUEUE_SLOTS * 64))
 		return fail(SE3CQ_EDEVICE, "device allocation failed");
 	if (hipDeviceGetAttribute(&w->num_cu, hipDeviceAttributeMultiprocessorCount, w->device) != hipSuccess ||
 	    w->num_cu <= 0)
@@ -1705,13 +1715,14 @@ int se3cq_translate_vanilla_pts_to_unit(const se3cq_world* w, int geo, const dou
 }
 
 /* ---- mixed batches (persistent kernel) ---- */
-static int batch_shape(se3cq_world* w, const DevScene& s, const se3cq_batch_config& c, int* block, int* grid)
+static int batch_shape(se3cq_world* w, const DevScene& s, const se3cq_batch_config& c, bool states, int* block,
+                       int* grid)
 {
-	const int bmax = batch_default_block(s);
+	const int bmax = batch_default_block(s, states);
 	int b = c.block_waves > 0 ? 64 * c.block_waves : bmax;
 	if (b > bmax)
 		b = bmax;
-	const int per_cu_max = batch_blocks_per_cu(s, b);
+	const int per_cu_max = batch_blocks_per_cu(s, b, states);
 	const int per_cu = (c.blocks_per_cu > 0 && c.blocks_per_cu < per_cu_max) ? c.blocks_per_cu : per_cu_max;
 	*block = b;
 	*grid = w->num_cu * per_cu;
@@ -1749,7 +1760,6 @@ static BatchArgs batch_args(se3cq_world* w, const double* d_seg_from, const doub
 	a.states = d_states;
 	a.n_states = n_states;
 	a.valid = d_valid;
-	a.queue = (unsigned long long*)w->d_queue.p;
 	a.states_per_lane = states_per_lane > 0 ? states_per_lane : 1;
 	return a;
 }
@@ -1758,7 +1768,7 @@ static int batch_launch(se3cq_world* w, const BatchArgs& a, const se3cq_batch_co
 {
 	const DevScene s = make_scene(w);
 	int block = 0, grid = 0;
-	batch_shape(w, s, c, &block, &grid);
+	batch_shape(w, s, c, a.n_states > 0, &block, &grid);
 	const hipError_t e = launch_batch(s, a, block, grid, st);
 	if (e != hipSuccess)
 		return fail(SE3CQ_EDEVICE, std::string("launch failed: ") + hipGetErrorString(e));
@@ -1813,7 +1823,7 @@ int se3cq_batch_host(se3cq_world* w, const double* seg_from, const double* seg_t
 	                               w->batch_cfg.states_per_lane);
 	const DevScene s = make_scene(w);
 	int block = 0, grid = 0;
-	batch_shape(w, s, w->batch_cfg, &block, &grid);
+	batch_shape(w, s, w->batch_cfg, n_states > 0, &block, &grid);
 	const int rc = sync_and_check(w, w->stream, launch_batch(s, a, block, grid, w->stream));
 	if (rc)
 		return rc;
@@ -1871,10 +1881,11 @@ int se3cq_batch_tune(se3cq_world* w, const double* d_seg_from, const double* d_s
 	if (n_seg == 0 && n_states == 0)
 		return fail(SE3CQ_EINVAL, "an empty batch cannot be timed");
 	const DevScene s = make_scene(w);
-	const int bmax = batch_default_block(s) / 64;
+	const bool states = n_states > 0;
+	const int bmax = batch_default_block(s, states) / 64;
 	std::vector<se3cq_batch_config> cands;
 	for (int bw = 1; bw <= bmax; bw *= 2) {
-		const int full = batch_blocks_per_cu(s, 64 * bw);
+		const int full = batch_blocks_per_cu(s, 64 * bw, states);
 		const int halves[2] = {full, (full + 1) / 2};
 		for (int h = 0; h < (full > 1 ? 2 : 1); h++)
 			for (int spl = 1; spl <= (n_states > 0 ? 4 : 1); spl *= 2)
