@@ -164,11 +164,19 @@ def run_states(args, torch, pyosr, W):
         from oracle import oracle as O
         O.set_libm(False)
         ow = O.OracleWorld().setup(*env, *rob)
-        m = 200_000
-        t0 = time.perf_counter()
-        c, nbv, ntri = ow.valid_states(qs[:m], nthreads=_sample_threads(), counters=True)
-        dt = time.perf_counter() - t0
-        extra["mismatches"] = int((out[:m] != c.astype(bool)).sum())
+        # the states in order, 200,000 at a time, until the time budget is spent
+        m, dt, mism, cs = 0, 0.0, 0, []
+        while dt < args.cpu_seconds and m < n:
+            sl = slice(m, min(n, m + 200_000))
+            t0 = time.perf_counter()
+            c, nbv, ntri = ow.valid_states(qs[sl], nthreads=_sample_threads(), counters=True)
+            dt += time.perf_counter() - t0
+            mism += int((out[sl] != c.astype(bool)).sum())
+            cs.append((nbv, ntri))
+            m = sl.stop
+        nbv = np.concatenate([a for a, _ in cs])
+        ntri = np.concatenate([b for _, b in cs])
+        extra["mismatches"] = mism
         extra["cpu_baseline"] = dict(value=m / dt, unit="queries/s", cores=_sample_threads(), kind="port",
                                      sample=f"first {m} of the {n} states ({dt:.2f} s)")
         fq = C_BV * nbv.mean() + C_TRI * ntri.mean()
@@ -192,7 +200,8 @@ def _free_states(uw, W, n, seed):
 
 
 def run_uneven(args, torch, pyosr, W):
-    """Advance-until-contact over uneven segments (lognormal step counts), shell scene."""
+    """Advance-until-contact over uneven segments (lognormal step counts),
+    shell scene, in the persistent segment kernel."""
     env, rob = W.shell_scene(0)
     uw = pyosr.create_unit_world_from_arrays(env, rob)
     n = args.segments or 4 * 1024 * 1024
@@ -208,7 +217,8 @@ def run_uneven(args, torch, pyosr, W):
     d_lt = torch.zeros(n, dtype=torch.float64, device="cuda")
     d_ta = torch.zeros(n, dtype=torch.float64, device="cuda")
     d_st = torch.zeros(n, dtype=torch.int32, device="cuda")
-    wall, kern_ms = _Timer(torch, stream).run(
+    timer = _Timer(torch, stream)
+    wall, kern_ms = timer.run(
         lambda: uw.transit_device(d_f.data_ptr(), d_t.data_ptr(), n, vd, d_done.data_ptr(), d_lt.data_ptr(),
                                   d_ta.data_ptr(), d_st.data_ptr(), stream=stream.cuda_stream),
         args.steps, args.warmup)
@@ -216,22 +226,26 @@ def run_uneven(args, torch, pyosr, W):
     steps = d_st.cpu().numpy().astype(np.int64)
     done = d_done.cpu().numpy().astype(bool)
     q = int(steps.sum())
-    extra = {"mismatches": None, "cpu_baseline": None, "kernel": "k_segments<transit>", "kernel_ms": round(kern_ms, 4)}
+    extra = {"mismatches": None, "cpu_baseline": None, "kernel": "k_segments<transit> (persistent)",
+             "kernel_ms": round(kern_ms, 4)}
     if not args.no_cpu:
         from oracle import oracle as O
         O.set_libm(False)
         ow = O.OracleWorld().setup(*env, *rob)
-        m = 4096
-        t0 = time.perf_counter()
-        o = ow.touch_batch(starts[:m], ends[:m], vd, 0, nthreads=_sample_threads())
-        dt = time.perf_counter() - t0
         lt = d_lt.cpu().numpy()
-        mism = int((o["done"].astype(bool) != done[:m]).sum() + (o["queries"] != steps[:m]).sum()
-                   + (o["last_tau"] != lt[:m]).sum())
+        done_n, cq, mism, dt = 0, 0, 0, 0.0
+        while dt < args.cpu_seconds and done_n < n:
+            sl = slice(done_n, min(n, done_n + 16384))
+            t0 = time.perf_counter()
+            o = ow.touch_batch(starts[sl], ends[sl], vd, 0, nthreads=_sample_threads())
+            dt += time.perf_counter() - t0
+            mism += int((o["done"].astype(bool) != done[sl]).sum() + (o["queries"] != steps[sl]).sum()
+                        + (o["last_tau"] != lt[sl]).sum())
+            cq += int(o["queries"].sum())
+            done_n = sl.stop
         extra["mismatches"] = mism
-        extra["cpu_baseline"] = dict(value=float(o["queries"].sum()) / dt, unit="queries/s", cores=_sample_threads(),
-                                     kind="port", sample=f"first {m} of the {n} segments "
-                                                         f"({int(o['queries'].sum())} queries, {dt:.2f} s)")
+        extra["cpu_baseline"] = dict(value=cq / dt, unit="queries/s", cores=_sample_threads(), kind="port",
+                                     sample=f"first {done_n} of the {n} segments ({cq} queries, {dt:.2f} s)")
     line = _line(f"uneven segments: shell env {len(env[1])} tris / bent rod {len(rob[1])} tris, {n} segments, "
                  f"lognormal(5, 1.2) step counts in [8, 4096] at step 1/1024", q * args.steps / wall, wall, args, extra)
     line["config"].update({"segments": n, "queries_per_step": q, "free_segments": int(done.sum())})
@@ -263,22 +277,105 @@ def run_visibility(args, torch, pyosr, W):
         from oracle import oracle as O
         O.set_libm(False)
         ow = O.OracleWorld().setup(*env, *rob)
-        m = 64
-        t0 = time.perf_counter()
-        c = ow.visibility_matrix(ms[:m], True, VD, nthreads=_sample_threads())
-        dt = time.perf_counter() - t0
+        # 64 x 64 diagonal blocks of the matrix, in order, until the time
+        # budget is spent
         flat = np.unpackbits(bits.view(np.uint8), bitorder="little")[: n * n].reshape(n, n)
-        extra["mismatches"] = int((flat[:m, :m].astype(np.int8) != c).sum())
-        iu = np.triu_indices(m, 1)
-        row, col = iu
-        pidx = row * (2 * n - row - 1) // 2 + (col - row - 1)
-        qs_cpu = int(steps[pidx].sum())
+        blk, k, dt, mism, qs_cpu = 64, 0, 0.0, 0, 0
+        row, col = np.triu_indices(blk, 1)
+        while dt < args.cpu_seconds and (k + 1) * blk <= n:
+            lo = k * blk
+            t0 = time.perf_counter()
+            c = ow.visibility_matrix(ms[lo:lo + blk], True, VD, nthreads=_sample_threads())
+            dt += time.perf_counter() - t0
+            mism += int((flat[lo:lo + blk, lo:lo + blk].astype(np.int8) != c).sum())
+            r, cc = row + lo, col + lo
+            qs_cpu += int(steps[r * (2 * n - r - 1) // 2 + (cc - r - 1)].sum())
+            k += 1
+        extra["mismatches"] = mism
         extra["cpu_baseline"] = dict(value=qs_cpu / dt, unit="queries/s", cores=_sample_threads(), kind="port",
-                                     sample=f"{m} x {m} block of the {n} x {n} matrix ({qs_cpu} queries, {dt:.2f} s)")
+                                     sample=f"{k} diagonal 64 x 64 blocks of the {n} x {n} matrix "
+                                            f"({qs_cpu} queries, {dt:.2f} s)")
     line = _line(f"visibility matrix: shell env {len(env[1])} tris / bent rod {len(rob[1])} tris, {n} milestones, "
                  f"{npairs} pairs at step 1/1024", q * args.steps / wall, wall, args, extra)
     line["config"].update({"milestones": n, "pairs": npairs, "queries_per_step": q,
                            "visible_pairs": int(np.unpackbits(bits.view(np.uint8)).sum()) // 2})
+    return line
+
+
+def run_batch(args, torch, pyosr, W):
+    """One persistent launch of uneven segments (lognormal step counts) plus
+    single states, shell scene; the launch shape is tuned on this batch first.
+    Also times the same work as two separate launches."""
+    env, rob = W.shell_scene(0)
+    uw = pyosr.create_unit_world_from_arrays(env, rob)
+    n = args.segments or 4 * 1024 * 1024
+    m = args.batch_states
+    vd = VD
+    starts = _free_states(uw, W, n, 5000)
+    steps_req = W.lognormal_steps(n, seed=5001)
+    ends = W.segment_ends(starts, seed=5002, lengths=steps_req * vd)
+    qs = W.random_states(m, seed=5003)
+    stream = torch.cuda.Stream()
+    torch.cuda.set_stream(stream)
+    dev = torch.device("cuda")
+    d_f, d_t, d_q = (torch.from_numpy(x).to(dev) for x in (starts, ends, qs))
+    d_done = torch.zeros(n, dtype=torch.uint8, device=dev)
+    d_lt = torch.zeros(n, dtype=torch.float64, device=dev)
+    d_ta = torch.zeros(n, dtype=torch.float64, device=dev)
+    d_st = torch.zeros(n, dtype=torch.int32, device=dev)
+    d_v = torch.zeros(m, dtype=torch.uint8, device=dev)
+    outs = (d_done.data_ptr(), d_lt.data_ptr(), d_ta.data_ptr(), d_st.data_ptr())
+    cfg, tuned_ms = uw.batch_tune(d_f.data_ptr(), d_t.data_ptr(), n, vd, *outs, d_q.data_ptr(), m, d_v.data_ptr(),
+                                  repeats=1)
+    timer = _Timer(torch, stream)
+    wall, kern_ms = timer.run(
+        lambda: uw.batch_device(d_f.data_ptr(), d_t.data_ptr(), n, vd, *outs, d_q.data_ptr(), m, d_v.data_ptr(),
+                                stream=stream.cuda_stream), args.steps, args.warmup)
+    uw.check_device()
+    steps = d_st.cpu().numpy().astype(np.int64)
+    done = d_done.cpu().numpy().astype(bool)
+    valid = d_v.cpu().numpy().astype(bool)
+    q = int(steps.sum()) + m
+    # the same work as two launches
+    d_done2 = torch.zeros_like(d_done)
+    d_v2 = torch.zeros_like(d_v)
+    d_st2 = torch.zeros_like(d_st)
+
+    def separate():
+        uw.transit_device(d_f.data_ptr(), d_t.data_ptr(), n, vd, d_done2.data_ptr(), d_lt.data_ptr(),
+                          d_ta.data_ptr(), d_st2.data_ptr(), stream=stream.cuda_stream)
+        uw.valid_states_device(d_q.data_ptr(), m, d_v2.data_ptr(), stream.cuda_stream)
+    _, sep_ms = timer.run(separate, max(1, args.steps // 2), 1)
+    uw.check_device()
+    same = bool(torch.equal(d_done, d_done2) and torch.equal(d_v, d_v2) and torch.equal(d_st, d_st2))
+    extra = {"mismatches": None, "cpu_baseline": None, "kernel": "k_batch", "kernel_ms": round(kern_ms, 4),
+             "tuned_config": cfg, "separate_launches_ms": round(sep_ms, 4), "same_as_separate": same}
+    if not args.no_cpu:
+        from oracle import oracle as O
+        O.set_libm(False)
+        ow = O.OracleWorld().setup(*env, *rob)
+        # chunks of 16,384 segments and 4,096 states (the batch's proportions)
+        # until the time budget is spent
+        ks = kq = 0
+        cq, mism, dt = 0, 0, 0.0
+        while dt < args.cpu_seconds and ks < n:
+            ss = slice(ks, min(n, ks + 16384))
+            sq = slice(kq, min(m, kq + 4096))
+            t0 = time.perf_counter()
+            o = ow.touch_batch(starts[ss], ends[ss], vd, 0, nthreads=_sample_threads())
+            c = ow.valid_states(qs[sq], nthreads=_sample_threads())
+            dt += time.perf_counter() - t0
+            mism += int((o["done"].astype(bool) != done[ss]).sum() + (o["queries"] != steps[ss]).sum()
+                        + (c.astype(bool) != valid[sq]).sum())
+            cq += int(o["queries"].sum()) + (sq.stop - sq.start)
+            ks, kq = ss.stop, sq.stop
+        extra["mismatches"] = mism
+        extra["cpu_baseline"] = dict(value=cq / dt, unit="queries/s", cores=_sample_threads(), kind="port",
+                                     sample=f"first {ks} segments and {kq} states ({cq} queries, {dt:.2f} s)")
+    line = _line(f"mixed batch, one persistent launch: shell env {len(env[1])} tris / bent rod {len(rob[1])} tris, "
+                 f"{n} segments with lognormal(5, 1.2) step counts in [8, 4096] at step 1/1024 plus {m} single "
+                 f"states", q * args.steps / wall, wall, args, extra)
+    line["config"].update({"segments": n, "states": m, "queries_per_step": q, "free_segments": int(done.sum())})
     return line
 
 
@@ -514,13 +611,16 @@ def main():
                     help="queries per launch (default: the workload's own size)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["touch", "states", "uneven", "visibility", "surrogate", "ompl"],
+    ap.add_argument("--batch-states", type=int, default=1024 * 1024,
+                    help="batch workload: single states in the launch beside the segments")
+    ap.add_argument("--workload", choices=["touch", "states", "uneven", "visibility", "surrogate", "ompl", "batch"],
                     default="touch",
                     help="touch (default, the headline): touch-configuration search on the torus scene; "
                          "states: 1M state validity; uneven: 4M lognormal segments on the shell scene; "
                          "visibility: 4,096-milestone matrix on the shell scene; "
                          "surrogate: learned prescreen on the shell scene; "
-                         "ompl: OMPL key screening (all pairs of 1,024 keys) on the alpha puzzle")
+                         "ompl: OMPL key screening (all pairs of 1,024 keys) on the alpha puzzle; "
+                         "batch: 4M uneven segments + 1M states in one persistent launch (shell scene)")
     ap.add_argument("--train-epochs", type=int, default=2)
     ap.add_argument("--threshold", type=float, default=0.5)
     ap.add_argument("--screen-segments", type=int, default=8 * 1024 * 1024,
@@ -549,7 +649,7 @@ def main():
         del src, dst
     if args.workload != "touch":
         fn = {"states": run_states, "uneven": run_uneven, "visibility": run_visibility,
-              "surrogate": run_surrogate, "ompl": run_ompl}[args.workload]
+              "surrogate": run_surrogate, "ompl": run_ompl, "batch": run_batch}[args.workload]
         print(json.dumps(fn(args, torch, pyosr, W)), flush=True)
         return
     env, rob = W.torus_scene(0)

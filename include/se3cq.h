@@ -194,14 +194,19 @@ int se3cq_translate_vanilla_pts_to_unit(const se3cq_world* w, int geo, const dou
  * se3cq_valid_states_* (valid[i] = 1 if free).  Persistent wavefronts take
  * segments and chunks of states from a work queue; the device form is one
  * fill kernel plus one kernel on the stream, so it can be captured in a HIP
- * graph.  Calls that overlap on one world race on its queue word.
- * se3cq_batch_tune times every launch shape on the given batch, keeps the
- * fastest for the world's later batch calls and leaves the batch's results
- * in the outputs; *ms = its time per launch. */
+ * graph.  A split batch (split = 1) runs the persistent segment kernel of
+ * se3cq_transit_* and the state kernel of se3cq_valid_states_* instead,
+ * which wins when the state traversal's LDS stacks would limit how many
+ * segment wavefronts a CU holds; a batch without states always does.  se3cq_batch_tune (batches with states) times every
+ * launch shape on the given batch, keeps the fastest for the world's later
+ * batch calls and leaves the batch's results in the outputs; *ms = its time
+ * per launch. */
 typedef struct se3cq_batch_config {
 	int32_t block_waves;     /* waves per block (1-4; 0 = as many as the LDS allows) */
 	int32_t blocks_per_cu;   /* resident blocks per CU to launch (0 = as many as fit) */
 	int32_t states_per_lane; /* states one lane checks per state work item (>= 1) */
+	int32_t split;           /* 1: the segment and state kernels, one after the other on the stream;
+	                          * 0: one kernel takes both kinds of work */
 } se3cq_batch_config;
 int se3cq_batch_device(se3cq_world* w, const double* d_seg_from, const double* d_seg_to, int64_t n_seg,
                        double verify_delta, double* d_last_free, double* d_state, uint8_t* d_done, double* d_last_tau,

@@ -23,7 +23,7 @@ vp = ctypes.c_void_p
 class BatchConfig(ctypes.Structure):
     """se3cq_batch_config."""
     _fields_ = [("block_waves", ctypes.c_int32), ("blocks_per_cu", ctypes.c_int32),
-                ("states_per_lane", ctypes.c_int32)]
+                ("states_per_lane", ctypes.c_int32), ("split", ctypes.c_int32)]
 i64 = ctypes.c_int64
 cint = ctypes.c_int
 cdbl = ctypes.c_double

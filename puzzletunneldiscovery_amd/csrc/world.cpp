@@ -119,7 +119,7 @@ struct se3cq_world {
 	DevBuf d_queue;
 	std::atomic<uint32_t> queue_slot{0};
 	int num_cu = 0;
-	se3cq_batch_config batch_cfg = {0, 0, 1};
+	se3cq_batch_config batch_cfg = {0, 0, 1, 1};
 	bool pert_uploaded = false;
 	DevBuf ws[12];
 	std::mutex mu;
@@ -1767,6 +1767,16 @@ static BatchArgs batch_args(se3cq_world* w, const double* d_seg_from, const doub
 static int batch_launch(se3cq_world* w, const BatchArgs& a, const se3cq_batch_config& c, hipStream_t st)
 {
 	const DevScene s = make_scene(w);
+	if (a.n_states == 0 || c.split) {
+		/* the persistent segment kernel (no traversal stacks in LDS), then
+		 * the state kernel */
+		hipError_t e = launch_segments(s, 0, a.seg_from, a.seg_to, a.n_seg, a.vd, 0, a.seg, st);
+		if (e == hipSuccess && a.n_states > 0)
+			e = launch_valid_states(s, a.states, a.n_states, a.valid, st);
+		if (e != hipSuccess)
+			return fail(SE3CQ_EDEVICE, std::string("launch failed: ") + hipGetErrorString(e));
+		return SE3CQ_OK;
+	}
 	int block = 0, grid = 0;
 	batch_shape(w, s, c, a.n_states > 0, &block, &grid);
 	const hipError_t e = launch_batch(s, a, block, grid, st);
@@ -1821,10 +1831,9 @@ int se3cq_batch_host(se3cq_world* w, const double* seg_from, const double* seg_t
 		HIPCHK(hipMemcpyAsync(dq, states, 7 * nq * sizeof(double), hipMemcpyHostToDevice, w->stream));
 	const BatchArgs a = batch_args(w, df, dt, n_seg, verify_delta, dlf, dst, dd, dlt, dta, dsp, dq, n_states, dv,
 	                               w->batch_cfg.states_per_lane);
-	const DevScene s = make_scene(w);
-	int block = 0, grid = 0;
-	batch_shape(w, s, w->batch_cfg, n_states > 0, &block, &grid);
-	const int rc = sync_and_check(w, w->stream, launch_batch(s, a, block, grid, w->stream));
+	int rc = batch_launch(w, a, w->batch_cfg, w->stream);
+	if (rc == SE3CQ_OK)
+		rc = sync_and_check(w, w->stream, hipSuccess);
 	if (rc)
 		return rc;
 	if (ns) {
@@ -1855,7 +1864,7 @@ int se3cq_batch_set_config(se3cq_world* w, const se3cq_batch_config* cfg)
 {
 	CHECK_WORLD(w);
 	if (!cfg || cfg->block_waves < 0 || cfg->block_waves > 4 || cfg->blocks_per_cu < 0 || cfg->states_per_lane < 1 ||
-	    cfg->states_per_lane > 64)
+	    cfg->states_per_lane > 64 || cfg->split < 0 || cfg->split > 1)
 		return fail(SE3CQ_EINVAL, "bad batch configuration");
 	w->batch_cfg = *cfg;
 	return SE3CQ_OK;
@@ -1867,29 +1876,31 @@ int se3cq_batch_tune(se3cq_world* w, const double* d_seg_from, const double* d_s
                      int32_t repeats, se3cq_batch_config* chosen, float* ms)
 {
 	/* Times every candidate launch shape on this batch (on the world's own
-	 * stream, synchronously) and keeps the fastest: waves per block (1, 2,
-	 * 4, as the LDS of the state stacks allows), resident blocks per CU (as
-	 * many as fit, or half), and states per lane of a state work item (1,
-	 * 2, 4; only when the batch has states).  The outputs hold the batch's
-	 * results afterwards, since every shape computes the same answers. */
+	 * stream, synchronously) and keeps the fastest: the split form (segment
+	 * kernel, then state kernel), and the one-kernel form with waves per
+	 * block 1, 2, 4 (as the LDS of the state stacks allows), resident blocks
+	 * per CU (as many as fit, or half) and states per lane of a state work
+	 * item (1, 2, 4).  The outputs hold the batch's results afterwards, since
+	 * every shape computes the same answers. */
 	CHECK_WORLD(w);
 	const int rc0 = batch_check(w, n_seg, verify_delta, d_last_free, d_state, d_done, n_states, d_valid);
 	if (rc0)
 		return rc0;
 	if (repeats < 1 || repeats > 100)
 		return fail(SE3CQ_EINVAL, "repeats must be in [1, 100]");
-	if (n_seg == 0 && n_states == 0)
-		return fail(SE3CQ_EINVAL, "an empty batch cannot be timed");
+	if (n_states == 0)
+		return fail(SE3CQ_EINVAL, "a batch without states has one launch shape (the segment kernel's); nothing to tune");
 	const DevScene s = make_scene(w);
 	const bool states = n_states > 0;
 	const int bmax = batch_default_block(s, states) / 64;
 	std::vector<se3cq_batch_config> cands;
+	cands.push_back(se3cq_batch_config{0, 0, 1, 1});
 	for (int bw = 1; bw <= bmax; bw *= 2) {
 		const int full = batch_blocks_per_cu(s, 64 * bw, states);
 		const int halves[2] = {full, (full + 1) / 2};
 		for (int h = 0; h < (full > 1 ? 2 : 1); h++)
 			for (int spl = 1; spl <= (n_states > 0 ? 4 : 1); spl *= 2)
-				cands.push_back(se3cq_batch_config{bw, halves[h], spl});
+				cands.push_back(se3cq_batch_config{bw, halves[h], spl, 0});
 	}
 	hipStream_t st = w->stream;
 	hipEvent_t e0, e1;

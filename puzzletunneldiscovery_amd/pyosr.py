@@ -512,7 +512,7 @@ class UnitWorld:
     @batch_config.setter
     def batch_config(self, c):
         cfg = N.BatchConfig(int(c.get("block_waves", 0)), int(c.get("blocks_per_cu", 0)),
-                            int(c.get("states_per_lane", 1)))
+                            int(c.get("states_per_lane", 1)), int(c.get("split", 0)))
         N.check(N.lib().se3cq_batch_set_config(self._w, ctypes.byref(cfg)))
 
     def check_device(self):

@@ -70,13 +70,13 @@ def test_every_launch_shape_agrees(gpu_world):
     ref = gpu_world.batch(starts, ends, VD, qs)
     keep = gpu_world.batch_config
     try:
-        for bw in (1, 2, 4):
-            for per_cu in (0, 1):
-                for spl in (1, 3):
-                    gpu_world.batch_config = {"block_waves": bw, "blocks_per_cu": per_cu, "states_per_lane": spl}
-                    got = gpu_world.batch(starts, ends, VD, qs)
-                    for k in ("done", "last_tau", "tau", "steps", "valid"):
-                        assert np.array_equal(got[k], ref[k]), (bw, per_cu, spl, k)
+        shapes = [{"split": 1}] + [{"block_waves": bw, "blocks_per_cu": per_cu, "states_per_lane": spl, "split": 0}
+                                   for bw in (1, 2, 4) for per_cu in (0, 1) for spl in (1, 3)]
+        for shape in shapes:
+            gpu_world.batch_config = shape
+            got = gpu_world.batch(starts, ends, VD, qs)
+            for k in ("done", "last_tau", "tau", "steps", "valid"):
+                assert np.array_equal(got[k], ref[k]), (shape, k)
     finally:
         gpu_world.batch_config = keep
     with pytest.raises(RuntimeError):
@@ -107,7 +107,7 @@ def test_batch_tune_keeps_results(gpu_world):
                                        *[o.data_ptr() for o in outs[:4]], ins[2].data_ptr(), len(qs),
                                        outs[4].data_ptr(), repeats=2)
         assert ms > 0 and cfg == gpu_world.batch_config
-        assert cfg["block_waves"] in (1, 2, 4) and cfg["states_per_lane"] in (1, 2, 4)
+        assert cfg["split"] == 1 or (cfg["block_waves"] in (1, 2, 4) and cfg["states_per_lane"] in (1, 2, 4))
         torch.cuda.synchronize()
         for o, k in zip(outs, ("done", "last_tau", "tau", "steps", "valid")):
             assert np.array_equal(o.cpu().numpy().astype(ref[k].dtype), ref[k]), k
