@@ -36,12 +36,12 @@ C_TRI = 753
 PEAK_FP32_TFLOPS = 157.3
 # HBM bytes per launch of the touch kernel on this workload, from rocprofv3
 # --pmc FETCH_SIZE and WRITE_SIZE (KiB per dispatch), each scaled by what the
-# same counter reads for a 256 MiB device copy (FETCH_SIZE 131,093 KiB, i.e.
+# same counter reads for a 256 MiB device copy (FETCH_SIZE 131,094 KiB, i.e.
 # x2.000; WRITE_SIZE 262,144 KiB, x1.000); see
 # profiles/k_segments_touch_torus65536.md.  Measured, not computed here.
-TRAFFIC_FETCH_KIB = 74615
-TRAFFIC_WRITE_KIB = 604210
-FETCH_SCALE = 262144 / 131093
+TRAFFIC_FETCH_KIB = 42874
+TRAFFIC_WRITE_KIB = 119098
+FETCH_SCALE = 262144 / 131094
 WRITE_SCALE = 1.0
 
 
