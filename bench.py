@@ -81,8 +81,19 @@ def cpu_baseline(ow, starts, ends, gpu_res, target_s):
         if key == "done":
             a = a.astype(bool)
         mism += int((a != b).sum())
-    return dict(value=q / dt, unit="queries/s", cores=threads, kind="port",
-                sample=f"first {n} of the {len(starts)} touch segments ({q} queries, {dt:.1f} s)"), mism, n
+    # a workload shorter than the budget is timed again until the budget is
+    # spent, so the rate is not a fraction-of-a-second reading
+    passes, q_all, dt_all = 1, q, dt
+    while dt_all < target_s:
+        t0 = time.perf_counter()
+        o = ow.touch_batch(starts[:n], ends[:n], VD, LEVELS, nthreads=threads)
+        dt_all += time.perf_counter() - t0
+        q_all += int(o["queries"].sum())
+        passes += 1
+    what = f"{passes} passes over " if passes > 1 else ""
+    return dict(value=q_all / dt_all, unit="queries/s", cores=threads, kind="port",
+                sample=f"{what}first {n} of the {len(starts)} touch segments ({q} queries per pass, "
+                       f"{dt_all:.1f} s)"), mism, n
 
 
 def oracle_flops_per_query(ow, starts, ends, n):
@@ -176,9 +187,18 @@ def run_states(args, torch, pyosr, W):
             m = sl.stop
         nbv = np.concatenate([a for a, _ in cs])
         ntri = np.concatenate([b for _, b in cs])
+        # the whole workload takes well under the budget: time further passes
+        passes, done_q = 1, m
+        while dt < args.cpu_seconds:
+            t0 = time.perf_counter()
+            ow.valid_states(qs[:m], nthreads=_sample_threads())
+            dt += time.perf_counter() - t0
+            done_q += m
+            passes += 1
         extra["mismatches"] = mism
-        extra["cpu_baseline"] = dict(value=m / dt, unit="queries/s", cores=_sample_threads(), kind="port",
-                                     sample=f"first {m} of the {n} states ({dt:.2f} s)")
+        what = f"{passes} passes over " if passes > 1 else ""
+        extra["cpu_baseline"] = dict(value=done_q / dt, unit="queries/s", cores=_sample_threads(), kind="port",
+                                     sample=f"{what}first {m} of the {n} states ({dt:.2f} s)")
         fq = C_BV * nbv.mean() + C_TRI * ntri.mean()
         extra["flops_per_query"] = round(float(fq), 1)
         extra["achieved_tflops"] = round(n * fq / (kern_ms * 1e-3) / 1e12, 3)

@@ -504,34 +504,214 @@ __device__ __forceinline__ void thread_stacks(const DevScene& s, uint32_t* smem,
 	leaves = PairStack<WIDE>{smem + threadIdx.x + (size_t)s.stack_depth * (WIDE ? 2 : 1) * stride, stride};
 }
 
+/* Where a state query finds its binary64 unit-world state i. */
+struct StatesPlain {
+	const double* qs;
+	__device__ __forceinline__ void get(int64_t i, double* o) const
+	{
+#pragma unroll
+		for (int k = 0; k < 7; k++)
+			o[k] = qs[7 * i + k];
+	}
+};
+
+struct StatesOmpl {
+	const double* qs; /* OMPL order (x, y, z, qx, qy, qz, qw) */
+	const OmplXf* x;
+	__device__ __forceinline__ void get(int64_t i, double* o) const
+	{
+		se3_ompl_to_unit(x->M, x->pert, x->delta, qs + 7 * i, o);
+	}
+};
+
+#ifndef SE3CQ_REFILL_MIN
+#define SE3CQ_REFILL_MIN 16
+#endif
+#ifndef SE3CQ_LEAF_PATIENCE
+#define SE3CQ_LEAF_PATIENCE 32
+#endif
+#ifndef SE3CQ_REFILL_CHUNK
+#define SE3CQ_REFILL_CHUNK 128
+#endif
+enum { REFILL_MIN = SE3CQ_REFILL_MIN, LEAF_PATIENCE = SE3CQ_LEAF_PATIENCE, REFILL_CHUNK = SE3CQ_REFILL_CHUNK };
+
+__device__ __forceinline__ int lane_rank(uint64_t m);
+
+/*
+ * State validity with lane refill.  Independent states traverse very
+ * differently: most are culled at the root within a step, a few near the
+ * surface walk deep, and a wavefront holding one state per lane runs until
+ * its slowest lane ends (7 % of VALU lanes were doing work).  Here every lane
+ * keeps one state in flight: once REFILL_MIN lanes are idle they take the
+ * next states from the wavefront's pool, which takes REFILL_CHUNK states at
+ * a time from the launch's counter (one atomic per chunk), and each
+ * pass of the loop is, wave-uniformly, either a traversal pass (one node
+ * pair per traversing lane) or a leaf pass (one exact binary64 SAT per lane
+ * whose leaf buffer is full or whose traversal is over), so the expensive
+ * leaf test never diverges against traversal.  A leaf pass runs once
+ * REFILL_MIN lanes wait for one, when no lane traverses, or after
+ * LEAF_PATIENCE traversal passes, so no lane waits without bound.  The
+ * answer, "some triangle pair intersects", does not depend on the order
+ * pairs are visited in.  set(i, hit) stores state i's answer.
+ */
+template <bool WIDE, class Loader, class Store>
+__device__ void states_refill(const DevScene& s, const Loader& ld, int64_t n, const Store& set,
+                              const PairStack<WIDE>& stk, const PairStack<WIDE>& leaves)
+{
+	const int lane = threadIdx.x & 63;
+	const int cap = s.stack_depth;
+	int64_t idx = -1;
+	int sp = 0, nleaf = 0, cached_b = -1;
+	QCull c;
+	RobBox B;
+	/* wave-uniform: the counter is spent, and the pool [pool_base,
+	 * pool_base + pool_left) of states taken but not yet started */
+	bool exhausted = false;
+	int64_t pool_base = 0;
+	int pool_left = 0;
+	int since_leaf = 0;
+	for (;;) {
+		const uint64_t idle = __ballot(idx < 0);
+		const int nidle = __popcll(idle);
+		if ((nidle >= REFILL_MIN || nidle == 64) && (pool_left > 0 || !exhausted)) {
+			int64_t base2 = 0;
+			int left2 = 0;
+			if (pool_left < nidle && !exhausted) {
+				unsigned long long got = 0;
+				if (lane == 0)
+					got = atomicAdd(s.queue, (unsigned long long)REFILL_CHUNK);
+				base2 = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(got >> 32)) << 32) |
+				                  (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)got));
+				if (base2 + REFILL_CHUNK >= n)
+					exhausted = true;
+				left2 = base2 < n ? (int)((n - base2) < REFILL_CHUNK ? (n - base2) : REFILL_CHUNK) : 0;
+			}
+			const int take = nidle < pool_left + left2 ? nidle : pool_left + left2;
+			if (idx < 0) {
+				const int r = lane_rank(idle);
+				int64_t i = -1;
+				if (r < pool_left)
+					i = pool_base + r;
+				else if (r - pool_left < left2)
+					i = base2 + (r - pool_left);
+				if (i >= 0) {
+					idx = i;
+					double q[7];
+					ld.get(i, q);
+					make_qcull(s, q, c);
+					sp = 0;
+					stk.put(sp++, 0, 0);
+					nleaf = 0;
+					cached_b = -1;
+				}
+			}
+			if (take <= pool_left) {
+				pool_base += take;
+				pool_left -= take;
+			} else {
+				pool_base = base2 + (take - pool_left);
+				pool_left = left2 - (take - pool_left);
+			}
+		}
+		const bool act = idx >= 0;
+		const uint64_t am = __ballot(act);
+		if (am == 0) {
+			if (exhausted && pool_left == 0)
+				break;
+			continue;
+		}
+		const bool want_leaf = act && (nleaf == LEAF_BUF || (sp == 0 && nleaf > 0));
+		const uint64_t wl = __ballot(want_leaf);
+		if (wl != 0 && (__popcll(wl) >= REFILL_MIN || wl == am || since_leaf >= LEAF_PATIENCE)) {
+			since_leaf = 0;
+			if (want_leaf) {
+				int te, tr;
+				leaves.get(--nleaf, te, tr);
+				double q[7];
+				ld.get(idx, q);
+				if (leaf_pair(s, q, te, tr)) {
+					set(idx, true);
+					idx = -1;
+				} else if (sp == 0 && nleaf == 0) {
+					set(idx, false);
+					idx = -1;
+				}
+			}
+			continue;
+		}
+		since_leaf++;
+		if (!act || want_leaf)
+			continue;
+		int a, b;
+		stk.get(--sp, a, b);
+		const GNode A = load_node(s.env.nodes, a);
+		if (b != cached_b) {
+			const GNode Bn = load_node(s.rob.nodes, b);
+			load_rob_box(Bn, c, B);
+			cached_b = b;
+		}
+		if (!obb_disjoint(A, B, c.eta)) {
+			const int ca = __float_as_int(A.a2_child.w);
+			const bool la = ca < 0, lb = B.child < 0;
+			if (la && lb) {
+				leaves.put(nleaf++, ~ca, ~B.child);
+			} else if (sp + 2 > cap) {
+				atomicOr(s.error_flag, 1u);
+				set(idx, true);
+				idx = -1;
+				continue;
+			} else {
+				/* FCL's rule: split the env node unless it is a leaf or the
+				 * robot box is the larger one (squared extent norms). */
+				const float sa = fmaf(A.c_ex.w, A.c_ex.w, fmaf(A.a0_ey.w, A.a0_ey.w, A.a1_ez.w * A.a1_ez.w));
+				const float eb0 = B.e[0] - c.infl, eb1 = B.e[1] - c.infl, eb2 = B.e[2] - c.infl;
+				const float sb = fmaf(eb0, eb0, fmaf(eb1, eb1, eb2 * eb2));
+				if (!la && (lb || sa > sb)) {
+					stk.put(sp++, ca + 1, b);
+					stk.put(sp++, ca, b);
+				} else {
+					stk.put(sp++, a, B.child + 1);
+					stk.put(sp++, a, B.child);
+				}
+			}
+		}
+		if (sp == 0 && nleaf == 0) {
+			set(idx, false);
+			idx = -1;
+		}
+	}
+}
+
+struct StoreValidU8 {
+	uint8_t* out;
+	__device__ __forceinline__ void operator()(int64_t i, bool hit) const { out[i] = hit ? 0 : 1; }
+};
+
+struct StoreValidI32 {
+	int32_t* out;
+	__device__ __forceinline__ void operator()(int64_t i, bool hit) const { out[i] = hit ? 0 : 1; }
+};
+
 template <bool WIDE>
 __global__ void __launch_bounds__(256, SE3CQ_MIN_WAVES) k_valid_states(DevScene s, const double* __restrict__ qs, int64_t n,
                                                       uint8_t* __restrict__ out)
 {
 	extern __shared__ uint32_t smem[];
-	const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= n)
-		return;
-	const double* qp = qs + 7 * i;
 	PairStack<WIDE> stk, leaves;
 	thread_stacks<WIDE>(s, smem, stk, leaves);
-	out[i] = collide_dfs<WIDE>(s, StateFromGlobal{qp}, stk, leaves) ? 0 : 1;
+	states_refill<WIDE>(s, StatesPlain{qs}, n, StoreValidU8{out}, stk, leaves);
 }
 
 /* OMPL-convention states (OmplDriver::validateStates,
- * lib/pyse3ompl/ompldriver.cc:566-589): converted to the unit world on load,
- * then the per-thread traversal.  out: 1 = valid. */
+ * lib/pyse3ompl/ompldriver.cc:566-589): converted to the unit world on load. */
 template <bool WIDE>
 __global__ void __launch_bounds__(256, SE3CQ_MIN_WAVES) k_ompl_states(DevScene s, OmplXf x, const double* __restrict__ qs,
                                                      int64_t n, int32_t* __restrict__ out)
 {
 	extern __shared__ uint32_t smem[];
-	const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= n)
-		return;
 	PairStack<WIDE> stk, leaves;
 	thread_stacks<WIDE>(s, smem, stk, leaves);
-	out[i] = collide_dfs<WIDE>(s, StateFromOmpl{qs + 7 * i, &x}, stk, leaves) ? 0 : 1;
+	states_refill<WIDE>(s, StatesOmpl{qs, &x}, n, StoreValidI32{out}, stk, leaves);
 }
 
 /* ---- packet traversal: one wavefront, one pose per lane ---- */
@@ -1775,12 +1955,17 @@ hipError_t launch_valid_states(const DevScene& s, const double* q, int64_t n, ui
 	if (n <= 0)
 		return hipSuccess;
 	const int block = pick_block(s);
-	const int64_t grid = (n + block - 1) / block;
 	const size_t lds = stack_bytes(s, block);
+	hipError_t e = zero_queue(s, st);
+	if (e != hipSuccess)
+		return e;
 	if (s.wide)
-		hipLaunchKernelGGL(k_valid_states<true>, dim3((unsigned)grid), dim3(block), lds, st, s, q, n, out);
+		hipLaunchKernelGGL(k_valid_states<true>, dim3(persistent_grid(s, k_valid_states<true>, block, lds, (n + 63) / 64)),
+		                   dim3(block), lds, st, s, q, n, out);
 	else
-		hipLaunchKernelGGL(k_valid_states<false>, dim3((unsigned)grid), dim3(block), lds, st, s, q, n, out);
+		hipLaunchKernelGGL(k_valid_states<false>,
+		                   dim3(persistent_grid(s, k_valid_states<false>, block, lds, (n + 63) / 64)), dim3(block), lds,
+		                   st, s, q, n, out);
 	return hipGetLastError();
 }
 
@@ -1790,12 +1975,17 @@ hipError_t launch_ompl_states(const DevScene& s, const OmplXf& x, const double* 
 	if (n <= 0)
 		return hipSuccess;
 	const int block = pick_block(s);
-	const int64_t grid = (n + block - 1) / block;
 	const size_t lds = stack_bytes(s, block);
+	hipError_t e = zero_queue(s, st);
+	if (e != hipSuccess)
+		return e;
 	if (s.wide)
-		hipLaunchKernelGGL(k_ompl_states<true>, dim3((unsigned)grid), dim3(block), lds, st, s, x, q, n, out);
+		hipLaunchKernelGGL(k_ompl_states<true>, dim3(persistent_grid(s, k_ompl_states<true>, block, lds, (n + 63) / 64)),
+		                   dim3(block), lds, st, s, x, q, n, out);
 	else
-		hipLaunchKernelGGL(k_ompl_states<false>, dim3((unsigned)grid), dim3(block), lds, st, s, x, q, n, out);
+		hipLaunchKernelGGL(k_ompl_states<false>,
+		                   dim3(persistent_grid(s, k_ompl_states<false>, block, lds, (n + 63) / 64)), dim3(block), lds,
+		                   st, s, x, q, n, out);
 	return hipGetLastError();
 }
 
