@@ -247,3 +247,82 @@ def test_contacts(gpu_world, torus_oracle):
     assert np.allclose(mag, np.linalg.norm(p1 - p0, axis=1))
     free = W.random_states(300, seed=121)[valid][:5]
     assert not gpu_world.contacts(free)[0].any()
+
+
+def test_concurrent_streams_share_a_world(gpu_world, torus_oracle):
+    """Persistent kernels take a fresh work counter per launch, so device
+    calls in flight on different streams of one world do not interfere."""
+    import torch
+    starts = _free_starts(gpu_world, 1500, 131)
+    ends = W.segment_ends(starts, seed=132)
+    qs = W.random_states(50_000, seed=133)
+    n, m = len(starts), len(qs)
+    dev = torch.device("cuda")
+    df, dt, dq = (torch.from_numpy(x).to(dev) for x in (starts, ends, qs))
+    streams = [torch.cuda.Stream() for _ in range(3)]
+    outs = []
+    for rep in range(2):
+        o = [torch.zeros(n, dtype=t, device=dev) for t in (torch.uint8, torch.float64, torch.float64,
+                                                             torch.float64, torch.float64, torch.int32)]
+        v = torch.zeros(m, dtype=torch.uint8, device=dev)
+        tr = [torch.zeros(n, dtype=t, device=dev) for t in (torch.uint8, torch.float64, torch.float64, torch.int32)]
+        torch.cuda.synchronize()
+        gpu_world.touch_search_device(df.data_ptr(), dt.data_ptr(), n, 1.0 / 1024, 20, *[x.data_ptr() for x in o],
+                                      stream=streams[0].cuda_stream)
+        gpu_world.valid_states_device(dq.data_ptr(), m, v.data_ptr(), streams[1].cuda_stream)
+        gpu_world.transit_device(df.data_ptr(), dt.data_ptr(), n, 1.0 / 1024, *[x.data_ptr() for x in tr],
+                                 stream=streams[2].cuda_stream)
+        torch.cuda.synchronize()
+        gpu_world.check_device()
+        outs.append((o, v, tr))
+    ref = torus_oracle.touch_batch(starts, ends, 1.0 / 1024, 20)
+    want_v = torus_oracle.valid_states(qs).astype(np.uint8)
+    for o, v, tr in outs:
+        assert np.array_equal(o[3].cpu().numpy(), ref["lo"]) and np.array_equal(o[4].cpu().numpy(), ref["hi"])
+        assert np.array_equal(v.cpu().numpy(), want_v)
+        assert np.array_equal(tr[0].cpu().numpy(), ref["done"].astype(np.uint8))
+
+
+def test_persistent_entry_points_in_hip_graph(gpu_world, torus_oracle):
+    """State validity (lane refill), OMPL motions and the bit-packed
+    visibility matrix captured in one graph: every replay matches eager."""
+    import torch
+    from puzzletunneldiscovery_amd import pyse3ompl
+    dev = torch.device("cuda")
+    qs = W.random_states(20_000, seed=141)
+    ms = _free_starts(gpu_world, 96, 142)
+    drv = pyse3ompl.OmplDriver.from_unit_world(gpu_world)
+    drv.set_bb([0, 0, 0], [1, 1, 1])
+    drv.set_cdres(0.002)
+    q0 = W.random_states(400, seed=143)[:, [0, 1, 2, 4, 5, 6, 3]].copy()
+    q1 = q0.copy()
+    q1[:, :3] += np.random.default_rng(144).normal(0, 0.1, (400, 3))
+    dq, dms, d0, d1 = (torch.from_numpy(np.ascontiguousarray(x)).to(dev) for x in (qs, ms, q0, q1))
+    v = torch.zeros(len(qs), dtype=torch.uint8, device=dev)
+    words = (len(ms) * len(ms) + 31) // 32
+    bits = torch.zeros(words, dtype=torch.int32, device=dev)
+    mo = torch.zeros(len(q0), dtype=torch.int32, device=dev)
+    s = torch.cuda.Stream()
+
+    def call():
+        gpu_world.valid_states_device(dq.data_ptr(), len(qs), v.data_ptr(), s.cuda_stream)
+        gpu_world.visibility_bits_device(dms.data_ptr(), len(ms), 1.0 / 256, bits.data_ptr(), None, s.cuda_stream)
+        drv.validate_motion_pairs_device(d0.data_ptr(), d1.data_ptr(), len(q0), mo.data_ptr(), None,
+                                         stream=s.cuda_stream)
+
+    with torch.cuda.stream(s):
+        call()
+    torch.cuda.synchronize()
+    eager = [x.clone() for x in (v, bits, mo)]
+    assert np.array_equal(eager[0].cpu().numpy(), torus_oracle.valid_states(qs))
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        call()
+    for rep in range(3):
+        for x in (v, bits, mo):
+            x.fill_(7 if x.dtype != torch.uint8 else 3)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        for a, b in zip((v, bits, mo), eager):
+            assert torch.equal(a, b), rep
