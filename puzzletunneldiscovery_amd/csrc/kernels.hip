@@ -778,8 +778,20 @@ __device__ __forceinline__ bool leaf_pair_x(const DevScene& s, const ExactTf& x,
  * lanes; leaf phases run the exact binary64 tests on the queue.  The culling
  * transform is rebuilt at the start of every traversal phase so it is dead
  * while the SATs run.
+ *
+ * With FIRST, pairs are taken in "focus" order: the topmost stack entry
+ * holding the lowest lane that still has pending pairs.  Lanes are then
+ * finished one after another from the bottom, so the pair that collides for
+ * the lowest colliding lane is reached early and its hit drops every lane
+ * above it.  Near contact, where the 64 consecutive poses straddle the first
+ * colliding one, this takes about a third fewer pair tests than plain depth
+ * first order, and half as many on the heaviest windows.  Focus order lets
+ * entries of the lanes above pile up, so once the stack holds more than
+ * 62 - stack_depth entries, pairs are popped from the top: a depth-first
+ * descent from there adds at most stack_depth - 2 entries, which keeps the
+ * stack within its 64 lanes.
  */
-enum { PACKET_LEAVES = 32 };
+enum { PACKET_LEAVES = 8 };
 
 /*
  * Where the traversal gets its pose.  The pose is recomputed at the two
@@ -825,6 +837,8 @@ __device__ uint64_t packet_collide(const DevScene& s, const Pose& pose, uint64_t
 	vmh = wrl((int)(uint32_t)(live >> 32), 0, vmh);
 	int sp = 1;
 	uint64_t hit = 0;
+	int focus = __ffsll((long long)live) - 1;
+	const int focus_cap = 62 - s.stack_depth;
 	for (;;) {
 		int lte = 0, ltr = 0, lml = 0, lmh = 0;
 		int nl = 0;
@@ -840,9 +854,35 @@ __device__ uint64_t packet_collide(const DevScene& s, const Pose& pose, uint64_t
 			int cached_b = -1;
 			RobBox B;
 			while (sp > 0 && nl < PACKET_LEAVES) {
+				int p = sp - 1;
+				if (FIRST && sp <= focus_cap) {
+					/* lanes below `focus` have no pending pairs; lanes at
+					 * or above the lowest hit need none */
+					uint64_t hb = 0;
+					const int stop = hit ? __ffsll((long long)hit) - 1 : 64;
+					while (focus < stop) {
+						const int word = (focus < 32) ? vml : vmh;
+						hb = __ballot(lane < sp && ((word >> (focus & 31)) & 1));
+						if (hb)
+							break;
+						focus++;
+					}
+					if (!hb) {
+						sp = 0;
+						break;
+					}
+					p = 63 - __clzll((long long)hb);
+				}
+				const int a = rdl(va, p), b = rdl(vb, p);
+				uint64_t m = ((uint64_t)(uint32_t)rdl(vmh, p) << 32) | (uint64_t)(uint32_t)rdl(vml, p);
 				sp--;
-				const int a = rdl(va, sp), b = rdl(vb, sp);
-				uint64_t m = ((uint64_t)(uint32_t)rdl(vmh, sp) << 32) | (uint64_t)(uint32_t)rdl(vml, sp);
+				if (p != sp) {
+					/* the top entry fills the hole */
+					va = wrl(rdl(va, sp), p, va);
+					vb = wrl(rdl(vb, sp), p, vb);
+					vml = wrl(rdl(vml, sp), p, vml);
+					vmh = wrl(rdl(vmh, sp), p, vmh);
+				}
 				m &= ~hit;
 				if (FIRST && hit)
 					m &= (hit & (~hit + 1ull)) - 1ull;
@@ -869,7 +909,7 @@ __device__ uint64_t packet_collide(const DevScene& s, const Pose& pose, uint64_t
 					nl++;
 					continue;
 				}
-				if (sp + 2 > s.stack_depth || sp + 2 > 64) {
+				if (sp + 2 > 64 || (!FIRST && sp + 2 > s.stack_depth)) {
 					if (lane == 0)
 						atomicOr(s.error_flag, 1u);
 					return live;
