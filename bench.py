@@ -39,8 +39,8 @@ PEAK_FP32_TFLOPS = 157.3
 # same counter reads for a 256 MiB device copy (FETCH_SIZE 131,094 KiB, i.e.
 # x2.000; WRITE_SIZE 262,144 KiB, x1.000); see
 # profiles/k_segments_touch_torus65536.md.  Measured, not computed here.
-TRAFFIC_FETCH_KIB = 42874
-TRAFFIC_WRITE_KIB = 119098
+TRAFFIC_FETCH_KIB = 45099
+TRAFFIC_WRITE_KIB = 127849
 FETCH_SCALE = 262144 / 131094
 WRITE_SCALE = 1.0
 
