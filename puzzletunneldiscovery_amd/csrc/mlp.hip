@@ -29,6 +29,7 @@
 namespace se3mlp {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 enum { IN = 32, HID = 256, OUT = 32, LAYERS = 5, OUT_USED = 2 };
@@ -261,16 +262,21 @@ __global__ void k_cast(const float* __restrict__ w, int rows, int cols, uint16_t
  * four hidden layers and the head, with the activations of the tile kept in
  * LDS (rows padded by 8 bf16 so the 32 rows a fragment load touches spread
  * over the banks; 67.6 KB, two blocks per CU).  Wave w computes output
- * features [64w, 64w + 64) of every hidden layer for all 128 states (4 x 2
+ * features [64w, 64w + 64) of every hidden layer for all 128 states (2 x 4
  * MFMA tiles of 32 x 32), so each block reads each weight matrix once from
- * L2 and every weight fragment feeds four MFMAs.  The next k-step's weight
- * fragments are loaded while the current step's MFMAs run.  The head (32
- * outputs, 2 used) gives each wave one 32-state tile.
+ * L2 and every weight fragment feeds four MFMAs.  The tiles are computed
+ * transposed (weights as the A operand, activations as B), so each lane's
+ * accumulator holds four consecutive features of one state: the epilogue
+ * converts them with v_cvt_pk_bf16_f32 (round to nearest even, two per
+ * instruction) and writes them with one 8-byte LDS store.  The next
+ * k-step's weight fragments are loaded while the current step's MFMAs run.
+ * The head (32 outputs, 2 used) gives each wave one 32-state tile.
  *
- * Measured on 64M samples: 71.7 ms with 64 states per block (four blocks
- * per CU), 47.8 ms with 128.  Computing the tiles transposed (so the
- * epilogue writes 8-byte stores), loading activation fragments a step
- * ahead, or weights two steps ahead all measured 48.5-51.7 ms.
+ * Measured on 64M samples: 71.7 ms with 64 states per block, 47.8 ms with
+ * 128, and 33.7 ms once the epilogue left software rounding (which had made
+ * the kernel VALU-bound: 17 VALU instructions per MFMA).  Loading
+ * activation fragments a step ahead, or weights two steps ahead, measured
+ * slower.
  */
 enum { FM = 128, ALD = HID + 8 };
 static const size_t MLP_LDS = (size_t)FM * ALD * sizeof(uint16_t);
@@ -327,8 +333,10 @@ __global__ void __launch_bounds__(256, 2) k_mlp_infer(const double* __restrict__
 				f[9 + k] = (float)(s[k] - 0.5);
 		}
 #pragma unroll
-		for (int k = 0; k < IN; k++)
-			act[threadIdx.x * ALD + k] = f2bf(f[k]);
+		for (int k = 0; k < IN; k += 2) {
+			const bf16x2 p = {(__bf16)f[k], (__bf16)f[k + 1]};
+			std::memcpy(act + threadIdx.x * ALD + k, &p, 4);
+		}
 	}
 	__syncthreads();
 	const int nb = 64 * wave;
@@ -336,35 +344,47 @@ __global__ void __launch_bounds__(256, 2) k_mlp_infer(const double* __restrict__
 		const int K = l == 0 ? IN : HID;
 		const uint16_t* W0 = w.W[l] + (int64_t)(nb + r) * K + kq;
 		const uint16_t* W1 = W0 + 32 * K;
-		f32x16 c[4][2];
+		/* c[h][t]: features nb + 32h + [0, 32) x states 32t + [0, 32) */
+		f32x16 c[2][4];
 #pragma unroll
 		for (int t = 0; t < 4; t++)
-			c[t][0] = c[t][1] = f32x16{};
-		bf16x8 b0 = ld8(W0), b1 = ld8(W1);
+			c[0][t] = c[1][t] = f32x16{};
+		bf16x8 w0 = ld8(W0), w1 = ld8(W1);
 		for (int k = 0; k < K; k += 16) {
-			bf16x8 n0 = b0, n1 = b1;
+			bf16x8 nw0 = w0, nw1 = w1;
 			if (k + 16 < K) {
-				n0 = ld8(W0 + k + 16);
-				n1 = ld8(W1 + k + 16);
+				nw0 = ld8(W0 + k + 16);
+				nw1 = ld8(W1 + k + 16);
 			}
 #pragma unroll
 			for (int t = 0; t < 4; t++) {
 				const bf16x8 a = ld8(act + (32 * t + r) * ALD + k + kq);
-				c[t][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b0, c[t][0], 0, 0, 0);
-				c[t][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b1, c[t][1], 0, 0, 0);
+				c[0][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w0, a, c[0][t], 0, 0, 0);
+				c[1][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w1, a, c[1][t], 0, 0, 0);
 			}
-			b0 = n0;
-			b1 = n1;
+			w0 = nw0;
+			w1 = nw1;
 		}
 		__syncthreads(); /* every wave has read this layer's input */
-		const float bias0 = w.b[l][nb + r], bias1 = w.b[l][nb + 32 + r];
+		/* lane (r, half) of tile (h, t): state 32t + r, features
+		 * nb + 32h + 8(i/4) + 4 half + (i%4) for accumulator element i */
+		const float* bl = w.b[l] + nb + 4 * (lane >> 5);
 #pragma unroll
-		for (int t = 0; t < 4; t++) {
+		for (int h = 0; h < 2; h++) {
 #pragma unroll
-			for (int i = 0; i < 16; i++) {
-				const int m = 32 * t + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
-				act[m * ALD + nb + r] = f2bf(fmaxf(c[t][0][i] + bias0, 0.0f));
-				act[m * ALD + nb + 32 + r] = f2bf(fmaxf(c[t][1][i] + bias1, 0.0f));
+			for (int g = 0; g < 4; g++) {
+				const float4 bias = *reinterpret_cast<const float4*>(bl + 32 * h + 8 * g);
+#pragma unroll
+				for (int t = 0; t < 4; t++) {
+					const f32x16& v = c[h][t];
+					/* v_cvt_pk_bf16_f32: round to nearest even, two at a time */
+					const bf16x2 p0 = {(__bf16)fmaxf(v[4 * g] + bias.x, 0.0f), (__bf16)fmaxf(v[4 * g + 1] + bias.y, 0.0f)};
+					const bf16x2 p1 = {(__bf16)fmaxf(v[4 * g + 2] + bias.z, 0.0f), (__bf16)fmaxf(v[4 * g + 3] + bias.w, 0.0f)};
+					uint2 u;
+					std::memcpy(&u.x, &p0, 4);
+					std::memcpy(&u.y, &p1, 4);
+					*reinterpret_cast<uint2*>(act + (32 * t + r) * ALD + nb + 32 * h + 8 * g + 4 * (lane >> 5)) = u;
+				}
 			}
 		}
 		__syncthreads();
