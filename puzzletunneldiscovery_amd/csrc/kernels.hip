@@ -554,6 +554,137 @@ __device__ __forceinline__ int lane_rank(uint64_t m);
  * answer, "some triangle pair intersects", does not depend on the order
  * pairs are visited in.  set(i, hit) stores state i's answer.
  */
+/* defined with the packet traversal below */
+__device__ __forceinline__ bool leaf_pair_x(const DevScene& s, const ExactTf& x, int te, int tr);
+__device__ __forceinline__ void wave_lds_sync();
+__device__ __forceinline__ int rdl(int v, int k);
+__device__ __forceinline__ int wrl(int val, int k, int old);
+
+/* Where a one-lane traversal finds state i of a state loader. */
+template <class Loader>
+struct StateFromLoader {
+	const Loader* ld;
+	int64_t i;
+	__device__ __forceinline__ void get(double* o) const { ld->get(i, o); }
+};
+
+/* Entry e of a pair stack shared by the whole wavefront: slot e / 64 of lane
+ * e % 64's traversal stack and leaf buffer (own = the calling lane's). */
+template <bool WIDE>
+__device__ __forceinline__ PairStack<WIDE> wave_entry(const PairStack<WIDE>& own, int e)
+{
+	return PairStack<WIDE>{own.base - (threadIdx.x & 63) + (e & 63), own.stride};
+}
+
+/*
+ * One state's traversal by the whole wavefront, for states whose one-lane
+ * traversal ran long.  Each step pops up to 64 pairs off a stack shared by
+ * the wavefront (its lanes' traversal stacks, free by then), one per lane,
+ * tests them with the state's culling transform, tests the leaf pairs among
+ * them exactly, and pushes the children of the overlapping inner pairs.
+ * The answer is the one-lane traversal's: whether some triangle pair
+ * intersects.  Host replays of the heaviest states of a million random
+ * torus states (up to 1,597 pair tests one lane at a time) take at most 29
+ * steps and 304 stack entries.  Should the shared stack fill up, lane 0
+ * walks the state alone.
+ */
+template <bool WIDE, class Loader>
+__device__ bool coop_collide(const DevScene& s, const Loader& ld, int64_t i, const PairStack<WIDE>& stk,
+                             const PairStack<WIDE>& leaves)
+{
+	const int lane = threadIdx.x & 63;
+	const int cap = 64 * (s.stack_depth + LEAF_BUF);
+	QCull c;
+	{
+		double q[7];
+		ld.get(i, q);
+		make_qcull(s, q, c);
+	}
+	if (lane == 0)
+		wave_entry(stk, 0).put(0, 0, 0);
+	int sp = 1;
+	wave_lds_sync();
+	for (;;) {
+		const int nproc = sp < 64 ? sp : 64;
+		const bool act = lane < nproc;
+		int a = 0, b = 0;
+		if (act) {
+			const int e = sp - 1 - lane;
+			wave_entry(stk, e).get(e >> 6, a, b);
+		}
+		sp -= nproc;
+		wave_lds_sync();
+		bool leaf = false, inner = false;
+		int a1 = 0, b1 = 0, a2 = 0, b2 = 0;
+		if (act) {
+			const GNode A = load_node(s.env.nodes, a);
+			RobBox B;
+			load_rob_box(load_node(s.rob.nodes, b), c, B);
+			if (!obb_disjoint(A, B, c.eta)) {
+				const int ca = __float_as_int(A.a2_child.w);
+				const bool la = ca < 0, lb = B.child < 0;
+				if (la && lb) {
+					leaf = true;
+					a1 = ~ca;
+					b1 = ~B.child;
+				} else {
+					inner = true;
+					/* FCL's rule, as in the one-lane traversal */
+					const float sa = fmaf(A.c_ex.w, A.c_ex.w, fmaf(A.a0_ey.w, A.a0_ey.w, A.a1_ez.w * A.a1_ez.w));
+					const float eb0 = B.e[0] - c.infl, eb1 = B.e[1] - c.infl, eb2 = B.e[2] - c.infl;
+					const float sb = fmaf(eb0, eb0, fmaf(eb1, eb1, eb2 * eb2));
+					if (!la && (lb || sa > sb)) {
+						a1 = ca + 1; b1 = b; a2 = ca; b2 = b;
+					} else {
+						a1 = a; b1 = B.child + 1; a2 = a; b2 = B.child;
+					}
+				}
+			}
+		}
+		if (__ballot(leaf)) {
+			ExactTf x;
+			{
+				double q[7];
+				ld.get(i, q);
+				make_exact(s, q, x);
+			}
+			const bool h = leaf && leaf_pair_x(s, x, a1, b1);
+			if (__ballot(h))
+				return true;
+		}
+		const uint64_t mi = __ballot(inner);
+		const int ni = __popcll(mi);
+		if (sp + 2 * ni > cap) {
+			/* the shared stack is full: lane 0 starts over alone */
+			bool r = false;
+			if (lane == 0)
+				r = collide_dfs<WIDE>(s, StateFromLoader<Loader>{&ld, i}, stk, leaves);
+			return __ballot(r) != 0;
+		}
+		if (inner) {
+			const int e = sp + 2 * lane_rank(mi);
+			wave_entry(stk, e).put(e >> 6, a1, b1);
+			wave_entry(stk, e + 1).put((e + 1) >> 6, a2, b2);
+		}
+		sp += 2 * ni;
+		wave_lds_sync();
+		if (sp == 0)
+			return false;
+	}
+}
+
+/* A lane whose state has taken more pair tests than SE3CQ_HEAVY_PER_LEVEL
+ * times the scene's stack depth (the two trees' depths plus 2) sets it aside
+ * (at most SE3CQ_HEAVY_MAX per wavefront) and takes a new one; the
+ * wavefront finishes the states set aside with coop_collide once its last
+ * lane is done. */
+#ifndef SE3CQ_HEAVY_PER_LEVEL
+#define SE3CQ_HEAVY_PER_LEVEL 14
+#endif
+#ifndef SE3CQ_HEAVY_MAX
+#define SE3CQ_HEAVY_MAX 12
+#endif
+
 template <bool WIDE, class Loader, class Store>
 __device__ void states_refill(const DevScene& s, const Loader& ld, int64_t n, const Store& set,
                               const PairStack<WIDE>& stk, const PairStack<WIDE>& leaves)
@@ -570,7 +701,26 @@ __device__ void states_refill(const DevScene& s, const Loader& ld, int64_t n, co
 	int64_t pool_base = 0;
 	int pool_left = 0;
 	int since_leaf = 0;
+	int tests = 0;
+	/* states set aside for coop_collide: lane j of (hv_hi, hv_lo) holds the
+	 * j-th index */
+	int hv_lo = 0, hv_hi = 0, nheavy = 0;
+	const int heavy_tests = SE3CQ_HEAVY_PER_LEVEL * s.stack_depth;
 	for (;;) {
+		{
+			uint64_t dm = __ballot(idx >= 0 && tests > heavy_tests);
+			uint64_t taken = 0;
+			while (dm && nheavy < SE3CQ_HEAVY_MAX) {
+				const int L = __ffsll((long long)dm) - 1;
+				dm &= dm - 1;
+				hv_lo = wrl(rdl((int)(uint32_t)idx, L), nheavy, hv_lo);
+				hv_hi = wrl(rdl((int)(uint32_t)((uint64_t)idx >> 32), L), nheavy, hv_hi);
+				nheavy++;
+				taken |= 1ull << L;
+			}
+			if ((taken >> lane) & 1ull)
+				idx = -1;
+		}
 		const uint64_t idle = __ballot(idx < 0);
 		const int nidle = __popcll(idle);
 		if ((nidle >= REFILL_MIN || nidle == 64) && (pool_left > 0 || !exhausted)) {
@@ -603,6 +753,7 @@ __device__ void states_refill(const DevScene& s, const Loader& ld, int64_t n, co
 					stk.put(sp++, 0, 0);
 					nleaf = 0;
 					cached_b = -1;
+					tests = 0;
 				}
 			}
 			if (take <= pool_left) {
@@ -620,6 +771,7 @@ __device__ void states_refill(const DevScene& s, const Loader& ld, int64_t n, co
 				break;
 			continue;
 		}
+
 		const bool want_leaf = act && (nleaf == LEAF_BUF || (sp == 0 && nleaf > 0));
 		const uint64_t wl = __ballot(want_leaf);
 		if (wl != 0 && (__popcll(wl) >= REFILL_MIN || wl == am || since_leaf >= LEAF_PATIENCE)) {
@@ -644,6 +796,7 @@ __device__ void states_refill(const DevScene& s, const Loader& ld, int64_t n, co
 			continue;
 		int a, b;
 		stk.get(--sp, a, b);
+		tests++;
 		const GNode A = load_node(s.env.nodes, a);
 		if (b != cached_b) {
 			const GNode Bn = load_node(s.rob.nodes, b);
@@ -679,6 +832,14 @@ __device__ void states_refill(const DevScene& s, const Loader& ld, int64_t n, co
 			set(idx, false);
 			idx = -1;
 		}
+	}
+	/* every lane is idle and the states have run out: the whole wavefront
+	 * now walks each state set aside */
+	for (int j = 0; j < nheavy; j++) {
+		const int64_t i = (int64_t)(((uint64_t)(uint32_t)rdl(hv_hi, j) << 32) | (uint64_t)(uint32_t)rdl(hv_lo, j));
+		const bool hit = coop_collide<WIDE>(s, ld, i, stk, leaves);
+		if (lane == 0)
+			set(i, hit);
 	}
 }
 

@@ -22,6 +22,27 @@ def test_states_torus(gpu_world, torus_oracle):
     assert int((g != c).sum()) == 0
 
 
+def test_heavy_states_whole_wavefront(gpu_world, torus_oracle):
+    """The states whose one-lane traversal runs longest are set aside and
+    walked by a whole wavefront: the heaviest 2,048 of 400k random states by
+    the oracle's own node-test count (so many per wavefront that some stay
+    with their lane), on their own and spread among light ones, must match
+    the oracle."""
+    qs = W.random_states(400_000, seed=55)
+    v, nbv, _ = torus_oracle.valid_states(qs, counters=True)
+    order = np.argsort(-nbv)
+    heavy = qs[order[:2048]]
+    assert nbv[order[2047]] > 14 * 23  # above the set-aside threshold of the torus scene
+    _flush("states: 2,048 heaviest")
+    g = gpu_world.is_valid_states(heavy)
+    assert np.array_equal(g, v[order[:2048]].astype(bool))
+    assert 0 < g.sum() < len(g)
+    mixed = np.concatenate([heavy, qs[order[-60_000:]]])[np.random.default_rng(56).permutation(62_048)]
+    _flush("states: heaviest among light ones")
+    g2 = gpu_world.is_valid_states(mixed)
+    assert np.array_equal(g2, torus_oracle.valid_states(mixed).astype(bool))
+
+
 def test_states_edge_cases(gpu_world, torus_oracle):
     rng = np.random.default_rng(7)
     qs = W.random_states(20_000, seed=8)
