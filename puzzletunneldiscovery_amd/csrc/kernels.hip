@@ -6,18 +6,28 @@
  * binary32 box culling uses explicit fmaf and is conservative by
  * construction (margins below), so it never changes a boolean.
  *
- * Kernels
- *   k_valid_states   one thread per state: BVH x BVH depth-first traversal
- *                    (UnitWorld::isValid, lib/osr/unit_world.cc:215-247 ->
- *                    fcl::collide via lib/osr/cdmodel.cc:162-178).
- *   k_segments       one wavefront per segment: the 64 lanes evaluate 64
- *                    consecutive discrete steps at once, a ballot finds the
- *                    first colliding step (transitStateToWithContact,
- *                    unit_world.cc:405-474; isValidTransition 477-486), and
- *                    the touch search refines contact with an exact
- *                    5-levels-per-round parallel bisection.
+ * Kernels (all persistent: the grid holds the resident wavefronts, which
+ * take work from a per-launch counter)
+ *   k_valid_states   one state per lane, lanes refilled as they finish:
+ *                    BVH x BVH depth-first traversal (UnitWorld::isValid,
+ *                    lib/osr/unit_world.cc:215-247 -> fcl::collide via
+ *                    lib/osr/cdmodel.cc:162-178); states that run long are
+ *                    finished by the whole wavefront.
+ *   k_segments       one wavefront per segment at a time: the 64 lanes
+ *                    evaluate 64 consecutive discrete steps in one packet
+ *                    traversal, which finds the first colliding step
+ *                    (transitStateToWithContact, unit_world.cc:405-474;
+ *                    isValidTransition 477-486), and the touch search
+ *                    refines contact with an exact 5-levels-per-round
+ *                    parallel bisection.
  *   k_visibility     one wavefront per state pair (calculateVisibility*,
  *                    unit_world.cc:740-840).
+ *   k_ompl_states,   OMPL-convention states and motions
+ *   k_ompl_motions   (lib/pyse3ompl/ompldriver.cc:566-620).
+ *   k_batch          uneven segments and single states in one launch.
+ *   k_contacts,      every intersecting triangle pair of a state and its
+ *   k_contact_segments  intersection segment (unit_world.cc:959-1012).
+ *   k_disentangled   the two models' boxes (unit_world.cc:249-264).
  */
 #include <hip/hip_runtime.h>
 
