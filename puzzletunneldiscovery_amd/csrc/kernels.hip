@@ -6,8 +6,9 @@
  * binary32 box culling uses explicit fmaf and is conservative by
  * construction (margins below), so it never changes a boolean.
  *
- * Kernels (all persistent: the grid holds the resident wavefronts, which
- * take work from a per-launch counter)
+ * Kernels (the query kernels up to k_batch are persistent: the grid holds
+ * the resident wavefronts, which take work from a per-launch counter; the
+ * last three are one thread per state)
  *   k_valid_states   one state per lane, lanes refilled as they finish:
  *                    BVH x BVH depth-first traversal (UnitWorld::isValid,
  *                    lib/osr/unit_world.cc:215-247 -> fcl::collide via
