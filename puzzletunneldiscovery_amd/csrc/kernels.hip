@@ -1389,22 +1389,26 @@ __device__ uint64_t sat_pass(const DevScene& s, const BisectShared* bs, int nb)
 
 /*
  * Mask of the lanes in `live` (all below lane 32) whose pose
- * se3_segment_at(sg, tau) collides, decided on the nc candidate pairs in bs,
- * which must cover every such pose.
+ * se3_segment_at(sg, tau, .) collides, decided on the nc candidate pairs in
+ * bs, which must cover every such pose.  Each pass tests two candidates
+ * against all probes: lane l takes probe l % 32 (the pose of lane l % 32)
+ * and candidate i + l / 32, so no lane idles while a round has at most 32
+ * probes.
  */
 __device__ uint64_t cands_collide(const DevScene& s, const se3_segment* sg, BisectShared* bs, int nc, double tau,
                                   uint64_t live)
 {
 	const int lane = threadIdx.x & 63;
-	const bool act = (live >> lane) & 1ull;
+	const int p = lane & 31, h = lane >> 5;
+	const bool act = (live >> p) & 1ull;
 	QCull c;
 	{
 		double q[7];
-		se3_segment_at(sg, tau, q);
+		se3_segment_at(sg, shfl_d(tau, p), q);
 		make_qcull(s, q, c);
-		ExactTf x;
-		make_exact(s, q, x);
-		if (act) {
+		if (h == 0 && act) {
+			ExactTf x;
+			make_exact(s, q, x);
 #pragma unroll
 			for (int t = 0; t < 9; t++)
 				bs->xf[lane][t] = x.R[t];
@@ -1416,16 +1420,14 @@ __device__ uint64_t cands_collide(const DevScene& s, const se3_segment* sg, Bise
 	wave_lds_sync();
 	uint64_t hitm = 0;
 	int nb = 0;
-	for (int i = 0; i < nc; i++) {
-		const int a = __builtin_amdgcn_readfirstlane((int)bs->ca[i]);
-		const int b = __builtin_amdgcn_readfirstlane((int)bs->cb[i]);
-		const GNode A = load_node_uniform(s.env.nodes, a);
-		const GNode Bn = load_node_uniform(s.rob.nodes, b);
-		bool ov;
-		{
+	for (int i = 0; i < nc; i += 2) {
+		const int ci = i + h;
+		bool ov = false;
+		if (act && ci < nc && !((hitm >> p) & 1ull)) {
+			const GNode A = load_node(s.env.nodes, (int)bs->ca[ci]);
 			RobBox B;
-			load_rob_box(Bn, c, B);
-			ov = act && !((hitm >> lane) & 1ull) && !obb_disjoint(A, B, c.eta);
+			load_rob_box(load_node(s.rob.nodes, (int)bs->cb[ci]), c, B);
+			ov = !obb_disjoint(A, B, c.eta);
 		}
 		const uint64_t m = __ballot(ov);
 		const int cnt = __popcll(m);
@@ -1437,7 +1439,7 @@ __device__ uint64_t cands_collide(const DevScene& s, const se3_segment* sg, Bise
 			wave_lds_sync();
 		}
 		if (ov)
-			bs->combo[nb + lane_rank(m)] = ((uint32_t)i << 8) | (uint32_t)lane;
+			bs->combo[nb + lane_rank(m)] = ((uint32_t)ci << 8) | (uint32_t)p;
 		nb += cnt;
 		wave_lds_sync();
 	}
