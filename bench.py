@@ -116,6 +116,22 @@ def oracle_flops_per_query(ow, starts, ends, n):
     return float(C_BV * nbv.mean() + C_TRI * ntri.mean()), float(nbv.mean()), float(ntri.mean())
 
 
+def _compute_roofline(flops_q, nbv, ntri, queries, kern_ms, kernel, sample):
+    """The bench line's roofline object for a VALU-bound traversal kernel:
+    algorithmic binary64 flops per query (the oracle's node and triangle
+    tests on `sample`) over the kernel's time, against the fp32 peak.  HBM
+    traffic is counted for the headline kernel only (null here)."""
+    achieved = queries * flops_q / (kern_ms * 1e-3) / 1e12
+    return dict(bound="mfma", achieved=round(achieved, 3), peak=PEAK_FP32_TFLOPS, unit="TFLOP/s",
+                frac=round(achieved / PEAK_FP32_TFLOPS, 5), traffic=None,
+                note="compute-bound on vector ALUs (no matrix instructions); the compute roof is MI355X's fp32 "
+                     "peak; algorithmic flops = oracle node/triangle test counts x their straight-line binary64 "
+                     "op counts",
+                flops_per_query=round(flops_q, 1), oracle_node_tests_per_query=round(nbv, 2),
+                oracle_tri_tests_per_query=round(ntri, 3), kernel=kernel, kernel_ms=round(kern_ms, 4),
+                flops_sample=sample)
+
+
 class _Timer:
     """HIP events on the stream the engine's kernels run on."""
 
@@ -200,8 +216,8 @@ def run_states(args, torch, pyosr, W):
         extra["cpu_baseline"] = dict(value=done_q / dt, unit="queries/s", cores=_sample_threads(), kind="port",
                                      sample=f"{what}first {m} of the {n} states ({dt:.2f} s)")
         fq = C_BV * nbv.mean() + C_TRI * ntri.mean()
-        extra["flops_per_query"] = round(float(fq), 1)
-        extra["achieved_tflops"] = round(n * fq / (kern_ms * 1e-3) / 1e12, 3)
+        extra["roofline"] = _compute_roofline(float(fq), float(nbv.mean()), float(ntri.mean()), n, kern_ms,
+                                              "k_valid_states", f"the first {m} states")
     line = _line(f"state validity: torus env 2,048 tris / robot 1,024 tris, {n} states", n * args.steps / wall,
                  wall, args, extra)
     line["config"]["states"] = n
@@ -266,6 +282,9 @@ def run_uneven(args, torch, pyosr, W):
         extra["mismatches"] = mism
         extra["cpu_baseline"] = dict(value=cq / dt, unit="queries/s", cores=_sample_threads(), kind="port",
                                      sample=f"first {done_n} of the {n} segments ({cq} queries, {dt:.2f} s)")
+        fq, nbv, ntri = oracle_flops_per_query(ow, starts, ends, 1024)
+        extra["roofline"] = _compute_roofline(fq, nbv, ntri, q, kern_ms, "k_segments<transit>",
+                                              "the states the first 1024 segments evaluate")
     line = _line(f"uneven segments: shell env {len(env[1])} tris / bent rod {len(rob[1])} tris, {n} segments, "
                  f"lognormal(5, 1.2) step counts in [8, 4096] at step 1/1024", q * args.steps / wall, wall, args, extra)
     line["config"].update({"segments": n, "queries_per_step": q, "free_segments": int(done.sum())})
@@ -315,6 +334,12 @@ def run_visibility(args, torch, pyosr, W):
         extra["cpu_baseline"] = dict(value=qs_cpu / dt, unit="queries/s", cores=_sample_threads(), kind="port",
                                      sample=f"{k} diagonal 64 x 64 blocks of the {n} x {n} matrix "
                                             f"({qs_cpu} queries, {dt:.2f} s)")
+        # states the pairs of the first rows evaluate (all at verify step
+        # 1/1024 unless a pair is closer than 2/1024)
+        fi, ti = np.triu_indices(n, 1)
+        fq, nbv, ntri = oracle_flops_per_query(ow, ms[fi[:512]], ms[ti[:512]], 512)
+        extra["roofline"] = _compute_roofline(fq, nbv, ntri, q, kern_ms, "k_visibility<matrix>",
+                                              "the states the first 512 pairs evaluate")
     line = _line(f"visibility matrix: shell env {len(env[1])} tris / bent rod {len(rob[1])} tris, {n} milestones, "
                  f"{npairs} pairs at step 1/1024", q * args.steps / wall, wall, args, extra)
     line["config"].update({"milestones": n, "pairs": npairs, "queries_per_step": q,
