@@ -41,6 +41,11 @@ PEAK_FP32_TFLOPS = 157.3
 # profiles/k_segments_touch_torus65536.md.  Measured, not computed here.
 TRAFFIC_FETCH_KIB = 45099
 TRAFFIC_WRITE_KIB = 127849
+# the same for one launch of the uneven-segment kernel (4M segments, shell
+# scene); see profiles/k_segments_transit_shell_uneven.md
+TRAFFIC_UNEVEN_FETCH_KIB = 15871168
+TRAFFIC_UNEVEN_WRITE_KIB = 10978818
+PEAK_HBM_GBS = 8000.0
 FETCH_SCALE = 262144 / 131094
 WRITE_SCALE = 1.0
 
@@ -283,8 +288,16 @@ def run_uneven(args, torch, pyosr, W):
         extra["cpu_baseline"] = dict(value=cq / dt, unit="queries/s", cores=_sample_threads(), kind="port",
                                      sample=f"first {done_n} of the {n} segments ({cq} queries, {dt:.2f} s)")
         fq, nbv, ntri = oracle_flops_per_query(ow, starts, ends, 1024)
-        extra["roofline"] = _compute_roofline(fq, nbv, ntri, q, kern_ms, "k_segments<transit>",
-                                              "the states the first 1024 segments evaluate")
+        roof = _compute_roofline(fq, nbv, ntri, q, kern_ms, "k_segments<transit>",
+                                 "the states the first 1024 segments evaluate")
+        if n == 4 * 1024 * 1024:
+            # both bounds, as the shell scene's trees exceed the L2: the
+            # compute fraction above and the measured HBM bytes per launch
+            traffic = int((TRAFFIC_UNEVEN_FETCH_KIB * FETCH_SCALE + TRAFFIC_UNEVEN_WRITE_KIB * WRITE_SCALE) * 1024)
+            roof["traffic"] = traffic
+            roof["hbm_gbs"] = round(traffic / (kern_ms * 1e-3) / 1e9, 1)
+            roof["hbm_frac"] = round(traffic / (kern_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 5)
+        extra["roofline"] = roof
     line = _line(f"uneven segments: shell env {len(env[1])} tris / bent rod {len(rob[1])} tris, {n} segments, "
                  f"lognormal(5, 1.2) step counts in [8, 4096] at step 1/1024", q * args.steps / wall, wall, args, extra)
     line["config"].update({"segments": n, "queries_per_step": q, "free_segments": int(done.sum())})
