@@ -39,8 +39,8 @@ PEAK_FP32_TFLOPS = 157.3
 # same counter reads for a 256 MiB device copy (FETCH_SIZE 131,094 KiB, i.e.
 # x2.000; WRITE_SIZE 262,144 KiB, x1.000); see
 # profiles/k_segments_touch_torus65536.md.  Measured, not computed here.
-TRAFFIC_FETCH_KIB = 45099
-TRAFFIC_WRITE_KIB = 127849
+TRAFFIC_FETCH_KIB = 50057
+TRAFFIC_WRITE_KIB = 142868
 # the same for one launch of the uneven-segment kernel (4M segments, shell
 # scene); see profiles/k_segments_transit_shell_uneven.md
 TRAFFIC_UNEVEN_FETCH_KIB = 15871168
