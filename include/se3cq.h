@@ -250,6 +250,9 @@ int se3cq_multi_distance(const double* origin7, const double* targets, int64_t n
 int se3cq_interpolate(const double* a7, const double* b7, double tau, double* out7);
 /* apply (osr_state.cc:243-259). */
 int se3cq_apply(const double* from7, const double* tr3, const double* aa3, double* out7);
+/* apply over n rows: out[i] = apply(from[i], tr[i], aa[i]), as the pipeline's
+ * touch sampling calls it once per sample (src/GP/pipeline/touchq_util.py:8-13). */
+int se3cq_apply_batch(const double* from, const double* tr, const double* aa, int64_t n, double* out);
 /* decompose_2 (osr_state.cc:94-99): translation and rotation matrix (row-major). */
 int se3cq_decompose_2(const double* q7, double* t3, double* R9);
 /* compose_from_angleaxis (osr_state.cc:121-128). */

@@ -1,0 +1,65 @@
+"""CPU restatement of the key-configuration clearance stage of the reference
+pipeline, as the checker of puzzletunneldiscovery_amd.clearance.
+
+Test infrastructure only (tests/ use it); the shipped package never imports
+it.  It follows the reference's procedure one sample at a time:
+
+- the sample directions: a point on the unit sphere by Marsaglia's rejection
+  method (two uniform draws per try) and a point in the ball of radius 2 pi
+  by rejection from the cube (three draws per try), drawn in that order per
+  sample, one ``rand`` call per try (src/GP/pipeline/uw_random.py:20-44,
+  touchq_util.py:8-13);
+- per key, per sample: target = apply(q0, tr, aa), the oracle's
+  transitStateToWithContact toward it at the given resolution, and the SE(3)
+  distance from q0 to the last free state (keyconf.py:309-315,
+  touchq_util.py:15-27, osr_state.cc:130-166);
+- the downsampling: mean squared distance per key, the first two keys
+  dropped, argsort, the first K (solve2.py:116-124).
+"""
+import math
+
+import numpy as np
+
+
+def sphere_point(rs):
+    while True:
+        u = rs.rand(2)
+        a, b = 2.0 * (u[0] - 0.5), 2.0 * (u[1] - 0.5)
+        s = a * a + b * b
+        if s < 1:
+            f = math.sqrt(1 - s)
+            return np.array([2 * a * f, 2 * b * f, 1 - 2 * s])
+
+
+def ball_point(rs, radius):
+    while True:
+        u = rs.rand(3)
+        a, b, c = 2.0 * (u[0] - 0.5), 2.0 * (u[1] - 0.5), 2.0 * (u[2] - 0.5)
+        if a * a + b * b + c * c <= 1:
+            return np.array([a, b, c]) * radius
+
+
+def directions(n, rs):
+    tr = np.zeros((n, 3))
+    aa = np.zeros((n, 3))
+    for i in range(n):
+        tr[i] = sphere_point(rs)
+        aa[i] = ball_point(rs, 2 * math.pi)
+    return tr, aa
+
+
+def key_distances(ow, oracle_mod, unit_key, tr, aa, stepping):
+    """Distances from one unit-world key to the last free state of each
+    touch query toward apply(key, tr[i], aa[i])."""
+    out = np.zeros(len(tr))
+    for i in range(len(tr)):
+        to = oracle_mod.apply(unit_key, tr[i], aa[i])
+        free = ow.transit(unit_key, to, stepping)[0]
+        out[i] = oracle_mod.distance(unit_key, free)
+    return out
+
+
+def least_visible(oskey, osc, k, reserved=2):
+    m = [float(np.mean(np.asarray(row) ** 2)) for row in osc]
+    order = np.asarray(m[reserved:]).argsort()[:k]
+    return np.asarray(oskey)[order + reserved]

@@ -2120,6 +2120,15 @@ int se3cq_apply(const double* from7, const double* tr3, const double* aa3, doubl
 	return SE3CQ_OK;
 }
 
+int se3cq_apply_batch(const double* from, const double* tr, const double* aa, int64_t n, double* out)
+{
+	if (n < 0 || (n > 0 && (!from || !tr || !aa || !out)))
+		return fail(SE3CQ_EINVAL, "bad arguments");
+	for (int64_t i = 0; i < n; i++)
+		se3cq_apply(from + 7 * i, tr + 3 * i, aa + 3 * i, out + 7 * i);
+	return SE3CQ_OK;
+}
+
 int se3cq_decompose_2(const double* q7, double* t3, double* R9)
 {
 	double q[4] = {q7[3], q7[4], q7[5], q7[6]};

@@ -84,6 +84,20 @@ def apply(frm, tr, aa):
     return out
 
 
+def apply_batch(frms, trs, aas):
+    """apply over rows: [n, 7] states moved by [n, 3] translations and
+    [n, 3] axis-angle rotations (the pipeline's touch sampling,
+    src/GP/pipeline/touchq_util.py:8-13, calls apply once per sample)."""
+    f = _states(frms)
+    t = np.ascontiguousarray(trs, dtype=np.float64).reshape(-1, 3)
+    a = np.ascontiguousarray(aas, dtype=np.float64).reshape(-1, 3)
+    if not (len(f) == len(t) == len(a)):
+        raise ValueError("apply_batch: row counts differ")
+    out = np.zeros((len(f), 7))
+    N.check(N.lib().se3cq_apply_batch(N.dptr(f), N.dptr(t), N.dptr(a), len(f), N.dptr(out)))
+    return out
+
+
 def decompose_2(state):
     """decompose_2 (osr_state.cc:94-99): (translation, 3x3 rotation)."""
     q = _state(state)
@@ -527,4 +541,20 @@ def create_unit_world_from_arrays(env, rob):
     uw.load_mesh(UnitWorld.GEO_ROB, *rob)
     uw.scaleToUnit()
     uw.angleModel(0.0, 0.0)
+    return uw
+
+
+def create_unit_world(puzzle_file):
+    """util.create_unit_world (src/GP/pipeline/util.py:122-140): the puzzle's
+    meshes scaled to the unit world, with recommended_cres = scene_scale x the
+    file's collision_resolution (0.001 when absent)."""
+    from .pyse3ompl import read_puzzle
+    cfg = read_puzzle(puzzle_file)
+    uw = UnitWorld()
+    uw.loadModelFromFile(cfg["env_fn"])
+    uw.loadRobotFromFile(cfg["rob_fn"])
+    uw.scaleToUnit()
+    uw.angleModel(0.0, 0.0)
+    cres = cfg["collision_resolution"]
+    uw.recommended_cres = uw.scene_scale * (0.001 if cres is None else cres)
     return uw

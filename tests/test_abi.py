@@ -136,6 +136,16 @@ def test_state_arithmetic_matches_oracle(oracle_mod):
         qn[3:] = a[i, 3:] / np.sqrt((x * x + z * z) + (y * y + w * w))
         Ro, to = oracle_mod.state_to_transform(qn)
         assert np.array_equal(R, Ro) and np.array_equal(t, to)
+    # apply over rows, with touch-sampling sized moves (unit translation, up to 2 pi)
+    trs = rng.normal(size=(300, 3))
+    trs /= np.linalg.norm(trs, axis=1, keepdims=True)
+    aas = rng.uniform(-1, 1, (300, 3)) * 2 * np.pi
+    aas[:5] = 0.0
+    got = pyosr.apply_batch(np.vstack([a, a])[:300], trs, aas)
+    assert np.array_equal(got, [oracle_mod.apply(q, t, x) for q, t, x in zip(np.vstack([a, a])[:300], trs, aas)])
+    assert pyosr.apply_batch(np.zeros((0, 7)), np.zeros((0, 3)), np.zeros((0, 3))).shape == (0, 7)
+    with pytest.raises(ValueError):
+        pyosr.apply_batch(a[:3], trs[:2], aas[:3])
 
 
 def test_compose_from_angleaxis_closed_form():

@@ -1,9 +1,13 @@
-"""Key-configuration screening on the GPU (puzzletunneldiscovery_amd.keyscreen
-over OmplDriver.validate_motion_pairs and UnitWorld disentanglement,
-src/GP/pipeline/solve2.py:164-300) against the oracle's step-by-step
-restatement, with the oracle's OMPL motion validator and isDisentangled as
-the predicates, on the alpha puzzle: the edge_batch files and the screened
-roots of every scheme range."""
+"""The pipeline stages around the collision queries, on the GPU, against the
+oracle's step-by-step restatements on the alpha puzzle:
+- key screening (puzzletunneldiscovery_amd.keyscreen over
+  OmplDriver.validate_motion_pairs and UnitWorld disentanglement,
+  src/GP/pipeline/solve2.py:164-300), with the oracle's OMPL motion validator
+  and isDisentangled as the predicates: the edge_batch files and the screened
+  roots of every scheme range;
+- key clearance (puzzletunneldiscovery_amd.clearance, keyconf.py:276-317,
+  solve2.py:104-125) with the oracle's transitStateToWithContact: the
+  DISTANCE_BATCH files and the downsampled keys."""
 import math
 import os
 
@@ -115,3 +119,58 @@ def test_screen_and_assemble_alpha(driver, alpha, tmp_path):
                 seen_first_dis |= bool(dis[c[0]] and not dis[c].all())
     # the cases that tell "first non-disentangled member" from "first member"
     assert seen_first_dis and seen_all_dis
+
+
+def write_alpha_cfg(path, cres=None):
+    lines = ["[problem]", f"world = {ENV}", f"robot = {ROB}"]
+    for key in ("start", "goal"):
+        lines += [f"{key}.x = 0", f"{key}.y = 0", f"{key}.z = 0",
+                  f"{key}.axis.x = 1", f"{key}.axis.y = 0", f"{key}.axis.z = 0", f"{key}.theta = 0"]
+    for key, v in (("volume.min", LO), ("volume.max", HI)):
+        lines += [f"{key}.{c} = {float(x)!r}" for c, x in zip("xyz", v)]
+    if cres is not None:
+        lines.append(f"collision_resolution = {cres!r}")
+    path.write_text("\n".join(lines) + "\n")
+    return str(path)
+
+
+def test_clearance_alpha(alpha, oracle_mod, tmp_path):
+    """estimate_keyconf_clearance (src/GP/pipeline/keyconf.py:276-317) then
+    least_visible_keyconf_fixed (solve2.py:104-125): DISTANCE_BATCH files
+    equal to the oracle's touch queries with the same random stream."""
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from oracle import clearance as OC
+    from puzzletunneldiscovery_amd import clearance as C
+    from puzzletunneldiscovery_amd import pyosr
+
+    uw = pyosr.create_unit_world(write_alpha_cfg(tmp_path / "alpha.cfg"))
+    assert uw.recommended_cres == uw.scene_scale * 0.001 and uw.scene_scale == alpha.scene_scale
+    cres = 0.002
+    uw2 = pyosr.create_unit_world(write_alpha_cfg(tmp_path / "alpha2.cfg", cres))
+    assert uw2.recommended_cres == uw2.scene_scale * cres
+
+    keys, _ = alpha_keys(alpha)
+    keys = np.vstack([keys[:2], keys[2:4], keys[40:52]])   # reservations, disentangled, path states
+    nsample = 24
+    nchunk = 3
+    for task in range(nchunk):
+        np.random.seed(500 + task)
+        C.estimate_keyconf_clearance_task(uw2, keys, nsample, nchunk, task, outdir=str(tmp_path))
+    osc = C.load_distance_batches(str(tmp_path))
+    assert osc.shape == (len(keys), nsample)
+
+    want = []
+    for task in range(nchunk):
+        rs = np.random.RandomState(500 + task)
+        for k in C.clearance_task(len(keys), nchunk, task):
+            tr, aa = OC.directions(nsample, rs)
+            q0 = alpha.translate_ompl_to_unit(keys[k])
+            want.append(OC.key_distances(alpha, oracle_mod, q0, tr, aa, uw2.recommended_cres))
+    want = np.array(want)
+    assert np.array_equal(osc, want)
+    # the path keys sit in the puzzle (short clearance), the disentangled ones do not
+    assert want[4:].mean() < want[2:4].mean()
+    top = C.least_visible_keys(keys, osc, 5)
+    assert np.array_equal(top, OC.least_visible(keys, want, 5))
