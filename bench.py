@@ -43,8 +43,8 @@ TRAFFIC_FETCH_KIB = 50057
 TRAFFIC_WRITE_KIB = 142868
 # the same for one launch of the uneven-segment kernel (4M segments, shell
 # scene); see profiles/k_segments_transit_shell_uneven.md
-TRAFFIC_UNEVEN_FETCH_KIB = 15871168
-TRAFFIC_UNEVEN_WRITE_KIB = 10978818
+TRAFFIC_UNEVEN_FETCH_KIB = 100288422
+TRAFFIC_UNEVEN_WRITE_KIB = 147817046
 PEAK_HBM_GBS = 8000.0
 FETCH_SCALE = 262144 / 131094
 WRITE_SCALE = 1.0

@@ -36,10 +36,26 @@
 #include "launch.h"
 #include "se3_math.h"
 
-/* Minimum resident wavefronts per SIMD the traversal kernels are compiled
- * for (caps their VGPR budget). */
+/* Minimum resident wavefronts per SIMD each traversal kernel is compiled
+ * for (caps its VGPR budget; 4 waves = 128 VGPRs, 6 = 80, 7 = 72).  The
+ * segment walks wait on dependent node loads, and for the plain segment and
+ * visibility walks more resident waves hide that better than the registers
+ * they give up, even with the spills that follow.  The per-lane state
+ * kernels, the touch search and the OMPL motions are fastest at 4. */
 #ifndef SE3CQ_MIN_WAVES
 #define SE3CQ_MIN_WAVES 4
+#endif
+#ifndef SE3CQ_SEG_WAVES
+#define SE3CQ_SEG_WAVES 7
+#endif
+#ifndef SE3CQ_VIS_WAVES
+#define SE3CQ_VIS_WAVES 6
+#endif
+#ifndef SE3CQ_TOUCH_WAVES
+#define SE3CQ_TOUCH_WAVES 4
+#endif
+#ifndef SE3CQ_OMPL_WAVES
+#define SE3CQ_OMPL_WAVES 4
 #endif
 
 namespace se3cq {
@@ -1691,7 +1707,7 @@ __device__ __forceinline__ void seg_store(const SegOut& o, int64_t seg, const Se
 }
 
 template <bool WIDE, int MODE>
-__global__ void __launch_bounds__(256, SE3CQ_MIN_WAVES) k_segments(DevScene s, const double* __restrict__ from_all,
+__global__ void __launch_bounds__(256, MODE == MODE_TOUCH ? SE3CQ_TOUCH_WAVES : SE3CQ_SEG_WAVES) k_segments(DevScene s, const double* __restrict__ from_all,
                                                   const double* __restrict__ to_all, int64_t n, double vd,
                                                   int levels, SegOut o)
 {
@@ -1784,7 +1800,7 @@ __device__ __forceinline__ void tri_pair(int64_t p, int64_t n, int64_t& fi, int6
  * bits: bit-packed row-major n x n (upper and lower triangles), either may be
  * null. */
 template <bool WIDE, int VIS>
-__global__ void __launch_bounds__(256, SE3CQ_MIN_WAVES) k_visibility(DevScene s, const double* __restrict__ qs0,
+__global__ void __launch_bounds__(256, SE3CQ_VIS_WAVES) k_visibility(DevScene s, const double* __restrict__ qs0,
                                                     const double* __restrict__ qs1, int64_t m, int64_t n,
                                                     int64_t npairs, double ivd, int8_t* __restrict__ out8,
                                                     uint32_t* __restrict__ bits, int32_t* __restrict__ steps)
@@ -1854,7 +1870,7 @@ __global__ void __launch_bounds__(256, SE3CQ_MIN_WAVES) k_visibility(DevScene s,
  * 2^31.  samples (optional): max(nd, 1), the states a valid motion costs.
  */
 template <bool WIDE>
-__global__ void __launch_bounds__(256, SE3CQ_MIN_WAVES) k_ompl_motions(DevScene s, OmplXf x,
+__global__ void __launch_bounds__(256, SE3CQ_OMPL_WAVES) k_ompl_motions(DevScene s, OmplXf x,
                                                       const double* __restrict__ q0,
                                                       const double* __restrict__ q1, int64_t n, double lvs,
                                                       int32_t* __restrict__ out, int32_t* __restrict__ samples)
