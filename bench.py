@@ -101,21 +101,21 @@ def cpu_baseline(ow, starts, ends, gpu_res, target_s):
                        f"{dt_all:.1f} s)"), mism, n
 
 
-def oracle_flops_per_query(ow, starts, ends, n):
+def oracle_flops_per_query(ow, starts, ends, n, vd=VD):
     """Mean algorithmic flops per query from the oracle's traversal counters
     over the states the first n segments evaluate."""
     from oracle import oracle as O
     qs = []
     for i in range(n):
-        lf, st, done, lt, t, ns = ow.transit(starts[i], ends[i], VD)
+        lf, st, done, lt, t, ns = ow.transit(starts[i], ends[i], vd)
         dist = O.distance(starts[i], ends[i])
-        d = VD
+        d = vd
         for k in range(ns):
             qs.append(O.interpolate(starts[i], ends[i], d * (1.0 / dist)))
             if d < dist:
-                d = min(d + VD, dist)
+                d = min(d + vd, dist)
             else:
-                d += VD
+                d += vd
     qs = np.array(qs)
     _, nbv, ntri = ow.valid_states(qs, counters=True)
     return float(C_BV * nbv.mean() + C_TRI * ntri.mean()), float(nbv.mean()), float(ntri.mean())
@@ -514,6 +514,112 @@ def run_ompl(args, torch, pyosr, W):
     return line
 
 
+def run_clearance(args, torch, pyosr, W):
+    """Key-configuration clearance (src/GP/pipeline/keyconf.py:276-317) on
+    the alpha puzzle at recommended_cres = scene_scale x 0.001: 128 touch
+    samples per key (envconfig.py:141) for 1,024 keys, a tenth of the
+    pipeline's 1,024 surface pairs x oversampling ratio 10.  At that step
+    (1.5e-5 of the unit world) a touch query averages about 28,000 steps,
+    so the full stage is about 4e10 queries.  Keys are free states on the
+    reference's solution path (between consecutive path states); the timed
+    step is the device launch of all 131,072 touch queries
+    (transitStateToWithContact)."""
+    from puzzletunneldiscovery_amd import clearance as C
+    from puzzletunneldiscovery_amd import pyse3ompl
+    uw = pyosr.UnitWorld()
+    uw.loadModelFromFile(os.path.join(ALPHA_DIR, "alpha_env-1.2.org.obj"))
+    uw.loadRobotFromFile(os.path.join(ALPHA_DIR, "alpha-1.2.org.obj"))
+    uw.scaleToUnit()
+    uw.angleModel(0.0, 0.0)
+    uw.recommended_cres = uw.scene_scale * 0.001
+    vd = uw.recommended_cres
+    drv = pyse3ompl.OmplDriver.from_unit_world(uw)
+    nkey = args.segments or 1024
+    nsample = 128
+    path = np.loadtxt(os.path.join(ALPHA_DIR, "alpha-1.2.org.path"))
+    rng = np.random.default_rng(7100)
+    keys, have = [], 0
+    while have < nkey:
+        # a normalised lerp between consecutive path states, kept where free
+        i = rng.integers(0, len(path) - 1, 2 * nkey)
+        t = rng.uniform(0, 1, (2 * nkey, 1))
+        a, b = path[i], path[i + 1].copy()
+        b[:, 3:] *= np.where((a[:, 3:] * b[:, 3:]).sum(axis=1, keepdims=True) < 0, -1.0, 1.0)
+        q = a * (1 - t) + b * t
+        q[:, 3:] /= np.linalg.norm(q[:, 3:], axis=1, keepdims=True)
+        q = q[drv.validate_states(q) == 1]
+        keys.append(q)
+        have += len(q)
+    keys = np.ascontiguousarray(np.vstack(keys)[:nkey])
+    print(f"clearance: {nkey} free keys on the solution path", file=sys.stderr, flush=True)
+    tr, aa = C.touch_directions(nkey * nsample, np.random.RandomState(7101))
+    unit = uw.translate_ompl_to_unit(keys)
+    starts = np.repeat(unit, nsample, axis=0)
+    ends = pyosr.apply_batch(starts, tr, aa)
+    n = len(starts)
+    stream = torch.cuda.Stream()
+    torch.cuda.set_stream(stream)
+    d_f = torch.from_numpy(starts).cuda()
+    d_t = torch.from_numpy(ends).cuda()
+    d_done = torch.zeros(n, dtype=torch.uint8, device="cuda")
+    d_lt = torch.zeros(n, dtype=torch.float64, device="cuda")
+    d_ta = torch.zeros(n, dtype=torch.float64, device="cuda")
+    d_st = torch.zeros(n, dtype=torch.int32, device="cuda")
+    wall, kern_ms = _Timer(torch, stream).run(
+        lambda: uw.transit_device(d_f.data_ptr(), d_t.data_ptr(), n, vd, d_done.data_ptr(), d_lt.data_ptr(),
+                                  d_ta.data_ptr(), d_st.data_ptr(), stream=stream.cuda_stream),
+        args.steps, args.warmup)
+    uw.check_device()
+    print(f"clearance: timed {args.steps} launches, {kern_ms:.1f} ms each", file=sys.stderr, flush=True)
+    steps = d_st.cpu().numpy().astype(np.int64)
+    done = d_done.cpu().numpy().astype(bool)
+    lt = d_lt.cpu().numpy()
+    q = int(steps.sum())
+    # the whole stage through the host API (host buffers, PCIe both ways,
+    # distances on the host), for comparison: not the timed step
+    t0 = time.perf_counter()
+    osc = C.touch_distances(uw, unit, tr, aa)
+    stage_s = time.perf_counter() - t0
+    extra = {"mismatches": None, "cpu_baseline": None, "kernel": "k_segments<transit> (persistent)",
+             "kernel_ms": round(kern_ms, 4), "stage_ms_host_api": round(stage_s * 1e3, 2)}
+    if not args.no_cpu:
+        from oracle import oracle as O
+        O.set_libm(False)
+        ow = O.OracleWorld()
+        ow.load_obj(0, os.path.join(ALPHA_DIR, "alpha_env-1.2.org.obj"))
+        ow.load_obj(1, os.path.join(ALPHA_DIR, "alpha-1.2.org.obj"))
+        ow.scale_to_unit()
+        ow.angle_model()
+        perm = np.random.default_rng(7102).permutation(n)
+        done_n, cq, mism, dt = 0, 0, 0, 0.0
+        while dt < args.cpu_seconds and done_n < n:
+            idx = perm[done_n:done_n + 512]
+            t0 = time.perf_counter()
+            o = ow.touch_batch(starts[idx], ends[idx], vd, 0, nthreads=_sample_threads())
+            dt += time.perf_counter() - t0
+            mism += int((o["done"].astype(bool) != done[idx]).sum() + (o["queries"] != steps[idx]).sum()
+                        + (o["last_tau"] != lt[idx]).sum())
+            cq += int(o["queries"].sum())
+            done_n += len(idx)
+        # DISTANCE_BATCH entries of the first two keys through the oracle's touch queries
+        for k in range(2):
+            free = [ow.transit(unit[k], ends[k * nsample + j], vd)[0] for j in range(16)]
+            mism += int((np.array([O.distance(unit[k], f) for f in free]) != osc[k, :16]).sum())
+        print(f"clearance: oracle sample done ({dt:.1f} s)", file=sys.stderr, flush=True)
+        extra["mismatches"] = mism
+        extra["cpu_baseline"] = dict(value=cq / dt, unit="queries/s", cores=_sample_threads(), kind="port",
+                                     sample=f"{done_n} random touch queries of the {n} ({cq} queries, {dt:.2f} s)")
+        fq, nbv, ntri = oracle_flops_per_query(ow, starts[perm[:32]], ends[perm[:32]], 32, vd)
+        extra["roofline"] = _compute_roofline(fq, nbv, ntri, q, kern_ms, "k_segments<transit>",
+                                              "the states of 32 random touch queries")
+    line = _line(f"key clearance: alpha puzzle (reference meshes), {nkey} keys x {nsample} touch samples at "
+                 f"recommended_cres {vd:.6g}", q * args.steps / wall, wall, args, extra)
+    line["config"].update({"keys": nkey, "samples_per_key": nsample, "segments": n, "queries_per_step": q,
+                           "contact_fraction": round(1.0 - float(done.mean()), 4),
+                           "keys_per_s": round(nkey / (kern_ms * 1e-3), 1)})
+    return line
+
+
 def _screen_segments(args, torch, W, uw, net, sq, sy, timer, stream):
     """Segments from exact-verified free states, lognormal(5, 1.2) step counts
     at 1/1024: isValidTransition on every segment, against the prescreen
@@ -671,14 +777,16 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-states", type=int, default=1024 * 1024,
                     help="batch workload: single states in the launch beside the segments")
-    ap.add_argument("--workload", choices=["touch", "states", "uneven", "visibility", "surrogate", "ompl", "batch"],
+    ap.add_argument("--workload", choices=["touch", "states", "uneven", "visibility", "surrogate", "ompl", "batch",
+                                           "clearance"],
                     default="touch",
                     help="touch (default, the headline): touch-configuration search on the torus scene; "
                          "states: 1M state validity; uneven: 4M lognormal segments on the shell scene; "
                          "visibility: 4,096-milestone matrix on the shell scene; "
                          "surrogate: learned prescreen on the shell scene; "
                          "ompl: OMPL key screening (all pairs of 1,024 keys) on the alpha puzzle; "
-                         "batch: 4M uneven segments + 1M states in one persistent launch (shell scene)")
+                         "batch: 4M uneven segments + 1M states in one persistent launch (shell scene); "
+                         "clearance: key clearance (1,024 keys x 128 touch samples) on the alpha puzzle")
     ap.add_argument("--train-epochs", type=int, default=2)
     ap.add_argument("--threshold", type=float, default=0.5)
     ap.add_argument("--screen-segments", type=int, default=8 * 1024 * 1024,
@@ -707,7 +815,8 @@ def main():
         del src, dst
     if args.workload != "touch":
         fn = {"states": run_states, "uneven": run_uneven, "visibility": run_visibility,
-              "surrogate": run_surrogate, "ompl": run_ompl, "batch": run_batch}[args.workload]
+              "surrogate": run_surrogate, "ompl": run_ompl, "batch": run_batch,
+              "clearance": run_clearance}[args.workload]
         print(json.dumps(fn(args, torch, pyosr, W)), flush=True)
         return
     env, rob = W.torus_scene(0)
