@@ -555,7 +555,7 @@ struct StatesOmpl {
 #define SE3CQ_REFILL_MIN 16
 #endif
 #ifndef SE3CQ_LEAF_PATIENCE
-#define SE3CQ_LEAF_PATIENCE 32
+#define SE3CQ_LEAF_PATIENCE 16
 #endif
 #ifndef SE3CQ_REFILL_CHUNK
 #define SE3CQ_REFILL_CHUNK 128
