@@ -979,7 +979,11 @@ __device__ __forceinline__ bool leaf_pair_x(const DevScene& s, const ExactTf& x,
  * descent from there adds at most stack_depth - 2 entries, which keeps the
  * stack within its 64 lanes.
  */
-enum { PACKET_LEAVES = 8 };
+#ifndef SE3CQ_PACKET_LEAVES
+#define SE3CQ_PACKET_LEAVES 8
+#endif
+enum { PACKET_LEAVES = SE3CQ_PACKET_LEAVES };  /* at most 64: one lane per entry */
+static_assert(PACKET_LEAVES >= 1 && PACKET_LEAVES <= 64, "leaf entries live one per lane");
 
 /*
  * Where the traversal gets its pose.  The pose is recomputed at the two
