@@ -24,6 +24,7 @@ the same seed gives the same targets and leaves the stream where the
 reference leaves it.  The touch queries of all keys go to the GPU in one
 batch (``UnitWorld.transit_states_to_with_contact``).
 """
+import itertools
 import math
 import os
 
@@ -61,6 +62,9 @@ def touch_directions(n, rs=None):
     - ball: x = 2 (u - 1/2) triples until x1^2 + x2^2 + x3^2 <= 1, times 2 pi.
     """
     rs = np.random if rs is None else rs
+    if not all(hasattr(rs, a) for a in ("get_state", "set_state", "random_sample")):
+        raise TypeError("touch_directions draws from numpy's legacy stream: pass np.random or a "
+                        "np.random.RandomState, as the reference's uw_random does")
     n = int(n)
     tr = np.empty((n, 3))
     aa = np.empty((n, 3))
@@ -169,7 +173,7 @@ def load_distance_batches(indir):
     missing file, concatenated without the empty ones (util.lsv2 and
     safe_concatente, util.py:634-679)."""
     arrs = []
-    for i in range(1 << 62):
+    for i in itertools.count():
         fn = os.path.join(indir, CLEARANCE_BATCH_FMT.format(i))
         if not os.path.exists(fn):
             if i == 0:

@@ -4,7 +4,6 @@ one-sample-at-a-time restatement; the oracle's direction sampler pinned by
 what the sampling methods fix.  No GPU: the touch queries themselves are
 checked in test_gpu_keyscreen.py."""
 import math
-import os
 
 import numpy as np
 import pytest
@@ -69,6 +68,9 @@ def test_touch_directions_global_stream():
     assert np.array_equal(tr, wtr) and np.array_equal(aa, waa)
     assert np.array_equal(after, np.random.rand(3))
     assert C.touch_directions(0)[0].shape == (0, 3)
+    # the reference draws from the legacy stream; a Generator has no such state
+    with pytest.raises(TypeError):
+        C.touch_directions(3, np.random.default_rng(0))
 
 
 def test_clearance_tasks():

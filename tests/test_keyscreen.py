@@ -5,7 +5,6 @@ oracle itself against hand-worked cases and brute force.  No GPU: motion and
 disentanglement checks come from stand-in predicates here; the GPU run of the
 same stages is in test_gpu_keyscreen.py."""
 import math
-import os
 
 import numpy as np
 import pytest
