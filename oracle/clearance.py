@@ -1,5 +1,5 @@
 """CPU restatement of the key-configuration clearance stage of the reference
-pipeline, as the checker of puzzletunneldiscovery_amd.clearance.
+pipeline, against which puzzletunneldiscovery_amd.clearance is tested.
 
 Test infrastructure only (tests/ use it); the shipped package never imports
 it.  It follows the reference's procedure one sample at a time:

@@ -1,5 +1,6 @@
 """CPU restatement of the key-configuration screening stages of the
-reference pipeline, as the checker of puzzletunneldiscovery_amd.keyscreen.
+reference pipeline, against which puzzletunneldiscovery_amd.keyscreen is
+tested.
 
 Test infrastructure only (tests/ use it); the shipped package never imports
 it.  It follows the reference's own procedure step by step rather than the

@@ -21,7 +21,7 @@
  *   - a bisection refinement of the contact parameter (no reference
  *     counterpart; defined here as the contract for the library's
  *     bisect_contact)
- *   - a brute-force all-triangle-pairs checker that pins the BVH traversal.
+ *   - a brute-force test over all triangle pairs that pins the BVH traversal.
  *
  * All arithmetic is binary64 with contraction off (-ffp-contract=off).
  */

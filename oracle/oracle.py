@@ -1,8 +1,8 @@
 """ctypes binding of the CPU oracle (oracle/oracle.c).
 
 Test infrastructure only: tests/, ``__graft_entry__.smoke()`` and the
-``cpu_baseline`` leg of ``bench.py`` use it as the checker and as the CPU
-baseline.  The shipped ``puzzletunneldiscovery_amd`` package never imports it.
+``cpu_baseline`` leg of ``bench.py`` test the GPU path against it and time
+it as the CPU baseline.  The shipped ``puzzletunneldiscovery_amd`` package never imports it.
 """
 import ctypes
 import os
