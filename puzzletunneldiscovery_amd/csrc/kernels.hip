@@ -37,13 +37,17 @@
 #include "se3_math.h"
 
 /* Minimum resident wavefronts per SIMD each traversal kernel is compiled
- * for (caps its VGPR budget; 4 waves = 128 VGPRs, 6 = 80, 7 = 72).  The
- * segment walks wait on dependent node loads, and for the plain segment and
- * visibility walks more resident waves hide that better than the registers
- * they give up, even with the spills that follow.  The per-lane state
- * kernels, the touch search and the OMPL motions are fastest at 4. */
+ * for (caps its VGPR budget; 3 waves = 168 VGPRs, 4 = 128, 6 = 80, 7 = 72).
+ * The segment walks wait on dependent node loads, and for the plain segment
+ * and visibility walks more resident waves hide that better than the
+ * registers they give up, even with the spills that follow.  The touch
+ * search, the OMPL motions and the mixed batch are fastest at 4, and the
+ * per-lane state kernels at 3, where they do not spill. */
 #ifndef SE3CQ_MIN_WAVES
 #define SE3CQ_MIN_WAVES 4
+#endif
+#ifndef SE3CQ_STATES_WAVES
+#define SE3CQ_STATES_WAVES 3
 #endif
 #ifndef SE3CQ_SEG_WAVES
 #define SE3CQ_SEG_WAVES 7
@@ -881,7 +885,7 @@ struct StoreValidI32 {
 };
 
 template <bool WIDE>
-__global__ void __launch_bounds__(256, SE3CQ_MIN_WAVES) k_valid_states(DevScene s, const double* __restrict__ qs, int64_t n,
+__global__ void __launch_bounds__(256, SE3CQ_STATES_WAVES) k_valid_states(DevScene s, const double* __restrict__ qs, int64_t n,
                                                       uint8_t* __restrict__ out)
 {
 	extern __shared__ uint32_t smem[];
@@ -893,7 +897,7 @@ __global__ void __launch_bounds__(256, SE3CQ_MIN_WAVES) k_valid_states(DevScene 
 /* OMPL-convention states (OmplDriver::validateStates,
  * lib/pyse3ompl/ompldriver.cc:566-589): converted to the unit world on load. */
 template <bool WIDE>
-__global__ void __launch_bounds__(256, SE3CQ_MIN_WAVES) k_ompl_states(DevScene s, OmplXf x, const double* __restrict__ qs,
+__global__ void __launch_bounds__(256, SE3CQ_STATES_WAVES) k_ompl_states(DevScene s, OmplXf x, const double* __restrict__ qs,
                                                      int64_t n, int32_t* __restrict__ out)
 {
 	extern __shared__ uint32_t smem[];
