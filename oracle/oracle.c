@@ -1038,6 +1038,207 @@ int or_is_valid_counted(const OrWorld* w, const double* q7, int64_t* n_bv, int64
 
 int or_is_valid(const OrWorld* w, const double* q7) { return or_is_valid_counted(w, q7, NULL, NULL); }
 
+/* ------------------------------------------------------------------ */
+/* Contact segments and bounding boxes (restated here, independently  */
+/* of the library's own versions, so the tests compare two codes)     */
+/* ------------------------------------------------------------------ */
+/* A point placed by a rigid transform, Eigen's `linear() * c + translation()`
+ * with the 3-term row sums taken left to right. */
+static void or_place(const double* R, const double* t, const double* c, double* out)
+{
+	double r0 = R[0] * c[0] + R[1] * c[1];
+	double r1 = R[3] * c[0] + R[4] * c[1];
+	double r2 = R[6] * c[0] + R[7] * c[1];
+	out[0] = (r0 + R[2] * c[2]) + t[0];
+	out[1] = (r1 + R[5] * c[2]) + t[1];
+	out[2] = (r2 + R[8] * c[2]) + t[2];
+}
+
+/* The boolean part of FCL's box-box test (boxBox2, ODE's dBoxBox): box 1
+ * (rotation R1, row-major, column k = axis k; centre c1; half sides A)
+ * against box 2 (R2, c2, B).  The fifteen candidate axes are written out one
+ * by one as dBoxBox lists them; each sum of radii runs left to right in
+ * that listing's term order.  The nine edge-edge axes use |R| plus a 1e-6
+ * guard.  Returns 1 when some axis separates the boxes. */
+static int or_box_box_disjoint(const double* R1, const double* c1, const double* A, const double* R2,
+                               const double* c2, const double* B)
+{
+	const double d[3] = {c2[0] - c1[0], c2[1] - c1[1], c2[2] - c1[2]};
+	double pp[3], M[3][3], Q[3][3];
+	for (int i = 0; i < 3; i++) {
+		/* column i of R1 dotted with d and with the columns of R2 */
+		pp[i] = (R1[0 + i] * d[0] + R1[3 + i] * d[1]) + R1[6 + i] * d[2];
+		for (int j = 0; j < 3; j++) {
+			M[i][j] = (R1[0 + i] * R2[0 + j] + R1[3 + i] * R2[3 + j]) + R1[6 + i] * R2[6 + j];
+			Q[i][j] = fabs(M[i][j]);
+		}
+	}
+	double v0 = (R2[0] * d[0] + R2[3] * d[1]) + R2[6] * d[2];
+	double v1 = (R2[1] * d[0] + R2[4] * d[1]) + R2[7] * d[2];
+	double v2 = (R2[2] * d[0] + R2[5] * d[1]) + R2[8] * d[2];
+#define OR_SEP(expr, rad) if (fabs(expr) - (rad) > 0.0) return 1
+	/* box 1's face normals */
+	OR_SEP(pp[0], A[0] + B[0] * Q[0][0] + B[1] * Q[0][1] + B[2] * Q[0][2]);
+	OR_SEP(pp[1], A[1] + B[0] * Q[1][0] + B[1] * Q[1][1] + B[2] * Q[1][2]);
+	OR_SEP(pp[2], A[2] + B[0] * Q[2][0] + B[1] * Q[2][1] + B[2] * Q[2][2]);
+	/* box 2's face normals */
+	OR_SEP(v0, A[0] * Q[0][0] + A[1] * Q[1][0] + A[2] * Q[2][0] + B[0]);
+	OR_SEP(v1, A[0] * Q[0][1] + A[1] * Q[1][1] + A[2] * Q[2][1] + B[1]);
+	OR_SEP(v2, A[0] * Q[0][2] + A[1] * Q[1][2] + A[2] * Q[2][2] + B[2]);
+	for (int i = 0; i < 3; i++)
+		for (int j = 0; j < 3; j++)
+			Q[i][j] += 1e-6;
+	/* edge of box 1 x edge of box 2 */
+	OR_SEP(pp[2] * M[1][0] - pp[1] * M[2][0], A[1] * Q[2][0] + A[2] * Q[1][0] + B[1] * Q[0][2] + B[2] * Q[0][1]);
+	OR_SEP(pp[2] * M[1][1] - pp[1] * M[2][1], A[1] * Q[2][1] + A[2] * Q[1][1] + B[0] * Q[0][2] + B[2] * Q[0][0]);
+	OR_SEP(pp[2] * M[1][2] - pp[1] * M[2][2], A[1] * Q[2][2] + A[2] * Q[1][2] + B[0] * Q[0][1] + B[1] * Q[0][0]);
+	OR_SEP(pp[0] * M[2][0] - pp[2] * M[0][0], A[0] * Q[2][0] + A[2] * Q[0][0] + B[1] * Q[1][2] + B[2] * Q[1][1]);
+	OR_SEP(pp[0] * M[2][1] - pp[2] * M[0][1], A[0] * Q[2][1] + A[2] * Q[0][1] + B[0] * Q[1][2] + B[2] * Q[1][0]);
+	OR_SEP(pp[0] * M[2][2] - pp[2] * M[0][2], A[0] * Q[2][2] + A[2] * Q[0][2] + B[0] * Q[1][1] + B[1] * Q[1][0]);
+	OR_SEP(pp[1] * M[0][0] - pp[0] * M[1][0], A[0] * Q[1][0] + A[1] * Q[0][0] + B[1] * Q[2][2] + B[2] * Q[2][1]);
+	OR_SEP(pp[1] * M[0][1] - pp[0] * M[1][1], A[0] * Q[1][1] + A[1] * Q[0][1] + B[0] * Q[2][2] + B[2] * Q[2][0]);
+	OR_SEP(pp[1] * M[0][2] - pp[0] * M[1][2], A[0] * Q[1][2] + A[1] * Q[0][2] + B[0] * Q[2][1] + B[1] * Q[2][0]);
+#undef OR_SEP
+	return 0;
+}
+
+/* Moller's triangle-triangle test with the intersection segment (JGT 1997,
+ * tri_tri_intersect_with_isectline as lib/tritri/Moller97d.cc:575-700 has
+ * it): signed plane distances below 1e-6 in magnitude snap to 0; each
+ * triangle's two crossing points lie on the edges from its lone vertex (the
+ * one on its own side of the other plane) to the remaining two, in index
+ * order; the overlap of the two intervals on the axis where the planes'
+ * common direction is largest gives the segment.  Returns 1 when the
+ * triangles meet; coplanar triangles return 1 with *coplanar = 1 and a zero
+ * segment. */
+static double or_dot3l(const double* a, const double* b) { return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2]; }
+
+static void or_cross3l(const double* a, const double* b, double* r)
+{
+	r[0] = a[1] * b[2] - a[2] * b[1];
+	r[1] = a[2] * b[0] - a[0] * b[2];
+	r[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+/* Signed distances of P0..P2 to the plane through T0..T2 (snapped), into dist[3]. */
+static void or_plane_dists(const double* T0, const double* T1, const double* T2, const double* const P[3],
+                           double* n, double* dist)
+{
+	double e1[3], e2[3];
+	for (int k = 0; k < 3; k++) {
+		e1[k] = T1[k] - T0[k];
+		e2[k] = T2[k] - T0[k];
+	}
+	or_cross3l(e1, e2, n);
+	const double off = -or_dot3l(n, T0);
+	for (int v = 0; v < 3; v++) {
+		dist[v] = or_dot3l(n, P[v]) + off;
+		if (fabs(dist[v]) < 0.000001)
+			dist[v] = 0.0;
+	}
+}
+
+/* Which vertex is alone: -1 if the triangle lies in the other plane. */
+static int or_lone_vertex(const double* dv)
+{
+	if (dv[0] * dv[1] > 0.0)
+		return 2;
+	if (dv[0] * dv[2] > 0.0)
+		return 1;
+	if (dv[1] * dv[2] > 0.0 || dv[0] != 0.0)
+		return 0;
+	if (dv[1] != 0.0)
+		return 1;
+	if (dv[2] != 0.0)
+		return 2;
+	return -1;
+}
+
+/* The two crossings of triangle T (distances dv, projections pr) from its
+ * lone vertex `a`: parameters s[2] on the axis and points X[2][3]. */
+static void or_crossings(const double* const T[3], const double* dv, const double* pr, int a, double* s,
+                         double X[2][3])
+{
+	const int others[3][2] = {{1, 2}, {0, 2}, {0, 1}};
+	for (int m = 0; m < 2; m++) {
+		const int b = others[a][m];
+		const double f = dv[a] / (dv[a] - dv[b]);
+		s[m] = pr[a] + (pr[b] - pr[a]) * f;
+		for (int k = 0; k < 3; k++)
+			X[m][k] = T[a][k] + (T[b][k] - T[a][k]) * f;
+	}
+}
+
+static int or_moller_segment(const double* V0, const double* V1, const double* V2, const double* U0,
+                             const double* U1, const double* U2, int* coplanar, double* p1, double* p2)
+{
+	const double* const Vt[3] = {V0, V1, V2};
+	const double* const Ut[3] = {U0, U1, U2};
+	double n1[3], n2[3], du[3], dv[3], dir[3];
+	*coplanar = 0;
+	for (int k = 0; k < 3; k++)
+		p1[k] = p2[k] = 0.0;
+	or_plane_dists(V0, V1, V2, Ut, n1, du);
+	if (du[0] * du[1] > 0.0 && du[0] * du[2] > 0.0)
+		return 0;
+	or_plane_dists(U0, U1, U2, Vt, n2, dv);
+	if (dv[0] * dv[1] > 0.0 && dv[0] * dv[2] > 0.0)
+		return 0;
+	or_cross3l(n1, n2, dir);
+	int ax = 0;
+	double big = fabs(dir[0]);
+	for (int k = 1; k < 3; k++)
+		if (fabs(dir[k]) > big) {
+			big = fabs(dir[k]);
+			ax = k;
+		}
+	const int lv = or_lone_vertex(dv);
+	if (lv < 0) {
+		*coplanar = 1;
+		return 1;
+	}
+	const int lu = or_lone_vertex(du);
+	if (lu < 0) {
+		/* U lies in V's plane (within the snap) while V does not lie in
+		 * U's: Moller's code leaves U's interval unset here; both codes
+		 * report the pair as coplanar */
+		*coplanar = 1;
+		return 1;
+	}
+	const double pv[3] = {V0[ax], V1[ax], V2[ax]}, pu[3] = {U0[ax], U1[ax], U2[ax]};
+	double sv[2], su[2], XV[2][3], XU[2][3];
+	or_crossings(Vt, dv, pv, lv, sv, XV);
+	or_crossings(Ut, du, pu, lu, su, XU);
+	/* each interval as (low, high) with its end points */
+	const int fv = sv[0] > sv[1], fu = su[0] > su[1];
+	const double vlo = fv ? sv[1] : sv[0], vhi = fv ? sv[0] : sv[1];
+	const double ulo = fu ? su[1] : su[0], uhi = fu ? su[0] : su[1];
+	const double* Vlo = XV[fv];
+	const double* Vhi = XV[1 - fv];
+	const double* Ulo = XU[fu];
+	const double* Uhi = XU[1 - fu];
+	if (vhi < ulo || uhi < vlo)
+		return 0;
+	const double* a;
+	const double* b;
+	if (ulo < vlo) {
+		/* the overlap starts at V's low end and ends at the lower high end
+		 * (V's on a tie) */
+		a = Vlo;
+		b = (uhi < vhi) ? Uhi : Vhi;
+	} else {
+		/* it starts at U's low end (also on a tie) and ends at the lower
+		 * high end (U's on a tie) */
+		a = Ulo;
+		b = (uhi > vhi) ? Vhi : Uhi;
+	}
+	for (int k = 0; k < 3; k++) {
+		p1[k] = a[k];
+		p2[k] = b[k];
+	}
+	return 1;
+}
+
 /* UnitWorld::intersectingSegments (unit_world.cc:959-1012): every
  * intersecting (env triangle, robot triangle) pair by brute force over all
  * pairs (the contacts CDModel::collideForDetails collects, cdmodel.cc:201-229),
@@ -1063,10 +1264,10 @@ int or_contacts(const OrWorld* w, const double* q7, int cap, int32_t* pairs, dou
 				double V[9], U[9], p1[3], p2[3];
 				int cop = 0;
 				for (int v = 0; v < 3; v++) {
-					se3_place(Re, te, env->V + 3 * env->F[3 * i + v], V + 3 * v);
-					se3_place(Rr, tr, rob->V + 3 * rob->F[3 * j + v], U + 3 * v);
+					or_place(Re, te, env->V + 3 * env->F[3 * i + v], V + 3 * v);
+					or_place(Rr, tr, rob->V + 3 * rob->F[3 * j + v], U + 3 * v);
 				}
-				const int hit = se3_tri_isect_segment(V, V + 3, V + 6, U, U + 3, U + 6, &cop, p1, p2);
+				const int hit = or_moller_segment(V, V + 3, V + 6, U, U + 3, U + 6, &cop, p1, p2);
 				pairs[2 * found] = (int32_t)i;
 				pairs[2 * found + 1] = (int32_t)j;
 				for (int k = 0; k < 3; k++) {
@@ -1083,7 +1284,7 @@ int or_contacts(const OrWorld* w, const double* q7, int cap, int32_t* pairs, dou
 /* The triangle-triangle segment test alone. */
 int or_tri_isect_segment(const double* V9, const double* U9, int* coplanar, double* p1, double* p2)
 {
-	return se3_tri_isect_segment(V9, V9 + 3, V9 + 6, U9, U9 + 3, U9 + 6, coplanar, p1, p2);
+	return or_moller_segment(V9, V9 + 3, V9 + 6, U9, U9 + 3, U9 + 6, coplanar, p1, p2);
 }
 
 /* UnitWorld::isDisentangled (unit_world.cc:249-264) -> CDModel::collideBB
@@ -1098,9 +1299,9 @@ int or_is_disentangled(const OrWorld* w, const double* q7)
 	double Re[9], te[3], Rr[9], tr[3], pe[3], pr[3];
 	or_state_to_transform(w->pert, Re, te);
 	or_state_to_transform(q7, Rr, tr);
-	se3_place(Re, te, w->cd[0].bc, pe);
-	se3_place(Rr, tr, w->cd[1].bc, pr);
-	return se3_boxes_disjoint(Rr, pr, w->cd[1].bh, Re, pe, w->cd[0].bh);
+	or_place(Re, te, w->cd[0].bc, pe);
+	or_place(Rr, tr, w->cd[1].bc, pr);
+	return or_box_box_disjoint(Rr, pr, w->cd[1].bh, Re, pe, w->cd[0].bh);
 }
 
 void or_disentangled_states(const OrWorld* w, const double* q, int64_t n, int8_t* out, int nthreads)
@@ -1117,7 +1318,7 @@ void or_disentangled_states(const OrWorld* w, const double* q, int64_t n, int8_t
 int or_boxes_disjoint(const double* R1, const double* p1, const double* A, const double* R2, const double* p2,
                       const double* B)
 {
-	return se3_boxes_disjoint(R1, p1, A, R2, p2, B);
+	return or_box_box_disjoint(R1, p1, A, R2, p2, B);
 }
 
 void or_model_box(const OrWorld* w, int geo, double* c3, double* h3)

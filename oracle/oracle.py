@@ -283,10 +283,12 @@ class OracleWorld:
                          ctypes.byref(lt), ctypes.byref(t), ctypes.byref(ns))
         return lf, st, bool(done.value), lt.value, t.value, ns.value
 
-    def is_valid_transition(self, frm, to, ivd):
+    def is_valid_transition(self, frm, to, ivd, with_steps=False):
         frm = _f64(frm, (7,))
         to = _f64(to, (7,))
-        return bool(lib().or_is_valid_transition(self._w, _d(frm), _d(to), ivd, None))
+        ns = ctypes.c_int64(0)
+        ok = bool(lib().or_is_valid_transition(self._w, _d(frm), _d(to), ivd, ctypes.byref(ns)))
+        return (ok, ns.value) if with_steps else ok
 
     def bisect(self, frm, to, lo, hi, levels):
         frm = _f64(frm, (7,))

@@ -329,12 +329,17 @@ SE3_HD static inline int se3_boxes_disjoint(const double* R1, const double* p1, 
 	}
 	for (int k = 0; k < 9; k++)
 		Q[k] = Q[k] + 1e-6;
-	/* u_i x v_j: with (i1, i2) the other two axes of box 1 and (j1, j2) of box 2 */
+	/* u_i x v_j: (i1, i2) are the other two axes of box 1 and (j1, j2) of
+	 * box 2, each pair in increasing order, which is the term order of the
+	 * dBoxBox listing (A[i1] before A[i2], B[j1] before B[j2]); the sign of
+	 * e follows the cyclic order of (i, i+1, i+2) */
 	for (int i = 0; i < 3; i++) {
-		const int i1 = (i + 1) % 3, i2 = (i + 2) % 3;
+		const int c1 = (i + 1) % 3, c2 = (i + 2) % 3;
+		const int i1 = c1 < c2 ? c1 : c2, i2 = c1 < c2 ? c2 : c1;
 		for (int j = 0; j < 3; j++) {
-			const int j1 = (j + 1) % 3, j2 = (j + 2) % 3;
-			const double e = pp[i2] * R[3 * i1 + j] - pp[i1] * R[3 * i2 + j];
+			const int k1 = (j + 1) % 3, k2 = (j + 2) % 3;
+			const int j1 = k1 < k2 ? k1 : k2, j2 = k1 < k2 ? k2 : k1;
+			const double e = pp[c2] * R[3 * c1 + j] - pp[c1] * R[3 * c2 + j];
 			const double r = ((A[i1] * Q[3 * i2 + j] + A[i2] * Q[3 * i1 + j]) + B[j1] * Q[3 * i + j2]) +
 			                 B[j2] * Q[3 * i + j1];
 			if (fabs(e) - r > 0.0)
@@ -450,7 +455,13 @@ SE3_HD static inline int se3_tri_isect_segment(const double* V0, const double* V
 		*coplanar = 1;
 		return 1;
 	}
-	se3_mol_interval(U0, U1, U2, U0[idx], U1[idx], U2[idx], du0, du1, du2, du0du1, du0du2, &i2a, &i2b, b1, b2);
+	if (se3_mol_interval(U0, U1, U2, U0[idx], U1[idx], U2[idx], du0, du1, du2, du0du1, du0du2, &i2a, &i2b, b1, b2)) {
+		/* U lies in V's plane within the snap while V does not lie in U's:
+		 * Moller's code leaves U's interval unset; report the pair as
+		 * coplanar */
+		*coplanar = 1;
+		return 1;
+	}
 	int s1 = 0, s2 = 0;
 	if (i1a > i1b) {
 		const double t = i1a; i1a = i1b; i1b = t;

@@ -123,6 +123,36 @@ def test_valid_transitions(gpu_world, torus_oracle):
         assert not g[:3].any()
 
 
+
+def test_valid_transitions_clamp(gpu_world, torus_oracle):
+    """Pairs shorter than 2 * initial_verify_delta walk at vd = min(d/2, ivd)
+    (unit_world.cc:477-486): two checks, the midpoint and the end.  Pairs of
+    SE(3) length 0.2-0.35, among them every free-to-free pair of a seeded pool
+    whose midpoint collides, at verify steps that put all, some or none of
+    them below 2 * ivd; answers and step counts must equal the oracle's."""
+    import torch
+    from oracle.cases import pairs_with_midpoint_hits
+    a, b, n_hit = pairs_with_midpoint_hits(torus_oracle, 197)
+    assert n_hit >= 5
+    n = len(a)
+    da, db = torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda()
+    out = torch.zeros(n, dtype=torch.uint8, device="cuda")
+    steps = torch.zeros(n, dtype=torch.int32, device="cuda")
+    for ivd in (0.2, 0.15, 0.125, 0.1, 1.0 / 256):
+        _flush(f"transitions: {n} pairs shorter than 2 ivd, ivd {ivd}")
+        torch.cuda.synchronize()
+        gpu_world.valid_transitions_device(da.data_ptr(), db.data_ptr(), n, ivd, out.data_ptr(), steps.data_ptr())
+        torch.cuda.synchronize()
+        gpu_world.check_device()
+        c = [torus_oracle.is_valid_transition(a[i], b[i], ivd, with_steps=True) for i in range(n)]
+        assert np.array_equal(out.cpu().numpy().astype(bool), np.array([x for x, _ in c]))
+        assert np.array_equal(steps.cpu().numpy(), np.array([k for _, k in c]))
+        if ivd == 0.2:
+            # every pair is below 2 * ivd: two checks, or one when the midpoint collides
+            assert set(steps.cpu().numpy().tolist()) == {1, 2}
+    g = gpu_world.is_valid_transitions(a, b, 0.2)
+    assert int((~g).sum()) == n_hit
+
 def test_touch_search(gpu_world, torus_oracle):
     starts = _free_starts(gpu_world, 2000, 51)
     ends = W.segment_ends(starts, seed=52)

@@ -177,3 +177,33 @@ def test_alpha_robot_center_matches_recorded_ompl_center(oracle_mod):
     ow.load_obj(1, os.path.join(REF, "alpha-1.2.org.obj"))
     want = np.array([16.973146438598633, 1.2278236150741577, 10.204807281494141])
     assert np.array_equal(ow.center(1).astype(np.float64), want)
+
+
+
+def test_valid_transition_clamp(torus_oracle):
+    """isValidTransition steps at vd = min(d/2, initial_vd) (unit_world.cc:477-486).
+    For d <= 2*ivd that is vd = d/2: exactly two checks, at tau = (d/2)*fl(1/d)
+    and tau = d*fl(1/d) (d/2 + d/2 = d exactly, so the min clamp lands on d),
+    and the answer is their conjunction.  Without the clamp a pair with
+    ivd >= d would take the early return and report false; with ivd a hair
+    below d/2 the walk takes a third check."""
+    from oracle import oracle as O
+    from oracle.cases import pairs_with_midpoint_hits
+    a, b, n_hit = pairs_with_midpoint_hits(torus_oracle, 97)
+    # short free-to-free pairs whose midpoint collides exist in this scene
+    assert n_hit >= 5
+    seen = set()
+    for i in range(len(a)):
+        d = O.distance(a[i], b[i])
+        assert d > 0
+        inv = 1.0 / d
+        v1 = torus_oracle.is_valid(O.interpolate(a[i], b[i], (d / 2) * inv))
+        v2 = torus_oracle.is_valid(O.interpolate(a[i], b[i], d * inv))
+        assert v2  # the end state itself (d * fl(1/d) rounds to 1 here or one ulp below)
+        seen.add(v1)
+        for ivd in (d / 2, 0.75 * d, d, 2 * d, 10.0):
+            ok, ns = torus_oracle.is_valid_transition(a[i], b[i], ivd, with_steps=True)
+            assert (ok, ns) == ((v1 and v2), 2 if v1 else 1), (i, ivd)
+        ok, ns = torus_oracle.is_valid_transition(a[i], b[i], np.nextafter(d / 2, 0.0), with_steps=True)
+        assert ns == 3 or (not ok and ns <= 3)
+    assert seen == {True, False}
