@@ -255,6 +255,10 @@ int se3cq_apply(const double* from7, const double* tr3, const double* aa3, doubl
 int se3cq_apply_batch(const double* from, const double* tr, const double* aa, int64_t n, double* out);
 /* decompose_2 (osr_state.cc:94-99): translation and rotation matrix (row-major). */
 int se3cq_decompose_2(const double* q7, double* t3, double* R9);
+/* decompose_3 (osr_state.cc:101-107, bound at lib/pyosr/module.cc:81-82):
+ * translation, rotation angle in [0, pi] and unit axis of the normalised
+ * quaternion (Eigen's AngleAxis of it). */
+int se3cq_decompose_3(const double* q7, double* t3, double* angle, double* axis3);
 /* compose_from_angleaxis (osr_state.cc:121-128). */
 int se3cq_compose_from_angleaxis(const double* t3, double angle, const double* axis3, double* out7);
 

@@ -99,6 +99,7 @@ SIGNATURES = {
     "se3cq_apply": [c_double_p, c_double_p, c_double_p, c_double_p],
     "se3cq_apply_batch": [c_double_p, c_double_p, c_double_p, i64, c_double_p],
     "se3cq_decompose_2": [c_double_p, c_double_p, c_double_p],
+    "se3cq_decompose_3": [c_double_p, c_double_p, c_double_p, c_double_p],
     "se3cq_compose_from_angleaxis": [c_double_p, cdbl, c_double_p, c_double_p],
 }
 # include/se3mlp.h: the learned prescreen

@@ -76,6 +76,7 @@ struct DevScene {
 	int32_t wide;          /* 1 if node ids need more than 16 bits */
 	uint32_t* error_flag;  /* device word: bit 0 traversal overflow, bit 1 a segment with 2^31+ steps */
 	unsigned long long* queue; /* this launch's work counter (persistent kernels zero it first) */
+	const int32_t* order;  /* segment kernels: item k walks segment order[k] (null: segment k) */
 	int32_t num_cu;        /* compute units of the device (persistent grid size) */
 };
 

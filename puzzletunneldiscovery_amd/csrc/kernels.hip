@@ -1725,9 +1725,10 @@ __global__ void __launch_bounds__(256, MODE == MODE_TOUCH ? SE3CQ_TOUCH_WAVES : 
 	SegShared* sh = &segsh[wave];
 	BisectShared* bs = (MODE == MODE_TOUCH) ? reinterpret_cast<BisectShared*>(seg_lds) + wave : nullptr;
 	for (;;) {
-		const int64_t seg = next_item(s.queue);
-		if (seg >= n)
+		const int64_t item = next_item(s.queue);
+		if (item >= n)
 			break;
+		const int64_t seg = s.order ? (int64_t)s.order[item] : item;
 		seg_setup<MODE>(sh, from_all + 7 * seg, to_all + 7 * seg, vd);
 		wave_lds_sync();
 		const SegResult r = segment_walk<WIDE, MODE>(s, sh, levels, bs);
@@ -2103,6 +2104,92 @@ static inline int pick_block(const DevScene& s)
 
 static inline size_t stack_bytes(const DevScene& s, int block) { return stack_bytes_per_thread(s) * (size_t)block; }
 
+/* ---- longest-first order of a segment launch ----
+ *
+ * Persistent wavefronts take segments in item order, so a long segment taken
+ * late in the launch walks alone while the other wavefronts idle.  A
+ * segment launch can instead walk its segments longest first: a counting
+ * sort on the estimated step count (distance / verify step), 16 bins per
+ * octave, builds order[] from one pass over the inputs.  Segments within a
+ * bin come in no fixed order.  The order changes which wavefront walks a
+ * segment and when, never what it computes. */
+constexpr int ORDER_BINS = 256;
+
+/* Bin of an estimated step count: 0 for the longest (2^16 steps and more),
+ * 255 for fewer than one step (the early return) or a NaN distance. */
+__device__ __forceinline__ int order_bin(double steps)
+{
+	if (!(steps >= 1.0))
+		return ORDER_BINS - 1;
+	if (steps >= 65536.0)
+		return 0;
+	const uint32_t b = __float_as_uint((float)steps);
+	const int octave = (int)(b >> 23) - 127;  /* 0 .. 15 */
+	const int sixteenth = (int)((b >> 19) & 15u);
+	return ORDER_BINS - 1 - (octave * 16 + sixteenth);
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(256) k_order_keys(const double* __restrict__ from, const double* __restrict__ to,
+                                                    int64_t n, double vd, uint8_t* __restrict__ key,
+                                                    uint32_t* __restrict__ hist)
+{
+	__shared__ uint32_t lh[ORDER_BINS];
+	lh[threadIdx.x] = 0;
+	__syncthreads();
+	const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) {
+		const double d = se3_distance(from + 7 * i, to + 7 * i);
+		double step = vd;
+		if (MODE == MODE_TRANSITION)
+			step = (vd < d / 2.0) ? vd : d / 2.0;
+		const int b = order_bin(d / step);
+		key[i] = (uint8_t)b;
+		atomicAdd(&lh[b], 1u);
+	}
+	__syncthreads();
+	if (lh[threadIdx.x])
+		atomicAdd(&hist[threadIdx.x], lh[threadIdx.x]);
+}
+
+/* Exclusive prefix sum of the 256 bin counts into the bins' first slots. */
+__global__ void __launch_bounds__(256) k_order_scan(const uint32_t* __restrict__ hist, uint32_t* __restrict__ cursor)
+{
+	__shared__ uint32_t v[ORDER_BINS];
+	const int t = threadIdx.x;
+	v[t] = hist[t];
+	__syncthreads();
+	for (int off = 1; off < ORDER_BINS; off <<= 1) {
+		const uint32_t add = t >= off ? v[t - off] : 0u;
+		__syncthreads();
+		v[t] += add;
+		__syncthreads();
+	}
+	cursor[t] = v[t] - hist[t];
+}
+
+/* Each block reserves one run of slots per bin, then its threads fill it. */
+__global__ void __launch_bounds__(256) k_order_scatter(const uint8_t* __restrict__ key, int64_t n,
+                                                       uint32_t* __restrict__ cursor, int32_t* __restrict__ order)
+{
+	__shared__ uint32_t cnt[ORDER_BINS], base[ORDER_BINS];
+	cnt[threadIdx.x] = 0;
+	__syncthreads();
+	const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	int b = 0;
+	uint32_t r = 0;
+	if (i < n) {
+		b = key[i];
+		r = atomicAdd(&cnt[b], 1u);
+	}
+	__syncthreads();
+	if (cnt[threadIdx.x])
+		base[threadIdx.x] = atomicAdd(&cursor[threadIdx.x], cnt[threadIdx.x]);
+	__syncthreads();
+	if (i < n)
+		order[base[b] + r] = (int32_t)i;
+}
+
 __global__ void k_fill_u32(uint32_t* p, int64_t n, uint32_t v);
 
 /* Zero the launch's work counter with a kernel (not hipMemsetAsync, so the
@@ -2177,6 +2264,18 @@ hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t st)
 	if (n <= 0)
 		return hipSuccess;
 	hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, v);
+	return hipGetLastError();
+}
+
+__global__ void k_take_u32(uint32_t* flag, uint32_t* out)
+{
+	if (threadIdx.x == 0)
+		*out = atomicExch(flag, 0u);
+}
+
+hipError_t launch_take_u32(uint32_t* flag, uint32_t* out, hipStream_t st)
+{
+	hipLaunchKernelGGL(k_take_u32, dim3(1), dim3(64), 0, st, flag, out);
 	return hipGetLastError();
 }
 
@@ -2283,36 +2382,78 @@ int batch_blocks_per_cu(const DevScene& s, int block, bool states)
 	return (e == hipSuccess && nb > 0) ? nb : 1;
 }
 
+static constexpr size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+static constexpr size_t order_bytes_for(int64_t n)
+{
+	return 2 * ORDER_BINS * sizeof(uint32_t) + align256((size_t)n) + (size_t)n * sizeof(int32_t);
+}
+
+size_t segment_order_bytes(int64_t n)
+{
+	if (n <= 0 || n >= ((int64_t)1 << 31))
+		return 0;
+	return order_bytes_for(n);
+}
+
+/* The longest-first order of n segments into `scratch` (segment_order_bytes(n)
+ * bytes: bin counts, bin cursors, one key byte per segment, the order). */
 template <int MODE>
-static hipError_t launch_seg_mode(const DevScene& s, const double* from, const double* to, int64_t n, double vd,
-                                  int levels, const SegOut& o, hipStream_t st)
+static hipError_t build_order(const double* from, const double* to, int64_t n, double vd, void* scratch,
+                              const int32_t** order, hipStream_t st)
+{
+	uint32_t* hist = static_cast<uint32_t*>(scratch);
+	uint32_t* cursor = hist + ORDER_BINS;
+	uint8_t* key = reinterpret_cast<uint8_t*>(cursor + ORDER_BINS);
+	int32_t* ord = reinterpret_cast<int32_t*>(key + align256((size_t)n));
+	const unsigned blocks = (unsigned)((n + 255) / 256);
+	hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(ORDER_BINS), 0, st, hist, (int64_t)ORDER_BINS, 0u);
+	hipLaunchKernelGGL(k_order_keys<MODE>, dim3(blocks), dim3(256), 0, st, from, to, n, vd, key, hist);
+	hipLaunchKernelGGL(k_order_scan, dim3(1), dim3(ORDER_BINS), 0, st, hist, cursor);
+	hipLaunchKernelGGL(k_order_scatter, dim3(blocks), dim3(256), 0, st, key, n, cursor, ord);
+	*order = ord;
+	return hipGetLastError();
+}
+
+template <int MODE>
+static hipError_t launch_seg_mode(const DevScene& s_in, const double* from, const double* to, int64_t n, double vd,
+                                  int levels, const SegOut& o, hipStream_t st, void* order_scratch)
 {
 	/* one wavefront per segment, four per block; the touch search keeps a
 	 * BisectShared record per wavefront in LDS */
 	const int block = 256;
 	const int per_block = block / 64;
 	const size_t lds = (MODE == MODE_TOUCH) ? per_block * sizeof(BisectShared) : 0;
+	DevScene s = s_in;
+	const unsigned grid = s.wide ? persistent_grid(s, k_segments<true, MODE>, block, lds, n)
+	                             : persistent_grid(s, k_segments<false, MODE>, block, lds, n);
+	/* with no more segments than resident wavefronts every segment starts
+	 * at once, and the order would change nothing */
+	if (order_scratch && !s.order && n > (int64_t)grid * per_block && segment_order_bytes(n)) {
+		hipError_t e = build_order<MODE>(from, to, n, vd, order_scratch, &s.order, st);
+		if (e != hipSuccess)
+			return e;
+	}
 	hipError_t e = zero_queue(s, st);
 	if (e != hipSuccess)
 		return e;
 	if (s.wide)
-		hipLaunchKernelGGL((k_segments<true, MODE>), dim3(persistent_grid(s, k_segments<true, MODE>, block, lds, n)),
-		                   dim3(block), lds, st, s, from, to, n, vd, levels, o);
+		hipLaunchKernelGGL((k_segments<true, MODE>), dim3(grid), dim3(block), lds, st, s, from, to, n, vd, levels, o);
 	else
-		hipLaunchKernelGGL((k_segments<false, MODE>), dim3(persistent_grid(s, k_segments<false, MODE>, block, lds, n)),
-		                   dim3(block), lds, st, s, from, to, n, vd, levels, o);
+		hipLaunchKernelGGL((k_segments<false, MODE>), dim3(grid), dim3(block), lds, st, s, from, to, n, vd, levels, o);
 	return hipGetLastError();
 }
 
 hipError_t launch_segments(const DevScene& s, int mode, const double* from, const double* to, int64_t n, double vd,
-                           int levels, const SegOut& o, hipStream_t st)
+                           int levels, const SegOut& o, hipStream_t st, void* order_scratch)
 {
 	if (n <= 0)
 		return hipSuccess;
 	switch (mode) {
-	case MODE_TRANSIT: return launch_seg_mode<MODE_TRANSIT>(s, from, to, n, vd, levels, o, st);
-	case MODE_TRANSITION: return launch_seg_mode<MODE_TRANSITION>(s, from, to, n, vd, levels, o, st);
-	case MODE_TOUCH: return launch_seg_mode<MODE_TOUCH>(s, from, to, n, vd, levels, o, st);
+	case MODE_TRANSIT: return launch_seg_mode<MODE_TRANSIT>(s, from, to, n, vd, levels, o, st, order_scratch);
+	case MODE_TRANSITION:
+		return launch_seg_mode<MODE_TRANSITION>(s, from, to, n, vd, levels, o, st, order_scratch);
+	case MODE_TOUCH: return launch_seg_mode<MODE_TOUCH>(s, from, to, n, vd, levels, o, st, order_scratch);
 	default: return hipErrorInvalidValue;
 	}
 }

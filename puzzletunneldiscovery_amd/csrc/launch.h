@@ -23,9 +23,14 @@ struct SegOut {
 };
 
 hipError_t launch_valid_states(const DevScene& s, const double* q, int64_t n, uint8_t* out, hipStream_t st);
-/* mode: 0 transit (fixed verify_delta), 1 transition (min(d/2, ivd)), 2 touch (transit + bisection) */
+/* mode: 0 transit (fixed verify_delta), 1 transition (min(d/2, ivd)), 2 touch (transit + bisection).
+ * With order_scratch (segment_order_bytes(n) bytes of device memory, or
+ * null), a launch of more segments than resident wavefronts first sorts
+ * them longest first into that scratch and walks them in that order. */
 hipError_t launch_segments(const DevScene& s, int mode, const double* from, const double* to, int64_t n,
-                           double vd, int levels, const SegOut& o, hipStream_t st);
+                           double vd, int levels, const SegOut& o, hipStream_t st, void* order_scratch = nullptr);
+/* Bytes of order scratch a launch of n segments uses (0: it walks in input order). */
+size_t segment_order_bytes(int64_t n);
 /* vis: 0 pair (row i vs row i), 1 symmetric matrix (fi < ti, mirrored), 2 full m x n matrix */
 hipError_t launch_visibility(const DevScene& s, int vis, const double* qs0, const double* qs1, int64_t m,
                              int64_t n, double ivd, int8_t* out8, uint32_t* bits, int32_t* steps, hipStream_t st);
@@ -77,6 +82,8 @@ hipError_t launch_disentangled(const BoxPair& b, const PertData* pert, const dou
  * device entry points in a HIP graph). */
 hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t st);
 hipError_t launch_fill_u8(uint8_t* p, int64_t n, uint8_t v, hipStream_t st);
+/* *out = *flag; *flag = 0 (one atomic exchange on the device). */
+hipError_t launch_take_u32(uint32_t* flag, uint32_t* out, hipStream_t st);
 
 } // namespace se3cq
 

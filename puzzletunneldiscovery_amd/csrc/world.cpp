@@ -27,6 +27,7 @@
 #include <cstdio>
 #include <cstring>
 #include <deque>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <unordered_map>
@@ -95,8 +96,53 @@ struct DevBuf {
 
 } // namespace
 
-/* Work counters of the persistent kernels per world. */
+/* Work counters of the persistent kernels per world: QUEUE_SLOTS words 64 B
+ * apart for eager launches, GRAPH_WORDS more for launches captured into HIP
+ * graphs (one each, never handed out again, since a graph may replay at
+ * any later time). */
 constexpr int QUEUE_SLOTS = 64;
+constexpr int GRAPH_WORDS = 4096;
+
+namespace {
+
+/* One eager work counter and what the world knows of its last launch: the
+ * stream and an event recorded after it.  A launch on another stream that
+ * takes the slot waits for that event first, so two launches never share a
+ * counter while both run.  `order` is the slot's segment-order scratch. */
+struct QueueSlot {
+	hipEvent_t ev = nullptr;
+	hipStream_t last = nullptr;
+	bool used = false;
+	DevBuf order;
+};
+
+/* What one blocking (_host) call works with: its own stream, staging
+ * buffers and error word, so calls from several threads run side by side. */
+struct HostCtx {
+	hipStream_t st = nullptr;
+	DevBuf ws[12];
+	DevBuf err;
+	~HostCtx()
+	{
+		for (auto& b : ws)
+			b.release();
+		err.release();
+		if (st)
+			(void)hipStreamDestroy(st);
+	}
+};
+
+/* A perturbation as the kernels read it.  Blocks are never overwritten:
+ * set_perturbation selects (or adds) the block of the new value, so a
+ * launch in flight, or a captured graph, keeps reading the value it was
+ * launched with. */
+struct PertBlock {
+	double q[7];
+	bool identity;
+	DevBuf buf;
+};
+
+} // namespace
 
 struct se3cq_world {
 	Mesh mesh[2];
@@ -109,20 +155,25 @@ struct se3cq_world {
 	/* device */
 	int device = -1;
 	bool uploaded = false;
-	hipStream_t stream = nullptr;
+	hipStream_t stream = nullptr; /* the world's own stream (tuning, error checks) */
 	DevBuf d_nodes[2], d_tris[2];
+	/* [0]: error bits of _device launches, kept until se3cq_check takes
+	 * them; [1]: where se3cq_check's kernel puts them */
 	DevBuf d_err;
-	DevBuf d_pert;
-	/* work counters of the persistent kernels: QUEUE_SLOTS words 64 B apart;
-	 * each launch takes the next (make_scene), so launches in flight on
-	 * different streams do not share one */
 	DevBuf d_queue;
-	std::atomic<uint32_t> queue_slot{0};
+	QueueSlot qslot[QUEUE_SLOTS];
+	uint32_t q_rr = 0;
+	DevBuf d_graph;
+	int graph_used = 0;
+	std::mutex qmu; /* slots, graph words and the current perturbation */
+	std::vector<std::unique_ptr<PertBlock>> perts;
+	const PertBlock* pert_cur = nullptr;
+	std::vector<std::unique_ptr<HostCtx>> hctx;
+	std::vector<HostCtx*> hfree;
+	std::mutex hmu;
 	int num_cu = 0;
 	se3cq_batch_config batch_cfg = {0, 0, 1, 1};
-	bool pert_uploaded = false;
-	DevBuf ws[12];
-	std::mutex mu;
+	std::mutex mu; /* set-up: geometry, upload, perturbation, configuration */
 
 	se3cq_world()
 	{
@@ -133,16 +184,28 @@ struct se3cq_world {
 	~se3cq_world() { release_device(); }
 	void release_device()
 	{
+		if (uploaded)
+			(void)hipDeviceSynchronize();
 		for (int g = 0; g < 2; g++) {
 			d_nodes[g].release();
 			d_tris[g].release();
 		}
 		d_err.release();
-		d_pert.release();
 		d_queue.release();
-		pert_uploaded = false;
-		for (auto& b : ws)
-			b.release();
+		d_graph.release();
+		graph_used = 0;
+		for (auto& q : qslot) {
+			if (q.ev)
+				(void)hipEventDestroy(q.ev);
+			q.ev = nullptr;
+			q.last = nullptr;
+			q.used = false;
+			q.order.release();
+		}
+		perts.clear();
+		pert_cur = nullptr;
+		hfree.clear();
+		hctx.clear();
 		if (stream)
 			(void)hipStreamDestroy(stream);
 		stream = nullptr;
@@ -586,7 +649,14 @@ void build_model(HostModel& m, const Mesh& me, const float* M)
 /* ---------------------------------------------------------------------- */
 /* Device scene                                                           */
 /* ---------------------------------------------------------------------- */
-DevScene make_scene(se3cq_world* w)
+bool pert_is_identity(const double* p)
+{
+	return p[0] == 0.0 && p[1] == 0.0 && p[2] == 0.0 && p[3] == 1.0 && p[4] == 0.0 && p[5] == 0.0 && p[6] == 0.0;
+}
+
+/* Everything of the scene but the launch's own work counter and error word
+ * (call with qmu held, so the perturbation is read once). */
+DevScene base_scene(const se3cq_world* w)
 {
 	DevScene s;
 	std::memset(&s, 0, sizeof(s));
@@ -598,16 +668,12 @@ DevScene make_scene(se3cq_world* w)
 		dm.ntris = (int32_t)(w->hm[g].tris.size() / 9);
 		dm.depth = w->hm[g].depth;
 	}
-	const double* p = w->pert;
-	s.pert = (const PertData*)w->d_pert.p;
-	s.pert_identity = (p[0] == 0.0 && p[1] == 0.0 && p[2] == 0.0 && p[3] == 1.0 && p[4] == 0.0 && p[5] == 0.0 &&
-	                   p[6] == 0.0) ? 1 : 0;
+	s.pert = (const PertData*)w->pert_cur->buf.p;
+	s.pert_identity = w->pert_cur->identity ? 1 : 0;
 	s.rob_reach = w->hm[1].reach;
 	s.scene_mag = w->hm[0].mag + w->hm[1].mag;
 	s.stack_depth = w->hm[0].depth + w->hm[1].depth + 2;
 	s.wide = (w->hm[0].nodes.size() > 65536 || w->hm[1].nodes.size() > 65536) ? 1 : 0;
-	s.error_flag = (uint32_t*)w->d_err.p;
-	s.queue = (unsigned long long*)((char*)w->d_queue.p + 64 * (w->queue_slot.fetch_add(1) % QUEUE_SLOTS));
 	s.num_cu = w->num_cu;
 	return s;
 }
@@ -637,12 +703,27 @@ PertData make_pert(const double* p)
 	return d;
 }
 
-int upload_pert(se3cq_world* w)
+/* Select the device block of w->pert, adding one if this value has none
+ * yet (call with mu held, after the upload). */
+int select_pert(se3cq_world* w)
 {
+	for (const auto& b : w->perts)
+		if (std::memcmp(b->q, w->pert, sizeof(b->q)) == 0) {
+			std::lock_guard<std::mutex> lk(w->qmu);
+			w->pert_cur = b.get();
+			return SE3CQ_OK;
+		}
+	std::unique_ptr<PertBlock> b(new (std::nothrow) PertBlock());
+	if (!b || !b->buf.reserve(sizeof(PertData)))
+		return fail(SE3CQ_ENOMEM, "device allocation failed");
+	std::memcpy(b->q, w->pert, sizeof(b->q));
+	b->identity = pert_is_identity(w->pert);
 	const PertData d = make_pert(w->pert);
-	if (hipMemcpy(w->d_pert.p, &d, sizeof(d), hipMemcpyHostToDevice) != hipSuccess)
+	if (hipMemcpy(b->buf.p, &d, sizeof(d), hipMemcpyHostToDevice) != hipSuccess)
 		return fail(SE3CQ_EDEVICE, "device upload failed");
-	w->pert_uploaded = true;
+	std::lock_guard<std::mutex> lk(w->qmu);
+	w->perts.push_back(std::move(b));
+	w->pert_cur = w->perts.back().get();
 	return SE3CQ_OK;
 }
 
@@ -661,15 +742,18 @@ int upload(se3cq_world* w)
 		    hipMemcpy(w->d_tris[g].p, m.tris.data(), tb, hipMemcpyHostToDevice) != hipSuccess)
 			return fail(SE3CQ_EDEVICE, "device upload failed");
 	}
-	if (!w->d_err.reserve(sizeof(uint32_t)) || hipMemset(w->d_err.p, 0, sizeof(uint32_t)) != hipSuccess)
+	if (!w->d_err.reserve(2 * sizeof(uint32_t)) || hipMemset(w->d_err.p, 0, 2 * sizeof(uint32_t)) != hipSuccess)
 		return fail(SE3CQ_EDEVICE, "device allocation failed");
-	if (!w->d_pert.reserve(sizeof(PertData)) || !w->d_queue.reserve(QUEUE_SLOTS * 64))
+	if (!w->d_queue.reserve(QUEUE_SLOTS * 64) || !w->d_graph.reserve((size_t)GRAPH_WORDS * 64))
 		return fail(SE3CQ_EDEVICE, "device allocation failed");
+	for (auto& q : w->qslot)
+		if (!q.ev && hipEventCreateWithFlags(&q.ev, hipEventDisableTiming) != hipSuccess)
+			return fail(SE3CQ_EDEVICE, "hipEventCreate failed");
 	if (hipDeviceGetAttribute(&w->num_cu, hipDeviceAttributeMultiprocessorCount, w->device) != hipSuccess ||
 	    w->num_cu <= 0)
 		return fail(SE3CQ_EDEVICE, "cannot read the device's CU count");
 	{
-		const int rc = upload_pert(w);
+		const int rc = select_pert(w);
 		if (rc)
 			return rc;
 	}
@@ -693,11 +777,14 @@ int ensure_device(se3cq_world* w)
 	return upload(w);
 }
 
-int check_device(se3cq_world* w)
+int setup_device(se3cq_world* w)
 {
-	uint32_t flag = 0;
-	if (hipMemcpy(&flag, w->d_err.p, sizeof(flag), hipMemcpyDeviceToHost) != hipSuccess)
-		return fail(SE3CQ_EDEVICE, "device error check failed");
+	std::lock_guard<std::mutex> lk(w->mu);
+	return ensure_device(w);
+}
+
+int error_bits(uint32_t flag)
+{
 	if (flag & 1u)
 		return fail(SE3CQ_EDEVICE, "traversal stack overflow on the device");
 	if (flag & 2u)
@@ -707,29 +794,175 @@ int check_device(se3cq_world* w)
 	return SE3CQ_OK;
 }
 
+/* Take (read and clear, atomically on the device) the error bits of the
+ * world's _device launches. */
+int take_device_errors(se3cq_world* w)
+{
+	uint32_t* e = (uint32_t*)w->d_err.p;
+	uint32_t flag = 0;
+	if (launch_take_u32(e, e + 1, w->stream) != hipSuccess ||
+	    hipMemcpyAsync(&flag, e + 1, sizeof(flag), hipMemcpyDeviceToHost, w->stream) != hipSuccess ||
+	    hipStreamSynchronize(w->stream) != hipSuccess)
+		return fail(SE3CQ_EDEVICE, "device error check failed");
+	return error_bits(flag);
+}
+
 hipStream_t pick_stream(se3cq_world* w, void* s) { return s ? (hipStream_t)s : w->stream; }
 
 /* A verify step that is not positive and finite never advances the walk
  * (the reference loops forever on it); such calls are rejected. */
 bool check_step(double vd) { return vd > 0.0 && std::isfinite(vd); }
 
-template <typename T>
-T* ws_get(se3cq_world* w, int slot, size_t count)
+/* The eager slot for a launch on st: the one st used last (stream order
+ * already separates its launches), else one never used or whose last
+ * launch has finished, else the next in turn, which st then waits for. */
+int pick_slot(se3cq_world* w, hipStream_t st)
 {
-	if (!w->ws[slot].reserve(std::max<size_t>(count, 1) * sizeof(T)))
-		return nullptr;
-	return (T*)w->ws[slot].p;
+	for (int k = 0; k < QUEUE_SLOTS; k++)
+		if (w->qslot[k].used && w->qslot[k].last == st)
+			return k;
+	for (int k = 0; k < QUEUE_SLOTS; k++)
+		if (!w->qslot[k].used)
+			return k;
+	for (int k = 0; k < QUEUE_SLOTS; k++)
+		if (hipEventQuery(w->qslot[k].ev) == hipSuccess)
+			return k;
+	return (int)(w->q_rr++ % QUEUE_SLOTS);
 }
 
-int sync_and_check(se3cq_world* w, hipStream_t st, hipError_t le)
+/*
+ * Enqueue one launch on st: fn(scene, order_scratch) with the scene's work
+ * counter and error word filled in.  An eager launch takes a slot (waiting
+ * on the slot's last launch if that ran on another stream) and, when
+ * order_bytes > 0, that slot's segment-order scratch.  A launch captured
+ * into a HIP graph gets a counter word of its own for the graph's lifetime
+ * and walks in input order (no scratch is allocated during a capture).
+ */
+template <class F>
+int with_launch(se3cq_world* w, hipStream_t st, uint32_t* err, size_t order_bytes, F&& fn)
 {
-	if (le != hipSuccess)
-		return fail(SE3CQ_EDEVICE, std::string("kernel launch failed: ") + hipGetErrorString(le));
-	const hipError_t e = hipStreamSynchronize(st);
+	hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+	if (hipStreamIsCapturing(st, &cs) != hipSuccess)
+		return fail(SE3CQ_EDEVICE, "hipStreamIsCapturing failed");
+	std::lock_guard<std::mutex> lk(w->qmu);
+	DevScene s = base_scene(w);
+	s.error_flag = err;
+	if (cs != hipStreamCaptureStatusNone) {
+		if (w->graph_used >= GRAPH_WORDS)
+			return fail(SE3CQ_ENOMEM, "too many launches captured on this world (" + std::to_string(GRAPH_WORDS) +
+			                              " work counters)");
+		s.queue = (unsigned long long*)((char*)w->d_graph.p + 64 * (size_t)w->graph_used++);
+		const hipError_t e = fn(s, (void*)nullptr);
+		if (e != hipSuccess)
+			return fail(SE3CQ_EDEVICE, std::string("launch failed: ") + hipGetErrorString(e));
+		return SE3CQ_OK;
+	}
+	const int k = pick_slot(w, st);
+	QueueSlot& q = w->qslot[k];
+	if (q.used && q.last != st && hipStreamWaitEvent(st, q.ev, 0) != hipSuccess)
+		return fail(SE3CQ_EDEVICE, "hipStreamWaitEvent failed");
+	void* ord = nullptr;
+	if (order_bytes > 0 && q.order.reserve(order_bytes))
+		ord = q.order.p;
+	s.queue = (unsigned long long*)((char*)w->d_queue.p + 64 * (size_t)k);
+	const hipError_t e = fn(s, ord);
 	if (e != hipSuccess)
-		return fail(SE3CQ_EDEVICE, std::string("device error: ") + hipGetErrorString(e));
-	return check_device(w);
+		return fail(SE3CQ_EDEVICE, std::string("launch failed: ") + hipGetErrorString(e));
+	if (hipEventRecord(q.ev, st) != hipSuccess)
+		return fail(SE3CQ_EDEVICE, "hipEventRecord failed");
+	q.last = st;
+	q.used = true;
+	return SE3CQ_OK;
 }
+
+/* A launch of a _device entry point: errors go to the world's word. */
+template <class F>
+int device_launch(se3cq_world* w, void* stream, size_t order_bytes, F&& fn)
+{
+	return with_launch(w, pick_stream(w, stream), (uint32_t*)w->d_err.p, order_bytes, std::forward<F>(fn));
+}
+
+/* A blocking call's context, taken from the world's pool for the call. */
+class HostCall {
+  public:
+	explicit HostCall(se3cq_world* w) : w_(w)
+	{
+		std::lock_guard<std::mutex> lk(w->hmu);
+		if (!w->hfree.empty()) {
+			c_ = w->hfree.back();
+			w->hfree.pop_back();
+			return;
+		}
+		std::unique_ptr<HostCtx> c(new (std::nothrow) HostCtx());
+		if (!c || hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess ||
+		    !c->err.reserve(sizeof(uint32_t)))
+			return;
+		c_ = c.get();
+		w->hctx.push_back(std::move(c));
+	}
+	~HostCall()
+	{
+		if (!c_)
+			return;
+		std::lock_guard<std::mutex> lk(w_->hmu);
+		w_->hfree.push_back(c_);
+	}
+	bool ok() const { return c_ != nullptr; }
+	hipStream_t stream() const { return c_->st; }
+	uint32_t* err() const { return (uint32_t*)c_->err.p; }
+	template <typename T>
+	T* buf(int slot, size_t count)
+	{
+		if (!c_->ws[slot].reserve(std::max<size_t>(count, 1) * sizeof(T)))
+			return nullptr;
+		return (T*)c_->ws[slot].p;
+	}
+	/* zero this call's error word (before its launches) */
+	int begin()
+	{
+		if (hipMemsetAsync(c_->err.p, 0, sizeof(uint32_t), c_->st) != hipSuccess)
+			return fail(SE3CQ_EDEVICE, "hipMemsetAsync failed");
+		return SE3CQ_OK;
+	}
+	template <class F>
+	int launch(size_t order_bytes, F&& fn)
+	{
+		return with_launch(w_, c_->st, (uint32_t*)c_->err.p, order_bytes, std::forward<F>(fn));
+	}
+	/* wait for the call's work and report its device-side errors */
+	int finish()
+	{
+		uint32_t flag = 0;
+		const hipError_t e = hipStreamSynchronize(c_->st);
+		if (e != hipSuccess)
+			return fail(SE3CQ_EDEVICE, std::string("device error: ") + hipGetErrorString(e));
+		if (hipMemcpy(&flag, c_->err.p, sizeof(flag), hipMemcpyDeviceToHost) != hipSuccess)
+			return fail(SE3CQ_EDEVICE, "device error check failed");
+		return error_bits(flag);
+	}
+	int h2d(void* dst, const void* src, size_t bytes)
+	{
+		if (bytes && hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c_->st) != hipSuccess)
+			return fail(SE3CQ_EDEVICE, "hipMemcpyAsync failed");
+		return SE3CQ_OK;
+	}
+	int d2h(void* dst, const void* src, size_t bytes)
+	{
+		if (bytes && hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c_->st) != hipSuccess)
+			return fail(SE3CQ_EDEVICE, "hipMemcpyAsync failed");
+		return SE3CQ_OK;
+	}
+	int sync()
+	{
+		if (hipStreamSynchronize(c_->st) != hipSuccess)
+			return fail(SE3CQ_EDEVICE, "hipStreamSynchronize failed");
+		return SE3CQ_OK;
+	}
+
+  private:
+	se3cq_world* w_;
+	HostCtx* c_ = nullptr;
+};
 
 #define CHECK_WORLD(w)                                                   \
 	do {                                                                 \
@@ -979,12 +1212,10 @@ int se3cq_set_perturbation(se3cq_world* w, const double* q7)
 	CHECK_WORLD(w);
 	std::lock_guard<std::mutex> lk(w->mu);
 	std::memcpy(w->pert, q7, 7 * sizeof(double));
-	if (w->uploaded) {
-		/* queries already issued on the world's stream see the old values */
-		if (hipStreamSynchronize(w->stream) != hipSuccess)
-			return fail(SE3CQ_EDEVICE, "stream synchronisation failed");
-		return upload_pert(w);
-	}
+	/* launches already issued, and graphs already captured, keep reading the
+	 * block of the value they were launched with */
+	if (w->uploaded)
+		return select_pert(w);
 	return SE3CQ_OK;
 }
 
@@ -1014,7 +1245,7 @@ int se3cq_check(se3cq_world* w)
 	CHECK_WORLD(w);
 	if (!w->ready() || !w->uploaded)
 		return SE3CQ_OK;
-	return check_device(w);
+	return take_device_errors(w);
 }
 
 /* ---- hot path ---- */
@@ -1023,22 +1254,17 @@ int se3cq_valid_states_device(se3cq_world* w, const double* d_q, int64_t n, uint
 	CHECK_WORLD(w);
 	if (n < 0)
 		return fail(SE3CQ_EINVAL, "negative count");
-	{
-		std::lock_guard<std::mutex> lk0(w->mu);
-		const int rc0 = ensure_device(w);
-		if (rc0 && w->ready())
-			return rc0;
-	}
-	hipStream_t st = pick_stream(w, stream);
 	if (!w->ready()) {
 		if (n > 0)
-			HIPCHK(launch_fill_u8(d_out, n, 1, st));
+			HIPCHK(launch_fill_u8(d_out, n, 1, pick_stream(w, stream)));
 		return SE3CQ_OK;
 	}
-	const hipError_t e = launch_valid_states(make_scene(w), d_q, n, d_out, st);
-	if (e != hipSuccess)
-		return fail(SE3CQ_EDEVICE, std::string("launch failed: ") + hipGetErrorString(e));
-	return SE3CQ_OK;
+	const int rc0 = setup_device(w);
+	if (rc0)
+		return rc0;
+	return device_launch(w, stream, 0, [&](const DevScene& s, void*) {
+		return launch_valid_states(s, d_q, n, d_out, pick_stream(w, stream));
+	});
 }
 
 int se3cq_valid_states_host(se3cq_world* w, const double* q, int64_t n, uint8_t* out)
@@ -1052,21 +1278,26 @@ int se3cq_valid_states_host(se3cq_world* w, const double* q, int64_t n, uint8_t*
 		std::memset(out, 1, (size_t)n);
 		return SE3CQ_OK;
 	}
-	std::lock_guard<std::mutex> lk(w->mu);
-	const int rc0 = ensure_device(w);
+	const int rc0 = setup_device(w);
 	if (rc0)
 		return rc0;
-	double* dq = ws_get<double>(w, 0, 7 * n);
-	uint8_t* dout = ws_get<uint8_t>(w, 1, n);
+	HostCall hc(w);
+	if (!hc.ok())
+		return fail(SE3CQ_EDEVICE, "cannot create a stream for the call");
+	double* dq = hc.buf<double>(0, 7 * n);
+	uint8_t* dout = hc.buf<uint8_t>(1, n);
 	if (!dq || !dout)
 		return fail(SE3CQ_ENOMEM, "device allocation failed");
-	HIPCHK(hipMemcpyAsync(dq, q, 7 * n * sizeof(double), hipMemcpyHostToDevice, w->stream));
-	const hipError_t le = launch_valid_states(make_scene(w), dq, n, dout, w->stream);
-	int rc = sync_and_check(w, w->stream, le);
-	if (rc)
-		return rc;
-	HIPCHK(hipMemcpy(out, dout, (size_t)n, hipMemcpyDeviceToHost));
-	return SE3CQ_OK;
+	int rc = hc.begin();
+	if (!rc)
+		rc = hc.h2d(dq, q, 7 * n * sizeof(double));
+	if (!rc)
+		rc = hc.launch(0, [&](const DevScene& s, void*) { return launch_valid_states(s, dq, n, dout, hc.stream()); });
+	if (!rc)
+		rc = hc.finish();
+	if (!rc)
+		rc = hc.d2h(out, dout, (size_t)n);
+	return rc ? rc : hc.sync();
 }
 
 int se3cq_contacts_host(se3cq_world* w, const double* q, int64_t n, int32_t cap, int32_t* pairs, double* segs,
@@ -1079,32 +1310,46 @@ int se3cq_contacts_host(se3cq_world* w, const double* q, int64_t n, int32_t cap,
 		return fail(SE3CQ_ESTATE, "models not loaded");
 	if (n == 0)
 		return SE3CQ_OK;
-	std::lock_guard<std::mutex> lk(w->mu);
-	const int rc0 = ensure_device(w);
+	const int rc0 = setup_device(w);
 	if (rc0)
 		return rc0;
+	HostCall hc(w);
+	if (!hc.ok())
+		return fail(SE3CQ_EDEVICE, "cannot create a stream for the call");
 	const size_t slots = (size_t)n * (size_t)(cap > 0 ? cap : 1);
-	double* dq = ws_get<double>(w, 0, 7 * n);
-	int32_t* dpairs = ws_get<int32_t>(w, 2, 2 * slots);
-	int32_t* dcount = ws_get<int32_t>(w, 3, (size_t)n);
-	double* dsegs = ws_get<double>(w, 4, 6 * slots);
-	int8_t* dflags = ws_get<int8_t>(w, 5, slots);
+	double* dq = hc.buf<double>(0, 7 * n);
+	int32_t* dpairs = hc.buf<int32_t>(2, 2 * slots);
+	int32_t* dcount = hc.buf<int32_t>(3, (size_t)n);
+	double* dsegs = hc.buf<double>(4, 6 * slots);
+	int8_t* dflags = hc.buf<int8_t>(5, slots);
 	if (!dq || !dpairs || !dcount || !dsegs || !dflags)
 		return fail(SE3CQ_ENOMEM, "device allocation failed");
-	HIPCHK(hipMemcpyAsync(dq, q, 7 * n * sizeof(double), hipMemcpyHostToDevice, w->stream));
-	const hipError_t le = launch_contacts(make_scene(w), dq, n, cap, dpairs, dcount, dsegs, dflags, w->stream);
-	int rc = sync_and_check(w, w->stream, le);
+	int rc = hc.begin();
+	if (!rc)
+		rc = hc.h2d(dq, q, 7 * n * sizeof(double));
+	if (!rc)
+		rc = hc.launch(0, [&](const DevScene& s, void*) {
+			return launch_contacts(s, dq, n, cap, dpairs, dcount, dsegs, dflags, hc.stream());
+		});
+	if (!rc)
+		rc = hc.finish();
 	if (rc)
 		return rc;
-	HIPCHK(hipMemcpy(counts, dcount, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost));
-	if (cap == 0)
-		return SE3CQ_OK;
-	std::vector<int32_t> hp(2 * slots);
-	std::vector<double> hs(6 * slots);
-	std::vector<int8_t> hf(slots);
-	HIPCHK(hipMemcpy(hp.data(), dpairs, hp.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
-	HIPCHK(hipMemcpy(hs.data(), dsegs, hs.size() * sizeof(double), hipMemcpyDeviceToHost));
-	HIPCHK(hipMemcpy(hf.data(), dflags, hf.size(), hipMemcpyDeviceToHost));
+	std::vector<int32_t> hp(cap > 0 ? 2 * slots : 0);
+	std::vector<double> hs(cap > 0 ? 6 * slots : 0);
+	std::vector<int8_t> hf(cap > 0 ? slots : 0);
+	rc = hc.d2h(counts, dcount, (size_t)n * sizeof(int32_t));
+	if (!rc && cap > 0) {
+		rc = hc.d2h(hp.data(), dpairs, hp.size() * sizeof(int32_t));
+		if (!rc)
+			rc = hc.d2h(hs.data(), dsegs, hs.size() * sizeof(double));
+		if (!rc)
+			rc = hc.d2h(hf.data(), dflags, hf.size());
+	}
+	if (!rc)
+		rc = hc.sync();
+	if (rc || cap == 0)
+		return rc;
 	/* report the listed pairs of each state in (env, robot) order */
 	std::vector<int> ord;
 	for (int64_t i = 0; i < n; i++) {
@@ -1147,17 +1392,12 @@ int se3cq_disentangled_device(se3cq_world* w, const double* d_q, int64_t n, uint
 		return fail(SE3CQ_EINVAL, "negative count");
 	if (!w->ready())
 		return fail(SE3CQ_ESTATE, "models not loaded");
-	{
-		std::lock_guard<std::mutex> lk0(w->mu);
-		const int rc0 = ensure_device(w);
-		if (rc0)
-			return rc0;
-	}
-	const hipError_t e =
-	    launch_disentangled(box_pair(w), (const PertData*)w->d_pert.p, d_q, n, d_out, pick_stream(w, stream));
-	if (e != hipSuccess)
-		return fail(SE3CQ_EDEVICE, std::string("launch failed: ") + hipGetErrorString(e));
-	return SE3CQ_OK;
+	const int rc0 = setup_device(w);
+	if (rc0)
+		return rc0;
+	return device_launch(w, stream, 0, [&](const DevScene& s, void*) {
+		return launch_disentangled(box_pair(w), s.pert, d_q, n, d_out, pick_stream(w, stream));
+	});
 }
 
 int se3cq_disentangled_host(se3cq_world* w, const double* q, int64_t n, uint8_t* out)
@@ -1169,30 +1409,100 @@ int se3cq_disentangled_host(se3cq_world* w, const double* q, int64_t n, uint8_t*
 		return fail(SE3CQ_ESTATE, "models not loaded");
 	if (n == 0)
 		return SE3CQ_OK;
-	std::lock_guard<std::mutex> lk(w->mu);
-	const int rc0 = ensure_device(w);
+	const int rc0 = setup_device(w);
 	if (rc0)
 		return rc0;
-	double* dq = ws_get<double>(w, 0, 7 * n);
-	uint8_t* dout = ws_get<uint8_t>(w, 1, n);
+	HostCall hc(w);
+	if (!hc.ok())
+		return fail(SE3CQ_EDEVICE, "cannot create a stream for the call");
+	double* dq = hc.buf<double>(0, 7 * n);
+	uint8_t* dout = hc.buf<uint8_t>(1, n);
 	if (!dq || !dout)
 		return fail(SE3CQ_ENOMEM, "device allocation failed");
-	HIPCHK(hipMemcpyAsync(dq, q, 7 * n * sizeof(double), hipMemcpyHostToDevice, w->stream));
-	const hipError_t le = launch_disentangled(box_pair(w), (const PertData*)w->d_pert.p, dq, n, dout, w->stream);
-	int rc = sync_and_check(w, w->stream, le);
-	if (rc)
-		return rc;
-	HIPCHK(hipMemcpy(out, dout, (size_t)n, hipMemcpyDeviceToHost));
-	return SE3CQ_OK;
+	int rc = hc.begin();
+	if (!rc)
+		rc = hc.h2d(dq, q, 7 * n * sizeof(double));
+	if (!rc)
+		rc = hc.launch(0, [&](const DevScene& s, void*) {
+			return launch_disentangled(box_pair(w), s.pert, dq, n, dout, hc.stream());
+		});
+	if (!rc)
+		rc = hc.finish();
+	if (!rc)
+		rc = hc.d2h(out, dout, (size_t)n);
+	return rc ? rc : hc.sync();
 }
 
 static int seg_device(se3cq_world* w, int mode, const double* d_from, const double* d_to, int64_t n, double vd,
-                      int levels, const SegOut& o, hipStream_t st)
+                      int levels, const SegOut& o, void* stream)
 {
-	const hipError_t e = launch_segments(make_scene(w), mode, d_from, d_to, n, vd, levels, o, st);
-	if (e != hipSuccess)
-		return fail(SE3CQ_EDEVICE, std::string("launch failed: ") + hipGetErrorString(e));
-	return SE3CQ_OK;
+	hipStream_t st = pick_stream(w, stream);
+	return device_launch(w, stream, segment_order_bytes(n), [&](const DevScene& s, void* ord) {
+		return launch_segments(s, mode, d_from, d_to, n, vd, levels, o, st, ord);
+	});
+}
+
+/* A blocking segment call: inputs in, one segment launch, outputs out. */
+struct SegHostOut {
+	double* last_free;
+	double* state;
+	uint8_t* done;
+	double* last_tau;
+	double* tau;
+	double* lo;
+	double* hi;
+	int32_t* steps;
+};
+
+static int seg_host(se3cq_world* w, int mode, const double* from, const double* to, int64_t n, double vd, int levels,
+                    const SegHostOut& h)
+{
+	HostCall hc(w);
+	if (!hc.ok())
+		return fail(SE3CQ_EDEVICE, "cannot create a stream for the call");
+	const size_t un = (size_t)n;
+	double* df = hc.buf<double>(0, 7 * un);
+	double* dt = hc.buf<double>(2, 7 * un);
+	double* dlf = h.last_free ? hc.buf<double>(3, 7 * un) : nullptr;
+	double* dst = h.last_free ? hc.buf<double>(4, 7 * un) : nullptr;
+	uint8_t* dd = hc.buf<uint8_t>(1, un);
+	double* dlt = h.last_tau ? hc.buf<double>(5, un) : nullptr;
+	double* dta = h.tau ? hc.buf<double>(6, un) : nullptr;
+	int32_t* dsp = h.steps ? hc.buf<int32_t>(7, un) : nullptr;
+	double* dlo = mode == 2 ? hc.buf<double>(8, un) : nullptr;
+	double* dhi = mode == 2 ? hc.buf<double>(9, un) : nullptr;
+	if (!df || !dt || !dd || (h.last_free && (!dlf || !dst)) || (h.last_tau && !dlt) || (h.tau && !dta) ||
+	    (h.steps && !dsp) || (mode == 2 && (!dlo || !dhi)))
+		return fail(SE3CQ_ENOMEM, "device allocation failed");
+	const SegOut o{dd, dlt, dta, dlf, dst, dlo, dhi, dsp};
+	int rc = hc.begin();
+	if (!rc)
+		rc = hc.h2d(df, from, 7 * un * sizeof(double));
+	if (!rc)
+		rc = hc.h2d(dt, to, 7 * un * sizeof(double));
+	if (!rc)
+		rc = hc.launch(segment_order_bytes(n), [&](const DevScene& s, void* ord) {
+			return launch_segments(s, mode, df, dt, n, vd, levels, o, hc.stream(), ord);
+		});
+	if (!rc)
+		rc = hc.finish();
+	if (!rc && h.done)
+		rc = hc.d2h(h.done, dd, un);
+	if (!rc && h.last_tau)
+		rc = hc.d2h(h.last_tau, dlt, un * sizeof(double));
+	if (!rc && h.tau)
+		rc = hc.d2h(h.tau, dta, un * sizeof(double));
+	if (!rc && h.lo)
+		rc = hc.d2h(h.lo, dlo, un * sizeof(double));
+	if (!rc && h.hi)
+		rc = hc.d2h(h.hi, dhi, un * sizeof(double));
+	if (!rc && h.steps)
+		rc = hc.d2h(h.steps, dsp, un * sizeof(int32_t));
+	if (!rc && h.last_free)
+		rc = hc.d2h(h.last_free, dlf, 7 * un * sizeof(double));
+	if (!rc && h.state)
+		rc = hc.d2h(h.state, dst, 7 * un * sizeof(double));
+	return rc ? rc : hc.sync();
 }
 
 int se3cq_transit_device(se3cq_world* w, const double* d_from, const double* d_to, int64_t n, double verify_delta,
@@ -1204,16 +1514,15 @@ int se3cq_transit_device(se3cq_world* w, const double* d_from, const double* d_t
 		return fail(SE3CQ_EINVAL, "the verify step must be positive and finite");
 	if (!w->ready())
 		return fail(SE3CQ_ESTATE, "models not loaded");
-	{
-		std::lock_guard<std::mutex> lk0(w->mu);
-		const int rc0 = ensure_device(w);
-		if (rc0)
-			return rc0;
-	}
 	if ((d_last_free == nullptr) != (d_state == nullptr))
 		return fail(SE3CQ_EINVAL, "last_free and state must both be given or both be null");
+	if (n < 0)
+		return fail(SE3CQ_EINVAL, "negative count");
+	const int rc0 = setup_device(w);
+	if (rc0)
+		return rc0;
 	SegOut o{d_done, d_last_tau, d_tau, d_last_free, d_state, nullptr, nullptr, d_steps};
-	return seg_device(w, 0, d_from, d_to, n, verify_delta, 0, o, pick_stream(w, stream));
+	return seg_device(w, 0, d_from, d_to, n, verify_delta, 0, o, stream);
 }
 
 int se3cq_transit_host(se3cq_world* w, const double* from, const double* to, int64_t n, double verify_delta,
@@ -1233,31 +1542,10 @@ int se3cq_transit_host(se3cq_world* w, const double* from, const double* to, int
 	}
 	if (n <= 0)
 		return n == 0 ? SE3CQ_OK : fail(SE3CQ_EINVAL, "negative count");
-	std::lock_guard<std::mutex> lk(w->mu);
-	double* df = ws_get<double>(w, 0, 7 * n);
-	double* dt = ws_get<double>(w, 2, 7 * n);
-	double* dlf = ws_get<double>(w, 3, 7 * n);
-	double* dst = ws_get<double>(w, 4, 7 * n);
-	uint8_t* dd = ws_get<uint8_t>(w, 1, n);
-	double* dlt = ws_get<double>(w, 5, n);
-	double* dta = ws_get<double>(w, 6, n);
-	int32_t* dsp = ws_get<int32_t>(w, 7, n);
-	if (!df || !dt || !dlf || !dst || !dd || !dlt || !dta || !dsp)
-		return fail(SE3CQ_ENOMEM, "device allocation failed");
-	HIPCHK(hipMemcpyAsync(df, from, 7 * n * sizeof(double), hipMemcpyHostToDevice, w->stream));
-	HIPCHK(hipMemcpyAsync(dt, to, 7 * n * sizeof(double), hipMemcpyHostToDevice, w->stream));
-	SegOut o{dd, dlt, dta, dlf, dst, nullptr, nullptr, dsp};
-	const hipError_t le = launch_segments(make_scene(w), 0, df, dt, n, verify_delta, 0, o, w->stream);
-	int rc = sync_and_check(w, w->stream, le);
-	if (rc)
-		return rc;
-	if (last_free) HIPCHK(hipMemcpy(last_free, dlf, 7 * n * sizeof(double), hipMemcpyDeviceToHost));
-	if (state) HIPCHK(hipMemcpy(state, dst, 7 * n * sizeof(double), hipMemcpyDeviceToHost));
-	if (done) HIPCHK(hipMemcpy(done, dd, n, hipMemcpyDeviceToHost));
-	if (last_tau) HIPCHK(hipMemcpy(last_tau, dlt, n * sizeof(double), hipMemcpyDeviceToHost));
-	if (tau) HIPCHK(hipMemcpy(tau, dta, n * sizeof(double), hipMemcpyDeviceToHost));
-	if (steps) HIPCHK(hipMemcpy(steps, dsp, n * sizeof(int32_t), hipMemcpyDeviceToHost));
-	return SE3CQ_OK;
+	if ((last_free == nullptr) != (state == nullptr))
+		return fail(SE3CQ_EINVAL, "last_free and state must both be given or both be null");
+	return seg_host(w, 0, from, to, n, verify_delta, 0,
+	                SegHostOut{last_free, state, done, last_tau, tau, nullptr, nullptr, steps});
 }
 
 int se3cq_valid_transitions_host(se3cq_world* w, const double* from, const double* to, int64_t n,
@@ -1276,21 +1564,8 @@ int se3cq_valid_transitions_host(se3cq_world* w, const double* from, const doubl
 	}
 	if (n <= 0)
 		return n == 0 ? SE3CQ_OK : fail(SE3CQ_EINVAL, "negative count");
-	std::lock_guard<std::mutex> lk(w->mu);
-	double* df = ws_get<double>(w, 0, 7 * n);
-	double* dt = ws_get<double>(w, 2, 7 * n);
-	uint8_t* dd = ws_get<uint8_t>(w, 1, n);
-	if (!df || !dt || !dd)
-		return fail(SE3CQ_ENOMEM, "device allocation failed");
-	HIPCHK(hipMemcpyAsync(df, from, 7 * n * sizeof(double), hipMemcpyHostToDevice, w->stream));
-	HIPCHK(hipMemcpyAsync(dt, to, 7 * n * sizeof(double), hipMemcpyHostToDevice, w->stream));
-	SegOut o{dd, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
-	const hipError_t le = launch_segments(make_scene(w), 1, df, dt, n, initial_verify_delta, 0, o, w->stream);
-	int rc = sync_and_check(w, w->stream, le);
-	if (rc)
-		return rc;
-	HIPCHK(hipMemcpy(out, dd, n, hipMemcpyDeviceToHost));
-	return SE3CQ_OK;
+	return seg_host(w, 1, from, to, n, initial_verify_delta, 0,
+	                SegHostOut{nullptr, nullptr, out, nullptr, nullptr, nullptr, nullptr, nullptr});
 }
 
 int se3cq_valid_transitions_device(se3cq_world* w, const double* d_from, const double* d_to, int64_t n,
@@ -1303,14 +1578,11 @@ int se3cq_valid_transitions_device(se3cq_world* w, const double* d_from, const d
 		return fail(SE3CQ_ESTATE, "models not loaded");
 	if (n < 0)
 		return fail(SE3CQ_EINVAL, "negative count");
-	{
-		std::lock_guard<std::mutex> lk0(w->mu);
-		const int rc0 = ensure_device(w);
-		if (rc0)
-			return rc0;
-	}
+	const int rc0 = setup_device(w);
+	if (rc0)
+		return rc0;
 	SegOut o{d_out, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, d_steps};
-	return seg_device(w, 1, d_from, d_to, n, initial_verify_delta, 0, o, pick_stream(w, stream));
+	return seg_device(w, 1, d_from, d_to, n, initial_verify_delta, 0, o, stream);
 }
 
 int se3cq_touch_search_device(se3cq_world* w, const double* d_from, const double* d_to, int64_t n,
@@ -1322,18 +1594,17 @@ int se3cq_touch_search_device(se3cq_world* w, const double* d_from, const double
 		return fail(SE3CQ_EINVAL, "the verify step must be positive and finite");
 	if (!w->ready())
 		return fail(SE3CQ_ESTATE, "models not loaded");
-	{
-		std::lock_guard<std::mutex> lk0(w->mu);
-		const int rc0 = ensure_device(w);
-		if (rc0)
-			return rc0;
-	}
 	if (levels < 0 || levels > 60)
 		return fail(SE3CQ_EINVAL, "levels must be in [0, 60]");
 	if (!d_lo || !d_hi)
 		return fail(SE3CQ_EINVAL, "lo and hi outputs are required");
+	if (n < 0)
+		return fail(SE3CQ_EINVAL, "negative count");
+	const int rc0 = setup_device(w);
+	if (rc0)
+		return rc0;
 	SegOut o{d_done, d_last_tau, d_tau, nullptr, nullptr, d_lo, d_hi, d_steps};
-	return seg_device(w, 2, d_from, d_to, n, verify_delta, levels, o, pick_stream(w, stream));
+	return seg_device(w, 2, d_from, d_to, n, verify_delta, levels, o, stream);
 }
 
 int se3cq_touch_search_host(se3cq_world* w, const double* from, const double* to, int64_t n, double verify_delta,
@@ -1355,31 +1626,8 @@ int se3cq_touch_search_host(se3cq_world* w, const double* from, const double* to
 		return fail(SE3CQ_EINVAL, "levels must be in [0, 60]");
 	if (n <= 0)
 		return n == 0 ? SE3CQ_OK : fail(SE3CQ_EINVAL, "negative count");
-	std::lock_guard<std::mutex> lk(w->mu);
-	double* df = ws_get<double>(w, 0, 7 * n);
-	double* dt = ws_get<double>(w, 2, 7 * n);
-	uint8_t* dd = ws_get<uint8_t>(w, 1, n);
-	double* dlt = ws_get<double>(w, 5, n);
-	double* dta = ws_get<double>(w, 6, n);
-	int32_t* dsp = ws_get<int32_t>(w, 7, n);
-	double* dlo = ws_get<double>(w, 8, n);
-	double* dhi = ws_get<double>(w, 9, n);
-	if (!df || !dt || !dd || !dlt || !dta || !dsp || !dlo || !dhi)
-		return fail(SE3CQ_ENOMEM, "device allocation failed");
-	HIPCHK(hipMemcpyAsync(df, from, 7 * n * sizeof(double), hipMemcpyHostToDevice, w->stream));
-	HIPCHK(hipMemcpyAsync(dt, to, 7 * n * sizeof(double), hipMemcpyHostToDevice, w->stream));
-	SegOut o{dd, dlt, dta, nullptr, nullptr, dlo, dhi, dsp};
-	const hipError_t le = launch_segments(make_scene(w), 2, df, dt, n, verify_delta, levels, o, w->stream);
-	int rc = sync_and_check(w, w->stream, le);
-	if (rc)
-		return rc;
-	if (done) HIPCHK(hipMemcpy(done, dd, n, hipMemcpyDeviceToHost));
-	if (last_tau) HIPCHK(hipMemcpy(last_tau, dlt, n * sizeof(double), hipMemcpyDeviceToHost));
-	if (tau) HIPCHK(hipMemcpy(tau, dta, n * sizeof(double), hipMemcpyDeviceToHost));
-	if (lo) HIPCHK(hipMemcpy(lo, dlo, n * sizeof(double), hipMemcpyDeviceToHost));
-	if (hi) HIPCHK(hipMemcpy(hi, dhi, n * sizeof(double), hipMemcpyDeviceToHost));
-	if (steps) HIPCHK(hipMemcpy(steps, dsp, n * sizeof(int32_t), hipMemcpyDeviceToHost));
-	return SE3CQ_OK;
+	return seg_host(w, 2, from, to, n, verify_delta, levels,
+	                SegHostOut{nullptr, nullptr, done, last_tau, tau, lo, hi, steps});
 }
 
 int se3cq_bisect_contact_host(se3cq_world* w, const double* from, const double* to, int64_t n, const double* lo_in,
@@ -1425,21 +1673,30 @@ int se3cq_bisect_contact_host(se3cq_world* w, const double* from, const double* 
 static int visibility_host(se3cq_world* w, int vis, const std::vector<double>& u0, const std::vector<double>& u1,
                            int64_t m, int64_t n, double vd, int8_t* out, size_t out_n)
 {
-	std::lock_guard<std::mutex> lk(w->mu);
-	double* d0 = ws_get<double>(w, 0, u0.size());
-	double* d1 = ws_get<double>(w, 2, u1.size());
-	int8_t* dout = ws_get<int8_t>(w, 1, out_n);
+	HostCall hc(w);
+	if (!hc.ok())
+		return fail(SE3CQ_EDEVICE, "cannot create a stream for the call");
+	double* d0 = hc.buf<double>(0, u0.size());
+	double* d1 = hc.buf<double>(2, u1.size());
+	int8_t* dout = hc.buf<int8_t>(1, out_n);
 	if (!d0 || !d1 || !dout)
 		return fail(SE3CQ_ENOMEM, "device allocation failed");
-	HIPCHK(hipMemcpyAsync(d0, u0.data(), u0.size() * sizeof(double), hipMemcpyHostToDevice, w->stream));
-	HIPCHK(hipMemcpyAsync(d1, u1.data(), u1.size() * sizeof(double), hipMemcpyHostToDevice, w->stream));
-	HIPCHK(hipMemsetAsync(dout, 0, out_n, w->stream));
-	const hipError_t le = launch_visibility(make_scene(w), vis, d0, d1, m, n, vd, dout, nullptr, nullptr, w->stream);
-	int rc = sync_and_check(w, w->stream, le);
-	if (rc)
-		return rc;
-	HIPCHK(hipMemcpy(out, dout, out_n, hipMemcpyDeviceToHost));
-	return SE3CQ_OK;
+	int rc = hc.begin();
+	if (!rc)
+		rc = hc.h2d(d0, u0.data(), u0.size() * sizeof(double));
+	if (!rc)
+		rc = hc.h2d(d1, u1.data(), u1.size() * sizeof(double));
+	if (!rc && launch_fill_u8((uint8_t*)dout, (int64_t)out_n, 0, hc.stream()) != hipSuccess)
+		rc = fail(SE3CQ_EDEVICE, "fill failed");
+	if (!rc)
+		rc = hc.launch(0, [&](const DevScene& s, void*) {
+			return launch_visibility(s, vis, d0, d1, m, n, vd, dout, nullptr, nullptr, hc.stream());
+		});
+	if (!rc)
+		rc = hc.finish();
+	if (!rc)
+		rc = hc.d2h(out, dout, out_n);
+	return rc ? rc : hc.sync();
 }
 
 int se3cq_visibility_matrix_host(se3cq_world* w, const double* qs, int64_t n, int qs_are_unit,
@@ -1526,11 +1783,9 @@ int se3cq_visibility_bits_device(se3cq_world* w, const double* d_qs, int64_t n, 
 	const size_t words = (size_t)((n * n + 31) / 32);
 	if (words)
 		HIPCHK(launch_fill_u32(d_bits, (int64_t)words, 0u, st));
-	const hipError_t e =
-	    launch_visibility(make_scene(w), 1, d_qs, d_qs, n, n, verify_magnitude, nullptr, d_bits, d_steps, st);
-	if (e != hipSuccess)
-		return fail(SE3CQ_EDEVICE, std::string("launch failed: ") + hipGetErrorString(e));
-	return SE3CQ_OK;
+	return device_launch(w, stream, 0, [&](const DevScene& s, void*) {
+		return launch_visibility(s, 1, d_qs, d_qs, n, n, verify_magnitude, nullptr, d_bits, d_steps, st);
+	});
 }
 
 /* ---- state arithmetic ---- */
@@ -1742,8 +1997,7 @@ static int batch_check(se3cq_world* w, int64_t n_seg, double vd, const double* d
 		return fail(SE3CQ_EINVAL, "missing output");
 	if (!w->ready())
 		return fail(SE3CQ_ESTATE, "models not loaded");
-	std::lock_guard<std::mutex> lk0(w->mu);
-	return ensure_device(w);
+	return setup_device(w);
 }
 
 static BatchArgs batch_args(se3cq_world* w, const double* d_seg_from, const double* d_seg_to, int64_t n_seg,
@@ -1764,25 +2018,24 @@ static BatchArgs batch_args(se3cq_world* w, const double* d_seg_from, const doub
 	return a;
 }
 
-static int batch_launch(se3cq_world* w, const BatchArgs& a, const se3cq_batch_config& c, hipStream_t st)
+/* One batch on st with error word err (the world's for _device calls). */
+static int batch_launch(se3cq_world* w, const BatchArgs& a, const se3cq_batch_config& c, hipStream_t st,
+                        uint32_t* err)
 {
-	const DevScene s = make_scene(w);
-	if (a.n_states == 0 || c.split) {
-		/* the persistent segment kernel (no traversal stacks in LDS), then
-		 * the state kernel */
-		hipError_t e = launch_segments(s, 0, a.seg_from, a.seg_to, a.n_seg, a.vd, 0, a.seg, st);
-		if (e == hipSuccess && a.n_states > 0)
-			e = launch_valid_states(s, a.states, a.n_states, a.valid, st);
-		if (e != hipSuccess)
-			return fail(SE3CQ_EDEVICE, std::string("launch failed: ") + hipGetErrorString(e));
-		return SE3CQ_OK;
-	}
-	int block = 0, grid = 0;
-	batch_shape(w, s, c, a.n_states > 0, &block, &grid);
-	const hipError_t e = launch_batch(s, a, block, grid, st);
-	if (e != hipSuccess)
-		return fail(SE3CQ_EDEVICE, std::string("launch failed: ") + hipGetErrorString(e));
-	return SE3CQ_OK;
+	const bool split = a.n_states == 0 || c.split;
+	return with_launch(w, st, err, split ? segment_order_bytes(a.n_seg) : 0, [&](const DevScene& s, void* ord) {
+		if (split) {
+			/* the persistent segment kernel (no traversal stacks in LDS),
+			 * then the state kernel */
+			hipError_t e = launch_segments(s, 0, a.seg_from, a.seg_to, a.n_seg, a.vd, 0, a.seg, st, ord);
+			if (e == hipSuccess && a.n_states > 0)
+				e = launch_valid_states(s, a.states, a.n_states, a.valid, st);
+			return e;
+		}
+		int block = 0, grid = 0;
+		batch_shape(w, s, c, a.n_states > 0, &block, &grid);
+		return launch_batch(s, a, block, grid, st);
+	});
 }
 
 int se3cq_batch_device(se3cq_world* w, const double* d_seg_from, const double* d_seg_to, int64_t n_seg,
@@ -1796,7 +2049,7 @@ int se3cq_batch_device(se3cq_world* w, const double* d_seg_from, const double* d
 		return rc;
 	const BatchArgs a = batch_args(w, d_seg_from, d_seg_to, n_seg, verify_delta, d_last_free, d_state, d_done,
 	                               d_last_tau, d_tau, d_steps, d_states, n_states, d_valid, w->batch_cfg.states_per_lane);
-	return batch_launch(w, a, w->batch_cfg, pick_stream(w, stream));
+	return batch_launch(w, a, w->batch_cfg, pick_stream(w, stream), (uint32_t*)w->d_err.p);
 }
 
 int se3cq_batch_host(se3cq_world* w, const double* seg_from, const double* seg_to, int64_t n_seg, double verify_delta,
@@ -1809,46 +2062,56 @@ int se3cq_batch_host(se3cq_world* w, const double* seg_from, const double* seg_t
 	const int rc0 = batch_check(w, n_seg, verify_delta, last_free, state, done, n_states, valid);
 	if (rc0 || (n_seg == 0 && n_states == 0))
 		return rc0;
-	std::lock_guard<std::mutex> lk(w->mu);
+	HostCall hc(w);
+	if (!hc.ok())
+		return fail(SE3CQ_EDEVICE, "cannot create a stream for the call");
 	const size_t ns = (size_t)n_seg, nq = (size_t)n_states;
-	double* df = ws_get<double>(w, 0, 7 * ns);
-	double* dt = ws_get<double>(w, 2, 7 * ns);
-	double* dlf = last_free ? ws_get<double>(w, 3, 7 * ns) : nullptr;
-	double* dst = last_free ? ws_get<double>(w, 4, 7 * ns) : nullptr;
-	uint8_t* dd = ws_get<uint8_t>(w, 1, ns);
-	double* dlt = ws_get<double>(w, 5, ns);
-	double* dta = ws_get<double>(w, 6, ns);
-	int32_t* dsp = ws_get<int32_t>(w, 7, ns);
-	double* dq = ws_get<double>(w, 8, 7 * nq);
-	uint8_t* dv = ws_get<uint8_t>(w, 9, nq);
+	double* df = hc.buf<double>(0, 7 * ns);
+	double* dt = hc.buf<double>(2, 7 * ns);
+	double* dlf = last_free ? hc.buf<double>(3, 7 * ns) : nullptr;
+	double* dst = last_free ? hc.buf<double>(4, 7 * ns) : nullptr;
+	uint8_t* dd = hc.buf<uint8_t>(1, ns);
+	double* dlt = hc.buf<double>(5, ns);
+	double* dta = hc.buf<double>(6, ns);
+	int32_t* dsp = hc.buf<int32_t>(7, ns);
+	double* dq = hc.buf<double>(8, 7 * nq);
+	uint8_t* dv = hc.buf<uint8_t>(9, nq);
 	if (!df || !dt || (last_free && (!dlf || !dst)) || !dd || !dlt || !dta || !dsp || !dq || !dv)
 		return fail(SE3CQ_ENOMEM, "device allocation failed");
-	if (ns) {
-		HIPCHK(hipMemcpyAsync(df, seg_from, 7 * ns * sizeof(double), hipMemcpyHostToDevice, w->stream));
-		HIPCHK(hipMemcpyAsync(dt, seg_to, 7 * ns * sizeof(double), hipMemcpyHostToDevice, w->stream));
+	int rc = hc.begin();
+	if (!rc)
+		rc = hc.h2d(df, seg_from, 7 * ns * sizeof(double));
+	if (!rc)
+		rc = hc.h2d(dt, seg_to, 7 * ns * sizeof(double));
+	if (!rc)
+		rc = hc.h2d(dq, states, 7 * nq * sizeof(double));
+	se3cq_batch_config cfg;
+	{
+		std::lock_guard<std::mutex> lk(w->mu);
+		cfg = w->batch_cfg;
 	}
-	if (nq)
-		HIPCHK(hipMemcpyAsync(dq, states, 7 * nq * sizeof(double), hipMemcpyHostToDevice, w->stream));
 	const BatchArgs a = batch_args(w, df, dt, n_seg, verify_delta, dlf, dst, dd, dlt, dta, dsp, dq, n_states, dv,
-	                               w->batch_cfg.states_per_lane);
-	int rc = batch_launch(w, a, w->batch_cfg, w->stream);
-	if (rc == SE3CQ_OK)
-		rc = sync_and_check(w, w->stream, hipSuccess);
-	if (rc)
-		return rc;
-	if (ns) {
-		HIPCHK(hipMemcpy(done, dd, ns, hipMemcpyDeviceToHost));
-		if (last_tau) HIPCHK(hipMemcpy(last_tau, dlt, ns * sizeof(double), hipMemcpyDeviceToHost));
-		if (tau) HIPCHK(hipMemcpy(tau, dta, ns * sizeof(double), hipMemcpyDeviceToHost));
-		if (steps) HIPCHK(hipMemcpy(steps, dsp, ns * sizeof(int32_t), hipMemcpyDeviceToHost));
-		if (last_free) {
-			HIPCHK(hipMemcpy(last_free, dlf, 7 * ns * sizeof(double), hipMemcpyDeviceToHost));
-			HIPCHK(hipMemcpy(state, dst, 7 * ns * sizeof(double), hipMemcpyDeviceToHost));
-		}
+	                               cfg.states_per_lane);
+	if (!rc)
+		rc = batch_launch(w, a, cfg, hc.stream(), hc.err());
+	if (!rc)
+		rc = hc.finish();
+	if (!rc && ns) {
+		rc = hc.d2h(done, dd, ns);
+		if (!rc && last_tau)
+			rc = hc.d2h(last_tau, dlt, ns * sizeof(double));
+		if (!rc && tau)
+			rc = hc.d2h(tau, dta, ns * sizeof(double));
+		if (!rc && steps)
+			rc = hc.d2h(steps, dsp, ns * sizeof(int32_t));
+		if (!rc && last_free)
+			rc = hc.d2h(last_free, dlf, 7 * ns * sizeof(double));
+		if (!rc && last_free)
+			rc = hc.d2h(state, dst, 7 * ns * sizeof(double));
 	}
-	if (nq)
-		HIPCHK(hipMemcpy(valid, dv, nq, hipMemcpyDeviceToHost));
-	return SE3CQ_OK;
+	if (!rc && nq)
+		rc = hc.d2h(valid, dv, nq);
+	return rc ? rc : hc.sync();
 }
 
 int se3cq_batch_get_config(const se3cq_world* w, se3cq_batch_config* cfg)
@@ -1890,7 +2153,11 @@ int se3cq_batch_tune(se3cq_world* w, const double* d_seg_from, const double* d_s
 		return fail(SE3CQ_EINVAL, "repeats must be in [1, 100]");
 	if (n_states == 0)
 		return fail(SE3CQ_EINVAL, "a batch without states has one launch shape (the segment kernel's); nothing to tune");
-	const DevScene s = make_scene(w);
+	DevScene s;
+	{
+		std::lock_guard<std::mutex> lk(w->qmu);
+		s = base_scene(w);
+	}
 	const bool states = n_states > 0;
 	const int bmax = batch_default_block(s, states) / 64;
 	std::vector<se3cq_batch_config> cands;
@@ -1919,7 +2186,7 @@ int se3cq_batch_tune(se3cq_world* w, const double* d_seg_from, const double* d_s
 			break;
 		}
 		for (int r = 0; r < (i == 0 ? 1 : repeats) && rc == SE3CQ_OK; r++)
-			rc = batch_launch(w, a, c, st);
+			rc = batch_launch(w, a, c, st, (uint32_t*)w->d_err.p);
 		if (rc)
 			break;
 		float t = 0.0f;
@@ -1937,7 +2204,7 @@ int se3cq_batch_tune(se3cq_world* w, const double* d_seg_from, const double* d_s
 	(void)hipEventDestroy(e1);
 	if (rc)
 		return rc;
-	rc = check_device(w);
+	rc = take_device_errors(w);
 	if (rc)
 		return rc;
 	/* leave the outputs as the chosen shape computes them (identical) */
@@ -1982,8 +2249,20 @@ static int ompl_prepare(se3cq_world* w)
 {
 	if (!w->ready())
 		return fail(SE3CQ_ESTATE, "models not loaded");
-	std::lock_guard<std::mutex> lk0(w->mu);
-	return ensure_device(w);
+	return setup_device(w);
+}
+
+/* The OMPL-to-unit transform with the perturbation of the launch's scene
+ * (the block the scene reads), so both come from one value. */
+static OmplXf ompl_xf_of(const se3cq_world* w, const DevScene& s)
+{
+	OmplXf x;
+	ompl_xf(w, &x);
+	for (const auto& b : w->perts)
+		if ((const PertData*)b->buf.p == s.pert)
+			for (int k = 0; k < 7; k++)
+				x.pert[k] = b->q[k];
+	return x;
 }
 
 int se3cq_ompl_validate_states_device(se3cq_world* w, const double* d_q, int64_t n, int32_t* d_out, void* stream)
@@ -1994,12 +2273,9 @@ int se3cq_ompl_validate_states_device(se3cq_world* w, const double* d_q, int64_t
 	const int rc0 = ompl_prepare(w);
 	if (rc0)
 		return rc0;
-	OmplXf x;
-	ompl_xf(w, &x);
-	const hipError_t e = launch_ompl_states(make_scene(w), x, d_q, n, d_out, pick_stream(w, stream));
-	if (e != hipSuccess)
-		return fail(SE3CQ_EDEVICE, std::string("launch failed: ") + hipGetErrorString(e));
-	return SE3CQ_OK;
+	return device_launch(w, stream, 0, [&](const DevScene& s, void*) {
+		return launch_ompl_states(s, ompl_xf_of(w, s), d_q, n, d_out, pick_stream(w, stream));
+	});
 }
 
 int se3cq_ompl_validate_states_host(se3cq_world* w, const double* q, int64_t n, int32_t* out)
@@ -2010,19 +2286,25 @@ int se3cq_ompl_validate_states_host(se3cq_world* w, const double* q, int64_t n, 
 	const int rc0 = ompl_prepare(w);
 	if (rc0 || n == 0)
 		return rc0;
-	std::lock_guard<std::mutex> lk(w->mu);
-	double* dq = ws_get<double>(w, 0, 7 * n);
-	int32_t* dout = ws_get<int32_t>(w, 3, n);
+	HostCall hc(w);
+	if (!hc.ok())
+		return fail(SE3CQ_EDEVICE, "cannot create a stream for the call");
+	double* dq = hc.buf<double>(0, 7 * n);
+	int32_t* dout = hc.buf<int32_t>(3, n);
 	if (!dq || !dout)
 		return fail(SE3CQ_ENOMEM, "device allocation failed");
-	OmplXf x;
-	ompl_xf(w, &x);
-	HIPCHK(hipMemcpyAsync(dq, q, 7 * n * sizeof(double), hipMemcpyHostToDevice, w->stream));
-	const int rc = sync_and_check(w, w->stream, launch_ompl_states(make_scene(w), x, dq, n, dout, w->stream));
-	if (rc)
-		return rc;
-	HIPCHK(hipMemcpy(out, dout, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost));
-	return SE3CQ_OK;
+	int rc = hc.begin();
+	if (!rc)
+		rc = hc.h2d(dq, q, 7 * n * sizeof(double));
+	if (!rc)
+		rc = hc.launch(0, [&](const DevScene& s, void*) {
+			return launch_ompl_states(s, ompl_xf_of(w, s), dq, n, dout, hc.stream());
+		});
+	if (!rc)
+		rc = hc.finish();
+	if (!rc)
+		rc = hc.d2h(out, dout, (size_t)n * sizeof(int32_t));
+	return rc ? rc : hc.sync();
 }
 
 int se3cq_ompl_validate_motions_device(se3cq_world* w, const double* d_q0, const double* d_q1, int64_t n, double lvs,
@@ -2034,13 +2316,9 @@ int se3cq_ompl_validate_motions_device(se3cq_world* w, const double* d_q0, const
 	const int rc0 = ompl_prepare(w);
 	if (rc0)
 		return rc0;
-	OmplXf x;
-	ompl_xf(w, &x);
-	const hipError_t e =
-	    launch_ompl_motions(make_scene(w), x, d_q0, d_q1, n, lvs, d_out, d_samples, pick_stream(w, stream));
-	if (e != hipSuccess)
-		return fail(SE3CQ_EDEVICE, std::string("launch failed: ") + hipGetErrorString(e));
-	return SE3CQ_OK;
+	return device_launch(w, stream, 0, [&](const DevScene& s, void*) {
+		return launch_ompl_motions(s, ompl_xf_of(w, s), d_q0, d_q1, n, lvs, d_out, d_samples, pick_stream(w, stream));
+	});
 }
 
 int se3cq_ompl_validate_motions_host(se3cq_world* w, const double* q0, const double* q1, int64_t n, double lvs,
@@ -2052,24 +2330,34 @@ int se3cq_ompl_validate_motions_host(se3cq_world* w, const double* q0, const dou
 	const int rc0 = ompl_prepare(w);
 	if (rc0 || n == 0)
 		return rc0;
-	std::lock_guard<std::mutex> lk(w->mu);
-	double* d0 = ws_get<double>(w, 0, 7 * n);
-	double* d1 = ws_get<double>(w, 6, 7 * n);
-	int32_t* dout = ws_get<int32_t>(w, 3, n);
-	int32_t* dsam = ws_get<int32_t>(w, 2, n);
+	HostCall hc(w);
+	if (!hc.ok())
+		return fail(SE3CQ_EDEVICE, "cannot create a stream for the call");
+	double* d0 = hc.buf<double>(0, 7 * n);
+	double* d1 = hc.buf<double>(6, 7 * n);
+	int32_t* dout = hc.buf<int32_t>(3, n);
+	int32_t* dsam = hc.buf<int32_t>(2, n);
 	if (!d0 || !d1 || !dout || !dsam)
 		return fail(SE3CQ_ENOMEM, "device allocation failed");
-	OmplXf x;
-	ompl_xf(w, &x);
-	HIPCHK(hipMemcpyAsync(d0, q0, 7 * n * sizeof(double), hipMemcpyHostToDevice, w->stream));
-	HIPCHK(hipMemcpyAsync(d1, q1, 7 * n * sizeof(double), hipMemcpyHostToDevice, w->stream));
-	const int rc =
-	    sync_and_check(w, w->stream, launch_ompl_motions(make_scene(w), x, d0, d1, n, lvs, dout, dsam, w->stream));
+	int rc = hc.begin();
+	if (!rc)
+		rc = hc.h2d(d0, q0, 7 * n * sizeof(double));
+	if (!rc)
+		rc = hc.h2d(d1, q1, 7 * n * sizeof(double));
+	if (!rc)
+		rc = hc.launch(0, [&](const DevScene& s, void*) {
+			return launch_ompl_motions(s, ompl_xf_of(w, s), d0, d1, n, lvs, dout, dsam, hc.stream());
+		});
+	if (!rc)
+		rc = hc.finish();
+	if (!rc)
+		rc = hc.d2h(out, dout, (size_t)n * sizeof(int32_t));
+	if (!rc && samples)
+		rc = hc.d2h(samples, dsam, (size_t)n * sizeof(int32_t));
+	if (!rc)
+		rc = hc.sync();
 	if (rc)
 		return rc;
-	HIPCHK(hipMemcpy(out, dout, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost));
-	if (samples)
-		HIPCHK(hipMemcpy(samples, dsam, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost));
 	for (int64_t i = 0; i < n; i++)
 		if (out[i] < 0)
 			return fail(SE3CQ_EINVAL, "motion " + std::to_string(i) +
@@ -2136,6 +2424,41 @@ int se3cq_decompose_2(const double* q7, double* t3, double* R9)
 	se3_quat_to_rotmat(q[0], q[1], q[2], q[3], R9);
 	for (int k = 0; k < 3; k++)
 		t3[k] = q7[k];
+	return SE3CQ_OK;
+}
+
+int se3cq_decompose_3(const double* q7, double* t3, double* angle, double* axis3)
+{
+	/* decompose (osr_state.cc:85-92) normalises the quaternion; Eigen 3.3's
+	 * AngleAxis from a quaternion: n = |vec| (the scaled norm when n is below
+	 * the binary64 epsilon), angle = 2 atan2(n, |w|), axis = vec / n with
+	 * n negated when w < 0; the identity gives angle 0 and axis (1, 0, 0). */
+	if (!q7 || !t3 || !angle || !axis3)
+		return fail(SE3CQ_EINVAL, "bad arguments");
+	double q[4] = {q7[3], q7[4], q7[5], q7[6]};
+	se3_quat_normalize(q);
+	for (int k = 0; k < 3; k++)
+		t3[k] = q7[k];
+	double n = std::sqrt((q[1] * q[1] + q[2] * q[2]) + q[3] * q[3]);
+	if (n < 0x1p-52) {
+		const double sc = std::max(std::fabs(q[1]), std::max(std::fabs(q[2]), std::fabs(q[3])));
+		if (sc > 0.0) {
+			const double a = q[1] / sc, b = q[2] / sc, c = q[3] / sc;
+			n = sc * std::sqrt((a * a + b * b) + c * c);
+		}
+	}
+	if (n != 0.0) {
+		*angle = 2.0 * se3_atan2(n, std::fabs(q[0]));
+		if (q[0] < 0.0)
+			n = -n;
+		for (int k = 0; k < 3; k++)
+			axis3[k] = q[1 + k] / n;
+	} else {
+		*angle = 0.0;
+		axis3[0] = 1.0;
+		axis3[1] = 0.0;
+		axis3[2] = 0.0;
+	}
 	return SE3CQ_OK;
 }
 

@@ -107,6 +107,17 @@ def decompose_2(state):
     return t, R
 
 
+def decompose_3(state):
+    """decompose_3 (osr_state.cc:101-107): (translation, angle, axis) of the
+    normalised state rotation."""
+    q = _state(state)
+    t = np.zeros(3)
+    a = np.zeros(1)
+    axis = np.zeros(3)
+    N.check(N.lib().se3cq_decompose_3(N.dptr(q), N.dptr(t), N.dptr(a), N.dptr(axis)))
+    return t, float(a[0]), axis
+
+
 def compose_from_angleaxis(tr, angle, axis):
     """compose_from_angleaxis (osr_state.cc:121-128)."""
     t, a = _vec3(tr), _vec3(axis)

@@ -155,6 +155,40 @@ def test_compose_from_angleaxis_closed_form():
     assert np.array_equal(pyosr.get_identity_state_vector(), [0, 0, 0, 1, 0, 0, 0])
 
 
+def test_decompose_3_closed_forms():
+    """decompose_3 (osr_state.cc:101-107): the state built from (angle, axis)
+    with angle in [0, pi] gives them back (within a few ulp: the angle goes
+    through atan2 and a square root); a quaternion with w < 0 is the same
+    rotation with axis and vector part negated; the identity and a rotation by
+    pi keep Eigen's conventions; the translation passes unchanged and the
+    quaternion is normalised first."""
+    rng = np.random.default_rng(12)
+    for _ in range(300):
+        ang = float(rng.uniform(1e-3, np.pi - 1e-3))
+        ax = rng.normal(size=3)
+        ax /= np.linalg.norm(ax)
+        tr = rng.normal(size=3)
+        q = np.concatenate([tr, [np.cos(ang / 2)], np.sin(ang / 2) * ax])
+        t, a, u = pyosr.decompose_3(q)
+        assert np.array_equal(t, tr)
+        assert abs(a - ang) <= 8 * np.spacing(np.pi) and np.allclose(u, ax, atol=1e-14)
+        qn = q.copy()
+        qn[3:] *= -float(rng.uniform(0.5, 2.0))      # same rotation, w < 0, not unit
+        t2, a2, u2 = pyosr.decompose_3(qn)
+        assert abs(a2 - ang) <= 8 * np.spacing(np.pi) and np.allclose(u2, ax, atol=1e-14)
+        assert np.linalg.norm(u2) == pytest.approx(1.0, abs=4e-16)
+    t, a, u = pyosr.decompose_3([1, 2, 3, 1, 0, 0, 0])
+    assert a == 0.0 and np.array_equal(u, [1.0, 0.0, 0.0])
+    t, a, u = pyosr.decompose_3([0, 0, 0, -2.0, 0, 0, 0])
+    assert a == 0.0 and np.array_equal(u, [1.0, 0.0, 0.0])
+    t, a, u = pyosr.decompose_3([0, 0, 0, 0.0, 0.0, 0.0, 1.0])   # rotation by pi about z
+    assert a == 2 * pyosr.distance([0] * 3 + [1, 0, 0, 0], [0] * 3 + [0, 0, 0, 1]) / 2
+    assert abs(a - np.pi) <= np.spacing(np.pi) and np.array_equal(u, [0.0, 0.0, 1.0])
+    # a tiny rotation: the axis still comes out of the vector part, scaled
+    t, a, u = pyosr.decompose_3([0, 0, 0, 1.0, 3e-17, 4e-17, 0.0])
+    assert a == pytest.approx(1e-16, rel=1e-12) and np.allclose(u, [0.6, 0.8, 0.0], atol=1e-15)
+
+
 def test_errors_are_reported_not_thrown():
     L = N.lib()
     h = ctypes.c_void_p()
