@@ -120,6 +120,23 @@ int se3cq_touch_search_device(se3cq_world* w, const double* d_from, const double
                               double verify_delta, int levels, uint8_t* d_done, double* d_last_tau,
                               double* d_tau, double* d_lo, double* d_hi, int32_t* d_steps, void* stream);
 
+/* se3cq_touch_search_device without any allocation, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  The
+ * launch's scratch (its work counter and the longest-first order of its
+ * segments) lives in one static array per device in the library's code
+ * object (5 MiB), not in memory allocated for the call; n may be at most
+ * SE3CQ_NOALLOC_MAX_SEGMENTS (larger n fails with SE3CQ_EINVAL).  The world
+ * must already be on the current device (any earlier query on it uploads
+ * it); otherwise the call fails with SE3CQ_ESTATE.  Device errors go to the
+ * world's error word, as for the other _device calls (se3cq_check).  Every
+ * call on one device shares that array: calls that overlap on one device
+ * (other streams, other threads) race and give wrong results. */
+#define SE3CQ_NOALLOC_MAX_SEGMENTS 1048576
+int se3cq_touch_search_device_noalloc(se3cq_world* w, const double* d_from, const double* d_to, int64_t n,
+                                      double verify_delta, int levels, uint8_t* d_done, double* d_last_tau,
+                                      double* d_tau, double* d_lo, double* d_hi, int32_t* d_steps, void* stream);
+
 /* Bisection alone on given brackets (one per row). */
 int se3cq_bisect_contact_host(se3cq_world* w, const double* from, const double* to, int64_t n,
                               const double* lo_in, const double* hi_in, int levels, double* lo, double* hi);

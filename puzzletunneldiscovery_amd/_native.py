@@ -62,6 +62,7 @@ SIGNATURES = {
     "se3cq_touch_search_host": [vp, c_double_p, c_double_p, i64, cdbl, cint, c_u8_p, c_double_p,
                                 c_double_p, c_double_p, c_double_p, c_i32_p],
     "se3cq_touch_search_device": [vp, vp, vp, i64, cdbl, cint, vp, vp, vp, vp, vp, vp, vp],
+    "se3cq_touch_search_device_noalloc": [vp, vp, vp, i64, cdbl, cint, vp, vp, vp, vp, vp, vp, vp],
     "se3cq_bisect_contact_host": [vp, c_double_p, c_double_p, i64, c_double_p, c_double_p, cint,
                                   c_double_p, c_double_p],
     "se3cq_visibility_matrix_host": [vp, c_double_p, i64, cint, cdbl, c_i8_p],

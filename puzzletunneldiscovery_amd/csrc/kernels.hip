@@ -32,6 +32,8 @@
  */
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "engine.h"
 #include "launch.h"
 #include "se3_math.h"
@@ -2394,6 +2396,38 @@ size_t segment_order_bytes(int64_t n)
 	if (n <= 0 || n >= ((int64_t)1 << 31))
 		return 0;
 	return order_bytes_for(n);
+}
+
+/* ---- scratch of the allocation-free touch search ----
+ * One array per device, in the code object: the launch's work counter
+ * (first 256 bytes), then the longest-first order of up to
+ * NOALLOC_MAX_SEGMENTS segments.  Host code never passes the array's name to
+ * a kernel: the host-side symbol is not a device address.  The device
+ * address comes from hipGetSymbolAddress, once per device. */
+static_assert(NOALLOC_MAX_SEGMENTS == SE3CQ_NOALLOC_MAX_SEGMENTS, "one limit");
+constexpr size_t NOALLOC_SCRATCH_BYTES = NOALLOC_COUNTER_BYTES + order_bytes_for(NOALLOC_MAX_SEGMENTS);
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+hipError_t noalloc_scratch(void** out)
+{
+	static std::mutex mu;
+	static void* ptr[64] = {};
+	int dev = -1;
+	hipError_t e = hipGetDevice(&dev);
+	if (e != hipSuccess)
+		return e;
+	if (dev < 0 || dev >= 64)
+		return hipErrorInvalidDevice;
+	std::lock_guard<std::mutex> lk(mu);
+	if (!ptr[dev]) {
+		void* p = nullptr;
+		e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+		if (e != hipSuccess)
+			return e;
+		ptr[dev] = p;
+	}
+	*out = ptr[dev];
+	return hipSuccess;
 }
 
 /* The longest-first order of n segments into `scratch` (segment_order_bytes(n)

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include "engine.h"
+#include "../../include/se3cq.h"
 
 namespace se3cq {
 
@@ -31,6 +32,13 @@ hipError_t launch_segments(const DevScene& s, int mode, const double* from, cons
                            double vd, int levels, const SegOut& o, hipStream_t st, void* order_scratch = nullptr);
 /* Bytes of order scratch a launch of n segments uses (0: it walks in input order). */
 size_t segment_order_bytes(int64_t n);
+
+/* The allocation-free touch search's scratch on the current device: the
+ * work counter in the first NOALLOC_COUNTER_BYTES, then order scratch for
+ * up to NOALLOC_MAX_SEGMENTS segments.  Looked up once per device. */
+constexpr int64_t NOALLOC_MAX_SEGMENTS = 1 << 20;
+constexpr size_t NOALLOC_COUNTER_BYTES = 256;
+hipError_t noalloc_scratch(void** out);
 /* vis: 0 pair (row i vs row i), 1 symmetric matrix (fi < ti, mirrored), 2 full m x n matrix */
 hipError_t launch_visibility(const DevScene& s, int vis, const double* qs0, const double* qs1, int64_t m,
                              int64_t n, double ivd, int8_t* out8, uint32_t* bits, int32_t* steps, hipStream_t st);

@@ -1607,6 +1607,55 @@ int se3cq_touch_search_device(se3cq_world* w, const double* d_from, const double
 	return seg_device(w, 2, d_from, d_to, n, verify_delta, levels, o, stream);
 }
 
+int se3cq_touch_search_device_noalloc(se3cq_world* w, const double* d_from, const double* d_to, int64_t n,
+                                      double verify_delta, int levels, uint8_t* d_done, double* d_last_tau,
+                                      double* d_tau, double* d_lo, double* d_hi, int32_t* d_steps, void* stream)
+{
+	/* Nothing here allocates: the work counter and the segment order live
+	 * in the library's static per-device scratch array, and the world's
+	 * buffers (and its error word) exist since its upload. */
+	CHECK_WORLD(w);
+	if (!check_step(verify_delta))
+		return fail(SE3CQ_EINVAL, "the verify step must be positive and finite");
+	if (levels < 0 || levels > 60)
+		return fail(SE3CQ_EINVAL, "levels must be in [0, 60]");
+	if (!d_lo || !d_hi || !d_done)
+		return fail(SE3CQ_EINVAL, "done, lo and hi outputs are required");
+	if (n < 0)
+		return fail(SE3CQ_EINVAL, "negative count");
+	if (n > SE3CQ_NOALLOC_MAX_SEGMENTS)
+		return fail(SE3CQ_EINVAL, "the allocation-free touch search takes at most " +
+		                              std::to_string(SE3CQ_NOALLOC_MAX_SEGMENTS) + " segments per call (got " +
+		                              std::to_string(n) + ")");
+	if (!w->ready())
+		return fail(SE3CQ_ESTATE, "models not loaded");
+	if (!w->uploaded)
+		return fail(SE3CQ_ESTATE, "the world is not on the device yet; any other query on it uploads it "
+		                          "(the allocation-free call does not)");
+	int dev = -1;
+	if (hipGetDevice(&dev) != hipSuccess || dev != w->device)
+		return fail(SE3CQ_EINVAL, "the current device is not the world's device");
+	if (n == 0)
+		return SE3CQ_OK;
+	void* scr = nullptr;
+	const hipError_t se = noalloc_scratch(&scr);
+	if (se != hipSuccess)
+		return fail(SE3CQ_EDEVICE, std::string("scratch lookup failed: ") + hipGetErrorString(se));
+	DevScene s;
+	{
+		std::lock_guard<std::mutex> lk(w->qmu);
+		s = base_scene(w);
+	}
+	s.error_flag = (uint32_t*)w->d_err.p;
+	s.queue = (unsigned long long*)scr;
+	const SegOut o{d_done, d_last_tau, d_tau, nullptr, nullptr, d_lo, d_hi, d_steps};
+	const hipError_t e = launch_segments(s, 2, d_from, d_to, n, verify_delta, levels, o, pick_stream(w, stream),
+	                                     (char*)scr + NOALLOC_COUNTER_BYTES);
+	if (e != hipSuccess)
+		return fail(SE3CQ_EDEVICE, std::string("launch failed: ") + hipGetErrorString(e));
+	return SE3CQ_OK;
+}
+
 int se3cq_touch_search_host(se3cq_world* w, const double* from, const double* to, int64_t n, double verify_delta,
                             int levels, uint8_t* done, double* last_tau, double* tau, double* lo, double* hi,
                             int32_t* steps)

@@ -480,6 +480,16 @@ class UnitWorld:
         N.check(N.lib().se3cq_touch_search_device(self._w, d_from, d_to, int(n), float(verify_delta), int(levels),
                                                   d_done, d_last_tau, d_tau, d_lo, d_hi, d_steps, stream))
 
+    def touch_search_device_noalloc(self, d_from, d_to, n, verify_delta, levels, d_done, d_last_tau, d_tau, d_lo,
+                                    d_hi, d_steps, stream=None):
+        """touch_search_device with no allocation: its scratch is the library's
+        static per-device array (at most SE3CQ_NOALLOC_MAX_SEGMENTS segments;
+        the world must have been queried before; overlapping calls on one
+        device race)."""
+        N.check(N.lib().se3cq_touch_search_device_noalloc(self._w, d_from, d_to, int(n), float(verify_delta),
+                                                          int(levels), d_done, d_last_tau, d_tau, d_lo, d_hi,
+                                                          d_steps, stream))
+
     def visibility_bits_device(self, d_qs, n, verify_magnitude, d_bits, d_steps=None, stream=None):
         N.check(N.lib().se3cq_visibility_bits_device(self._w, d_qs, int(n), float(verify_magnitude), d_bits, d_steps,
                                                      stream))
