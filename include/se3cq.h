@@ -15,9 +15,12 @@
  *     back in them.  "_device" entry points take device pointers (HBM) and
  *     enqueue on `stream` (a hipStream_t; NULL = the world's own stream)
  *     without synchronising; call se3cq_check(w) after synchronising to
- *     surface device-side errors.
- *   - A world is immutable after se3cq_angle_model; calls on one world are
- *     serialised internally, calls on different worlds may run concurrently.
+ *     surface (and clear) device-side errors of _device calls.  A _host call
+ *     reports its own launches' errors.
+ *   - A world is immutable after se3cq_angle_model.  Queries on one world may
+ *     come from several threads at once (the reference releases the GIL on
+ *     them, lib/pyosr/module.cc:138-184): each _host call runs on a stream
+ *     and staging buffers of its own.  Set-up calls must not overlap queries.
  *   - geo: 0 = environment (UnitWorld::GEO_ENV), 1 = robot (GEO_ROB).
  */
 #ifndef SE3CQ_H
