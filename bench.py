@@ -622,11 +622,12 @@ def run_clearance(args, torch, pyosr, W):
 
 def _screen_segments(args, torch, W, uw, net, sq, sy, timer, stream):
     """Segments from exact-verified free states, lognormal(5, 1.2) step counts
-    at 1/1024: isValidTransition on every segment, against the prescreen
-    (P(free) at K points along the segment, all above the threshold) followed
-    by isValidTransition on the accepted ones."""
+    at 1/1024: isValidTransition on every segment, against the product's
+    prescreen (surrogate.screen_segments: P(free) at 8 points along each
+    segment in the fused inference kernel, then isValidTransition on the
+    selected segments, read from the device-side list)."""
+    from puzzletunneldiscovery_amd import surrogate as S
     n_seg = args.screen_segments
-    K = 8
     free_idx = torch.nonzero(sy.bool()).squeeze(1)[:n_seg]
     starts = sq.index_select(0, free_idx).contiguous()
     n_seg = starts.shape[0]
@@ -640,43 +641,27 @@ def _screen_segments(args, torch, W, uw, net, sq, sy, timer, stream):
                             args.steps, args.warmup)
     free_plain = int(ok.sum().item())
     queries = int(nsteps.to(torch.int64).sum().item())
-    t = torch.arange(1, K + 1, dtype=torch.float64, device="cuda") / K
-    pk = torch.zeros(n_seg * K, dtype=torch.float32, device="cuda")
     state = {}
 
     def screened():
-        # K points per segment: linear translation, normalised linear
-        # quaternion blend (an approximation is enough to rank)
-        q0, q1 = starts[:, 3:], ends[:, 3:]
-        sgn = torch.where((q0 * q1).sum(1, keepdim=True) < 0, -1.0, 1.0)
-        p = starts[:, None, :3] + t[None, :, None] * (ends[:, None, :3] - starts[:, None, :3])
-        qq = q0[:, None, :] + t[None, :, None] * (sgn * q1 - q0)[:, None, :]
-        qq = qq / qq.norm(dim=2, keepdim=True)
-        pts = torch.cat([p, qq], dim=2).reshape(-1, 7).contiguous()
-        net.forward_device(pts.data_ptr(), n_seg * K, pk.data_ptr(), stream=sp)
-        acc_idx = torch.nonzero(pk.view(n_seg, K).min(1).values > args.threshold).squeeze(1)
-        a, b = starts.index_select(0, acc_idx), ends.index_select(0, acc_idx)
-        out = torch.zeros(a.shape[0], dtype=torch.uint8, device="cuda")
-        if a.shape[0]:
-            uw.valid_transitions_device(a.data_ptr(), b.data_ptr(), a.shape[0], VD, out.data_ptr(), None, sp)
-        state["idx"], state["out"] = acc_idx, out
+        state["r"] = S.screen_segments(uw, net, starts, ends, VD, args.threshold, points=8, stream=sp)
 
     _, scr_ms = timer.run(screened, args.steps, args.warmup)
     uw.check_device()
-    free_scr = int(state["out"].sum().item())
+    r = state["r"]
     truth = ok.bool()
-    acc = torch.zeros(n_seg, dtype=torch.bool, device="cuda")
-    acc[state["idx"]] = True
-    tp = int((truth & acc).sum().item())
+    sel = S.selected_mask(r, n_seg)
+    free_scr = int(r["free"].sum().item())
+    recall, precision = S.recall_precision(r, truth)
     return {
-        "segments": n_seg, "points_per_segment": K, "exact_queries": queries, "free_true": free_plain,
-        "accepted": int(acc.sum().item()), "free_verified_screened": free_scr,
-        "recall": round(tp / max(free_plain, 1), 4), "precision": round(tp / max(int(acc.sum().item()), 1), 4),
+        "segments": n_seg, "points_per_segment": 8, "exact_queries": queries, "free_true": free_plain,
+        "accepted": int(sel.sum().item()), "free_verified_screened": free_scr,
+        "recall": round(recall, 4), "precision": round(precision, 4),
         "exact_ms": round(plain_ms, 3), "screen_total_ms": round(scr_ms, 3),
         "plain_free_per_s": round(free_plain / (plain_ms * 1e-3), 1),
         "screened_free_per_s": round(free_scr / (scr_ms * 1e-3), 1),
         "speedup": round((free_scr / scr_ms) / max(free_plain / plain_ms, 1e-9), 3),
-        "mismatches": 0 if free_scr == int(truth[state["idx"]].sum().item()) else 1,
+        "mismatches": int((r["free"].bool() != (sel & truth)).sum().item()),
     }
 
 
@@ -684,6 +669,7 @@ def run_surrogate(args, torch, pyosr, W):
     """Learned prescreen on the shell scene: exact labels, BF16-MFMA training,
     then free samples per second with and without the prescreen (every
     accepted sample exact-verified)."""
+    from puzzletunneldiscovery_amd import surrogate as S
     from puzzletunneldiscovery_amd.surrogate import Surrogate
     env, rob = W.shell_scene(0)
     uw = pyosr.create_unit_world_from_arrays(env, rob)
@@ -721,34 +707,29 @@ def run_surrogate(args, torch, pyosr, W):
     state = {}
 
     def screened():
-        idx = torch.nonzero(pf > thr).squeeze(1)
-        acc = sq.index_select(0, idx)
-        out = torch.zeros(acc.shape[0], dtype=torch.uint8, device="cuda")
-        if acc.shape[0]:
-            exact(acc, acc.shape[0], out)
-        state["acc"], state["out"], state["idx"] = acc.shape[0], out, idx
+        # the product's prescreen: selection fused into the inference kernel,
+        # exact re-verification of the device-side list, no host round trip
+        state["r"] = S.screen_states(uw, net, sq, thr, stream=sp)
 
-    wall_scr, scr_ms = timer.run(lambda: (net.forward_device(sq.data_ptr(), n_screen, pf.data_ptr(), stream=sp),
-                                          screened()), args.steps, args.warmup)
+    wall_scr, scr_ms = timer.run(screened, args.steps, args.warmup)
     uw.check_device()
-    free_scr = int(state["out"].sum().item())
+    r = state["r"]
+    free_scr = int(r["free"].sum().item())
     seg = _screen_segments(args, torch, W, uw, net, sq, sy, timer, stream) if args.screen_segments else None
     truth = sy.bool()
-    pred = pf > thr
-    tp = int((truth & pred).sum().item())
-    recall = tp / max(free_plain, 1)
-    precision = tp / max(int(pred.sum().item()), 1)
+    sel = S.selected_mask(r, n_screen)
+    recall, precision = S.recall_precision(r, truth)
     flops_sample = 2 * (12 * 256 + 3 * 256 * 256 + 256 * 2)
     rate_plain = free_plain / (plain_ms * 1e-3)
     rate_scr = free_scr / (scr_ms * 1e-3)
     extra = {
-        "mismatches": 0 if free_scr == int((truth[state["idx"]]).sum().item()) else 1,
+        "mismatches": int((r["free"].bool() != (sel & truth)).sum().item()),
         "cpu_baseline": None,
         "surrogate": {
             "labels": n_train, "label_ms": round(label_ms, 2), "train_epochs": args.train_epochs,
             "train_steps": steps_done, "train_s": round(train_s, 2), "loss_first_epoch": round(losses[0], 4),
             "loss_last_epoch": round(losses[-1], 4), "threshold": thr, "recall": round(recall, 4),
-            "precision": round(precision, 4), "accepted": state["acc"], "free_true": free_plain,
+            "precision": round(precision, 4), "accepted": int(sel.sum().item()), "free_true": free_plain,
             "free_verified_screened": free_scr,
             "plain_free_per_s": round(rate_plain, 1), "screened_free_per_s": round(rate_scr, 1),
             "speedup": round(rate_scr / max(rate_plain, 1e-9), 3),

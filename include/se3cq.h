@@ -91,6 +91,15 @@ int se3cq_get_recommended_cres(const se3cq_world* w, double* cres);
 int se3cq_valid_states_host(se3cq_world* w, const double* q, int64_t n, uint8_t* out);
 int se3cq_valid_states_device(se3cq_world* w, const double* d_q, int64_t n, uint8_t* d_out, void* stream);
 
+/* isValid on the states a prescreen kept (no reference counterpart: it
+ * replaces the samplers' isValid calls on accepted candidates, e.g.
+ * src/GP/uw_random.py:46-52): for k < min(*d_count, n), with i = d_idx[k],
+ * d_out[i] = 1 if state i is free.  Entries of d_out not listed are not
+ * written; indices outside [0, n) are skipped.  The count is read on the
+ * device, so a prescreen's output list needs no synchronisation. */
+int se3cq_valid_states_subset_device(se3cq_world* w, const double* d_states, int64_t n, const int32_t* d_idx,
+                                     const uint32_t* d_count, uint8_t* d_out, void* stream);
+
 /* UnitWorld::transitStateToWithContact (unit_world.cc:405-474), batched over
  * n (from, to) rows with one verify_delta.  Outputs per row: last_free [7],
  * first colliding state [7], done (1 = free), last_tau and tau in binary64
@@ -109,6 +118,13 @@ int se3cq_valid_transitions_host(se3cq_world* w, const double* from, const doubl
 
 int se3cq_valid_transitions_device(se3cq_world* w, const double* d_from, const double* d_to, int64_t n,
                                    double initial_verify_delta, uint8_t* d_out, int32_t* d_steps, void* stream);
+
+/* isValidTransition on the segments a prescreen kept (unit_world.cc:477-486,
+ * listed as se3cq_valid_states_subset_device lists states): d_out[i] (and
+ * d_steps[i], may be NULL) for i = d_idx[k], k < min(*d_count, n). */
+int se3cq_valid_transitions_subset_device(se3cq_world* w, const double* d_from, const double* d_to, int64_t n,
+                                          const int32_t* d_idx, const uint32_t* d_count, double initial_verify_delta,
+                                          uint8_t* d_out, int32_t* d_steps, void* stream);
 
 /* Touch-configuration search: transitStateToWithContact followed, for rows
  * that hit an obstacle, by `levels` bisection halvings of [last_tau, tau]

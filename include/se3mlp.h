@@ -56,6 +56,24 @@ int se3mlp_set_params(se3mlp_net* net, int layer, const float* w, const float* b
 int se3mlp_forward_device(se3mlp_net* net, const double* d_states, int64_t n, float* d_pfree, float* d_margin,
                           float* d_logits, void* stream);
 
+/* The prescreen of n < 2^31 states in one kernel: inference, then the index
+ * of every state with P(free) > threshold appended to d_idx (room for n
+ * int32), *d_count (device uint32, zeroed by the call) their number.  The
+ * order of the indices is not fixed.  d_pfree (may be NULL) receives P(free)
+ * of every state.  Re-verify the survivors with
+ * se3cq_valid_states_subset_device, which reads the count on the device. */
+int se3mlp_screen_states_device(se3mlp_net* net, const double* d_states, int64_t n, float threshold, int32_t* d_idx,
+                                uint32_t* d_count, float* d_pfree, void* stream);
+/* The prescreen of n segments from d_from[i] to d_to[i]: `points` (a divisor
+ * of 32) states per segment at t = j / points, j = 1 .. points, with the
+ * translation interpolated linearly and the quaternion blended linearly on
+ * the short arc (an approximation of the slerp, for ranking only); a
+ * segment is appended to d_idx when all its points have P(free) >
+ * threshold.  n * points < 2^31.  Re-verify with
+ * se3cq_valid_transitions_subset_device. */
+int se3mlp_screen_segments_device(se3mlp_net* net, const double* d_from, const double* d_to, int64_t n, int points,
+                                  float threshold, int32_t* d_idx, uint32_t* d_count, void* stream);
+
 /* One Adam step (lr; betas 0.9, 0.999; eps 1e-8) on the mean two-class
  * cross-entropy of a batch of n <= max_batch labelled states.  loss_out, if
  * not NULL, receives the batch's mean loss before the step (synchronises). */

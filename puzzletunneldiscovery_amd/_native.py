@@ -63,6 +63,8 @@ SIGNATURES = {
                                 c_double_p, c_double_p, c_double_p, c_i32_p],
     "se3cq_touch_search_device": [vp, vp, vp, i64, cdbl, cint, vp, vp, vp, vp, vp, vp, vp],
     "se3cq_touch_search_device_noalloc": [vp, vp, vp, i64, cdbl, cint, vp, vp, vp, vp, vp, vp, vp],
+    "se3cq_valid_states_subset_device": [vp, vp, i64, vp, vp, vp, vp],
+    "se3cq_valid_transitions_subset_device": [vp, vp, vp, i64, vp, vp, cdbl, vp, vp, vp],
     "se3cq_bisect_contact_host": [vp, c_double_p, c_double_p, i64, c_double_p, c_double_p, cint,
                                   c_double_p, c_double_p],
     "se3cq_visibility_matrix_host": [vp, c_double_p, i64, cint, cdbl, c_i8_p],
@@ -112,6 +114,8 @@ MLP_SIGNATURES = {
     "se3mlp_get_params": [vp, cint, c_float_p, c_float_p],
     "se3mlp_set_params": [vp, cint, c_float_p, c_float_p],
     "se3mlp_forward_device": [vp, vp, i64, vp, vp, vp, vp],
+    "se3mlp_screen_states_device": [vp, vp, i64, ctypes.c_float, vp, vp, vp, vp],
+    "se3mlp_screen_segments_device": [vp, vp, vp, i64, cint, ctypes.c_float, vp, vp, vp],
     "se3mlp_train_step_device": [vp, vp, vp, i64, ctypes.c_float, c_float_p, vp],
     "se3mlp_gemm_device": [vp, i64, vp, i64, cint, cint, cint, vp, i64, vp],
     "se3mlp_features_device": [vp, i64, vp, vp],

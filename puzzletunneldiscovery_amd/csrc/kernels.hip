@@ -548,6 +548,23 @@ struct StatesPlain {
 	}
 };
 
+/* States picked by an index list (a prescreen's survivors): item i is
+ * state idx[i]; an index outside [0, n) reads state 0 and is never stored. */
+struct StatesSubset {
+	const double* qs;
+	const int32_t* idx;
+	int64_t n;
+	__device__ __forceinline__ void get(int64_t i, double* o) const
+	{
+		int64_t j = idx[i];
+		if (j < 0 || j >= n)
+			j = 0;
+#pragma unroll
+		for (int k = 0; k < 7; k++)
+			o[k] = qs[7 * j + k];
+	}
+};
+
 struct StatesOmpl {
 	const double* qs; /* OMPL order (x, y, z, qx, qy, qz, qw) */
 	const OmplXf* x;
@@ -881,6 +898,18 @@ struct StoreValidU8 {
 	__device__ __forceinline__ void operator()(int64_t i, bool hit) const { out[i] = hit ? 0 : 1; }
 };
 
+struct StoreValidSubset {
+	uint8_t* out;
+	const int32_t* idx;
+	int64_t n;
+	__device__ __forceinline__ void operator()(int64_t i, bool hit) const
+	{
+		const int64_t j = idx[i];
+		if (j >= 0 && j < n)
+			out[j] = hit ? 0 : 1;
+	}
+};
+
 struct StoreValidI32 {
 	int32_t* out;
 	__device__ __forceinline__ void operator()(int64_t i, bool hit) const { out[i] = hit ? 0 : 1; }
@@ -894,6 +923,21 @@ __global__ void __launch_bounds__(256, SE3CQ_STATES_WAVES) k_valid_states(DevSce
 	PairStack<WIDE> stk, leaves;
 	thread_stacks<WIDE>(s, smem, stk, leaves);
 	states_refill<WIDE>(s, StatesPlain{qs}, n, StoreValidU8{out}, stk, leaves);
+}
+
+/* The states a prescreen kept: items [0, min(*count, n)) of idx, each an
+ * index into qs; out[idx[i]] is written, other entries are left alone. */
+template <bool WIDE>
+__global__ void __launch_bounds__(256, SE3CQ_STATES_WAVES) k_valid_states_subset(DevScene s, const double* __restrict__ qs,
+                                                      int64_t n, const int32_t* __restrict__ idx,
+                                                      const uint32_t* __restrict__ count, uint8_t* __restrict__ out)
+{
+	extern __shared__ uint32_t smem[];
+	PairStack<WIDE> stk, leaves;
+	thread_stacks<WIDE>(s, smem, stk, leaves);
+	const int64_t c = (int64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)*count);
+	const int64_t items = c < n ? c : n;
+	states_refill<WIDE>(s, StatesSubset{qs, idx, n}, items, StoreValidSubset{out, idx, n}, stk, leaves);
 }
 
 /* OMPL-convention states (OmplDriver::validateStates,
@@ -1726,11 +1770,18 @@ __global__ void __launch_bounds__(256, MODE == MODE_TOUCH ? SE3CQ_TOUCH_WAVES : 
 	const int wave = threadIdx.x >> 6;
 	SegShared* sh = &segsh[wave];
 	BisectShared* bs = (MODE == MODE_TOUCH) ? reinterpret_cast<BisectShared*>(seg_lds) + wave : nullptr;
+	int64_t items = n;
+	if (s.n_items) {
+		const int64_t c = (int64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)*s.n_items);
+		items = c < n ? c : n;
+	}
 	for (;;) {
 		const int64_t item = next_item(s.queue);
-		if (item >= n)
+		if (item >= items)
 			break;
 		const int64_t seg = s.order ? (int64_t)s.order[item] : item;
+		if (seg < 0 || seg >= n)
+			continue; /* an index list from the caller may hold anything */
 		seg_setup<MODE>(sh, from_all + 7 * seg, to_all + 7 * seg, vd);
 		wave_lds_sync();
 		const SegResult r = segment_walk<WIDE, MODE>(s, sh, levels, bs);
@@ -2305,6 +2356,27 @@ hipError_t launch_valid_states(const DevScene& s, const double* q, int64_t n, ui
 		hipLaunchKernelGGL(k_valid_states<false>,
 		                   dim3(persistent_grid(s, k_valid_states<false>, block, lds, (n + 63) / 64)), dim3(block), lds,
 		                   st, s, q, n, out);
+	return hipGetLastError();
+}
+
+hipError_t launch_valid_states_subset(const DevScene& s, const double* q, int64_t n, const int32_t* idx,
+                                      const uint32_t* count, uint8_t* out, hipStream_t st)
+{
+	if (n <= 0)
+		return hipSuccess;
+	const int block = pick_block(s);
+	const size_t lds = stack_bytes(s, block);
+	hipError_t e = zero_queue(s, st);
+	if (e != hipSuccess)
+		return e;
+	if (s.wide)
+		hipLaunchKernelGGL(k_valid_states_subset<true>,
+		                   dim3(persistent_grid(s, k_valid_states_subset<true>, block, lds, (n + 63) / 64)), dim3(block),
+		                   lds, st, s, q, n, idx, count, out);
+	else
+		hipLaunchKernelGGL(k_valid_states_subset<false>,
+		                   dim3(persistent_grid(s, k_valid_states_subset<false>, block, lds, (n + 63) / 64)),
+		                   dim3(block), lds, st, s, q, n, idx, count, out);
 	return hipGetLastError();
 }
 

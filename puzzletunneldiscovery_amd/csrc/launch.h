@@ -70,6 +70,11 @@ hipError_t launch_batch(const DevScene& s, const BatchArgs& a, int block, int gr
 int batch_default_block(const DevScene& s, bool states);
 int batch_blocks_per_cu(const DevScene& s, int block, bool states);
 
+/* out[idx[k]] = 1 if state idx[k] is free, for k < min(*count, n); indices
+ * outside [0, n) are skipped; other entries of out are not written. */
+hipError_t launch_valid_states_subset(const DevScene& s, const double* q, int64_t n, const int32_t* idx,
+                                      const uint32_t* count, uint8_t* out, hipStream_t st);
+
 /* OMPL-convention validity (OmplDriver::validateStates / validateMotionPairs):
  * out 1 valid, 0 invalid; motions: -1 when the sample count is not finite
  * below 2^31, samples = max(nd, 1) (may be null). */

@@ -294,11 +294,35 @@ __device__ __forceinline__ bf16x8 ld8(const uint16_t* p)
 	return r;
 }
 
+/*
+ * Prescreen modes of the same kernel.  INFER_PLAIN writes P(free), margin or
+ * logits per row.  INFER_SCREEN_STATES also appends the index of every row
+ * with P(free) > threshold to idx (one atomic per block reserves the block's
+ * run of slots in *count).  INFER_SCREEN_SEGMENTS evaluates `points` rows per
+ * segment, the states at t = j / points (j = 1 .. points) of the segment
+ * from -> to (translation interpolated linearly, the quaternion blended
+ * linearly on the short arc and normalised: a cheap stand-in for the slerp
+ * that is only used to rank), and appends the segment when every one of its
+ * rows passes.  The order of the appended indices is not fixed.
+ */
+enum { INFER_PLAIN = 0, INFER_SCREEN_STATES = 1, INFER_SCREEN_SEGMENTS = 2 };
+
+struct ScreenArgs {
+	const double* to;  /* segments: end states (q holds the start states) */
+	int points;        /* segments: rows per segment, a divisor of 32 */
+	float threshold;
+	int32_t* idx;
+	uint32_t* count;
+	int64_t base;      /* index of this launch's first row (state) or segment */
+};
+
+template <int MODE>
 __global__ void __launch_bounds__(256, 2) k_mlp_infer(const double* __restrict__ q, int64_t n, MlpW w,
                                                       float* __restrict__ pfree, float* __restrict__ margin,
-                                                      float* __restrict__ logits)
+                                                      float* __restrict__ logits, ScreenArgs sa)
 {
 	extern __shared__ __align__(16) uint16_t act[]; /* FM x ALD */
+	__shared__ uint32_t wcount[4], wbase[4];
 	const int wave = threadIdx.x >> 6;
 	const int lane = threadIdx.x & 63;
 	const int r = lane & 31, kq = (lane >> 5) * 8;
@@ -310,7 +334,25 @@ __global__ void __launch_bounds__(256, 2) k_mlp_infer(const double* __restrict__
 		for (int k = 0; k < IN; k++)
 			f[k] = 0.0f;
 		if (i < n) {
-			const double* s = q + 7 * i;
+			double st[7];
+			if (MODE == INFER_SCREEN_SEGMENTS) {
+				const int64_t seg = i / sa.points;
+				const double t = (double)(i - seg * sa.points + 1) / (double)sa.points;
+				const double* a = q + 7 * seg;
+				const double* b = sa.to + 7 * seg;
+				const double sg = ((a[3] * b[3] + a[4] * b[4]) + (a[5] * b[5] + a[6] * b[6])) < 0.0 ? -1.0 : 1.0;
+#pragma unroll
+				for (int k = 0; k < 3; k++)
+					st[k] = a[k] + t * (b[k] - a[k]);
+#pragma unroll
+				for (int k = 3; k < 7; k++)
+					st[k] = a[k] + t * (sg * b[k] - a[k]);
+			} else {
+#pragma unroll
+				for (int k = 0; k < 7; k++)
+					st[k] = q[7 * i + k];
+			}
+			const double* s = st;
 			double qw = s[3], qx = s[4], qy = s[5], qz = s[6];
 			const double nn = sqrt((qx * qx + qz * qz) + (qy * qy + qw * qw));
 			if (nn > 0.0) {
@@ -396,24 +438,70 @@ __global__ void __launch_bounds__(256, 2) k_mlp_infer(const double* __restrict__
 			c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ld8(act + (32 * wave + r) * ALD + k + kq),
 			                                            ld8(W + (int64_t)r * HID + k + kq), c, 0, 0, 0);
 		const float bias = w.b[LAYERS - 1][r];
+		/* lanes 0 and 32: bit p of `pass` = row 32 wave + p of the tile passes */
+		uint32_t pass = 0;
 #pragma unroll
 		for (int i = 0; i < 16; i++) {
 			const float z = c[i] + bias;
 			const float z1 = __shfl(z, lane + 1, 64); /* column 1 of the same row */
-			const int64_t m = row0 + 32 * wave + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
+			const int p = 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
+			const int64_t m = row0 + 32 * wave + p;
 			if (r == 0 && m < n) {
 				const float d = z - z1;
+				const float pf = 1.0f / (1.0f + expf(-d));
 				if (pfree)
-					pfree[m] = 1.0f / (1.0f + expf(-d));
+					pfree[m] = pf;
 				if (margin)
 					margin[m] = d;
 				if (logits) {
 					logits[2 * m] = z;
 					logits[2 * m + 1] = z1;
 				}
+				if (pf > sa.threshold)
+					pass |= 1u << p;
+			}
+		}
+		if (MODE != INFER_PLAIN) {
+			const uint32_t rows = (uint32_t)__builtin_amdgcn_readlane((int)pass, 0) |
+			                      (uint32_t)__builtin_amdgcn_readlane((int)pass, 32);
+			/* items of this wave's tile that pass: rows, or segments whose
+			 * rows all pass (a segment's rows never straddle tiles) */
+			uint32_t sel = rows;
+			int per_tile = 32;
+			if (MODE == INFER_SCREEN_SEGMENTS) {
+				const int k = sa.points;
+				const uint32_t all = k == 32 ? 0xffffffffu : ((1u << k) - 1u);
+				per_tile = 32 / k;
+				sel = 0;
+				for (int g = 0; g < per_tile; g++)
+					if (((rows >> (g * k)) & all) == all)
+						sel |= 1u << g;
+			}
+			if (lane == 0)
+				wcount[wave] = (uint32_t)__popc(sel);
+			__syncthreads();
+			if (threadIdx.x == 0) {
+				const uint32_t tot = wcount[0] + wcount[1] + wcount[2] + wcount[3];
+				const uint32_t b0 = tot ? atomicAdd(sa.count, tot) : 0u;
+				wbase[0] = b0;
+				wbase[1] = b0 + wcount[0];
+				wbase[2] = b0 + wcount[0] + wcount[1];
+				wbase[3] = b0 + wcount[0] + wcount[1] + wcount[2];
+			}
+			__syncthreads();
+			if (lane < per_tile && ((sel >> lane) & 1u)) {
+				const uint32_t slot = wbase[wave] + (uint32_t)__popc(sel & ((1u << lane) - 1u));
+				const int64_t first = (MODE == INFER_SCREEN_STATES) ? row0 + 32 * wave : (row0 + 32 * wave) / sa.points;
+				sa.idx[slot] = (int32_t)(sa.base + first + lane);
 			}
 		}
 	}
+}
+
+__global__ void k_zero_u32(uint32_t* p)
+{
+	if (threadIdx.x == 0)
+		*p = 0u;
 }
 
 /* ---- host side ------------------------------------------------------------ */
@@ -636,11 +724,61 @@ int se3mlp_forward_device(se3mlp_net* net, const double* d_states, int64_t n, fl
 	const int64_t per_launch = (int64_t)FM * 0x40000000LL;
 	for (int64_t off = 0; off < n; off += per_launch) {
 		const int64_t cnt = (n - off < per_launch) ? (n - off) : per_launch;
-		hipLaunchKernelGGL(k_mlp_infer, dim3((unsigned)((cnt + FM - 1) / FM)), dim3(256), MLP_LDS, (hipStream_t)stream,
-		                   d_states + 7 * off, cnt, mw, d_pfree ? d_pfree + off : nullptr,
-		                   d_margin ? d_margin + off : nullptr, d_logits ? d_logits + 2 * off : nullptr);
+		hipLaunchKernelGGL(k_mlp_infer<INFER_PLAIN>, dim3((unsigned)((cnt + FM - 1) / FM)), dim3(256), MLP_LDS,
+		                   (hipStream_t)stream, d_states + 7 * off, cnt, mw, d_pfree ? d_pfree + off : nullptr,
+		                   d_margin ? d_margin + off : nullptr, d_logits ? d_logits + 2 * off : nullptr, ScreenArgs{});
 		HCHK(hipGetLastError());
 	}
+	return SE3MLP_OK;
+}
+
+int se3mlp_screen_states_device(se3mlp_net* net, const double* d_states, int64_t n, float threshold, int32_t* d_idx,
+                                uint32_t* d_count, float* d_pfree, void* stream)
+{
+	if (!net || n < 0 || n >= ((int64_t)1 << 31) || !d_count || (n > 0 && (!d_states || !d_idx)) ||
+	    !std::isfinite(threshold))
+		return fail(SE3MLP_EINVAL, "se3mlp_screen_states_device: bad arguments");
+	std::lock_guard<std::mutex> lk(net->mu);
+	hipStream_t st = (hipStream_t)stream;
+	hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, d_count);
+	HCHK(hipGetLastError());
+	if (n == 0)
+		return SE3MLP_OK;
+	MlpW mw;
+	for (int l = 0; l < LAYERS; l++) {
+		mw.W[l] = net->W[l].as<uint16_t>();
+		mw.b[l] = net->b[l].as<float>();
+	}
+	const ScreenArgs sa{nullptr, 1, threshold, d_idx, d_count, 0};
+	hipLaunchKernelGGL(k_mlp_infer<INFER_SCREEN_STATES>, dim3((unsigned)((n + FM - 1) / FM)), dim3(256), MLP_LDS, st,
+	                   d_states, n, mw, d_pfree, nullptr, nullptr, sa);
+	HCHK(hipGetLastError());
+	return SE3MLP_OK;
+}
+
+int se3mlp_screen_segments_device(se3mlp_net* net, const double* d_from, const double* d_to, int64_t n, int points,
+                                  float threshold, int32_t* d_idx, uint32_t* d_count, void* stream)
+{
+	if (!net || n < 0 || !d_count || (n > 0 && (!d_from || !d_to || !d_idx)) || !std::isfinite(threshold) ||
+	    points < 1 || points > 32 || (32 % points) != 0 || n * (int64_t)points >= ((int64_t)1 << 31))
+		return fail(SE3MLP_EINVAL, "se3mlp_screen_segments_device: bad arguments (points must divide 32, "
+		                           "n * points must stay below 2^31)");
+	std::lock_guard<std::mutex> lk(net->mu);
+	hipStream_t st = (hipStream_t)stream;
+	hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, d_count);
+	HCHK(hipGetLastError());
+	if (n == 0)
+		return SE3MLP_OK;
+	MlpW mw;
+	for (int l = 0; l < LAYERS; l++) {
+		mw.W[l] = net->W[l].as<uint16_t>();
+		mw.b[l] = net->b[l].as<float>();
+	}
+	const int64_t rows = n * points;
+	const ScreenArgs sa{d_to, points, threshold, d_idx, d_count, 0};
+	hipLaunchKernelGGL(k_mlp_infer<INFER_SCREEN_SEGMENTS>, dim3((unsigned)((rows + FM - 1) / FM)), dim3(256), MLP_LDS,
+	                   st, d_from, rows, mw, nullptr, nullptr, nullptr, sa);
+	HCHK(hipGetLastError());
 	return SE3MLP_OK;
 }
 

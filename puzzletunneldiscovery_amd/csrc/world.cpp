@@ -1267,6 +1267,24 @@ int se3cq_valid_states_device(se3cq_world* w, const double* d_q, int64_t n, uint
 	});
 }
 
+int se3cq_valid_states_subset_device(se3cq_world* w, const double* d_states, int64_t n, const int32_t* d_idx,
+                                     const uint32_t* d_count, uint8_t* d_out, void* stream)
+{
+	CHECK_WORLD(w);
+	if (n < 0 || n >= ((int64_t)1 << 31))
+		return fail(SE3CQ_EINVAL, "n must lie in [0, 2^31)");
+	if (n > 0 && (!d_states || !d_idx || !d_count || !d_out))
+		return fail(SE3CQ_EINVAL, "bad arguments");
+	if (!w->ready())
+		return fail(SE3CQ_ESTATE, "models not loaded");
+	const int rc0 = setup_device(w);
+	if (rc0 || n == 0)
+		return rc0;
+	return device_launch(w, stream, 0, [&](const DevScene& s, void*) {
+		return launch_valid_states_subset(s, d_states, n, d_idx, d_count, d_out, pick_stream(w, stream));
+	});
+}
+
 int se3cq_valid_states_host(se3cq_world* w, const double* q, int64_t n, uint8_t* out)
 {
 	CHECK_WORLD(w);
@@ -1583,6 +1601,32 @@ int se3cq_valid_transitions_device(se3cq_world* w, const double* d_from, const d
 		return rc0;
 	SegOut o{d_out, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, d_steps};
 	return seg_device(w, 1, d_from, d_to, n, initial_verify_delta, 0, o, stream);
+}
+
+int se3cq_valid_transitions_subset_device(se3cq_world* w, const double* d_from, const double* d_to, int64_t n,
+                                          const int32_t* d_idx, const uint32_t* d_count, double initial_verify_delta,
+                                          uint8_t* d_out, int32_t* d_steps, void* stream)
+{
+	CHECK_WORLD(w);
+	if (!check_step(initial_verify_delta))
+		return fail(SE3CQ_EINVAL, "the verify step must be positive and finite");
+	if (n < 0 || n >= ((int64_t)1 << 31))
+		return fail(SE3CQ_EINVAL, "n must lie in [0, 2^31)");
+	if (n > 0 && (!d_from || !d_to || !d_idx || !d_count || !d_out))
+		return fail(SE3CQ_EINVAL, "bad arguments");
+	if (!w->ready())
+		return fail(SE3CQ_ESTATE, "models not loaded");
+	const int rc0 = setup_device(w);
+	if (rc0 || n == 0)
+		return rc0;
+	hipStream_t st = pick_stream(w, stream);
+	const SegOut o{d_out, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, d_steps};
+	return device_launch(w, stream, 0, [&](const DevScene& s0, void*) {
+		DevScene s = s0;
+		s.order = d_idx;
+		s.n_items = d_count;
+		return launch_segments(s, 1, d_from, d_to, n, initial_verify_delta, 0, o, st, nullptr);
+	});
 }
 
 int se3cq_touch_search_device(se3cq_world* w, const double* d_from, const double* d_to, int64_t n,

@@ -464,6 +464,16 @@ class UnitWorld:
     def valid_states_device(self, d_q, n, d_out, stream=None):
         N.check(N.lib().se3cq_valid_states_device(self._w, d_q, int(n), d_out, stream))
 
+    def valid_states_subset_device(self, d_q, n, d_idx, d_count, d_out, stream=None):
+        """out[idx[k]] = isValid(q[idx[k]]) for k < *count (device count)."""
+        N.check(N.lib().se3cq_valid_states_subset_device(self._w, d_q, int(n), d_idx, d_count, d_out, stream))
+
+    def valid_transitions_subset_device(self, d_from, d_to, n, d_idx, d_count, initial_verify_delta, d_out,
+                                        d_steps=None, stream=None):
+        """out[idx[k]] = isValidTransition(from[idx[k]], to[idx[k]]) for k < *count."""
+        N.check(N.lib().se3cq_valid_transitions_subset_device(self._w, d_from, d_to, int(n), d_idx, d_count,
+                                                              float(initial_verify_delta), d_out, d_steps, stream))
+
     def transit_device(self, d_from, d_to, n, verify_delta, d_done, d_last_tau, d_tau, d_steps,
                        d_last_free=None, d_state=None, stream=None):
         """Device-pointer form of transit_states_to_with_contact (asynchronous)."""

@@ -56,6 +56,14 @@ class Surrogate:
     def forward_device(self, d_states, n, d_pfree=None, d_margin=None, d_logits=None, stream=None):
         N.check_mlp(N.lib().se3mlp_forward_device(self._h, d_states, int(n), d_pfree, d_margin, d_logits, stream))
 
+    def screen_states_device(self, d_states, n, threshold, d_idx, d_count, d_pfree=None, stream=None):
+        N.check_mlp(N.lib().se3mlp_screen_states_device(self._h, d_states, int(n), ctypes.c_float(threshold), d_idx,
+                                                       d_count, d_pfree, stream))
+
+    def screen_segments_device(self, d_from, d_to, n, points, threshold, d_idx, d_count, stream=None):
+        N.check_mlp(N.lib().se3mlp_screen_segments_device(self._h, d_from, d_to, int(n), int(points),
+                                                         ctypes.c_float(threshold), d_idx, d_count, stream))
+
     def train_step_device(self, d_states, d_labels, n, lr, want_loss=False, stream=None):
         loss = ctypes.c_float(0.0)
         N.check_mlp(N.lib().se3mlp_train_step_device(self._h, d_states, d_labels, int(n), ctypes.c_float(lr),
@@ -102,6 +110,102 @@ class Surrogate:
                 log(f"epoch {ep}: loss {losses[-1]:.4f}")
         torch.cuda.synchronize()
         return losses
+
+
+class _one_stream:
+    """The stream both halves of a screen run on, so the exact pass reads the
+    list the inference kernel wrote.  A NULL handle means different streams
+    to the two libraries (the legacy default stream to the prescreen, the
+    world's own stream to the engine), so with torch's default stream current
+    the screen runs on a side stream ordered after it, and the default stream
+    waits for the screen."""
+
+    def __init__(self, stream):
+        self.stream = stream
+
+    def __enter__(self):
+        import torch
+        if self.stream:
+            return self.stream
+        self.cur = torch.cuda.current_stream()
+        if self.cur.cuda_stream:
+            self.side = None
+            return self.cur.cuda_stream
+        self.side = torch.cuda.Stream()
+        self.side.wait_stream(self.cur)
+        return self.side.cuda_stream
+
+    def __exit__(self, *exc):
+        if not self.stream and self.side is not None:
+            self.cur.wait_stream(self.side)
+        return False
+
+
+def screen_states(uw, net, d_states, threshold=0.5, stream=None, d_pfree=None):
+    """The prescreen of the sampler loops (src/GP/uw_random.py:46-52,
+    src/GP/pipeline/keyconf.py:127-131): the network selects the states with
+    P(free) > threshold in one kernel, and the exact engine re-verifies every
+    selected state (UnitWorld::isValid, lib/osr/unit_world.cc:215-247), so
+    nothing the network accepts is reported free unless it is.  Everything is
+    enqueued on `stream` (default: torch's current stream) without a host
+    synchronisation.  d_states: float64 CUDA tensor [n, 7].  Returns a dict of
+    CUDA tensors: idx (int32 [n], the first count entries are the selected
+    states, in no fixed order), count (uint32 [1]) and free (uint8 [n], 1 for
+    a selected state that is exactly free, else 0)."""
+    import torch
+    n = int(d_states.shape[0])
+    idx = torch.empty(max(n, 1), dtype=torch.int32, device=d_states.device)
+    count = torch.zeros(1, dtype=torch.int32, device=d_states.device)
+    free = torch.zeros(n, dtype=torch.uint8, device=d_states.device)
+    with _one_stream(stream) as sp:
+        net.screen_states_device(d_states.data_ptr(), n, threshold, idx.data_ptr(), count.data_ptr(),
+                                 d_pfree.data_ptr() if d_pfree is not None else None, stream=sp)
+        if n:
+            uw.valid_states_subset_device(d_states.data_ptr(), n, idx.data_ptr(), count.data_ptr(),
+                                          free.data_ptr(), sp)
+    return {"idx": idx, "count": count, "free": free}
+
+
+def screen_segments(uw, net, d_from, d_to, initial_verify_delta, threshold=0.5, points=8, stream=None):
+    """The prescreen of candidate edges (the visibility checks of
+    src/GP/pipeline/solve2.py:164-233): a segment is selected when the
+    network gives P(free) > threshold at `points` states along it, and every
+    selected segment is re-verified exactly (UnitWorld::isValidTransition,
+    lib/osr/unit_world.cc:477-486).  Returns idx, count and free as
+    screen_states does, per segment."""
+    import torch
+    n = int(d_from.shape[0])
+    idx = torch.empty(max(n, 1), dtype=torch.int32, device=d_from.device)
+    count = torch.zeros(1, dtype=torch.int32, device=d_from.device)
+    free = torch.zeros(n, dtype=torch.uint8, device=d_from.device)
+    with _one_stream(stream) as sp:
+        net.screen_segments_device(d_from.data_ptr(), d_to.data_ptr(), n, points, threshold, idx.data_ptr(),
+                                   count.data_ptr(), stream=sp)
+        if n:
+            uw.valid_transitions_subset_device(d_from.data_ptr(), d_to.data_ptr(), n, idx.data_ptr(),
+                                               count.data_ptr(), initial_verify_delta, free.data_ptr(), None, sp)
+    return {"idx": idx, "count": count, "free": free}
+
+
+def selected_mask(result, n):
+    """Bool CUDA tensor [n]: the items a screen selected (synchronises)."""
+    import torch
+    k = int(result["count"].item())
+    m = torch.zeros(n, dtype=torch.bool, device=result["idx"].device)
+    m[result["idx"][:k].long()] = True
+    return m
+
+
+def recall_precision(result, truth):
+    """Against exact labels truth (bool CUDA tensor [n], True = free): recall
+    = free items the screen kept / free items, precision = free items among
+    the selected ones."""
+    n = int(truth.shape[0])
+    sel = selected_mask(result, n)
+    kept = result["free"].bool()
+    t = int(truth.sum().item())
+    s = int(sel.sum().item())
+    return int(kept.sum().item()) / max(t, 1), int((sel & truth).sum().item()) / max(s, 1)
 
 
 def features_reference(states):

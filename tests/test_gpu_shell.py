@@ -78,3 +78,78 @@ def test_shell_visibility(shell):
     c = ow.visibility_matrix(ms, True, 1.0 / 1024)
     assert np.array_equal(g, c)
     assert 0 < g.sum() < g.size - len(ms)
+
+
+def test_shell_prescreen(shell):
+    """The learned prescreen as a sampler would use it (src/GP/uw_random.py:46-52):
+    trained on 2M exact labels of the shell scene, it selects among 200k new
+    states and 2,000 lognormal segments, and the exact engine re-verifies
+    every selection.  No state or segment is reported free unless the oracle
+    finds it free, every selected free one is reported, the selection equals
+    P(free) > threshold of the plain forward pass, and recall is at least 0.9."""
+    import torch
+    from puzzletunneldiscovery_amd import surrogate as S
+    uw, ow = shell
+    _flush("prescreen: 2M labels, training")
+    tq = torch.from_numpy(W.random_states(2 * 1024 * 1024, seed=311)).cuda()
+    ty = torch.zeros(tq.shape[0], dtype=torch.uint8, device="cuda")
+    uw.valid_states_device(tq.data_ptr(), tq.shape[0], ty.data_ptr())
+    torch.cuda.synchronize()
+    uw.check_device()
+    net = S.Surrogate(max_batch=16384, seed=312)
+    net.fit(tq, ty, epochs=3, batch=16384, lr=2e-3)
+    del tq, ty
+    _flush("prescreen: 200k states")
+    qs = W.random_states(200_000, seed=313)
+    dq = torch.from_numpy(qs).cuda()
+    pf = torch.zeros(len(qs), dtype=torch.float32, device="cuda")
+    r = S.screen_states(uw, net, dq, 0.5, d_pfree=pf)
+    torch.cuda.synchronize()
+    uw.check_device()
+    truth = ow.valid_states(qs).astype(bool)
+    sel = S.selected_mask(r, len(qs)).cpu().numpy()
+    free = r["free"].cpu().numpy().astype(bool)
+    assert np.array_equal(free, sel & truth)
+    assert np.array_equal(sel, pf.cpu().numpy() > 0.5)
+    k = int(r["count"].item())
+    assert len(np.unique(r["idx"][:k].cpu().numpy())) == k
+    recall, precision = S.recall_precision(r, torch.from_numpy(truth).cuda())
+    _flush(f"prescreen: states recall {recall:.4f} precision {precision:.4f}")
+    assert recall >= 0.9 and precision > truth.mean()
+    _flush("prescreen: 2,000 segments")
+    a = _free(uw, 2000, 314)
+    steps = W.lognormal_steps(len(a), seed=315)
+    b = W.segment_ends(a, seed=316, lengths=steps * (1.0 / 1024))
+    da, db = torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda()
+    rs = S.screen_segments(uw, net, da, db, 1.0 / 1024, 0.5, points=8)
+    torch.cuda.synchronize()
+    uw.check_device()
+    tseg = np.array([ow.is_valid_transition(x, y, 1.0 / 1024) for x, y in zip(a, b)])
+    ssel = S.selected_mask(rs, len(a)).cpu().numpy()
+    sfree = rs["free"].cpu().numpy().astype(bool)
+    assert np.array_equal(sfree, ssel & tseg)
+    srecall, sprec = S.recall_precision(rs, torch.from_numpy(tseg).cuda())
+    _flush(f"prescreen: segments recall {srecall:.4f} precision {sprec:.4f}")
+    assert srecall >= 0.9 and 0.1 < tseg.mean() < 0.95
+
+
+def test_subset_lists_from_the_caller(shell):
+    """The subset entry points read their index list and count on the device:
+    entries outside [0, n) are skipped, a count above n is cut to n, and
+    entries of the output that are not listed keep their contents."""
+    import torch
+    uw, ow = shell
+    qs = W.random_states(64, seed=321)
+    dq = torch.from_numpy(qs).cuda()
+    idx = torch.tensor([5, -1, 64, 7, 63, 1000] + [0] * 58, dtype=torch.int32, device="cuda")
+    for cnt in (6, 500):
+        count = torch.tensor([cnt], dtype=torch.int32, device="cuda")
+        out = torch.full((64,), 9, dtype=torch.uint8, device="cuda")
+        uw.valid_states_subset_device(dq.data_ptr(), 64, idx.data_ptr(), count.data_ptr(), out.data_ptr())
+        torch.cuda.synchronize()
+        uw.check_device()
+        o = out.cpu().numpy()
+        want = ow.valid_states(qs).astype(np.uint8)
+        listed = [5, 7, 63] + ([0] if cnt > 6 else [])
+        for i in range(64):
+            assert o[i] == (want[i] if i in listed else 9), (cnt, i)
