@@ -11,11 +11,18 @@ decision made under the reference's semantics: every evaluated discrete step
 plus every bisection probe.  Inputs sit in HBM before the timed region.
 
 Prints one JSON line (metric, value in queries/s, roofline of the segment
-kernel, CPU-oracle baseline on a bounded sample).
+kernel, CPU-oracle baseline on a bounded sample).  The default run then
+measures the other configurations the same way and adds them under
+"workloads": 1M state validity (torus), 4M uneven segments and a 4,096-
+milestone visibility matrix (shell scene), and the learned prescreen (shell
+scene), each with its kernel time, roofline, oracle baseline and mismatch
+count (--headline-only skips them; --workload runs one of them alone).
 """
 import argparse
+import copy
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -34,6 +41,10 @@ LEVELS = 20
 C_BV = 286
 C_TRI = 753
 PEAK_FP32_TFLOPS = 157.3
+# The bound field takes "hbm" or "mfma"; these kernels are compute-bound on
+# the vector ALUs and issue no matrix instructions, so the roof they are
+# measured against is the fp32 vector peak, named here.
+VALU_ROOF = "fp32 VALU (vector ALUs; the kernel issues no matrix instructions)"
 # HBM bytes per launch of the touch kernel on this workload, from rocprofv3
 # --pmc FETCH_SIZE and WRITE_SIZE (KiB per dispatch), each scaled by what the
 # same counter reads for a 256 MiB device copy (FETCH_SIZE 131,094 KiB, i.e.
@@ -65,10 +76,10 @@ def build_workload(uw, W, n_seg, seed=0):
     return np.ascontiguousarray(starts), np.ascontiguousarray(ends)
 
 
-def cpu_baseline(ow, starts, ends, gpu_res, target_s):
+def cpu_baseline(ow, starts, ends, gpu_res, target_s, single_s):
     """Time the oracle on a bounded sample of the same segments (all threads
-    OMP allows) and check the GPU's answers for that sample."""
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    OMP allows, then one thread) and check the GPU's answers for that sample."""
+    threads = _sample_threads()
     n = 256
     t0 = time.perf_counter()
     o = ow.touch_batch(starts[:n], ends[:n], VD, LEVELS, nthreads=threads)
@@ -96,9 +107,14 @@ def cpu_baseline(ow, starts, ends, gpu_res, target_s):
         q_all += int(o["queries"].sum())
         passes += 1
     what = f"{passes} passes over " if passes > 1 else ""
-    return dict(value=q_all / dt_all, unit="queries/s", cores=threads, kind="port",
-                sample=f"{what}first {n} of the {len(starts)} touch segments ({q} queries per pass, "
-                       f"{dt_all:.1f} s)"), mism, n
+
+    def one(k):
+        sl = slice((k % 256) * 16, (k % 256 + 1) * 16)
+        return int(ow.touch_batch(starts[sl], ends[sl], VD, LEVELS, nthreads=1)["queries"].sum())
+    st, k = _single_thread(one, single_s)
+    return _cpu_baseline(q_all / dt_all, "queries/s",
+                         f"{what}first {n} of the {len(starts)} touch segments ({q} queries per pass, "
+                         f"{dt_all:.1f} s)", st, f"{k} chunks of 16 of the same segments, one thread"), mism, n
 
 
 def oracle_flops_per_query(ow, starts, ends, n, vd=VD):
@@ -127,7 +143,7 @@ def _compute_roofline(flops_q, nbv, ntri, queries, kern_ms, kernel, sample):
     tests on `sample`) over the kernel's time, against the fp32 peak.  HBM
     traffic is counted for the headline kernel only (null here)."""
     achieved = queries * flops_q / (kern_ms * 1e-3) / 1e12
-    return dict(bound="mfma", achieved=round(achieved, 3), peak=PEAK_FP32_TFLOPS, unit="TFLOP/s",
+    return dict(bound="mfma", roof=VALU_ROOF, achieved=round(achieved, 3), peak=PEAK_FP32_TFLOPS, unit="TFLOP/s",
                 frac=round(achieved / PEAK_FP32_TFLOPS, 5), traffic=None,
                 note="compute-bound on vector ALUs (no matrix instructions); the compute roof is MI355X's fp32 "
                      "peak; algorithmic flops = oracle node/triangle test counts x their straight-line binary64 "
@@ -173,8 +189,75 @@ def _line(workload, value, wall, args, extra):
     return line
 
 
+_HOST = None
+
+
+def host_cpu():
+    """The host CPU the oracle's baseline runs on: lscpu's model and topology
+    (of the whole host), the CPUs this process may use (a container or
+    cgroup slice may hold fewer) and the OpenMP settings in force."""
+    global _HOST
+    if _HOST is not None:
+        return _HOST
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            v = v.strip()
+            if k == "Model name":
+                info["model"] = v
+            elif k == "Socket(s)":
+                info["sockets"] = int(v)
+            elif k == "Core(s) per socket":
+                info["cores_per_socket"] = int(v)
+            elif k == "Thread(s) per core":
+                info["threads_per_core"] = int(v)
+            elif k == "CPU(s)":
+                info["cpus"] = int(v)
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
+    if "model" not in info:
+        try:
+            with open("/proc/cpuinfo") as f:
+                for line in f:
+                    if line.startswith("model name"):
+                        info["model"] = line.split(":", 1)[1].strip()
+                        break
+        except OSError:
+            pass
+    tpc = info.get("threads_per_core", 1)
+    info["smt"] = "on" if tpc > 1 else "off"
+    info["cpus_usable"] = len(os.sched_getaffinity(0))
+    info["physical_cores_usable"] = max(1, info["cpus_usable"] // max(tpc, 1))
+    info["omp"] = {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OMP_PROC_BIND", "OMP_PLACES")}
+    _HOST = info
+    return info
+
+
 def _sample_threads():
-    return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    """Threads of the oracle's multi-threaded baseline: OMP_NUM_THREADS when
+    set, else the physical cores this process may use."""
+    if os.environ.get("OMP_NUM_THREADS"):
+        return int(os.environ["OMP_NUM_THREADS"])
+    return host_cpu()["physical_cores_usable"]
+
+
+def _single_thread(fn, budget_s):
+    """fn(k) runs chunk k of a workload on one thread and returns its units;
+    chunks run until budget_s seconds are spent.  Returns (rate, chunks)."""
+    done, dt, k = 0, 0.0, 0
+    while dt < budget_s:
+        t0 = time.perf_counter()
+        done += fn(k)
+        dt += time.perf_counter() - t0
+        k += 1
+    return done / dt, k
+
+
+def _cpu_baseline(value, unit, sample, single, single_sample):
+    return dict(value=value, unit=unit, cores=_sample_threads(), kind="port", sample=sample,
+                single_thread=dict(value=single, unit=unit, sample=single_sample), host=host_cpu())
 
 
 def run_states(args, torch, pyosr, W):
@@ -218,8 +301,12 @@ def run_states(args, torch, pyosr, W):
             passes += 1
         extra["mismatches"] = mism
         what = f"{passes} passes over " if passes > 1 else ""
-        extra["cpu_baseline"] = dict(value=done_q / dt, unit="queries/s", cores=_sample_threads(), kind="port",
-                                     sample=f"{what}first {m} of the {n} states ({dt:.2f} s)")
+        nch = max(1, min(m, n) // 2000)
+        st, k = _single_thread(lambda k: int(ow.valid_states(qs[(k % nch) * 2000:(k % nch + 1) * 2000],
+                                                               nthreads=1).size), args.single_seconds)
+        extra["cpu_baseline"] = _cpu_baseline(done_q / dt, "queries/s",
+                                              f"{what}first {m} of the {n} states ({dt:.2f} s)", st,
+                                              f"{k} chunks of 2,000 of the same states, one thread")
         fq = C_BV * nbv.mean() + C_TRI * ntri.mean()
         extra["roofline"] = _compute_roofline(float(fq), float(nbv.mean()), float(ntri.mean()), n, kern_ms,
                                               "k_valid_states", f"the first {m} states")
@@ -285,8 +372,14 @@ def run_uneven(args, torch, pyosr, W):
             cq += int(o["queries"].sum())
             done_n = sl.stop
         extra["mismatches"] = mism
-        extra["cpu_baseline"] = dict(value=cq / dt, unit="queries/s", cores=_sample_threads(), kind="port",
-                                     sample=f"first {done_n} of the {n} segments ({cq} queries, {dt:.2f} s)")
+
+        def one(k):
+            sl = slice((k % 512) * 32, (k % 512 + 1) * 32)
+            return int(ow.touch_batch(starts[sl], ends[sl], vd, 0, nthreads=1)["queries"].sum())
+        st, k = _single_thread(one, args.single_seconds)
+        extra["cpu_baseline"] = _cpu_baseline(cq / dt, "queries/s",
+                                              f"first {done_n} of the {n} segments ({cq} queries, {dt:.2f} s)", st,
+                                              f"{k} chunks of 32 of the same segments, one thread")
         fq, nbv, ntri = oracle_flops_per_query(ow, starts, ends, 1024)
         roof = _compute_roofline(fq, nbv, ntri, q, kern_ms, "k_segments<transit>",
                                  "the states the first 1024 segments evaluate")
@@ -344,9 +437,18 @@ def run_visibility(args, torch, pyosr, W):
             qs_cpu += int(steps[r * (2 * n - r - 1) // 2 + (cc - r - 1)].sum())
             k += 1
         extra["mismatches"] = mism
-        extra["cpu_baseline"] = dict(value=qs_cpu / dt, unit="queries/s", cores=_sample_threads(), kind="port",
-                                     sample=f"{k} diagonal 64 x 64 blocks of the {n} x {n} matrix "
-                                            f"({qs_cpu} queries, {dt:.2f} s)")
+        r16, c16 = np.triu_indices(16, 1)
+
+        def one(kk):
+            lo = (kk % (n // 16)) * 16
+            ow.visibility_matrix(ms[lo:lo + 16], True, VD, nthreads=1)
+            rr, cc = r16 + lo, c16 + lo
+            return int(steps[rr * (2 * n - rr - 1) // 2 + (cc - rr - 1)].sum())
+        st, kk = _single_thread(one, args.single_seconds)
+        extra["cpu_baseline"] = _cpu_baseline(qs_cpu / dt, "queries/s",
+                                              f"{k} diagonal 64 x 64 blocks of the {n} x {n} matrix "
+                                              f"({qs_cpu} queries, {dt:.2f} s)", st,
+                                              f"{kk} diagonal 16 x 16 blocks, one thread")
         # states the pairs of the first rows evaluate (all at verify step
         # 1/1024 unless a pair is closer than 2/1024)
         fi, ti = np.triu_indices(n, 1)
@@ -428,8 +530,16 @@ def run_batch(args, torch, pyosr, W):
             cq += int(o["queries"].sum()) + (sq.stop - sq.start)
             ks, kq = ss.stop, sq.stop
         extra["mismatches"] = mism
-        extra["cpu_baseline"] = dict(value=cq / dt, unit="queries/s", cores=_sample_threads(), kind="port",
-                                     sample=f"first {ks} segments and {kq} states ({cq} queries, {dt:.2f} s)")
+
+        def one(k):
+            ss, sq = slice((k % 256) * 32, (k % 256 + 1) * 32), slice((k % 256) * 8, (k % 256 + 1) * 8)
+            o = ow.touch_batch(starts[ss], ends[ss], vd, 0, nthreads=1)
+            ow.valid_states(qs[sq], nthreads=1)
+            return int(o["queries"].sum()) + 8
+        st, k1 = _single_thread(one, args.single_seconds)
+        extra["cpu_baseline"] = _cpu_baseline(cq / dt, "queries/s",
+                                              f"first {ks} segments and {kq} states ({cq} queries, {dt:.2f} s)", st,
+                                              f"{k1} chunks of 32 segments and 8 states, one thread")
     line = _line(f"mixed batch, one persistent launch: shell env {len(env[1])} tris / bent rod {len(rob[1])} tris, "
                  f"{n} segments with lognormal(5, 1.2) step counts in [8, 4096] at step 1/1024 plus {m} single "
                  f"states", q * args.steps / wall, wall, args, extra)
@@ -502,8 +612,15 @@ def run_ompl(args, torch, pyosr, W):
             mism += int((c != out[idx]).sum() + (cs != sam[idx]).sum())
             done += len(idx)
         extra["mismatches"] = mism
-        extra["cpu_baseline"] = dict(value=done / dt, unit="motions/s", cores=_sample_threads(), kind="port",
-                                     sample=f"{done} random pairs of the {npairs} ({dt:.2f} s)")
+
+        def one(k):
+            idx = perm[(k % 64) * 64:(k % 64 + 1) * 64]
+            ow.ompl_validate_motions(keys[fi[idx]], keys[ti[idx]], drv.longest_valid_segment, nthreads=1)
+            return len(idx)
+        st, k1 = _single_thread(one, args.single_seconds)
+        extra["cpu_baseline"] = _cpu_baseline(done / dt, "motions/s",
+                                              f"{done} random pairs of the {npairs} ({dt:.2f} s)", st,
+                                              f"{k1} chunks of 64 of the same pairs, one thread")
     line = _line(f"OMPL key screening: alpha puzzle (reference meshes), {n} free keys, {npairs} motion pairs "
                  f"(np.tril_indices), resolution 1e-4", npairs * args.steps / wall, wall, args, extra)
     line["unit"] = "motions/s"
@@ -607,8 +724,14 @@ def run_clearance(args, torch, pyosr, W):
             mism += int((np.array([O.distance(unit[k], f) for f in free]) != osc[k, :16]).sum())
         print(f"clearance: oracle sample done ({dt:.1f} s)", file=sys.stderr, flush=True)
         extra["mismatches"] = mism
-        extra["cpu_baseline"] = dict(value=cq / dt, unit="queries/s", cores=_sample_threads(), kind="port",
-                                     sample=f"{done_n} random touch queries of the {n} ({cq} queries, {dt:.2f} s)")
+
+        def one(k):
+            idx = perm[(k % 64) * 8:(k % 64 + 1) * 8]
+            return int(ow.touch_batch(starts[idx], ends[idx], vd, 0, nthreads=1)["queries"].sum())
+        st, k1 = _single_thread(one, args.single_seconds)
+        extra["cpu_baseline"] = _cpu_baseline(cq / dt, "queries/s",
+                                              f"{done_n} random touch queries of the {n} ({cq} queries, {dt:.2f} s)",
+                                              st, f"{k1} chunks of 8 of the same queries, one thread")
         fq, nbv, ntri = oracle_flops_per_query(ow, starts[perm[:32]], ends[perm[:32]], 32, vd)
         extra["roofline"] = _compute_roofline(fq, nbv, ntri, q, kern_ms, "k_segments<transit>",
                                               "the states of 32 random touch queries")
@@ -739,6 +862,32 @@ def run_surrogate(args, torch, pyosr, W):
         },
         "surrogate_segments": seg,
     }
+    if not args.no_cpu:
+        # the oracle's exact check of the same random samples: verified free
+        # samples per second on the host, and parity with the GPU's labels
+        from oracle import oracle as O
+        O.set_libm(False)
+        ow = O.OracleWorld().setup(*env, *rob)
+        hq = sq[: 4 * 1024 * 1024].cpu().numpy()
+        hy = sy[: 4 * 1024 * 1024].cpu().numpy().astype(bool)
+        m, dt, fr, mism = 0, 0.0, 0, 0
+        while dt < args.cpu_seconds and m < len(hq):
+            sl = slice(m, min(len(hq), m + 100_000))
+            t0 = time.perf_counter()
+            c = ow.valid_states(hq[sl], nthreads=_sample_threads()).astype(bool)
+            dt += time.perf_counter() - t0
+            fr += int(c.sum())
+            mism += int((c != hy[sl]).sum())
+            m = sl.stop
+
+        def one(k):
+            sl = slice((k % 64) * 1000, (k % 64 + 1) * 1000)
+            return int(ow.valid_states(hq[sl], nthreads=1).sum())
+        st, k1 = _single_thread(one, args.single_seconds)
+        extra["mismatches"] += mism
+        extra["cpu_baseline"] = _cpu_baseline(fr / dt, "verified free samples/s",
+                                              f"exact check of the first {m} screened samples ({dt:.2f} s)", st,
+                                              f"{k1} chunks of 1,000 of the same samples, one thread")
     line = _line(f"learned prescreen: shell env {len(env[1])} tris / bent rod {len(rob[1])} tris, "
                  f"{n_train} exact labels, {n_screen} screened samples", rate_scr, wall_scr, args, extra)
     line["unit"] = "verified free samples/s"
@@ -747,59 +896,9 @@ def run_surrogate(args, torch, pyosr, W):
     return line
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--segments", type=int, default=None,
-                    help="queries per launch (default: the workload's own size)")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--batch-states", type=int, default=1024 * 1024,
-                    help="batch workload: single states in the launch beside the segments")
-    ap.add_argument("--workload", choices=["touch", "states", "uneven", "visibility", "surrogate", "ompl", "batch",
-                                           "clearance"],
-                    default="touch",
-                    help="touch (default, the headline): touch-configuration search on the torus scene; "
-                         "states: 1M state validity; uneven: 4M lognormal segments on the shell scene; "
-                         "visibility: 4,096-milestone matrix on the shell scene; "
-                         "surrogate: learned prescreen on the shell scene; "
-                         "ompl: OMPL key screening (all pairs of 1,024 keys) on the alpha puzzle; "
-                         "batch: 4M uneven segments + 1M states in one persistent launch (shell scene); "
-                         "clearance: key clearance (1,024 keys x 128 touch samples) on the alpha puzzle")
-    ap.add_argument("--train-epochs", type=int, default=2)
-    ap.add_argument("--threshold", type=float, default=0.5)
-    ap.add_argument("--screen-segments", type=int, default=8 * 1024 * 1024,
-                    help="surrogate workload: segments screened after the samples (0 = none)")
-    ap.add_argument("--copy-mib", type=int, default=0,
-                    help="first copy a buffer of this many MiB three times (a known-size "
-                         "kernel that calibrates the FETCH_SIZE / WRITE_SIZE counters)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
-              "(each query is independent; see DESIGN.md for how the path shards).", file=sys.stderr)
-        sys.exit(2)
-
-    import torch
-
-    from puzzletunneldiscovery_amd import pyosr, workloads as W
-
-    assert torch.cuda.is_available(), "bench.py needs a GPU"
-    torch.cuda.set_device(0)
-    if args.copy_mib:
-        src = torch.ones(args.copy_mib << 18, dtype=torch.float32, device="cuda:0")
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
-    if args.workload != "touch":
-        fn = {"states": run_states, "uneven": run_uneven, "visibility": run_visibility,
-              "surrogate": run_surrogate, "ompl": run_ompl, "batch": run_batch,
-              "clearance": run_clearance}[args.workload]
-        print(json.dumps(fn(args, torch, pyosr, W)), flush=True)
-        return
+def run_touch(args, torch, pyosr, W):
+    """The headline: touch-configuration search over 65,536 segments of the
+    torus scene (one step = one launch)."""
     env, rob = W.torus_scene(0)
     uw = pyosr.create_unit_world_from_arrays(env, rob)
     starts, ends = build_workload(uw, W, args.segments or N_SEG)
@@ -866,13 +965,13 @@ def main():
         from oracle import oracle as O
         O.set_libm(False)
         ow = O.OracleWorld().setup(*env, *rob)
-        cpu, mism, n_checked = cpu_baseline(ow, starts, ends, res, args.cpu_seconds)
+        cpu, mism, n_checked = cpu_baseline(ow, starts, ends, res, args.cpu_seconds, args.single_seconds)
         flops_q, nbv, ntri = oracle_flops_per_query(ow, starts, ends, min(2048, n))
     roof = None
     if flops_q is not None:
         achieved = queries * flops_q / (kern_ms * 1e-3) / 1e12
         traffic = int((TRAFFIC_FETCH_KIB * FETCH_SCALE + TRAFFIC_WRITE_KIB * WRITE_SCALE) * 1024)
-        roof = dict(bound="mfma", achieved=round(achieved, 3), peak=PEAK_FP32_TFLOPS, unit="TFLOP/s",
+        roof = dict(bound="mfma", roof=VALU_ROOF, achieved=round(achieved, 3), peak=PEAK_FP32_TFLOPS, unit="TFLOP/s",
                     frac=round(achieved / PEAK_FP32_TFLOPS, 5), traffic=traffic,
                     note="compute-bound on vector ALUs (no matrix instructions); the compute roof is "
                          "MI355X's fp32 peak; algorithmic flops = oracle node/triangle test counts x "
@@ -902,6 +1001,107 @@ def main():
         "cpu_baseline": cpu,
         "pcie_inclusive": pcie,
     }
+    return line
+
+
+SUB_WORKLOADS = (
+    # name, function, steps cap, warmup cap: each config timed over a few
+    # launches so the whole default run stays within a few minutes
+    ("states", "run_states", 20, 3),
+    ("uneven", "run_uneven", 3, 1),
+    ("visibility", "run_visibility", 2, 1),
+    ("surrogate", "run_surrogate", 3, 1),
+)
+SUB_KEYS = ("value", "unit", "metric", "steps", "warmup", "ms_per_step", "kernel", "kernel_ms", "mismatches",
+            "roofline", "cpu_baseline", "dtype", "config", "surrogate", "surrogate_segments")
+
+
+def run_all_configs(args, torch, pyosr, W):
+    """The other configurations, each measured as its own --workload run
+    would, with the steps capped; returned as one object per workload."""
+    out = {}
+    for name, fn, steps_cap, warm_cap in SUB_WORKLOADS:
+        sub = copy.copy(args)
+        sub.steps = max(1, min(args.steps, steps_cap))
+        sub.warmup = min(args.warmup, warm_cap)
+        sub.segments = None
+        t0 = time.perf_counter()
+        print(f"bench: {name} ...", file=sys.stderr, flush=True)
+        r = globals()[fn](sub, torch, pyosr, W)
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
+        print(f"bench: {name} done ({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+        out[name] = {k: r[k] for k in SUB_KEYS if k in r}
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--segments", type=int, default=None,
+                    help="queries per launch (default: the workload's own size)")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--single-seconds", type=float, default=3.0,
+                    help="oracle seconds of each single-thread baseline")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="only the touch-search headline (the default run also measures the states, uneven, "
+                         "visibility and surrogate configurations)")
+    ap.add_argument("--batch-states", type=int, default=1024 * 1024,
+                    help="batch workload: single states in the launch beside the segments")
+    ap.add_argument("--workload", choices=["touch", "states", "uneven", "visibility", "surrogate", "ompl", "batch",
+                                           "clearance"],
+                    default="touch",
+                    help="touch (default, the headline): touch-configuration search on the torus scene; "
+                         "states: 1M state validity; uneven: 4M lognormal segments on the shell scene; "
+                         "visibility: 4,096-milestone matrix on the shell scene; "
+                         "surrogate: learned prescreen on the shell scene; "
+                         "ompl: OMPL key screening (all pairs of 1,024 keys) on the alpha puzzle; "
+                         "batch: 4M uneven segments + 1M states in one persistent launch (shell scene); "
+                         "clearance: key clearance (1,024 keys x 128 touch samples) on the alpha puzzle")
+    ap.add_argument("--train-epochs", type=int, default=2)
+    ap.add_argument("--threshold", type=float, default=0.5)
+    ap.add_argument("--screen-segments", type=int, default=8 * 1024 * 1024,
+                    help="surrogate workload: segments screened after the samples (0 = none)")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="first copy a buffer of this many MiB three times (a known-size "
+                         "kernel that calibrates the FETCH_SIZE / WRITE_SIZE counters)")
+    args = ap.parse_args()
+    # the oracle's OpenMP threads stay on their cores (read when its library
+    # loads, so set before any oracle import)
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
+    # read the CPU set before OpenMP binds this thread to its first place
+    host_cpu()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(each query is independent; see DESIGN.md for how the path shards).", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from puzzletunneldiscovery_amd import pyosr, workloads as W
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.set_device(0)
+    if args.copy_mib:
+        src = torch.ones(args.copy_mib << 18, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+    if args.workload != "touch":
+        fn = {"states": run_states, "uneven": run_uneven, "visibility": run_visibility,
+              "surrogate": run_surrogate, "ompl": run_ompl, "batch": run_batch,
+              "clearance": run_clearance}[args.workload]
+        print(json.dumps(fn(args, torch, pyosr, W)), flush=True)
+        return
+    line = run_touch(args, torch, pyosr, W)
+    if not args.headline_only:
+        line["workloads"] = run_all_configs(args, torch, pyosr, W)
     print(json.dumps(line), flush=True)
 
 
