@@ -79,7 +79,10 @@ def build_workload(uw, W, n_seg, seed=0):
 def cpu_baseline(ow, starts, ends, gpu_res, target_s, single_s):
     """Time the oracle on a bounded sample of the same segments (all threads
     OMP allows, then one thread) and check the GPU's answers for that sample."""
+    from oracle import oracle as O
     threads = _sample_threads()
+    # timed with the host's C library, as the reference runs
+    O.set_libm(1)
     n = 256
     t0 = time.perf_counter()
     o = ow.touch_batch(starts[:n], ends[:n], VD, LEVELS, nthreads=threads)
@@ -90,13 +93,6 @@ def cpu_baseline(ow, starts, ends, gpu_res, target_s, single_s):
         o = ow.touch_batch(starts[:n], ends[:n], VD, LEVELS, nthreads=threads)
         dt = time.perf_counter() - t0
     q = int(o["queries"].sum())
-    mism = 0
-    for key, gk in (("done", "done"), ("lo", "lo"), ("hi", "hi"), ("queries", "steps")):
-        a = o[key]
-        b = gpu_res[gk][:n]
-        if key == "done":
-            a = a.astype(bool)
-        mism += int((a != b).sum())
     # a workload shorter than the budget is timed again until the budget is
     # spent, so the rate is not a fraction-of-a-second reading
     passes, q_all, dt_all = 1, q, dt
@@ -112,9 +108,20 @@ def cpu_baseline(ow, starts, ends, gpu_res, target_s, single_s):
         sl = slice((k % 256) * 16, (k % 256 + 1) * 16)
         return int(ow.touch_batch(starts[sl], ends[sl], VD, LEVELS, nthreads=1)["queries"].sum())
     st, k = _single_thread(one, single_s)
+    # parity: the correctly rounded restatement on the same segments
+    O.set_libm(0)
+    o = ow.touch_batch(starts[:n], ends[:n], VD, LEVELS, nthreads=threads)
+    mism = 0
+    for key, gk in (("done", "done"), ("lo", "lo"), ("hi", "hi"), ("queries", "steps")):
+        a = o[key]
+        b = gpu_res[gk][:n]
+        if key == "done":
+            a = a.astype(bool)
+        mism += int((a != b).sum())
     return _cpu_baseline(q_all / dt_all, "queries/s",
                          f"{what}first {n} of the {len(starts)} touch segments ({q} queries per pass, "
-                         f"{dt_all:.1f} s)", st, f"{k} chunks of 16 of the same segments, one thread"), mism, n
+                         f"{dt_all:.1f} s; the host C library's sin/acos/atan2, as the reference runs)", st,
+                         f"{k} chunks of 16 of the same segments, one thread"), mism, n
 
 
 def oracle_flops_per_query(ow, starts, ends, n, vd=VD):
@@ -362,23 +369,31 @@ def run_uneven(args, torch, pyosr, W):
         ow = O.OracleWorld().setup(*env, *rob)
         lt = d_lt.cpu().numpy()
         done_n, cq, mism, dt = 0, 0, 0, 0.0
+        O.set_libm(1)  # timed with the host's C library, as the reference runs
         while dt < args.cpu_seconds and done_n < n:
             sl = slice(done_n, min(n, done_n + 16384))
             t0 = time.perf_counter()
             o = ow.touch_batch(starts[sl], ends[sl], vd, 0, nthreads=_sample_threads())
             dt += time.perf_counter() - t0
-            mism += int((o["done"].astype(bool) != done[sl]).sum() + (o["queries"] != steps[sl]).sum()
-                        + (o["last_tau"] != lt[sl]).sum())
             cq += int(o["queries"].sum())
             done_n = sl.stop
+        O.set_libm(0)  # parity with the correctly rounded restatement
+        par = min(done_n, 65536)
+        o = ow.touch_batch(starts[:par], ends[:par], vd, 0, nthreads=_sample_threads())
+        mism = int((o["done"].astype(bool) != done[:par]).sum() + (o["queries"] != steps[:par]).sum()
+                   + (o["last_tau"] != lt[:par]).sum())
         extra["mismatches"] = mism
+        extra["parity_sample"] = f"the first {par} segments"
+        O.set_libm(1)
 
         def one(k):
             sl = slice((k % 512) * 32, (k % 512 + 1) * 32)
             return int(ow.touch_batch(starts[sl], ends[sl], vd, 0, nthreads=1)["queries"].sum())
         st, k = _single_thread(one, args.single_seconds)
+        O.set_libm(0)
         extra["cpu_baseline"] = _cpu_baseline(cq / dt, "queries/s",
-                                              f"first {done_n} of the {n} segments ({cq} queries, {dt:.2f} s)", st,
+                                              f"first {done_n} of the {n} segments ({cq} queries, {dt:.2f} s; the "
+                                              "host C library's sin/acos/atan2, as the reference runs)", st,
                                               f"{k} chunks of 32 of the same segments, one thread")
         fq, nbv, ntri = oracle_flops_per_query(ow, starts, ends, 1024)
         roof = _compute_roofline(fq, nbv, ntri, q, kern_ms, "k_segments<transit>",
@@ -427,16 +442,23 @@ def run_visibility(args, torch, pyosr, W):
         flat = np.unpackbits(bits.view(np.uint8), bitorder="little")[: n * n].reshape(n, n)
         blk, k, dt, mism, qs_cpu = 64, 0, 0.0, 0, 0
         row, col = np.triu_indices(blk, 1)
+        O.set_libm(1)  # timed with the host's C library, as the reference runs
         while dt < args.cpu_seconds and (k + 1) * blk <= n:
             lo = k * blk
             t0 = time.perf_counter()
             c = ow.visibility_matrix(ms[lo:lo + blk], True, VD, nthreads=_sample_threads())
             dt += time.perf_counter() - t0
-            mism += int((flat[lo:lo + blk, lo:lo + blk].astype(np.int8) != c).sum())
             r, cc = row + lo, col + lo
             qs_cpu += int(steps[r * (2 * n - r - 1) // 2 + (cc - r - 1)].sum())
             k += 1
+        O.set_libm(0)  # parity with the correctly rounded restatement
+        for kb in range(min(k, 8)):
+            lo = kb * blk
+            c = ow.visibility_matrix(ms[lo:lo + blk], True, VD, nthreads=_sample_threads())
+            mism += int((flat[lo:lo + blk, lo:lo + blk].astype(np.int8) != c).sum())
         extra["mismatches"] = mism
+        extra["parity_sample"] = f"the first {min(k, 8)} diagonal 64 x 64 blocks"
+        O.set_libm(1)
         r16, c16 = np.triu_indices(16, 1)
 
         def one(kk):
@@ -445,6 +467,7 @@ def run_visibility(args, torch, pyosr, W):
             rr, cc = r16 + lo, c16 + lo
             return int(steps[rr * (2 * n - rr - 1) // 2 + (cc - rr - 1)].sum())
         st, kk = _single_thread(one, args.single_seconds)
+        O.set_libm(0)
         extra["cpu_baseline"] = _cpu_baseline(qs_cpu / dt, "queries/s",
                                               f"{k} diagonal 64 x 64 blocks of the {n} x {n} matrix "
                                               f"({qs_cpu} queries, {dt:.2f} s)", st,
@@ -1012,7 +1035,7 @@ SUB_WORKLOADS = (
     ("visibility", "run_visibility", 2, 1),
     ("surrogate", "run_surrogate", 3, 1),
 )
-SUB_KEYS = ("value", "unit", "metric", "steps", "warmup", "ms_per_step", "kernel", "kernel_ms", "mismatches",
+SUB_KEYS = ("value", "unit", "metric", "steps", "warmup", "ms_per_step", "kernel", "kernel_ms", "mismatches", "parity_sample",
             "roofline", "cpu_baseline", "dtype", "config", "surrogate", "surrogate_segments")
 
 

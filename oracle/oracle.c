@@ -14,6 +14,8 @@
 #include "oracle.h"
 #include "../puzzletunneldiscovery_amd/csrc/se3_math.h"
 
+#include <stdint.h>
+
 #include <float.h>
 #include <math.h>
 #include <stdio.h>
@@ -26,13 +28,21 @@
 /* ------------------------------------------------------------------ */
 /* libm selection                                                     */
 /* ------------------------------------------------------------------ */
-static int g_glibc = 0;
-void or_set_libm(int use_glibc) { g_glibc = use_glibc ? 1 : 0; }
-int or_get_libm(void) { return g_glibc; }
-static double o_sin(double x) { return g_glibc ? sin(x) : se3_sin(x); }
-static double o_cos(double x) { return g_glibc ? cos(x) : se3_cos(x); }
-static double o_acos(double x) { return g_glibc ? acos(x) : se3_acos(x); }
-static double o_atan2(double y, double x) { return g_glibc ? atan2(y, x) : se3_atan2(y, x); }
+/* 0: correctly rounded sin, cos, acos, atan2 (the reference platform's
+ * glibc 2.27 rounds these correctly; se3_crmath.h computes them);
+ * 1: this host's glibc;
+ * 2: as 0, except that the states a walk only tests (not the ones it
+ *    returns) get their slerp weights from se3_sin_legacy, as the GPU
+ *    kernels compute them inside their traversals. */
+static int g_libm = 0;
+void or_set_libm(int mode) { g_libm = (mode == 1 || mode == 2) ? mode : 0; }
+int or_get_libm(void) { return g_libm; }
+static double o_sin(double x) { return g_libm == 1 ? sin(x) : se3_sin(x); }
+static double o_cos(double x) { return g_libm == 1 ? cos(x) : se3_cos(x); }
+static double o_acos(double x) { return g_libm == 1 ? acos(x) : se3_acos(x); }
+static double o_atan2(double y, double x) { return g_libm == 1 ? atan2(y, x) : se3_atan2(y, x); }
+/* the sine of the weights of a state that is only tested */
+static double o_sin_tested(double x) { return g_libm == 2 ? se3_sin_legacy(x) : o_sin(x); }
 
 /* ------------------------------------------------------------------ */
 /* Data structures                                                    */
@@ -178,8 +188,19 @@ double or_distance(const double* a, const double* b)
 	return trdist + rotdist;
 }
 
-/* interpolate (lib/osr/osr_state.cc:27-44) with Eigen's slerp. */
-void or_interpolate(const double* a, const double* b, double tau, double* out)
+/* interpolate (lib/osr/osr_state.cc:27-44) with Eigen's slerp; tested_only
+ * selects the weights' sine of o_sin_tested. */
+static void interp(const double* a, const double* b, double tau, double* out, int tested_only);
+
+void or_interpolate(const double* a, const double* b, double tau, double* out) { interp(a, b, tau, out, 0); }
+
+/* the state a walk tests at tau (or_interpolate outside libm mode 2) */
+static void or_interpolate_tested(const double* a, const double* b, double tau, double* out)
+{
+	interp(a, b, tau, out, 1);
+}
+
+static void interp(const double* a, const double* b, double tau, double* out, int tested_only)
 {
 	double omt = 1.0 - tau;
 	for (int i = 0; i < 3; i++)
@@ -197,8 +218,13 @@ void or_interpolate(const double* a, const double* b, double tau, double* out)
 	} else {
 		double theta = o_acos(absd);
 		double st = o_sin(theta);
-		s0 = o_sin((1.0 - tau) * theta) / st;
-		s1 = o_sin(tau * theta) / st;
+		if (tested_only) {
+			s0 = o_sin_tested((1.0 - tau) * theta) / st;
+			s1 = o_sin_tested(tau * theta) / st;
+		} else {
+			s0 = o_sin((1.0 - tau) * theta) / st;
+			s1 = o_sin(tau * theta) / st;
+		}
 	}
 	if (d < 0.0)
 		s1 = -s1;
@@ -1387,7 +1413,9 @@ void or_transit(const OrWorld* w, const double* from, const double* to, double v
 		double tau = delta * inv_dist;
 		or_interpolate(from, to, tau, st);
 		steps++;
-		if (!or_is_valid(w, st)) {
+		double tested[7];
+		or_interpolate_tested(from, to, tau, tested);
+		if (!or_is_valid(w, tested)) {
 			memcpy(last_free, lf, sizeof(lf));
 			memcpy(state, st, sizeof(st));
 			*done = 0;
@@ -1435,7 +1463,7 @@ void or_bisect(const OrWorld* w, const double* from, const double* to,
 	double st[7];
 	for (int i = 0; i < levels; i++) {
 		double mid = 0.5 * (lo + hi);
-		or_interpolate(from, to, mid, st);
+		or_interpolate_tested(from, to, mid, st);
 		if (or_is_valid(w, st))
 			lo = mid;
 		else
@@ -1589,15 +1617,15 @@ double or_ompl_longest_segment(const double* lo, const double* hi, double resolu
 	return ext * resolution;
 }
 
-void or_ompl_interpolate(const double* from, const double* to, double t, double* out)
+static void ompl_interp(const double* from, const double* to, double t, double* out, int tested_only)
 {
 	for (int i = 0; i < 3; i++)
 		out[i] = from[i] + (to[i] - from[i]) * t;
 	double theta = ompl_arc(from, to);
 	if (theta > DBL_EPSILON) {
 		double d = 1.0 / o_sin(theta);
-		double s0 = o_sin((1.0 - t) * theta);
-		double s1 = o_sin(t * theta);
+		double s0 = tested_only ? o_sin_tested((1.0 - t) * theta) : o_sin((1.0 - t) * theta);
+		double s1 = tested_only ? o_sin_tested(t * theta) : o_sin(t * theta);
 		double dq = from[3] * to[3] + from[4] * to[4] + from[5] * to[5] + from[6] * to[6];
 		if (dq < 0)
 			s1 = -s1;
@@ -1607,6 +1635,11 @@ void or_ompl_interpolate(const double* from, const double* to, double t, double*
 		for (int k = 3; k < 7; k++)
 			out[k] = from[k];
 	}
+}
+
+void or_ompl_interpolate(const double* from, const double* to, double t, double* out)
+{
+	ompl_interp(from, to, t, out, 0);
 }
 
 int or_ompl_is_valid(const OrWorld* w, const double* q)
@@ -1642,7 +1675,7 @@ int or_ompl_check_motion(const OrWorld* w, const double* s1, const double* s2, d
 	while (head < tail) {
 		int64_t first = qa[2 * (head % nd)], second = qa[2 * (head % nd) + 1];
 		int64_t mid = (first + second) / 2;
-		or_ompl_interpolate(s1, s2, (double)mid / (double)nd, test);
+		ompl_interp(s1, s2, (double)mid / (double)nd, test, 1);
 		if (!or_ompl_is_valid(w, test)) {
 			result = 0;
 			break;
@@ -1760,6 +1793,12 @@ void or_visibility_pair(const OrWorld* w, const double* qs0, const double* qs1, 
 /* ------------------------------------------------------------------ */
 /* The portable transcendental functions, exported for their tests.   */
 /* ------------------------------------------------------------------ */
+/* The slow (256-bit) paths and the legacy sine on their own, for tests. */
+double or_cr_sin_slow(double x) { return se3_cr_sincos_slow(x, 0); }
+double or_cr_cos_slow(double x) { return se3_cr_sincos_slow(x, 1); }
+double or_cr_acos_slow(double x) { return se3_cr_acos_slow(x); }
+double or_cr_atan2_slow(double y, double x) { return se3_cr_atan2_slow(y, x); }
+double or_se3_sin_legacy(double x) { return se3_sin_legacy(x); }
 double or_se3_sin(double x) { return se3_sin(x); }
 double or_se3_cos(double x) { return se3_cos(x); }
 double or_se3_acos(double x) { return se3_acos(x); }

@@ -36,9 +36,11 @@ extern "C" {
 
 typedef struct OrWorld OrWorld;
 
-/* libm used for slerp/angularDistance: 0 = the library's portable
- * se3_sin/se3_acos/se3_atan2 (bit-identical to the GPU), 1 = glibc. */
-void or_set_libm(int use_glibc);
+/* sin, cos, acos, atan2 of slerp, angularDistance and AngleAxis:
+ * 0 = correctly rounded (what the reference platform's glibc 2.27 returns),
+ * 1 = this host's glibc, 2 = as 0 except the slerp weights of the states a
+ * walk only tests, which use the GPU kernels' faster sine. */
+void or_set_libm(int mode);
 int or_get_libm(void);
 
 OrWorld* or_world_new(void);

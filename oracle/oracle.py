@@ -100,6 +100,11 @@ def lib():
                                                  _c_double_p]),
             "or_model_box": (None, [vp, ctypes.c_int, _c_double_p, _c_double_p]),
             "or_se3_sin": (ctypes.c_double, [ctypes.c_double]),
+            "or_se3_sin_legacy": (ctypes.c_double, [ctypes.c_double]),
+            "or_cr_sin_slow": (ctypes.c_double, [ctypes.c_double]),
+            "or_cr_cos_slow": (ctypes.c_double, [ctypes.c_double]),
+            "or_cr_acos_slow": (ctypes.c_double, [ctypes.c_double]),
+            "or_cr_atan2_slow": (ctypes.c_double, [ctypes.c_double, ctypes.c_double]),
             "or_se3_cos": (ctypes.c_double, [ctypes.c_double]),
             "or_se3_acos": (ctypes.c_double, [ctypes.c_double]),
             "or_se3_atan2": (ctypes.c_double, [ctypes.c_double, ctypes.c_double]),
@@ -123,8 +128,11 @@ def _f64(a, shape=None):
     return a
 
 
-def set_libm(use_glibc):
-    lib().or_set_libm(1 if use_glibc else 0)
+def set_libm(mode):
+    """0 or False: correctly rounded functions (the reference platform's
+    libm); 1 or True: this host's glibc; 2: as 0, with the GPU kernels'
+    faster sine for the states a walk only tests."""
+    lib().or_set_libm(int(mode))
 
 
 class OracleWorld:

@@ -446,7 +446,7 @@ struct StateFromOmpl {
 struct StateFromSegment {
 	const se3_segment* sg; /* LDS */
 	double tau;
-	__device__ __forceinline__ void get(double* o) const { se3_segment_at(sg, tau, o); }
+	__device__ __forceinline__ void get(double* o) const { se3_segment_at_fast(sg, tau, o); }
 };
 
 /*
@@ -1050,7 +1050,7 @@ struct SegPose {
 	{
 		double t = tau;
 		asm volatile("" : "+v"(t));
-		se3_segment_at(sg, t, o);
+		se3_segment_at_fast(sg, t, o);
 	}
 };
 
@@ -1063,7 +1063,7 @@ struct OmplPose {
 		int64_t kk = k;
 		asm volatile("" : "+v"(kk));
 		double st[7];
-		se3_ompl_motion_sample(m, kk, st);
+		se3_ompl_motion_sample(m, kk, st, 1);
 		se3_ompl_to_unit(x->M, x->pert, x->delta, st, o);
 	}
 };
@@ -1334,7 +1334,7 @@ __device__ __forceinline__ int lane_rank(uint64_t m)
 }
 
 /*
- * Candidate leaf pairs for all poses se3_segment_at(sg, t), t in [t0, t1]:
+ * Candidate leaf pairs for all poses se3_segment_at_fast(sg, t), t in [t0, t1]:
  * breadth first, one node pair per lane.  Poses are placed at the middle
  * parameter tm; a slerp moves the rotation by at most 2 * theta * |t - tm|
  * radians and the translation by |p1 - p0| * |t - tm|, so growing the robot
@@ -1348,7 +1348,7 @@ __device__ int sweep_candidates(const DevScene& s, const se3_segment* sg, double
 	QCull c;
 	{
 		double q[7];
-		se3_segment_at(sg, 0.5 * (t0 + t1), q);
+		se3_segment_at_fast(sg, 0.5 * (t0 + t1), q);
 		make_qcull(s, q, c);
 	}
 	const double half = 0.5 * (t1 - t0);
@@ -1459,7 +1459,7 @@ __device__ uint64_t sat_pass(const DevScene& s, const BisectShared* bs, int nb)
 
 /*
  * Mask of the lanes in `live` (all below lane 32) whose pose
- * se3_segment_at(sg, tau, .) collides, decided on the nc candidate pairs in
+ * se3_segment_at_fast(sg, tau, .) collides, decided on the nc candidate pairs in
  * bs, which must cover every such pose.  Each pass tests two candidates
  * against all probes: lane l takes probe l % 32 (the pose of lane l % 32)
  * and candidate i + l / 32, so no lane idles while a round has at most 32
@@ -1474,7 +1474,7 @@ __device__ uint64_t cands_collide(const DevScene& s, const se3_segment* sg, Bise
 	QCull c;
 	{
 		double q[7];
-		se3_segment_at(sg, shfl_d(tau, p), q);
+		se3_segment_at_fast(sg, shfl_d(tau, p), q);
 		make_qcull(s, q, c);
 		if (h == 0 && act) {
 			ExactTf x;
@@ -2182,6 +2182,19 @@ __device__ __forceinline__ int order_bin(double steps)
 	return ORDER_BINS - 1 - (octave * 16 + sixteenth);
 }
 
+/* The SE(3) distance of lib/osr/osr_state.cc:130-154 to a few digits, for
+ * the order's bins only: translation length plus twice the relative
+ * rotation's half-angle, with the binary32 atan2 of the device library. */
+__device__ __forceinline__ double order_distance(const double* a, const double* b)
+{
+	const double dx = a[0] - b[0], dy = a[1] - b[1], dz = a[2] - b[2];
+	const double na = sqrt(a[3] * a[3] + a[4] * a[4] + a[5] * a[5] + a[6] * a[6]);
+	const double nb = sqrt(b[3] * b[3] + b[4] * b[4] + b[5] * b[5] + b[6] * b[6]);
+	const double dot = fabs(a[3] * b[3] + a[4] * b[4] + a[5] * b[5] + a[6] * b[6]) / (na * nb);
+	const double vn = sqrt(fmax(0.0, 1.0 - dot * dot));
+	return sqrt(dx * dx + dy * dy + dz * dz) + 2.0 * (double)atan2f((float)vn, (float)dot);
+}
+
 template <int MODE>
 __global__ void __launch_bounds__(256) k_order_keys(const double* __restrict__ from, const double* __restrict__ to,
                                                     int64_t n, double vd, uint8_t* __restrict__ key,
@@ -2192,7 +2205,7 @@ __global__ void __launch_bounds__(256) k_order_keys(const double* __restrict__ f
 	__syncthreads();
 	const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
 	if (i < n) {
-		const double d = se3_distance(from + 7 * i, to + 7 * i);
+		const double d = order_distance(from + 7 * i, to + 7 * i);
 		double step = vd;
 		if (MODE == MODE_TRANSITION)
 			step = (vd < d / 2.0) ? vd : d / 2.0;

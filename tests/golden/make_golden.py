@@ -5,7 +5,8 @@ collision-query path on the seeded two-torus scene, small enough to commit.
 
 The meshes are not stored; they are regenerated from their seeds by
 puzzletunneldiscovery_amd.workloads.torus_scene(0).  The oracle runs with the
-library's portable transcendental functions (the ones the GPU uses)."""
+correctly rounded transcendental functions (se3_crmath.h, the values the
+reference platform's C library returns; the GPU uses the same)."""
 import os
 import sys
 

@@ -10,57 +10,118 @@ import pytest
 mp.mp.prec = 120
 
 
-def ulps(got, want):
-    """Distance in units of the last place of the correctly rounded value."""
-    want_d = float(want)
-    if want_d == 0.0:
-        return abs(got) / 5e-324
-    u = math.ulp(want_d)
-    return abs(mp.mpf(got) - want) / u
+def cr(f, *args):
+    """The correctly rounded (nearest, ties to even) binary64 value of f.
+    mpmath has no signed zero, so zero arguments take math's value."""
+    if all(float(a) == 0.0 for a in args):
+        return {mp.sin: math.sin, mp.cos: math.cos, mp.acos: math.acos, mp.atan2: math.atan2}[f](*args)
+    with mp.workprec(320):
+        return float(f(*[mp.mpf(float(a)) for a in args]))
 
 
-@pytest.mark.parametrize("fn,ref,lo,hi", [
-    ("or_se3_sin", mp.sin, -12.0, 12.0),
-    ("or_se3_cos", mp.cos, -12.0, 12.0),
-    ("or_se3_acos", mp.acos, -1.0, 1.0),
-])
-def test_unary_accuracy(oracle_mod, fn, ref, lo, hi):
-    f = getattr(oracle_mod.lib(), fn)
-    rng = np.random.default_rng(1)
-    xs = np.concatenate([rng.uniform(lo, hi, 3000), rng.uniform(-1e-3, 1e-3, 300),
-                         [lo, hi, 0.0, 0.5, -0.5, math.pi / 4, -math.pi / 4, math.pi / 2]])
-    xs = xs[(xs >= lo) & (xs <= hi)]
-    worst = max(ulps(f(float(x)), ref(mp.mpf(float(x)))) for x in xs if abs(float(ref(mp.mpf(float(x))))) > 1e-300)
-    assert worst <= 2.0, (fn, worst)
+def same(a, b):
+    """Bitwise equality of two binary64 values (NaN equals NaN)."""
+    return np.float64(a).tobytes() == np.float64(b).tobytes() or (a != a and b != b)
 
 
-def test_atan2_accuracy(oracle_mod):
-    f = oracle_mod.lib().or_se3_atan2
+def _sin_args(rng):
+    k = np.arange(1, 3001)
+    near = np.nextafter(k * (math.pi / 2), np.inf) + rng.integers(-3, 4, len(k)) * np.spacing(k * (math.pi / 2))
+    xs = np.concatenate([
+        rng.uniform(-math.pi, math.pi, 3000),
+        rng.uniform(0.0, math.pi / 2, 2000),                                   # slerp weights
+        np.ldexp(rng.uniform(1, 2, 2000), rng.integers(-30, 20, 2000)) * rng.choice([-1, 1], 2000),
+        near,                                                                  # next to multiples of pi/2
+        np.ldexp(rng.uniform(1, 2, 200), rng.integers(-40, -27, 200)),        # tiny
+        [0.0, -0.0, math.pi / 4, -math.pi / 4, math.pi / 2, 2.0 ** -27, 2.0 ** 20 * 0.999999],
+    ])
+    return xs
+
+
+def test_sin_cos_correctly_rounded(oracle_mod):
+    """se3_sin and se3_cos round correctly: bit-equal to mpmath's value
+    rounded to nearest, on random arguments, the slerp domain [0, pi/2],
+    magnitudes 2^-30 .. 2^20, arguments next to multiples of pi/2 (where the
+    reduction cancels most bits) and tiny ones."""
+    L = oracle_mod.lib()
+    for x in _sin_args(np.random.default_rng(1)):
+        x = float(x)
+        assert same(L.or_se3_sin(x), cr(mp.sin, x)), ("sin", x)
+        assert same(L.or_se3_cos(x), cr(mp.cos, x)), ("cos", x)
+
+
+def test_acos_correctly_rounded(oracle_mod):
+    L = oracle_mod.lib()
     rng = np.random.default_rng(2)
-    ys = rng.uniform(-2, 2, 3000)
-    xs = rng.uniform(-2, 2, 3000)
-    worst = 0.0
-    for y, x in zip(ys, xs):
-        worst = max(worst, ulps(f(float(y), float(x)), mp.atan2(mp.mpf(float(y)), mp.mpf(float(x)))))
-    # the (z - 1) / (z + 1) reduction adds up to ~1 ulp above tan(pi/8)
-    assert worst <= 3.0
-    assert f(0.0, 0.0) == 0.0 and f(0.0, 1.0) == 0.0
-    assert abs(f(1.0, 0.0) - math.pi / 2) <= math.ulp(math.pi / 2)
+    xs = np.concatenate([
+        rng.uniform(-1, 1, 4000),
+        1 - np.ldexp(rng.uniform(0, 1, 1000), -rng.integers(1, 53, 1000)),    # next to 1 (slerp's |dot|)
+        -1 + np.ldexp(rng.uniform(0, 1, 300), -rng.integers(1, 53, 300)),
+        np.ldexp(rng.uniform(-1, 1, 300), -rng.integers(0, 80, 300)),         # next to 0
+        [0.0, -0.0, 0.5, -0.5, 1.0, -1.0, np.nextafter(1.0, 0.0), np.nextafter(-1.0, 0.0)],
+    ])
+    for x in xs:
+        x = float(x)
+        assert same(L.or_se3_acos(x), cr(mp.acos, x)), x
+    assert math.isnan(L.or_se3_acos(1.5)) and math.isnan(L.or_se3_acos(float("nan")))
 
 
-def test_against_glibc(oracle_mod):
-    """The portable functions and glibc agree to within 2 ulps (they are
-    different implementations of the same functions)."""
+def test_atan2_correctly_rounded(oracle_mod):
+    """All four quadrants, ratios from 2^-80 to 2^80 (angularDistance calls
+    atan2(|vec|, |w|) with both in [0, 1]), and C99's special values."""
     L = oracle_mod.lib()
     rng = np.random.default_rng(3)
-    xs = rng.uniform(0, math.pi / 2, 5000)
-    for fn, g in (("or_se3_sin", math.sin), ("or_se3_cos", math.cos)):
-        f = getattr(L, fn)
-        d = [abs(f(float(x)) - g(float(x))) / math.ulp(g(float(x))) for x in xs]
-        assert max(d) <= 2.0
-    xs = rng.uniform(0, 1, 5000)
-    d = [abs(L.or_se3_acos(float(x)) - math.acos(float(x))) / math.ulp(math.acos(float(x))) for x in xs]
-    assert max(d) <= 2.0
+    ys = np.concatenate([rng.uniform(-2, 2, 3000), rng.uniform(0, 1, 2000),
+                         np.ldexp(rng.uniform(1, 2, 2000), rng.integers(-80, 80, 2000)) * rng.choice([-1, 1], 2000)])
+    xs = np.concatenate([rng.uniform(-2, 2, 3000), rng.uniform(0, 1, 2000),
+                         rng.uniform(-2, 2, 2000)])
+    for y, x in zip(ys, xs):
+        y, x = float(y), float(x)
+        assert same(L.or_se3_atan2(y, x), cr(mp.atan2, y, x)), (y, x)
+    inf, nan = float("inf"), float("nan")
+    for y in (0.0, -0.0, 1.5, -1.5, inf, -inf, nan, 1e-310, -1e300):
+        for x in (0.0, -0.0, 2.0, -2.0, inf, -inf, nan, 1e-310, -1e300):
+            assert same(L.or_se3_atan2(y, x), math.atan2(y, x)), (y, x)
+
+
+def test_slow_paths_correctly_rounded(oracle_mod):
+    """The 256-bit fixed-point evaluations that decide when the double-double
+    one cannot (about once in 2^40 calls) are correctly rounded themselves;
+    they are called here directly."""
+    L = oracle_mod.lib()
+    rng = np.random.default_rng(4)
+    for x in np.concatenate([rng.uniform(-4, 4, 400), np.ldexp(rng.uniform(1, 2, 200), rng.integers(-26, 19, 200))]):
+        x = float(x)
+        if abs(x) < 2.0 ** -26:
+            continue
+        assert same(L.or_cr_sin_slow(x), cr(mp.sin, x)), x
+        assert same(L.or_cr_cos_slow(x), cr(mp.cos, x)), x
+    for x in np.concatenate([rng.uniform(-0.999, 0.999, 400), 1 - np.ldexp(rng.uniform(0, 1, 100), -rng.integers(1, 50, 100))]):
+        x = float(x)
+        if x != 0.0:
+            assert same(L.or_cr_acos_slow(x), cr(mp.acos, x)), x
+    for y, x in zip(np.ldexp(rng.uniform(1, 2, 400), rng.integers(-29, 3, 400)) * rng.choice([-1, 1], 400),
+                    rng.uniform(-3, 3, 400)):
+        y, x = float(y), float(x)
+        r = min(abs(y), abs(x)) / max(abs(y), abs(x))
+        if r >= 2.0 ** -30:
+            assert same(L.or_cr_atan2_slow(y, x), cr(mp.atan2, y, x)), (y, x)
+
+
+def test_host_libm_mode_differs_only_in_last_bit(oracle_mod):
+    """Libm mode 1 calls the host's sin/cos/acos/atan2 (glibc's are within an
+    ulp and usually correctly rounded): each value is within one ulp of the
+    correctly rounded one and differs from it rarely."""
+    L = oracle_mod.lib()
+    rng = np.random.default_rng(5)
+    xs = rng.uniform(-math.pi, math.pi, 20000)
+    diff = 0
+    for x in xs:
+        x = float(x)
+        for mine, host in ((L.or_se3_sin(x), math.sin(x)), (L.or_se3_cos(x), math.cos(x))):
+            assert abs(mine - host) <= np.spacing(abs(mine)), x
+            diff += mine != host
+    assert diff < 0.01 * 2 * len(xs)
 
 
 def _rand_states(n, seed):
