@@ -1100,7 +1100,8 @@ def main():
     host_cpu()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
-              "(each query is independent; see DESIGN.md for how the path shards).", file=sys.stderr)
+              "(each query is independent, so several GPUs would each take a contiguous slice of the "
+              "batch against their own copy of the scene).", file=sys.stderr)
         sys.exit(2)
 
     import torch
