@@ -160,6 +160,16 @@ def _compute_roofline(flops_q, nbv, ntri, queries, kern_ms, kernel, sample):
                 flops_sample=sample)
 
 
+def _world(args, pyosr, env, rob):
+    """The workload's world, with the traversal knobs the command line set."""
+    uw = pyosr.create_unit_world_from_arrays(env, rob)
+    knobs = {k: v for k, v in (("leaf_size", args.leaf_size), ("pool_chunk", args.pool_chunk),
+                               ("queue_depth", args.queue_depth)) if v}
+    if knobs:
+        uw.tuning = knobs
+    return uw
+
+
 class _Timer:
     """HIP events on the stream the engine's kernels run on."""
 
@@ -270,7 +280,7 @@ def _cpu_baseline(value, unit, sample, single, single_sample):
 def run_states(args, torch, pyosr, W):
     """1M single-state validity queries, torus scene."""
     env, rob = W.torus_scene(0)
-    uw = pyosr.create_unit_world_from_arrays(env, rob)
+    uw = _world(args, pyosr, env, rob)
     n = args.segments or 1_000_000
     qs = W.random_states(n, seed=1)
     stream = torch.cuda.Stream()
@@ -338,7 +348,7 @@ def run_uneven(args, torch, pyosr, W):
     """Advance-until-contact over uneven segments (lognormal step counts),
     shell scene, in the persistent segment kernel."""
     env, rob = W.shell_scene(0)
-    uw = pyosr.create_unit_world_from_arrays(env, rob)
+    uw = _world(args, pyosr, env, rob)
     n = args.segments or 4 * 1024 * 1024
     vd = VD
     starts = _free_states(uw, W, n, 5000)
@@ -415,7 +425,7 @@ def run_uneven(args, torch, pyosr, W):
 def run_visibility(args, torch, pyosr, W):
     """Bit-packed visibility matrix of 4,096 free milestones, shell scene."""
     env, rob = W.shell_scene(0)
-    uw = pyosr.create_unit_world_from_arrays(env, rob)
+    uw = _world(args, pyosr, env, rob)
     n = args.segments or 4096
     ms = _free_states(uw, W, n, 6000)
     stream = torch.cuda.Stream()
@@ -490,7 +500,7 @@ def run_batch(args, torch, pyosr, W):
     single states, shell scene; the launch shape is tuned on this batch first.
     Also times the same work as two separate launches."""
     env, rob = W.shell_scene(0)
-    uw = pyosr.create_unit_world_from_arrays(env, rob)
+    uw = _world(args, pyosr, env, rob)
     n = args.segments or 4 * 1024 * 1024
     m = args.batch_states
     vd = VD
@@ -818,7 +828,7 @@ def run_surrogate(args, torch, pyosr, W):
     from puzzletunneldiscovery_amd import surrogate as S
     from puzzletunneldiscovery_amd.surrogate import Surrogate
     env, rob = W.shell_scene(0)
-    uw = pyosr.create_unit_world_from_arrays(env, rob)
+    uw = _world(args, pyosr, env, rob)
     n_train = 16 * 1024 * 1024
     n_screen = args.segments or 64 * 1024 * 1024
     stream = torch.cuda.Stream()
@@ -923,7 +933,7 @@ def run_touch(args, torch, pyosr, W):
     """The headline: touch-configuration search over 65,536 segments of the
     torus scene (one step = one launch)."""
     env, rob = W.torus_scene(0)
-    uw = pyosr.create_unit_world_from_arrays(env, rob)
+    uw = _world(args, pyosr, env, rob)
     starts, ends = build_workload(uw, W, args.segments or N_SEG)
     n = starts.shape[0]
     dev = torch.device("cuda:0")
@@ -1088,6 +1098,9 @@ def main():
     ap.add_argument("--threshold", type=float, default=0.5)
     ap.add_argument("--screen-segments", type=int, default=8 * 1024 * 1024,
                     help="surrogate workload: segments screened after the samples (0 = none)")
+    ap.add_argument("--leaf-size", type=int, default=0, help="triangles per BVH leaf, at most (default: 1)")
+    ap.add_argument("--pool-chunk", type=int, default=0, help="states per counter add of the state kernels")
+    ap.add_argument("--queue-depth", type=int, default=0, help="segments per counter add of the segment kernels")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first copy a buffer of this many MiB three times (a known-size "
                          "kernel that calibrates the FETCH_SIZE / WRITE_SIZE counters)")

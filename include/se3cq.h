@@ -233,10 +233,20 @@ int se3cq_translate_vanilla_pts_to_unit(const se3cq_world* w, int geo, const dou
  * graph.  A split batch (split = 1) runs the persistent segment kernel of
  * se3cq_transit_* and the state kernel of se3cq_valid_states_* instead,
  * which wins when the state traversal's LDS stacks would limit how many
- * segment wavefronts a CU holds; a batch without states always does.  se3cq_batch_tune (batches with states) times every
- * launch shape on the given batch, keeps the fastest for the world's later
- * batch calls and leaves the batch's results in the outputs; *ms = its time
- * per launch. */
+ * segment wavefronts a CU holds; a batch without states always does.
+ *
+ * se3cq_batch_tune (batches with states) searches, one knob at a time and
+ * keeping the best value of each before the next: the world's leaf size
+ * (1, 2, 4), the launch shapes (the split form; the one-kernel form with
+ * waves per block 1, 2, 4, resident blocks per CU all or half, states per
+ * lane 1, 2, 4), the pool chunk (64, 128, 256) and the queue depth (1, 2,
+ * 4) of se3cq_tuning.  Each candidate is timed over `repeats` launches on
+ * the world's own stream, and its outputs are compared byte for byte on the
+ * device with the first run's; a difference fails the call with
+ * SE3CQ_EDEVICE (every candidate computes the same answers).  The fastest
+ * configuration stays set (se3cq_batch_get_config, se3cq_get_tuning), the
+ * outputs hold the batch's results, *ms = the time per launch of the
+ * choice. */
 typedef struct se3cq_batch_config {
 	int32_t block_waves;     /* waves per block (1-4; 0 = as many as the LDS allows) */
 	int32_t blocks_per_cu;   /* resident blocks per CU to launch (0 = as many as fit) */
@@ -257,6 +267,28 @@ int se3cq_batch_tune(se3cq_world* w, const double* d_seg_from, const double* d_s
                      int32_t repeats, se3cq_batch_config* chosen, float* ms);
 int se3cq_batch_get_config(const se3cq_world* w, se3cq_batch_config* cfg);
 int se3cq_batch_set_config(se3cq_world* w, const se3cq_batch_config* cfg);
+
+/* World-wide traversal knobs (no reference counterpart: FCL's BVH, built by
+ * CDModel, lib/osr/cdmodel.cc:85-114, holds one triangle per leaf and the
+ * reference has no work queue).  Results never depend on them.
+ *   leaf_size   triangles per BVH leaf, at most (1-8, default 1); a change
+ *               rebuilds both trees (about 2 s for 215k triangles) after
+ *               waiting for the device.  A set-up call: no query of the
+ *               world may run meanwhile, and HIP graphs captured before it
+ *               must be captured again (a deeper tree than the one they were
+ *               captured with can overflow their traversal stacks, which is
+ *               reported as an error).
+ *   pool_chunk  states a wavefront of the state kernels takes per counter
+ *               add (16-1024, default 128).
+ *   queue_depth segments (or batch items) a wavefront of the segment and
+ *               batch kernels takes per counter add (1-16, default 1). */
+typedef struct se3cq_tuning {
+	int32_t leaf_size;
+	int32_t pool_chunk;
+	int32_t queue_depth;
+} se3cq_tuning;
+int se3cq_get_tuning(const se3cq_world* w, se3cq_tuning* t);
+int se3cq_set_tuning(se3cq_world* w, const se3cq_tuning* t);
 
 /* OMPL-side validity, the two queries of pyse3ompl's OmplDriver that do not
  * plan (lib/pyse3ompl/pyse3ompl.cc:77-81), over this world's geometry.

@@ -24,6 +24,11 @@ class BatchConfig(ctypes.Structure):
     """se3cq_batch_config."""
     _fields_ = [("block_waves", ctypes.c_int32), ("blocks_per_cu", ctypes.c_int32),
                 ("states_per_lane", ctypes.c_int32), ("split", ctypes.c_int32)]
+
+
+class Tuning(ctypes.Structure):
+    """se3cq_tuning."""
+    _fields_ = [("leaf_size", ctypes.c_int32), ("pool_chunk", ctypes.c_int32), ("queue_depth", ctypes.c_int32)]
 i64 = ctypes.c_int64
 cint = ctypes.c_int
 cdbl = ctypes.c_double
@@ -91,6 +96,8 @@ SIGNATURES = {
                          ctypes.c_int32, ctypes.POINTER(BatchConfig), ctypes.POINTER(ctypes.c_float)],
     "se3cq_batch_get_config": [vp, ctypes.POINTER(BatchConfig)],
     "se3cq_batch_set_config": [vp, ctypes.POINTER(BatchConfig)],
+    "se3cq_get_tuning": [vp, ctypes.POINTER(Tuning)],
+    "se3cq_set_tuning": [vp, ctypes.POINTER(Tuning)],
     "se3cq_ompl_longest_valid_segment": [c_double_p, c_double_p, ctypes.c_double, c_double_p],
     "se3cq_ompl_validate_states_host": [vp, c_double_p, i64, c_i32_p],
     "se3cq_ompl_validate_states_device": [vp, vp, i64, vp, vp],

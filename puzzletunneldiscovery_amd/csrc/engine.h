@@ -6,9 +6,11 @@
  *   nodes : GNode[2*nf-1], 64 B each, children of an internal node are
  *           adjacent (first, first+1), nodes in breadth-first order so the
  *           top levels shared by every query sit in a few cache lines.
- *   tris  : binary64 xyz of the three vertices of every triangle in the
- *           mesh's own face order (72 B per triangle); the exact leaf test
- *           reads these.
+ *   tris  : binary64 xyz of the three vertices of every triangle, in leaf
+ *           order (the triangles of a leaf are consecutive; 72 B per
+ *           triangle); the exact leaf test reads these.
+ *   tri_id / tri_pos : the mesh's face index of each triangle in leaf order,
+ *           and its inverse (contact enumeration reports face indices).
  * Node boxes are binary32 and conservative: every vertex of every triangle
  * under a node lies inside the box defined by the stored binary32 centre,
  * axes and extents, checked in binary64 at build time.
@@ -30,7 +32,9 @@ namespace se3cq {
 
 /* c_ex = (cx, cy, cz, ex), a0_ey = (axis0, ey), a1_ez = (axis1, ez),
  * a2_child = (axis2, child) with child >= 0 an internal node whose children
- * are child and child+1, and child < 0 a leaf holding triangle ~child. */
+ * are child and child+1, and child < 0 a leaf whose code ~child is
+ * 8 * first + count - 1: triangles first .. first + count - 1 of the leaf
+ * order, 1 <= count <= 8 (the world's leaf size bounds it). */
 struct alignas(16) GNode {
 	float4 c_ex;
 	float4 a0_ey;
@@ -41,6 +45,8 @@ struct alignas(16) GNode {
 struct DevModel {
 	const GNode* nodes;
 	const double* tris;
+	const int32_t* tri_id;  /* leaf order -> mesh face index */
+	const int32_t* tri_pos; /* mesh face index -> leaf order */
 	int32_t nnodes;
 	int32_t ntris;
 	int32_t depth;
@@ -79,6 +85,8 @@ struct DevScene {
 	const int32_t* order;  /* segment kernels: item k walks segment order[k] (null: segment k) */
 	const uint32_t* n_items; /* segment kernels: device count of items (null: one per segment) */
 	int32_t num_cu;        /* compute units of the device (persistent grid size) */
+	int32_t pool_chunk;    /* state kernels: states a wavefront takes per counter add */
+	int32_t queue_depth;   /* segment and batch kernels: items a wavefront takes per counter add */
 };
 
 } // namespace se3cq

@@ -368,34 +368,22 @@ __device__ __forceinline__ void make_exact(const DevScene& s, const double* q, E
 	}
 }
 
-/*
- * One (env triangle, robot triangle) pair at state q: robot vertices R q + T,
- * both triangles translated by -P1, then the SAT.  P1 - P1 is the zero vector
- * for the finite coordinates the host admits, so p1 is the literal 0.
- */
-__device__ __forceinline__ bool leaf_pair(const DevScene& s, const double* q, int te, int tr)
+/* Exact test of two leaves (defined with the packet traversal below). */
+__device__ __forceinline__ bool leaf_pair_x(const DevScene& s, const ExactTf& x, int le, int lr);
+
+/* The leaves (codes le, lr) at state q. */
+__device__ __forceinline__ bool leaf_pair(const DevScene& s, const double* q, int le, int lr)
 {
 	ExactTf x;
 	make_exact(s, q, x);
-	const double* P = s.env.tris + 9 * (int64_t)te;
-	const double* qv = s.rob.tris + 9 * (int64_t)tr;
-	const double P1[3] = {P[0], P[1], P[2]};
-	const double p1[3] = {0.0, 0.0, 0.0};
-	double p2[3], p3[3], q1[3], q2[3], q3[3];
-#pragma unroll
-	for (int k = 0; k < 3; k++) {
-		p2[k] = P[3 + k] - P1[k];
-		p3[k] = P[6 + k] - P1[k];
-	}
-	double* qq[3] = {q1, q2, q3};
-#pragma unroll
-	for (int v = 0; v < 3; v++) {
-		const double a = qv[3 * v], b = qv[3 * v + 1], c = qv[3 * v + 2];
-#pragma unroll
-		for (int r = 0; r < 3; r++)
-			qq[v][r] = (((x.R[3 * r] * a + x.R[3 * r + 1] * b) + x.R[3 * r + 2] * c) + x.T[r]) - P1[r];
-	}
-	return sat17(p1, p2, p3, q1, q2, q3);
+	return leaf_pair_x(s, x, le, lr);
+}
+
+/* Leaf code (the first triangle and the triangle count, see GNode) of leaf
+ * node i. */
+__device__ __forceinline__ int leaf_code(const GNode* __restrict__ nodes, int i)
+{
+	return ~__float_as_int(nodes[i].a2_child.w);
 }
 
 /* ---- depth-first BVH x BVH traversal, one thread per query ---- */
@@ -493,7 +481,7 @@ __device__ bool collide_dfs(const DevScene& s, const Src& src, const PairStack<W
 				const int ca = __float_as_int(A.a2_child.w);
 				const bool la = ca < 0, lb = B.child < 0;
 				if (la && lb) {
-					leaves.put(nleaf++, ~ca, ~B.child);
+					leaves.put(nleaf++, a, b);
 					continue;
 				}
 				if (sp + 2 > cap) {
@@ -515,11 +503,11 @@ __device__ bool collide_dfs(const DevScene& s, const Src& src, const PairStack<W
 			}
 		}
 		for (int i = 0; i < nleaf; i++) {
-			int te, tr;
-			leaves.get(i, te, tr);
+			int a, b;
+			leaves.get(i, a, b);
 			double q[7];
 			src.get(q);
-			if (leaf_pair(s, q, te, tr))
+			if (leaf_pair(s, q, leaf_code(s.env.nodes, a), leaf_code(s.rob.nodes, b)))
 				return true;
 		}
 		if (sp == 0)
@@ -762,6 +750,8 @@ __device__ void states_refill(const DevScene& s, const Loader& ld, int64_t n, co
 	 * j-th index */
 	int hv_lo = 0, hv_hi = 0, nheavy = 0;
 	const int heavy_tests = SE3CQ_HEAVY_PER_LEVEL * s.stack_depth;
+	/* states per counter add (at least 1; the launch sets it) */
+	const int chunk = s.pool_chunk > 0 ? s.pool_chunk : REFILL_CHUNK;
 	for (;;) {
 		{
 			uint64_t dm = __ballot(idx >= 0 && tests > heavy_tests);
@@ -785,12 +775,12 @@ __device__ void states_refill(const DevScene& s, const Loader& ld, int64_t n, co
 			if (pool_left < nidle && !exhausted) {
 				unsigned long long got = 0;
 				if (lane == 0)
-					got = atomicAdd(s.queue, (unsigned long long)REFILL_CHUNK);
+					got = atomicAdd(s.queue, (unsigned long long)chunk);
 				base2 = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(got >> 32)) << 32) |
 				                  (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)got));
-				if (base2 + REFILL_CHUNK >= n)
+				if (base2 + chunk >= n)
 					exhausted = true;
-				left2 = base2 < n ? (int)((n - base2) < REFILL_CHUNK ? (n - base2) : REFILL_CHUNK) : 0;
+				left2 = base2 < n ? (int)((n - base2) < chunk ? (n - base2) : chunk) : 0;
 			}
 			const int take = nidle < pool_left + left2 ? nidle : pool_left + left2;
 			if (idx < 0) {
@@ -833,11 +823,11 @@ __device__ void states_refill(const DevScene& s, const Loader& ld, int64_t n, co
 		if (wl != 0 && (__popcll(wl) >= REFILL_MIN || wl == am || since_leaf >= LEAF_PATIENCE)) {
 			since_leaf = 0;
 			if (want_leaf) {
-				int te, tr;
-				leaves.get(--nleaf, te, tr);
+				int a, b;
+				leaves.get(--nleaf, a, b);
 				double q[7];
 				ld.get(idx, q);
-				if (leaf_pair(s, q, te, tr)) {
+				if (leaf_pair(s, q, leaf_code(s.env.nodes, a), leaf_code(s.rob.nodes, b))) {
 					set(idx, true);
 					idx = -1;
 				} else if (sp == 0 && nleaf == 0) {
@@ -863,7 +853,7 @@ __device__ void states_refill(const DevScene& s, const Loader& ld, int64_t n, co
 			const int ca = __float_as_int(A.a2_child.w);
 			const bool la = ca < 0, lb = B.child < 0;
 			if (la && lb) {
-				leaves.put(nleaf++, ~ca, ~B.child);
+				leaves.put(nleaf++, a, b);
 			} else if (sp + 2 > cap) {
 				atomicOr(s.error_flag, 1u);
 				set(idx, true);
@@ -989,8 +979,13 @@ __device__ __forceinline__ int rdl(int v, int k) { return __builtin_amdgcn_readl
 __device__ int se3cq_amdgcn_writelane(int val, int lane, int old) __asm("llvm.amdgcn.writelane.i32");
 __device__ __forceinline__ int wrl(int val, int k, int old) { return se3cq_amdgcn_writelane(val, k, old); }
 
-/* Exact leaf test with uniform triangle ids and per-lane exact transform. */
-__device__ __forceinline__ bool leaf_pair_x(const DevScene& s, const ExactTf& x, int te, int tr)
+/*
+ * One (env triangle, robot triangle) pair under the exact transform x:
+ * robot vertices R q + T, both triangles translated by -P1, then the SAT.
+ * P1 - P1 is the zero vector for the finite coordinates the host admits, so
+ * p1 is the literal 0.  Triangles are numbered in leaf order.
+ */
+__device__ __forceinline__ bool tri_pair_x(const DevScene& s, const ExactTf& x, int te, int tr)
 {
 	const double* P = s.env.tris + 9 * (int64_t)te;
 	const double* qv = s.rob.tris + 9 * (int64_t)tr;
@@ -1011,6 +1006,18 @@ __device__ __forceinline__ bool leaf_pair_x(const DevScene& s, const ExactTf& x,
 			qq[v][r] = (((x.R[3 * r] * a + x.R[3 * r + 1] * b) + x.R[3 * r + 2] * c) + x.T[r]) - P1[r];
 	}
 	return sat17(p1, p2, p3, q1, q2, q3);
+}
+
+/* Every triangle pair of two leaves (codes le, lr); true at the first that
+ * intersects. */
+__device__ __forceinline__ bool leaf_pair_x(const DevScene& s, const ExactTf& x, int le, int lr)
+{
+	const int e0 = le >> 3, ne = (le & 7) + 1, r0 = lr >> 3, nr = (lr & 7) + 1;
+	for (int i = 0; i < ne; i++)
+		for (int j = 0; j < nr; j++)
+			if (tri_pair_x(s, x, e0 + i, r0 + j))
+				return true;
+	return false;
 }
 
 /*
@@ -1331,11 +1338,11 @@ __device__ __forceinline__ void wave_lds_sync()
  * (lane 0 adds, the wave reads the value in scalar registers) until the
  * items run out.  A long item then holds one wavefront, not a whole block,
  * and a finished wavefront starts its next item at once. */
-__device__ __forceinline__ int64_t next_item(unsigned long long* queue)
+__device__ __forceinline__ int64_t next_item(unsigned long long* queue, int take = 1)
 {
 	unsigned long long got = 0;
 	if ((threadIdx.x & 63) == 0)
-		got = atomicAdd(queue, 1ull);
+		got = atomicAdd(queue, (unsigned long long)take);
 	return (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(got >> 32)) << 32) |
 	                 (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)got));
 }
@@ -1902,8 +1909,17 @@ __global__ void __launch_bounds__(256, MODE == MODE_TOUCH ? SE3CQ_TOUCH_WAVES : 
 		const int64_t c = (int64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)*s.n_items);
 		items = c < n ? c : n;
 	}
+	/* items [got, got + left) are this wavefront's (queue_depth per add) */
+	const int depth = s.queue_depth > 0 ? s.queue_depth : 1;
+	int64_t got = 0;
+	int left = 0;
 	for (;;) {
-		const int64_t item = next_item(s.queue);
+		if (left == 0) {
+			got = next_item(s.queue, depth);
+			left = depth;
+		}
+		const int64_t item = got++;
+		left--;
 		if (item >= items)
 			break;
 		const int64_t seg = s.order ? (int64_t)s.order[item] : item;
@@ -1946,8 +1962,16 @@ __global__ void __launch_bounds__(256, SE3CQ_MIN_WAVES) k_batch(DevScene s, Batc
 		thread_stacks<WIDE>(s, smem, stk, leaves);
 	const int64_t per_item = 64 * (int64_t)a.states_per_lane;
 	const int64_t total = a.n_seg + (a.n_states + per_item - 1) / per_item;
+	const int depth = s.queue_depth > 0 ? s.queue_depth : 1;
+	int64_t got = 0;
+	int left = 0;
 	for (;;) {
-		const int64_t item = next_item(s.queue);
+		if (left == 0) {
+			got = next_item(s.queue, depth);
+			left = depth;
+		}
+		const int64_t item = got++;
+		left--;
 		if (item >= total)
 			break;
 		if (item < a.n_seg) {
@@ -2173,7 +2197,7 @@ __global__ void __launch_bounds__(256, SE3CQ_MIN_WAVES) k_contacts(DevScene s, c
 				const int ca = __float_as_int(A.a2_child.w);
 				const bool la = ca < 0, lb = B.child < 0;
 				if (la && lb) {
-					leaves.put(nleaf++, ~ca, ~B.child);
+					leaves.put(nleaf++, a, b);
 					continue;
 				}
 				if (sp + 2 > depth_cap) {
@@ -2193,18 +2217,25 @@ __global__ void __launch_bounds__(256, SE3CQ_MIN_WAVES) k_contacts(DevScene s, c
 			}
 		}
 		for (int j = 0; j < nleaf; j++) {
-			int te, tr;
-			leaves.get(j, te, tr);
+			int a, b;
+			leaves.get(j, a, b);
+			const int le = leaf_code(s.env.nodes, a), lr = leaf_code(s.rob.nodes, b);
 			double q[7];
 			for (int k = 0; k < 7; k++)
 				q[k] = qp[k];
-			if (leaf_pair(s, q, te, tr)) {
-				if (found < cap) {
-					pairs[(i * cap + found) * 2] = te;
-					pairs[(i * cap + found) * 2 + 1] = tr;
+			ExactTf x;
+			make_exact(s, q, x);
+			for (int u = 0; u < (le & 7) + 1; u++)
+				for (int v = 0; v < (lr & 7) + 1; v++) {
+					const int te = (le >> 3) + u, tr = (lr >> 3) + v;
+					if (tri_pair_x(s, x, te, tr)) {
+						if (found < cap) {
+							pairs[(i * cap + found) * 2] = s.env.tri_id[te];
+							pairs[(i * cap + found) * 2 + 1] = s.rob.tri_id[tr];
+						}
+						found++;
+					}
 				}
-				found++;
-			}
 		}
 	}
 	count[i] = found;
@@ -2230,8 +2261,8 @@ __global__ void k_contact_segments(DevScene s, const double* __restrict__ qs, in
 	const double* q = qs + 7 * i;
 	double Rr[9];
 	se3_quat_to_rotmat(q[3], q[4], q[5], q[6], Rr);
-	const double* E = s.env.tris + 9 * (int64_t)pairs[2 * t];
-	const double* Q = s.rob.tris + 9 * (int64_t)pairs[2 * t + 1];
+	const double* E = s.env.tris + 9 * (int64_t)s.env.tri_pos[pairs[2 * t]];
+	const double* Q = s.rob.tris + 9 * (int64_t)s.rob.tri_pos[pairs[2 * t + 1]];
 	double V[9], U[9];
 	for (int v = 0; v < 3; v++) {
 		se3_place(s.pert->Re, s.pert->te, E + 3 * v, V + 3 * v);
@@ -2471,6 +2502,26 @@ __global__ void k_take_u32(uint32_t* flag, uint32_t* out)
 hipError_t launch_take_u32(uint32_t* flag, uint32_t* out, hipStream_t st)
 {
 	hipLaunchKernelGGL(k_take_u32, dim3(1), dim3(64), 0, st, flag, out);
+	return hipGetLastError();
+}
+
+/* *flag |= 1 if bytes [0, n) of a and b differ (grid-stride, any grid). */
+__global__ void k_differ(const uint8_t* __restrict__ a, const uint8_t* __restrict__ b, int64_t n, uint32_t* flag)
+{
+	bool d = false;
+	for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+		d |= a[i] != b[i];
+	if (__ballot(d) && (threadIdx.x & 63) == 0)
+		atomicOr(flag, 1u);
+}
+
+hipError_t launch_differ(const void* a, const void* b, int64_t n, uint32_t* flag, hipStream_t st)
+{
+	if (n <= 0)
+		return hipSuccess;
+	const int64_t blocks = (n + 255) / 256;
+	hipLaunchKernelGGL(k_differ, dim3((unsigned)(blocks < 4096 ? blocks : 4096)), dim3(256), 0, st,
+	                   static_cast<const uint8_t*>(a), static_cast<const uint8_t*>(b), n, flag);
 	return hipGetLastError();
 }
 

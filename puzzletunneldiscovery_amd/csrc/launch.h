@@ -97,6 +97,8 @@ hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t st);
 hipError_t launch_fill_u8(uint8_t* p, int64_t n, uint8_t v, hipStream_t st);
 /* *out = *flag; *flag = 0 (one atomic exchange on the device). */
 hipError_t launch_take_u32(uint32_t* flag, uint32_t* out, hipStream_t st);
+/* *flag |= 1 if the n bytes at a and b differ. */
+hipError_t launch_differ(const void* a, const void* b, int64_t n, uint32_t* flag, hipStream_t st);
 
 } // namespace se3cq
 

@@ -539,8 +539,10 @@ class UnitWorld:
 
     def batch_tune(self, d_from, d_to, n_seg, verify_delta, d_done, d_last_tau, d_tau, d_steps, d_states, n_states,
                    d_valid, d_last_free=None, d_state=None, repeats=3):
-        """Time every launch shape on this batch and keep the fastest for later
-        batch calls.  Returns (config dict, ms per launch)."""
+        """Search the world's leaf size, the launch shapes, the pool chunk and
+        the queue depth on this batch (each candidate's outputs checked equal
+        on the device) and keep the fastest for later calls.  Returns
+        (config dict, ms per launch); the chosen knobs are in `tuning`."""
         cfg = N.BatchConfig()
         ms = ctypes.c_float(0.0)
         N.check(N.lib().se3cq_batch_tune(self._w, d_from, d_to, int(n_seg), float(verify_delta), d_last_free,
@@ -559,6 +561,22 @@ class UnitWorld:
         cfg = N.BatchConfig(int(c.get("block_waves", 0)), int(c.get("blocks_per_cu", 0)),
                             int(c.get("states_per_lane", 1)), int(c.get("split", 0)))
         N.check(N.lib().se3cq_batch_set_config(self._w, ctypes.byref(cfg)))
+
+    @property
+    def tuning(self):
+        """World-wide traversal knobs: leaf_size (triangles per BVH leaf, at
+        most), pool_chunk (states per counter add of the state kernels) and
+        queue_depth (segments per counter add of the segment kernels)."""
+        t = N.Tuning()
+        N.check(N.lib().se3cq_get_tuning(self._w, ctypes.byref(t)))
+        return {k: getattr(t, k) for k, _ in N.Tuning._fields_}
+
+    @tuning.setter
+    def tuning(self, c):
+        cur = self.tuning
+        cur.update(c)
+        t = N.Tuning(int(cur["leaf_size"]), int(cur["pool_chunk"]), int(cur["queue_depth"]))
+        N.check(N.lib().se3cq_set_tuning(self._w, ctypes.byref(t)))
 
     def check_device(self):
         N.check(N.lib().se3cq_check(self._w))
