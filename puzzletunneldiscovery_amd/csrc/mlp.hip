@@ -31,6 +31,7 @@ namespace se3mlp {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef short s16x2 __attribute__((ext_vector_type(2)));
 
 enum { IN = 32, HID = 256, OUT = 32, LAYERS = 5, OUT_USED = 2 };
 
@@ -111,6 +112,64 @@ __global__ void __launch_bounds__(256) k_gemm_nt(Gemm g)
 	}
 }
 
+/*
+ * Split-K form of the same product for long reductions (the weight
+ * gradients reduce over the batch, K = rows, while M x N is only 256 x 256:
+ * one wavefront per tile would leave the device nearly idle).  Block
+ * (x, y, z) computes its four 32 x 32 tiles over k in [z SPLIT_K,
+ * (z + 1) SPLIT_K), loading each k-step's fragments one step ahead, and
+ * stores the f32 partial P[z][m][n]; k_sum_parts adds the partials in z
+ * order, so the sum does not depend on scheduling.
+ */
+enum { SPLIT_K = 256 };
+__global__ void __launch_bounds__(256) k_gemm_nt_part(Gemm g, float* __restrict__ P)
+{
+	const int wave = threadIdx.x >> 6;
+	const int lane = threadIdx.x & 63;
+	const int m0 = (blockIdx.y * 2 + (wave >> 1)) * 32;
+	const int n0 = (blockIdx.x * 2 + (wave & 1)) * 32;
+	if (m0 >= g.M || n0 >= g.N)
+		return;
+	const int k0 = blockIdx.z * SPLIT_K;
+	const int k1 = (g.K - k0 < SPLIT_K) ? g.K : k0 + SPLIT_K;
+	const int r = lane & 31, kq = (lane >> 5) * 8;
+	const uint16_t* pa = g.A + (int64_t)(m0 + r) * g.lda + kq;
+	const uint16_t* pb = g.Bt + (int64_t)(n0 + r) * g.ldb + kq;
+	f32x16 acc = {};
+	uint4 va = *reinterpret_cast<const uint4*>(pa + k0);
+	uint4 vb = *reinterpret_cast<const uint4*>(pb + k0);
+	for (int k = k0; k < k1; k += 16) {
+		bf16x8 a, b;
+		std::memcpy(&a, &va, 16);
+		std::memcpy(&b, &vb, 16);
+		if (k + 16 < k1) {
+			va = *reinterpret_cast<const uint4*>(pa + k + 16);
+			vb = *reinterpret_cast<const uint4*>(pb + k + 16);
+		}
+		acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+	}
+	float* out = P + (int64_t)blockIdx.z * g.M * g.N;
+	const int n = n0 + r;
+#pragma unroll
+	for (int i = 0; i < 16; i++) {
+		const int m = m0 + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
+		out[(int64_t)m * g.N + n] = acc[i];
+	}
+}
+
+/* C[m][n] = sum over z < parts of P[z][m][n], in z order. */
+__global__ void k_sum_parts(const float* __restrict__ P, int parts, int M, int N, float* __restrict__ C, int64_t ldc)
+{
+	const int64_t mn = (int64_t)M * N;
+	const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (e >= mn)
+		return;
+	float s = 0.0f;
+	for (int z = 0; z < parts; z++)
+		s += P[(int64_t)z * mn + e];
+	C[(e / N) * ldc + e % N] = s;
+}
+
 /* ---- features, loss, reductions, optimiser ---------------------------------- */
 /* Rotation of the normalised quaternion (Eigen toRotationMatrix on q/|q|) and
  * translation - 0.5; rows >= n (padding up to the tile size) are zero. */
@@ -185,20 +244,34 @@ __global__ void k_xent(const float* __restrict__ Z, int64_t n, int64_t rows, con
 	loss[i] = l;
 }
 
-/* out[j] = sum_i XT[j][i] over i < rows (bias gradient), one wave per j. */
-__global__ void k_rowsum_bf16(const uint16_t* __restrict__ XT, int64_t rows, int cols, float* __restrict__ out)
+/* out[j] = sum_i XT[j][i] over i < rows (bias gradient; rows a multiple of
+ * 8): one block per j, each thread summing runs of eight, then a tree of
+ * fixed order in LDS. */
+__global__ void __launch_bounds__(256) k_rowsum_bf16(const uint16_t* __restrict__ XT, int64_t rows, int cols,
+                                                     float* __restrict__ out)
 {
-	const int j = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-	const int lane = threadIdx.x & 63;
+	__shared__ float part[256];
+	const int j = blockIdx.x;
 	if (j >= cols)
 		return;
+	const uint16_t* x = XT + (int64_t)j * rows;
 	float s = 0.0f;
-	for (int64_t i = lane; i < rows; i += 64)
-		s += bf2f(XT[(int64_t)j * rows + i]);
-	for (int off = 32; off > 0; off >>= 1)
-		s += __shfl_down(s, off, 64);
-	if (lane == 0)
-		out[j] = s;
+	for (int64_t i = 8 * (int64_t)threadIdx.x; i < rows; i += 8 * 256) {
+		const uint4 v = *reinterpret_cast<const uint4*>(x + i);
+		const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+		for (int k = 0; k < 4; k++)
+			s += bf2f((uint16_t)(w[k] & 0xffffu)) + bf2f((uint16_t)(w[k] >> 16));
+	}
+	part[threadIdx.x] = s;
+	__syncthreads();
+	for (int off = 128; off > 0; off >>= 1) {
+		if ((int)threadIdx.x < off)
+			part[threadIdx.x] += part[threadIdx.x + off];
+		__syncthreads();
+	}
+	if (threadIdx.x == 0)
+		out[j] = part[0];
 }
 
 __global__ void k_sum_f32(const float* __restrict__ x, int64_t n, float* __restrict__ out)
@@ -386,11 +459,28 @@ __global__ void __launch_bounds__(256, 2) k_mlp_infer(const double* __restrict__
 		const int K = l == 0 ? IN : HID;
 		const uint16_t* W0 = w.W[l] + (int64_t)(nb + r) * K + kq;
 		const uint16_t* W1 = W0 + 32 * K;
-		/* c[h][t]: features nb + 32h + [0, 32) x states 32t + [0, 32) */
+		/* c[h][t]: features nb + 32h + [0, 32) x states 32t + [0, 32),
+		 * starting from the bias: lane (r, half) of tile (h, t) holds
+		 * features nb + 32h + 8(i/4) + 4 half + (i%4) in element i */
 		f32x16 c[2][4];
+		{
+			const float* bl = w.b[l] + nb + 4 * (lane >> 5);
 #pragma unroll
-		for (int t = 0; t < 4; t++)
-			c[0][t] = c[1][t] = f32x16{};
+			for (int h = 0; h < 2; h++) {
+				f32x16 b0;
+#pragma unroll
+				for (int g = 0; g < 4; g++) {
+					const float4 bias = *reinterpret_cast<const float4*>(bl + 32 * h + 8 * g);
+					b0[4 * g] = bias.x;
+					b0[4 * g + 1] = bias.y;
+					b0[4 * g + 2] = bias.z;
+					b0[4 * g + 3] = bias.w;
+				}
+#pragma unroll
+				for (int t = 0; t < 4; t++)
+					c[h][t] = b0;
+			}
+		}
 		bf16x8 w0 = ld8(W0), w1 = ld8(W1);
 		for (int k = 0; k < K; k += 16) {
 			bf16x8 nw0 = w0, nw1 = w1;
@@ -408,23 +498,28 @@ __global__ void __launch_bounds__(256, 2) k_mlp_infer(const double* __restrict__
 			w1 = nw1;
 		}
 		__syncthreads(); /* every wave has read this layer's input */
-		/* lane (r, half) of tile (h, t): state 32t + r, features
-		 * nb + 32h + 8(i/4) + 4 half + (i%4) for accumulator element i */
-		const float* bl = w.b[l] + nb + 4 * (lane >> 5);
+		/* lane (r, half) of tile (h, t): state 32t + r.  Each pair is
+		 * rounded to bf16 (v_cvt_pk_bf16_f32, nearest even) and then put
+		 * through the ReLU as two int16: a bf16 with its sign bit set is a
+		 * negative int16, so max(., 0) maps it to +0 and keeps the rest
+		 * (the same values as rounding max(x, 0)). */
 #pragma unroll
 		for (int h = 0; h < 2; h++) {
 #pragma unroll
 			for (int g = 0; g < 4; g++) {
-				const float4 bias = *reinterpret_cast<const float4*>(bl + 32 * h + 8 * g);
 #pragma unroll
 				for (int t = 0; t < 4; t++) {
 					const f32x16& v = c[h][t];
-					/* v_cvt_pk_bf16_f32: round to nearest even, two at a time */
-					const bf16x2 p0 = {(__bf16)fmaxf(v[4 * g] + bias.x, 0.0f), (__bf16)fmaxf(v[4 * g + 1] + bias.y, 0.0f)};
-					const bf16x2 p1 = {(__bf16)fmaxf(v[4 * g + 2] + bias.z, 0.0f), (__bf16)fmaxf(v[4 * g + 3] + bias.w, 0.0f)};
+					const bf16x2 p0 = {(__bf16)v[4 * g], (__bf16)v[4 * g + 1]};
+					const bf16x2 p1 = {(__bf16)v[4 * g + 2], (__bf16)v[4 * g + 3]};
+					s16x2 s0, s1;
+					std::memcpy(&s0, &p0, 4);
+					std::memcpy(&s1, &p1, 4);
+					s0 = __builtin_elementwise_max(s0, s16x2{0, 0});
+					s1 = __builtin_elementwise_max(s1, s16x2{0, 0});
 					uint2 u;
-					std::memcpy(&u.x, &p0, 4);
-					std::memcpy(&u.y, &p1, 4);
+					std::memcpy(&u.x, &s0, 4);
+					std::memcpy(&u.y, &s1, 4);
 					*reinterpret_cast<uint2*>(act + (32 * t + r) * ALD + nb + 32 * h + 8 * g + 4 * (lane >> 5)) = u;
 				}
 			}
@@ -548,6 +643,7 @@ struct se3mlp_net {
 	Buf Z;                      /* f32 output logits */
 	Buf D[LAYERS], DT[LAYERS];  /* bf16 gradient w.r.t. the pre-activation of layer l */
 	Buf loss, lsum;
+	Buf part;                   /* split-K partials of the weight gradients */
 	hipStream_t stream = nullptr;
 	std::mutex mu;
 };
@@ -642,7 +738,8 @@ int se3mlp_create(int64_t max_batch, uint64_t seed, se3mlp_net** out)
 		ok = hipMemcpy(net->w[l].p, hw.data(), nw * 4, hipMemcpyHostToDevice) == hipSuccess;
 	}
 	ok = ok && net->X.alloc((size_t)R * IN * 2) && net->XT.alloc((size_t)R * IN * 2) &&
-	     net->Z.alloc((size_t)R * OUT * 4) && net->loss.alloc((size_t)R * 4) && net->lsum.alloc(4);
+	     net->Z.alloc((size_t)R * OUT * 4) && net->loss.alloc((size_t)R * 4) && net->lsum.alloc(4) &&
+	     net->part.alloc((size_t)((R + SPLIT_K - 1) / SPLIT_K) * HID * HID * 4);
 	if (!ok) {
 		delete net;
 		return fail(SE3MLP_ENOMEM, "se3mlp_create: device allocation failed");
@@ -803,11 +900,19 @@ int se3mlp_train_step_device(se3mlp_net* net, const double* d_states, const uint
 	for (int l = LAYERS - 1; l >= 0; l--) {
 		const int no = kOut[l], ni = kIn[l];
 		const uint16_t* inT = l == 0 ? net->XT.as<uint16_t>() : net->HT[l - 1].as<uint16_t>();
-		/* weight gradient: gW[no x ni] = D_l^T (no x rows) . X_l (rows x ni) */
-		HCHK(gemm(st, net->DT[l].as<uint16_t>(), rows, inT, rows, no, ni, (int)rows, nullptr, 0, nullptr, 0,
-		          net->gw[l].as<float>(), ni, nullptr, 0, nullptr, 0));
-		hipLaunchKernelGGL(k_rowsum_bf16, dim3((unsigned)((no + 3) / 4)), dim3(256), 0, st, net->DT[l].as<uint16_t>(),
-		                   rows, no, net->gb[l].as<float>());
+		/* weight gradient: gW[no x ni] = D_l^T (no x rows) . X_l (rows x ni),
+		 * split over the rows */
+		{
+			Gemm g{net->DT[l].as<uint16_t>(), inT, rows, rows, no, ni, (int)rows, nullptr, 0, nullptr, 0,
+			       nullptr, 0, nullptr, 0, nullptr, 0};
+			const int parts = (int)((rows + SPLIT_K - 1) / SPLIT_K);
+			hipLaunchKernelGGL(k_gemm_nt_part, dim3((unsigned)((ni + 63) / 64), (unsigned)((no + 63) / 64), (unsigned)parts),
+			                   dim3(256), 0, st, g, net->part.as<float>());
+			hipLaunchKernelGGL(k_sum_parts, dim3(blocks((int64_t)no * ni)), dim3(256), 0, st, net->part.as<float>(),
+			                   parts, no, ni, net->gw[l].as<float>(), (int64_t)ni);
+		}
+		hipLaunchKernelGGL(k_rowsum_bf16, dim3((unsigned)no), dim3(256), 0, st, net->DT[l].as<uint16_t>(), rows, no,
+		                   net->gb[l].as<float>());
 		HCHK(hipGetLastError());
 		if (l > 0) {
 			/* input gradient through ReLU: D_{l-1} = (D_l . W_l) * [H_{l-1} > 0] */
