@@ -1717,11 +1717,16 @@ struct SegResult {
 	double last_tau, tau, lo, hi;
 };
 
-/* Fill the wave's segment record (every lane computes it; lane 0 stores). */
+/* Fill the wave's segment record.  Lane 0 alone computes it: the correctly
+ * rounded atan2, acos and sin it calls are out-of-line functions whose
+ * register saves go to scratch memory, so the other lanes sit out (their
+ * saves would be 63 times the traffic for the same record). */
 template <int MODE>
 __device__ __forceinline__ void seg_setup(SegShared* sh, const double* __restrict__ from,
                                           const double* __restrict__ to, double vd_in)
 {
+	if ((threadIdx.x & 63) != 0)
+		return;
 	double f[7], t[7];
 #pragma unroll
 	for (int k = 0; k < 7; k++) {
@@ -1736,7 +1741,7 @@ __device__ __forceinline__ void seg_setup(SegShared* sh, const double* __restric
 	}
 	se3_segment sg;
 	se3_segment_init(&sg, f, t);
-	if ((threadIdx.x & 63) == 0) {
+	{
 		sh->sg = sg;
 		sh->dist = dist;
 		sh->inv_dist = 1.0 / dist;
@@ -1869,7 +1874,9 @@ __device__ __forceinline__ void seg_store(const SegOut& o, int64_t seg, const Se
 		if (o.hi) o.hi[seg] = r.hi;
 		if (o.steps) o.steps[seg] = r.steps;
 	}
-	if (o.last_free && lane < 7) {
+	/* lane 0 alone again: se3_segment_at calls the out-of-line correctly
+	 * rounded sin and acos */
+	if (o.last_free && lane == 0) {
 		double a[7], b[7];
 		if (r.lo_is_from) {
 			for (int k = 0; k < 7; k++) a[k] = from[k];
@@ -1883,14 +1890,10 @@ __device__ __forceinline__ void seg_store(const SegOut& o, int64_t seg, const Se
 		} else {
 			se3_segment_at(&sh->sg, r.tau, b);
 		}
-		double va = a[0], vb = b[0];
-		for (int k = 1; k < 7; k++)
-			if (lane == k) {
-				va = a[k];
-				vb = b[k];
-			}
-		o.last_free[7 * seg + lane] = va;
-		o.state[7 * seg + lane] = vb;
+		for (int k = 0; k < 7; k++) {
+			o.last_free[7 * seg + k] = a[k];
+			o.state[7 * seg + k] = b[k];
+		}
 	}
 }
 
