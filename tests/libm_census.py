@@ -26,7 +26,7 @@ oracle (which the GPU parity tests show to agree with the engine):
 
 Usage:  python tests/libm_census.py [--workloads touch,uneven] [--threads 8]
         [--uneven-segments N] [--vis-milestones N] [--modes 0,1,2]
-Prints one JSON object.
+Prints one JSON line per workload as it finishes, then all of them.
 """
 import argparse
 import json
@@ -152,6 +152,7 @@ def main():
             out[wl] = census_visibility(a.threads, modes, a.vis_milestones)
         else:
             raise SystemExit(f"unknown workload {wl}")
+        print(json.dumps({wl: out[wl]}), flush=True)
     print(json.dumps(out))
 
 
