@@ -31,7 +31,6 @@ namespace se3mlp {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
-typedef short s16x2 __attribute__((ext_vector_type(2)));
 
 enum { IN = 32, HID = 256, OUT = 32, LAYERS = 5, OUT_USED = 2 };
 
@@ -396,6 +395,7 @@ __global__ void __launch_bounds__(256, 2) k_mlp_infer(const double* __restrict__
 {
 	extern __shared__ __align__(16) uint16_t act[]; /* FM x ALD */
 	__shared__ uint32_t wcount[4], wbase[4];
+	__shared__ float head[4][2 * 32]; /* per wave: the 32 rows' two logits */
 	const int wave = threadIdx.x >> 6;
 	const int lane = threadIdx.x & 63;
 	const int r = lane & 31, kq = (lane >> 5) * 8;
@@ -455,34 +455,48 @@ __global__ void __launch_bounds__(256, 2) k_mlp_infer(const double* __restrict__
 	}
 	__syncthreads();
 	const int nb = 64 * wave;
+	/* layer l's bias tile (lane (r, half) of tile h holds features nb + 32h
+	 * + 8(i/4) + 4 half + (i%4) in element i) and the weight fragments of
+	 * its first two k-steps, loaded while the previous layer finishes */
+	f32x16 pbt[2];
+	bf16x8 pw[4];
+	auto prefetch = [&](int l) {
+		const int K = l == 0 ? IN : HID;
+		const uint16_t* W0 = w.W[l] + (int64_t)(nb + r) * K + kq;
+		pw[0] = ld8(W0);
+		pw[1] = ld8(W0 + 32 * K);
+		pw[2] = ld8(W0 + 16); /* K >= 32 */
+		pw[3] = ld8(W0 + 32 * K + 16);
+		const float* bl = w.b[l] + nb + 4 * (lane >> 5);
+#pragma unroll
+		for (int h = 0; h < 2; h++) {
+#pragma unroll
+			for (int g = 0; g < 4; g++) {
+				const float4 bias = *reinterpret_cast<const float4*>(bl + 32 * h + 8 * g);
+				pbt[h][4 * g] = bias.x;
+				pbt[h][4 * g + 1] = bias.y;
+				pbt[h][4 * g + 2] = bias.z;
+				pbt[h][4 * g + 3] = bias.w;
+			}
+		}
+	};
+	prefetch(0);
 	for (int l = 0; l < LAYERS - 1; l++) {
 		const int K = l == 0 ? IN : HID;
 		const uint16_t* W0 = w.W[l] + (int64_t)(nb + r) * K + kq;
 		const uint16_t* W1 = W0 + 32 * K;
-		/* c[h][t]: features nb + 32h + [0, 32) x states 32t + [0, 32),
-		 * starting from the bias: lane (r, half) of tile (h, t) holds
-		 * features nb + 32h + 8(i/4) + 4 half + (i%4) in element i */
+		/* c[h][t]: features nb + 32h + [0, 32) x states 32t + [0, 32).
+		 * The first k-step adds its products to the bias tile, so no
+		 * accumulator is initialised separately. */
 		f32x16 c[2][4];
-		{
-			const float* bl = w.b[l] + nb + 4 * (lane >> 5);
+		bf16x8 w0 = pw[2], w1 = pw[3];
 #pragma unroll
-			for (int h = 0; h < 2; h++) {
-				f32x16 b0;
-#pragma unroll
-				for (int g = 0; g < 4; g++) {
-					const float4 bias = *reinterpret_cast<const float4*>(bl + 32 * h + 8 * g);
-					b0[4 * g] = bias.x;
-					b0[4 * g + 1] = bias.y;
-					b0[4 * g + 2] = bias.z;
-					b0[4 * g + 3] = bias.w;
-				}
-#pragma unroll
-				for (int t = 0; t < 4; t++)
-					c[h][t] = b0;
-			}
+		for (int t = 0; t < 4; t++) {
+			const bf16x8 a = ld8(act + (32 * t + r) * ALD + kq);
+			c[0][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pw[0], a, pbt[0], 0, 0, 0);
+			c[1][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pw[1], a, pbt[1], 0, 0, 0);
 		}
-		bf16x8 w0 = ld8(W0), w1 = ld8(W1);
-		for (int k = 0; k < K; k += 16) {
+		for (int k = 16; k < K; k += 16) {
 			bf16x8 nw0 = w0, nw1 = w1;
 			if (k + 16 < K) {
 				nw0 = ld8(W0 + k + 16);
@@ -497,6 +511,8 @@ __global__ void __launch_bounds__(256, 2) k_mlp_infer(const double* __restrict__
 			w0 = nw0;
 			w1 = nw1;
 		}
+		if (l + 1 < LAYERS - 1)
+			prefetch(l + 1);
 		__syncthreads(); /* every wave has read this layer's input */
 		/* lane (r, half) of tile (h, t): state 32t + r.  Each pair is
 		 * rounded to bf16 (v_cvt_pk_bf16_f32, nearest even) and then put
@@ -512,14 +528,13 @@ __global__ void __launch_bounds__(256, 2) k_mlp_infer(const double* __restrict__
 					const f32x16& v = c[h][t];
 					const bf16x2 p0 = {(__bf16)v[4 * g], (__bf16)v[4 * g + 1]};
 					const bf16x2 p1 = {(__bf16)v[4 * g + 2], (__bf16)v[4 * g + 3]};
-					s16x2 s0, s1;
-					std::memcpy(&s0, &p0, 4);
-					std::memcpy(&s1, &p1, 4);
-					s0 = __builtin_elementwise_max(s0, s16x2{0, 0});
-					s1 = __builtin_elementwise_max(s1, s16x2{0, 0});
 					uint2 u;
-					std::memcpy(&u.x, &s0, 4);
-					std::memcpy(&u.y, &s1, 4);
+					std::memcpy(&u.x, &p0, 4);
+					std::memcpy(&u.y, &p1, 4);
+					/* one instruction per pair; written out so that the
+					 * compiler does not split it into masks */
+					asm("v_pk_max_i16 %0, %0, 0" : "+v"(u.x));
+					asm("v_pk_max_i16 %0, %0, 0" : "+v"(u.y));
 					*reinterpret_cast<uint2*>(act + (32 * t + r) * ALD + nb + 32 * h + 8 * g + 4 * (lane >> 5)) = u;
 				}
 			}
@@ -527,38 +542,49 @@ __global__ void __launch_bounds__(256, 2) k_mlp_infer(const double* __restrict__
 		__syncthreads();
 	}
 	{
-		const uint16_t* W = w.W[LAYERS - 1];
-		f32x16 c = {};
-		for (int k = 0; k < HID; k += 16)
-			c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ld8(act + (32 * wave + r) * ALD + k + kq),
-			                                            ld8(W + (int64_t)r * HID + k + kq), c, 0, 0, 0);
-		const float bias = w.b[LAYERS - 1][r];
-		/* lanes 0 and 32: bit p of `pass` = row 32 wave + p of the tile passes */
-		uint32_t pass = 0;
+		const uint16_t* W = w.W[LAYERS - 1] + (int64_t)r * HID + kq;
+		/* all sixteen weight fragments at once: one load latency, not 16 */
+		bf16x8 hw[HID / 16];
 #pragma unroll
-		for (int i = 0; i < 16; i++) {
-			const float z = c[i] + bias;
-			const float z1 = __shfl(z, lane + 1, 64); /* column 1 of the same row */
-			const int p = 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
-			const int64_t m = row0 + 32 * wave + p;
-			if (r == 0 && m < n) {
-				const float d = z - z1;
-				const float pf = 1.0f / (1.0f + expf(-d));
-				if (pfree)
-					pfree[m] = pf;
-				if (margin)
-					margin[m] = d;
-				if (logits) {
-					logits[2 * m] = z;
-					logits[2 * m + 1] = z1;
-				}
-				if (pf > sa.threshold)
-					pass |= 1u << p;
+		for (int j = 0; j < HID / 16; j++)
+			hw[j] = ld8(W + 16 * j);
+		f32x16 c = {};
+#pragma unroll
+		for (int j = 0; j < HID / 16; j++)
+			c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ld8(act + (32 * wave + r) * ALD + 16 * j + kq), hw[j], c, 0,
+			                                            0, 0);
+		const float bias = w.b[LAYERS - 1][r];
+		/* the two used logit columns of the tile's 32 rows go to LDS (lanes
+		 * of columns 0 and 1: element i is row 8(i/4) + 4 half + i%4), then
+		 * lane p < 32 takes row 32 wave + p */
+		float* zs = head[wave];
+		if (r < OUT_USED) {
+#pragma unroll
+			for (int i = 0; i < 16; i++)
+				zs[2 * (8 * (i >> 2) + 4 * (lane >> 5) + (i & 3)) + r] = c[i] + bias;
+		}
+		__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+		__builtin_amdgcn_wave_barrier();
+		__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+		const int64_t m = row0 + 32 * wave + lane;
+		bool ok = false;
+		if (lane < 32 && m < n) {
+			const float z = zs[2 * lane], z1 = zs[2 * lane + 1];
+			const float d = z - z1;
+			const float pf = 1.0f / (1.0f + expf(-d));
+			if (pfree)
+				pfree[m] = pf;
+			if (margin)
+				margin[m] = d;
+			if (logits) {
+				logits[2 * m] = z;
+				logits[2 * m + 1] = z1;
 			}
+			ok = pf > sa.threshold;
 		}
 		if (MODE != INFER_PLAIN) {
-			const uint32_t rows = (uint32_t)__builtin_amdgcn_readlane((int)pass, 0) |
-			                      (uint32_t)__builtin_amdgcn_readlane((int)pass, 32);
+			/* bit p: row 32 wave + p of the tile passes */
+			const uint32_t rows = (uint32_t)__ballot(ok);
 			/* items of this wave's tile that pass: rows, or segments whose
 			 * rows all pass (a segment's rows never straddle tiles) */
 			uint32_t sel = rows;
