@@ -37,6 +37,7 @@
 #include "engine.h"
 #include "launch.h"
 #include "se3_math.h"
+#include "cull.h"
 
 using namespace se3cq;
 
@@ -446,13 +447,6 @@ void sym_eigen3(double A[3][3], double E[3][3])
 	}
 }
 
-float round_up_f(double x)
-{
-	float f = (float)x;
-	if ((double)f < x)
-		f = std::nextafter(f, INFINITY);
-	return f;
-}
 
 /* Fit a box to the vertices of triangles perm[lo..hi) and store it in
  * binary32 such that every vertex v satisfies |a_i . (v - c)| <= e_i for the
@@ -705,30 +699,6 @@ DevScene base_scene(const se3cq_world* w)
 	return s;
 }
 
-/* Perturbation data for the device (see PertData). */
-PertData make_pert(const double* p)
-{
-	PertData d;
-	std::memset(&d, 0, sizeof(d));
-	se3_quat_to_rotmat(p[3], p[4], p[5], p[6], d.Re);
-	for (int i = 0; i < 3; i++)
-		d.te[i] = p[i];
-	double qn[4] = {p[3], p[4], p[5], p[6]};
-	se3_quat_normalize(qn);
-	double Rh[9];
-	se3_quat_to_rotmat(qn[0], qn[1], qn[2], qn[3], Rh);
-	double fr = 0.0, fd = 0.0;
-	for (int i = 0; i < 3; i++)
-		for (int j = 0; j < 3; j++) {
-			d.PeT[3 * i + j] = (float)Rh[3 * j + i];
-			fr += d.Re[3 * i + j] * d.Re[3 * i + j];
-			const double e = d.Re[3 * i + j] - Rh[3 * i + j];
-			fd += e * e;
-		}
-	d.fro_Re = round_up_f(std::sqrt(fr) * (1.0 + 1e-9));
-	d.fro_dRe = round_up_f(std::sqrt(3.0 * fd) * (1.0 + 1e-9) + (fd > 0 ? 1e-15 : 0.0));
-	return d;
-}
 
 /* Select the device block of w->pert, adding one if this value has none
  * yet (call with mu held, after the upload). */
