@@ -568,6 +568,156 @@ __device__ bool coop_collide(const DevScene& s, const Loader& ld, int64_t i, con
 #ifndef SE3CQ_HEAVY_MAX
 #define SE3CQ_HEAVY_MAX 12
 #endif
+/* Once the launch's counter is spent and the wavefront's pool is empty, no
+ * lane takes a new state, and a lane still walking one alone holds the
+ * wavefront: states past SE3CQ_HEAVY_TAIL times the stack depth (if that is
+ * the lower limit) are set aside then too, up to SE3CQ_HEAVY_TAIL_MAX per
+ * wavefront, and all states set aside are walked together by
+ * coop_collide_many. */
+#ifndef SE3CQ_HEAVY_TAIL
+#define SE3CQ_HEAVY_TAIL 8
+#endif
+#ifndef SE3CQ_HEAVY_TAIL_MAX
+#define SE3CQ_HEAVY_TAIL_MAX 64
+#endif
+/* states walked together at most (the rest wait for the next group) */
+#ifndef SE3CQ_MANY_GROUP
+#define SE3CQ_MANY_GROUP 64
+#endif
+static_assert(SE3CQ_MANY_GROUP >= 1 && SE3CQ_MANY_GROUP <= 64, "a group's slots live one per lane");
+static_assert(SE3CQ_HEAVY_TAIL_MAX <= 64 && SE3CQ_HEAVY_MAX <= SE3CQ_HEAVY_TAIL_MAX, "set-aside states live one per lane");
+
+/* OR of a 64-bit value over the wavefront's lanes. */
+__device__ __forceinline__ uint64_t wave_or64(uint64_t v)
+{
+	uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+#pragma unroll
+	for (int off = 32; off >= 1; off >>= 1) {
+		lo |= (uint32_t)__shfl_xor((int)lo, off, 64);
+		hi |= (uint32_t)__shfl_xor((int)hi, off, 64);
+	}
+	return ((uint64_t)hi << 32) | lo;
+}
+
+/*
+ * Several states, each "some triangle pair intersects", decided together
+ * by the whole wavefront: slot j (j < nh) is the state whose index lane j
+ * of (hv_hi, hv_lo) holds.  A LIFO stack of (environment node, slot and
+ * robot node) entries, two words each, fills the lanes' traversal stacks
+ * (free by then); every pass pops up to 64 entries of any slots, one per
+ * lane, reads the entry's culling transform from its slot's lane (computed
+ * once per slot), tests the pair, tests it exactly if both are leaves, and
+ * pushes the children of an overlapping inner pair.  Entries of a slot
+ * already found colliding are dropped.  Returns the mask of colliding slots; *ok is false when the
+ * stack would overflow (the mask then holds only the collisions found).
+ * Robot node ids must stay below 2^26.
+ */
+template <bool WIDE, class Loader>
+__device__ uint64_t coop_collide_many(const DevScene& s, const Loader& ld, int hv_lo, int hv_hi, int nh,
+                                      const PairStack<WIDE>& stk, bool* ok)
+{
+	const int lane = threadIdx.x & 63;
+	uint32_t* col0 = stk.base - lane; /* lane 0's column of this wavefront */
+	const int stride = stk.stride;
+	const int rows = (s.stack_depth + LEAF_BUF) * (WIDE ? 2 : 1);
+	const int cap = 32 * rows; /* entries of two words */
+	const uint64_t all = nh >= 64 ? ~0ull : ((1ull << nh) - 1ull);
+	auto word = [&](int w) -> uint32_t* { return col0 + (w & 63) + (size_t)(w >> 6) * stride; };
+	/* lane j keeps slot j's culling transform; a pass reads each entry's
+	 * from its slot's lane */
+	QCull mc{};
+	if (lane < nh) {
+		const int64_t i = (int64_t)(((uint64_t)(uint32_t)hv_hi << 32) | (uint64_t)(uint32_t)hv_lo);
+		double q[7];
+		ld.get(i, q);
+		make_qcull(s, q, mc);
+		*word(2 * lane) = 0u;
+		*word(2 * lane + 1) = (uint32_t)lane << 26;
+	}
+	int sp = nh;
+	uint64_t hit = 0;
+	wave_lds_sync();
+	for (;;) {
+		const int nproc = sp < 64 ? sp : 64;
+		int a = 0, b = 0, slot = 0;
+		if (lane < nproc) {
+			const int e = sp - 1 - lane;
+			a = (int)*word(2 * e);
+			const uint32_t w1 = *word(2 * e + 1);
+			slot = (int)(w1 >> 26);
+			b = (int)(w1 & ((1u << 26) - 1u));
+		}
+		sp -= nproc;
+		wave_lds_sync();
+		/* every lane takes part in the shuffles */
+		QCull c;
+#pragma unroll
+		for (int k = 0; k < 9; k++)
+			c.O[k] = __shfl(mc.O[k], slot, 64);
+#pragma unroll
+		for (int k = 0; k < 3; k++)
+			c.T[k] = __shfl(mc.T[k], slot, 64);
+		c.infl = __shfl(mc.infl, slot, 64);
+		c.eta = __shfl(mc.eta, slot, 64);
+		const int ilo = __shfl(hv_lo, slot, 64), ihi = __shfl(hv_hi, slot, 64);
+		const bool live = lane < nproc && !((hit >> slot) & 1ull);
+		bool inner = false, h = false;
+		int a1 = 0, b1 = 0, a2 = 0, b2 = 0;
+		if (live) {
+			const GNode A = load_node(s.env.nodes, a);
+			RobBox B;
+			load_rob_box(load_node(s.rob.nodes, b), c, B);
+			if (!obb_disjoint(A, B, c.eta)) {
+				const int ca = __float_as_int(A.a2_child.w);
+				const bool la = ca < 0, lb = B.child < 0;
+				if (la && lb) {
+					const int64_t i = (int64_t)(((uint64_t)(uint32_t)ihi << 32) | (uint64_t)(uint32_t)ilo);
+					double q[7];
+					ld.get(i, q);
+					h = leaf_pair(s, q, ~ca, ~B.child);
+				} else {
+					inner = true;
+					/* FCL's rule, as in the one-lane traversal */
+					const float sa = fmaf(A.c_ex.w, A.c_ex.w, fmaf(A.a0_ey.w, A.a0_ey.w, A.a1_ez.w * A.a1_ez.w));
+					const float eb0 = B.e[0] - c.infl, eb1 = B.e[1] - c.infl, eb2 = B.e[2] - c.infl;
+					const float sb = fmaf(eb0, eb0, fmaf(eb1, eb1, eb2 * eb2));
+					if (!la && (lb || sa > sb)) {
+						a1 = ca + 1; b1 = b; a2 = ca; b2 = b;
+					} else {
+						a1 = a; b1 = B.child + 1; a2 = a; b2 = B.child;
+					}
+				}
+			}
+		}
+		if (__ballot(h)) {
+			hit |= wave_or64(h ? (1ull << slot) : 0ull);
+			if ((hit & all) == all) {
+				*ok = true;
+				return hit;
+			}
+		}
+		const uint64_t mi = __ballot(inner);
+		const int ni = __popcll(mi);
+		if (sp + 2 * ni > cap) {
+			*ok = false;
+			return hit;
+		}
+		if (inner) {
+			const int e = sp + 2 * lane_rank(mi);
+			const uint32_t tag = (uint32_t)slot << 26;
+			*word(2 * e) = (uint32_t)a1;
+			*word(2 * e + 1) = tag | (uint32_t)b1;
+			*word(2 * e + 2) = (uint32_t)a2;
+			*word(2 * e + 3) = tag | (uint32_t)b2;
+		}
+		sp += 2 * ni;
+		wave_lds_sync();
+		if (sp == 0) {
+			*ok = true;
+			return hit;
+		}
+	}
+}
 
 
 template <bool WIDE, class Loader, class Store>
@@ -591,13 +741,19 @@ __device__ void states_refill(const DevScene& s, const Loader& ld, int64_t n, co
 	 * j-th index */
 	int hv_lo = 0, hv_hi = 0, nheavy = 0;
 	const int heavy_tests = SE3CQ_HEAVY_PER_LEVEL * s.stack_depth;
+	const int tail_tests = SE3CQ_HEAVY_TAIL * s.stack_depth;
+	/* the multi-state walk tags entries with 26-bit robot node ids */
+	const bool many = s.rob.nnodes < (1 << 26);
 	/* states per counter add (at least 1; the launch sets it) */
 	const int chunk = s.pool_chunk > 0 ? s.pool_chunk : REFILL_CHUNK;
 	for (;;) {
 		{
-			uint64_t dm = __ballot(idx >= 0 && tests > heavy_tests);
+			const bool tail = many && exhausted && pool_left == 0;
+			const int lim = tail && tail_tests < heavy_tests ? tail_tests : heavy_tests;
+			const int hmax = tail ? SE3CQ_HEAVY_TAIL_MAX : SE3CQ_HEAVY_MAX;
+			uint64_t dm = __ballot(idx >= 0 && tests > lim);
 			uint64_t taken = 0;
-			while (dm && nheavy < SE3CQ_HEAVY_MAX) {
+			while (dm && nheavy < hmax) {
 				const int L = __ffsll((long long)dm) - 1;
 				dm &= dm - 1;
 				hv_lo = wrl(rdl((int)(uint32_t)idx, L), nheavy, hv_lo);
@@ -721,12 +877,36 @@ __device__ void states_refill(const DevScene& s, const Loader& ld, int64_t n, co
 		}
 	}
 	/* every lane is idle and the states have run out: the whole wavefront
-	 * now walks each state set aside */
+	 * now walks the states set aside, together; should their stack
+	 * overflow, each undecided one alone */
+	if (nheavy == 0)
+		return;
+	/* decided: slots whose answer is known; hit: those found colliding */
+	uint64_t decided = 0, hit = 0;
+	if (many) {
+		for (int g0 = 0; g0 < nheavy; g0 += SE3CQ_MANY_GROUP) {
+			const int gn = nheavy - g0 < SE3CQ_MANY_GROUP ? nheavy - g0 : SE3CQ_MANY_GROUP;
+			const int glo = __shfl(hv_lo, (lane + g0) & 63, 64), ghi = __shfl(hv_hi, (lane + g0) & 63, 64);
+			bool ok = false;
+			const uint64_t h = coop_collide_many<WIDE>(s, ld, glo, ghi, gn, stk, &ok);
+			const uint64_t gm = gn >= 64 ? ~0ull : ((1ull << gn) - 1ull);
+			hit |= h << g0;
+			decided |= (ok ? gm : h) << g0;
+			wave_lds_sync();
+		}
+	}
+	if (lane < nheavy && ((decided >> lane) & 1ull)) {
+		const int64_t i = (int64_t)(((uint64_t)(uint32_t)hv_hi << 32) | (uint64_t)(uint32_t)hv_lo);
+		set(i, (hit >> lane) & 1ull);
+	}
+	/* should the shared stack have overflowed: each undecided state alone */
 	for (int j = 0; j < nheavy; j++) {
+		if ((decided >> j) & 1ull)
+			continue;
 		const int64_t i = (int64_t)(((uint64_t)(uint32_t)rdl(hv_hi, j) << 32) | (uint64_t)(uint32_t)rdl(hv_lo, j));
-		const bool hit = coop_collide<WIDE>(s, ld, i, stk, leaves);
+		const bool h = coop_collide<WIDE>(s, ld, i, stk, leaves);
 		if (lane == 0)
-			set(i, hit);
+			set(i, h);
 	}
 }
 
