@@ -1070,37 +1070,43 @@ enum { PACKET_LEAVES = SE3CQ_PACKET_LEAVES };  /* at most 64: one lane per entry
 static_assert(PACKET_LEAVES >= 1 && PACKET_LEAVES <= 64, "leaf entries live one per lane");
 
 /*
- * Where the traversal gets its pose.  The pose is recomputed at the two
- * places that need it (the culling transform of each traversal phase and the
- * exact transform of each leaf phase) instead of staying live in 14 VGPRs
- * through the traversal, which cuts the spills of the packet kernels by two
- * thirds.  The empty asm makes the parameter opaque, so the compiler cannot
- * hoist the recomputation back out of the traversal loop.
+ * Where the traversal gets its pose.  The pose is read at the two places
+ * that need it (the culling transform of each traversal phase and the exact
+ * transform of each leaf phase) instead of staying live in 14 VGPRs through
+ * the traversal, which cuts the spills of the packet kernels by two thirds.
+ *
+ * LdsPose: the wavefront's 64 poses, evaluated once per packet and staged in
+ * LDS as 7 rows of 64 binary64 values (lane l's pose is column l, so a row
+ * read touches consecutive banks).  A read costs seven LDS loads instead of
+ * the interpolation (two sines, two divisions) per phase.  The empty asm
+ * makes the lane index opaque, so the compiler cannot hoist the loads out of
+ * the traversal loop.
  */
-struct SegPose {
-	const se3_segment* sg; /* LDS */
-	double tau;
+enum { POSE_LDS_DOUBLES = 7 * 64, POSE_LDS_BYTES = POSE_LDS_DOUBLES * 8 };
+
+struct LdsPose {
+	const double* pq; /* LDS, POSE_LDS_DOUBLES */
 	__device__ __forceinline__ void get(double* o) const
 	{
-		double t = tau;
-		asm volatile("" : "+v"(t));
-		se3_segment_at_fast(sg, t, o);
+		int l = threadIdx.x & 63;
+		asm volatile("" : "+v"(l));
+#pragma unroll
+		for (int k = 0; k < 7; k++)
+			o[k] = pq[64 * k + l];
 	}
 };
 
-struct OmplPose {
-	const se3_ompl_motion* m; /* LDS */
-	const OmplXf* x;          /* LDS */
-	int64_t k;                /* the validator's sample index */
-	__device__ __forceinline__ void get(double* o) const
-	{
-		int64_t kk = k;
-		asm volatile("" : "+v"(kk));
-		double st[7];
-		se3_ompl_motion_sample(m, kk, st, 1);
-		se3_ompl_to_unit(x->M, x->pert, x->delta, st, o);
-	}
-};
+/* Stores each lane's pose q into the wavefront's pose rows (every lane of
+ * the wavefront calls it). */
+__device__ __forceinline__ void wave_lds_sync();
+__device__ __forceinline__ void stage_pose(double* pq, const double* q)
+{
+	const int lane = threadIdx.x & 63;
+#pragma unroll
+	for (int k = 0; k < 7; k++)
+		pq[64 * k + lane] = q[k];
+	wave_lds_sync();
+}
 
 template <bool FIRST, class Pose>
 __device__ uint64_t packet_collide(const DevScene& s, const Pose& pose, uint64_t live)
@@ -1246,11 +1252,17 @@ struct SegShared {
 };
 
 /* Wave-level check of one pose per lane (lanes outside `live` are ignored):
- * returns the mask of colliding lanes. */
+ * returns the mask of colliding lanes.  pq: the wavefront's pose rows. */
 template <bool FIRST>
-__device__ __forceinline__ uint64_t seg_poses_collide(const DevScene& s, const SegShared* sh, double tau, uint64_t live)
+__device__ __forceinline__ uint64_t seg_poses_collide(const DevScene& s, const SegShared* sh, double* pq, double tau,
+                                                      uint64_t live)
 {
-	return packet_collide<FIRST>(s, SegPose{&sh->sg, tau}, live);
+	{
+		double q[7];
+		se3_segment_at_fast(&sh->sg, tau, q);
+		stage_pose(pq, q);
+	}
+	return packet_collide<FIRST>(s, LdsPose{pq}, live);
 }
 
 /*
@@ -1279,7 +1291,7 @@ __device__ __forceinline__ int steps_exact(double vd, double dist)
  * same 0.5*(lo+hi) sequence the sequential search would use), then walks the
  * tree with the ballot of results. */
 template <bool WIDE>
-__device__ void bisect_wave(const DevScene& s, const SegShared* sh, double& lo, double& hi, int levels)
+__device__ void bisect_wave(const DevScene& s, const SegShared* sh, double* pq, double& lo, double& hi, int levels)
 {
 	const int lane = threadIdx.x & 63;
 	for (int done = 0; done < levels;) {
@@ -1302,7 +1314,7 @@ __device__ void bisect_wave(const DevScene& s, const SegShared* sh, double& lo, 
 			mid = 0.5 * (l0 + h0);
 		}
 		const uint64_t live = __ballot(act);
-		const uint64_t hitm = seg_poses_collide<false>(s, sh, act ? mid : lo, live);
+		const uint64_t hitm = seg_poses_collide<false>(s, sh, pq, act ? mid : lo, live);
 		ok = !((hitm >> lane) & 1ull);
 		const uint64_t vm = __ballot(act && ok);
 		int h = 1;
@@ -1345,6 +1357,7 @@ struct BisectShared {
 	double xf[32][12];                           /* exact transform per probe: R row-major, T */
 	SweepShared sw;
 };
+static_assert(sizeof(BisectShared) >= POSE_LDS_BYTES, "a touch search's pose rows live in its BisectShared");
 
 /* Make this wave's LDS stores visible to its own later LDS loads. */
 __device__ __forceinline__ void wave_lds_sync()
@@ -1613,7 +1626,7 @@ __device__ void bisect_touch(const DevScene& s, const SegShared* sh, double& lo,
 	if (nc >= 0)
 		bisect_cands(s, sh, lo, hi, levels, bs, nc);
 	else
-		bisect_wave<WIDE>(s, sh, lo, hi, levels);
+		bisect_wave<WIDE>(s, sh, reinterpret_cast<double*>(bs), lo, hi, levels);
 }
 
 /* ---- walk chunks: the same candidate search over 64 consecutive steps ---- */
@@ -1715,8 +1728,8 @@ __device__ uint64_t walk_cands_collide(const DevScene& s, const se3_segment* sg,
 /* Colliding lanes of a walk chunk (lowest one exact): with SWEEP by
  * candidate lists when they fit, else by the packet traversal. */
 template <bool SWEEP>
-__device__ __forceinline__ uint64_t walk_collide(const DevScene& s, const SegShared* sh, SweepShared* sw, double tau,
-                                                 uint64_t live)
+__device__ __forceinline__ uint64_t walk_collide(const DevScene& s, const SegShared* sh, SweepShared* sw, double* pq,
+                                                 double tau, uint64_t live)
 {
 	if (SWEEP) {
 		const int l0 = __ffsll((long long)live) - 1;
@@ -1726,7 +1739,7 @@ __device__ __forceinline__ uint64_t walk_collide(const DevScene& s, const SegSha
 		if (nc >= 0)
 			return walk_cands_collide(s, &sh->sg, sw, nc, tau, live);
 	}
-	return seg_poses_collide<true>(s, sh, tau, live);
+	return seg_poses_collide<true>(s, sh, pq, tau, live);
 }
 
 /* Wave-uniform outcome of one segment walk. */
@@ -1777,7 +1790,7 @@ __device__ __forceinline__ void seg_setup(SegShared* sh, const double* __restric
 }
 
 template <bool WIDE, int MODE, bool SWEEP = false>
-__device__ SegResult segment_walk(const DevScene& s, const SegShared* sh, int levels, BisectShared* bs,
+__device__ SegResult segment_walk(const DevScene& s, const SegShared* sh, double* pq, int levels, BisectShared* bs,
                                   SweepShared* sw = nullptr)
 {
 	const int lane = threadIdx.x & 63;
@@ -1837,7 +1850,7 @@ __device__ SegResult segment_walk(const DevScene& s, const SegShared* sh, int le
 		}
 		const double tau = d * inv_dist;
 		const uint64_t am = __ballot(active);
-		const uint64_t cm = am ? walk_collide<SWEEP>(s, sh, sw, active ? tau : sh->vd * inv_dist, am) : 0ull;
+		const uint64_t cm = am ? walk_collide<SWEEP>(s, sh, sw, pq, active ? tau : sh->vd * inv_dist, am) : 0ull;
 		if (cm) {
 			const int f = __ffsll((long long)cm) - 1;
 			const double thi = shfl_d(tau, f);
@@ -1928,6 +1941,9 @@ __global__ void __launch_bounds__(256, MODE == MODE_TOUCH ? SE3CQ_TOUCH_WAVES : 
 	const int wave = threadIdx.x >> 6;
 	SegShared* sh = &segsh[wave];
 	BisectShared* bs = (MODE == MODE_TOUCH) ? reinterpret_cast<BisectShared*>(seg_lds) + wave : nullptr;
+	/* the pose rows: a touch search's walk and its candidate bisection take
+	 * turns on the wavefront's BisectShared */
+	double* pq = (MODE == MODE_TOUCH) ? reinterpret_cast<double*>(bs) : seg_lds + wave * POSE_LDS_DOUBLES;
 	int64_t items = n;
 	if (s.n_items) {
 		const int64_t c = (int64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)*s.n_items);
@@ -1951,7 +1967,7 @@ __global__ void __launch_bounds__(256, MODE == MODE_TOUCH ? SE3CQ_TOUCH_WAVES : 
 			continue; /* an index list from the caller may hold anything */
 		seg_setup<MODE>(sh, from_all + 7 * seg, to_all + 7 * seg, vd);
 		wave_lds_sync();
-		const SegResult r = segment_walk<WIDE, MODE>(s, sh, levels, bs);
+		const SegResult r = segment_walk<WIDE, MODE>(s, sh, pq, levels, bs);
 		seg_store(o, seg, r, sh, from_all + 7 * seg, to_all + 7 * seg);
 		wave_lds_sync();
 	}
@@ -1984,6 +2000,9 @@ __global__ void __launch_bounds__(256, SE3CQ_MIN_WAVES) k_batch(DevScene s, Batc
 	PairStack<WIDE> stk, leaves;
 	if (STATES)
 		thread_stacks<WIDE>(s, smem, stk, leaves);
+	/* the pose rows follow the traversal stacks */
+	double* pq = reinterpret_cast<double*>(smem + (STATES ? (size_t)lds_words_per_thread(s) * blockDim.x : 0)) +
+	             wave * POSE_LDS_DOUBLES;
 	const int64_t per_item = 64 * (int64_t)a.states_per_lane;
 	const int64_t total = a.n_seg + (a.n_states + per_item - 1) / per_item;
 	const int depth = s.queue_depth > 0 ? s.queue_depth : 1;
@@ -2003,7 +2022,7 @@ __global__ void __launch_bounds__(256, SE3CQ_MIN_WAVES) k_batch(DevScene s, Batc
 			const double* to = a.seg_to + 7 * item;
 			seg_setup<MODE_TRANSIT>(sh, from, to, a.vd);
 			wave_lds_sync();
-			const SegResult r = segment_walk<WIDE, MODE_TRANSIT>(s, sh, 0, nullptr);
+			const SegResult r = segment_walk<WIDE, MODE_TRANSIT>(s, sh, pq, 0, nullptr);
 			seg_store(a.seg, item, r, sh, from, to);
 			/* every lane is done with the record before the next item
 			 * overwrites it */
@@ -2036,6 +2055,10 @@ __device__ __forceinline__ void tri_pair(int64_t p, int64_t n, int64_t& fi, int6
 /* Visibility: one wavefront per pair.  out8: int8 matrix (or pair vector),
  * bits: bit-packed row-major n x n (upper and lower triangles), either may be
  * null. */
+/* A visibility wavefront's LDS: its pose rows or its swept-search lists. */
+enum { VIS_WAVE_DOUBLES = (sizeof(SweepShared) > POSE_LDS_BYTES ? sizeof(SweepShared) : POSE_LDS_BYTES) / 8 };
+static_assert(sizeof(SweepShared) % 8 == 0, "the regions are whole binary64 values");
+
 template <bool WIDE, int VIS>
 __global__ void __launch_bounds__(256, SE3CQ_VIS_WAVES) k_visibility(DevScene s, const double* __restrict__ qs0,
                                                     const double* __restrict__ qs1, int64_t m, int64_t n,
@@ -2047,7 +2070,10 @@ __global__ void __launch_bounds__(256, SE3CQ_VIS_WAVES) k_visibility(DevScene s,
 	const int wave = threadIdx.x >> 6;
 	const int lane = threadIdx.x & 63;
 	SegShared* sh = &segsh[wave];
-	SweepShared* sw = reinterpret_cast<SweepShared*>(vis_lds) + wave;
+	/* the pose rows of the packet traversal and the candidate lists of the
+	 * swept search take turns on the wavefront's region */
+	double* pq = vis_lds + wave * VIS_WAVE_DOUBLES;
+	SweepShared* sw = reinterpret_cast<SweepShared*>(pq);
 	for (;;) {
 		const int64_t p = next_item(s.queue);
 		if (p >= npairs)
@@ -2064,7 +2090,7 @@ __global__ void __launch_bounds__(256, SE3CQ_VIS_WAVES) k_visibility(DevScene s,
 		}
 		seg_setup<MODE_TRANSITION>(sh, qs0 + 7 * fi, qs1 + 7 * ti, ivd);
 		wave_lds_sync();
-		const SegResult r = segment_walk<WIDE, MODE_TRANSITION, SE3CQ_WALK_SWEEP != 0>(s, sh, 0, nullptr, sw);
+		const SegResult r = segment_walk<WIDE, MODE_TRANSITION, SE3CQ_WALK_SWEEP != 0>(s, sh, pq, 0, nullptr, sw);
 		const uint8_t done = (uint8_t)r.done;
 		if (lane == 0) {
 			if (VIS == VIS_MATRIX) {
@@ -2115,9 +2141,10 @@ __global__ void __launch_bounds__(256, SE3CQ_OMPL_WAVES) k_ompl_motions(DevScene
                                                       int32_t* __restrict__ out, int32_t* __restrict__ samples)
 {
 	/* the motion record and the transform sit in LDS (broadcast reads), so
-	 * the packet traversal keeps its registers */
+	 * the packet traversal keeps its registers; so do the pose rows */
 	__shared__ se3_ompl_motion msh[4];
 	__shared__ OmplXf xsh;
+	extern __shared__ double ompl_lds[];
 	const int wave = threadIdx.x >> 6;
 	const int lane = threadIdx.x & 63;
 	if (threadIdx.x < 16)
@@ -2128,6 +2155,14 @@ __global__ void __launch_bounds__(256, SE3CQ_OMPL_WAVES) k_ompl_motions(DevScene
 		xsh.delta[threadIdx.x - 23] = x.delta[threadIdx.x - 23];
 	__syncthreads();
 	se3_ompl_motion* m = &msh[wave];
+	double* pq = ompl_lds + wave * POSE_LDS_DOUBLES;
+	/* sample k of the motion, in the unit world, staged as the pose rows */
+	auto stage = [&](int64_t k) {
+		double st[7], o[7];
+		se3_ompl_motion_sample(m, k, st, 1);
+		se3_ompl_to_unit(xsh.M, xsh.pert, xsh.delta, st, o);
+		stage_pose(pq, o);
+	};
 	for (;;) {
 		const int64_t p = next_item(s.queue);
 		if (p >= n)
@@ -2154,7 +2189,8 @@ __global__ void __launch_bounds__(256, SE3CQ_OMPL_WAVES) k_ompl_motions(DevScene
 				const int64_t k = 1 + C * lane;
 				const bool active = k <= K;
 				const uint64_t am = __ballot(active);
-				if (packet_collide<true>(s, OmplPose{m, &xsh, active ? k : K}, am))
+				stage(active ? k : K);
+				if (packet_collide<true>(s, LdsPose{pq}, am))
 					valid = 0;
 			}
 			/* then every sample, 64 neighbours per pass */
@@ -2162,7 +2198,8 @@ __global__ void __launch_bounds__(256, SE3CQ_OMPL_WAVES) k_ompl_motions(DevScene
 				const int64_t k = 1 + 64 * c + lane;
 				const bool active = k <= K;
 				const uint64_t am = __ballot(active);
-				if (packet_collide<true>(s, OmplPose{m, &xsh, active ? k : K}, am))
+				stage(active ? k : K);
+				if (packet_collide<true>(s, LdsPose{pq}, am))
 					valid = 0;
 			}
 		}
@@ -2626,13 +2663,21 @@ hipError_t launch_ompl_motions(const DevScene& s, const OmplXf& x, const double*
 	hipError_t e = zero_queue(s, st);
 	if (e != hipSuccess)
 		return e;
+	const size_t lds = (block / 64) * (size_t)POSE_LDS_BYTES;
 	if (s.wide)
-		hipLaunchKernelGGL(k_ompl_motions<true>, dim3(persistent_grid(s, k_ompl_motions<true>, block, 0, n)),
-		                   dim3(block), 0, st, s, x, q0, q1, n, lvs, out, samples);
+		hipLaunchKernelGGL(k_ompl_motions<true>, dim3(persistent_grid(s, k_ompl_motions<true>, block, lds, n)),
+		                   dim3(block), lds, st, s, x, q0, q1, n, lvs, out, samples);
 	else
-		hipLaunchKernelGGL(k_ompl_motions<false>, dim3(persistent_grid(s, k_ompl_motions<false>, block, 0, n)),
-		                   dim3(block), 0, st, s, x, q0, q1, n, lvs, out, samples);
+		hipLaunchKernelGGL(k_ompl_motions<false>, dim3(persistent_grid(s, k_ompl_motions<false>, block, lds, n)),
+		                   dim3(block), lds, st, s, x, q0, q1, n, lvs, out, samples);
 	return hipGetLastError();
+}
+
+/* k_batch's LDS: the traversal stacks of a batch with states, then one
+ * wavefront's pose rows per wavefront. */
+static inline size_t batch_lds_bytes(const DevScene& s, int block, bool states)
+{
+	return (states ? stack_bytes(s, block) : 0) + (size_t)(block / 64) * POSE_LDS_BYTES;
 }
 
 hipError_t launch_batch(const DevScene& s, const BatchArgs& a, int block, int grid, hipStream_t st)
@@ -2645,7 +2690,7 @@ hipError_t launch_batch(const DevScene& s, const BatchArgs& a, int block, int gr
 	if (e != hipSuccess)
 		return e;
 	const bool states = a.n_states > 0;
-	const size_t lds = states ? stack_bytes(s, block) : 0;
+	const size_t lds = batch_lds_bytes(s, block, states);
 	if (states && s.wide)
 		hipLaunchKernelGGL((k_batch<true, true>), dim3((unsigned)grid), dim3(block), lds, st, s, a);
 	else if (states)
@@ -2662,7 +2707,7 @@ int batch_default_block(const DevScene& s, bool states) { return states ? pick_b
 int batch_blocks_per_cu(const DevScene& s, int block, bool states)
 {
 	int nb = 0;
-	const size_t lds = states ? stack_bytes(s, block) : 0;
+	const size_t lds = batch_lds_bytes(s, block, states);
 	hipError_t e;
 	if (states)
 		e = s.wide ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch<true, true>, block, lds)
@@ -2742,11 +2787,11 @@ template <int MODE>
 static hipError_t launch_seg_mode(const DevScene& s_in, const double* from, const double* to, int64_t n, double vd,
                                   int levels, const SegOut& o, hipStream_t st, void* order_scratch)
 {
-	/* one wavefront per segment, four per block; the touch search keeps a
-	 * BisectShared record per wavefront in LDS */
+	/* one wavefront per segment, four per block; each wavefront keeps its
+	 * pose rows in LDS, within a BisectShared record for the touch search */
 	const int block = 256;
 	const int per_block = block / 64;
-	const size_t lds = (MODE == MODE_TOUCH) ? per_block * sizeof(BisectShared) : 0;
+	const size_t lds = per_block * ((MODE == MODE_TOUCH) ? sizeof(BisectShared) : (size_t)POSE_LDS_BYTES);
 	DevScene s = s_in;
 	const unsigned grid = s.wide ? persistent_grid(s, k_segments<true, MODE>, block, lds, n)
 	                             : persistent_grid(s, k_segments<false, MODE>, block, lds, n);
@@ -2787,7 +2832,7 @@ static hipError_t launch_vis_mode(const DevScene& s, const double* qs0, const do
                                   hipStream_t st)
 {
 	const int block = 256;
-	const size_t lds = (block / 64) * sizeof(SweepShared);
+	const size_t lds = (block / 64) * (size_t)VIS_WAVE_DOUBLES * sizeof(double);
 	hipError_t e = zero_queue(s, st);
 	if (e != hipSuccess)
 		return e;
