@@ -647,6 +647,28 @@ void build_model(HostModel& m, const Mesh& me, const float* M, int leaf_size)
 		q.push_back({first + 1, mid, it.hi, it.depth + 1});
 	}
 	m.nodes.resize(next);
+	/* each box's axes stored thinnest first: the culling test tries its
+	 * face axes in storage order and a wavefront leaves it once every lane
+	 * has found a separating axis, which the thinnest axis is the likeliest
+	 * to be (the separating-axis test itself does not depend on the order
+	 * or the handedness of the axes) */
+#ifndef SE3CQ_AXES_THIN_FIRST
+#define SE3CQ_AXES_THIN_FIRST 1
+#endif
+	if (SE3CQ_AXES_THIN_FIRST)
+		for (GNode& g : m.nodes) {
+			float ax[3][4] = {{g.a0_ey.x, g.a0_ey.y, g.a0_ey.z, g.c_ex.w},
+			                  {g.a1_ez.x, g.a1_ez.y, g.a1_ez.z, g.a0_ey.w},
+			                  {g.a2_child.x, g.a2_child.y, g.a2_child.z, g.a1_ez.w}};
+			int o[3] = {0, 1, 2};
+			std::stable_sort(o, o + 3, [&](int a, int b) { return ax[a][3] < ax[b][3]; });
+			g.a0_ey = {ax[o[0]][0], ax[o[0]][1], ax[o[0]][2], ax[o[1]][3]};
+			g.a1_ez = {ax[o[1]][0], ax[o[1]][1], ax[o[1]][2], ax[o[2]][3]};
+			g.c_ex.w = ax[o[0]][3];
+			g.a2_child.x = ax[o[2]][0];
+			g.a2_child.y = ax[o[2]][1];
+			g.a2_child.z = ax[o[2]][2];
+		}
 	/* triangles in leaf order */
 	std::vector<double> mesh_order;
 	mesh_order.swap(m.tris);
