@@ -580,6 +580,12 @@ __device__ bool coop_collide(const DevScene& s, const Loader& ld, int64_t i, con
 #ifndef SE3CQ_HEAVY_TAIL_MAX
 #define SE3CQ_HEAVY_TAIL_MAX 64
 #endif
+/* In that tail, once at most SE3CQ_TAIL_LANES lanes still walk a state, all
+ * of those states are set aside as well: the joint walk takes 64 node
+ * pairs per pass, where a lone lane's walk is one dependent chain. */
+#ifndef SE3CQ_TAIL_LANES
+#define SE3CQ_TAIL_LANES 8
+#endif
 /* states walked together at most (the rest wait for the next group) */
 #ifndef SE3CQ_MANY_GROUP
 #define SE3CQ_MANY_GROUP 64
@@ -749,7 +755,11 @@ __device__ void states_refill(const DevScene& s, const Loader& ld, int64_t n, co
 	for (;;) {
 		{
 			const bool tail = many && exhausted && pool_left == 0;
-			const int lim = tail && tail_tests < heavy_tests ? tail_tests : heavy_tests;
+			const uint64_t busy = __ballot(idx >= 0);
+			/* in the tail, once few lanes still walk, all of them stop and
+			 * the wavefront walks their states together */
+			const bool few = tail && __popcll(busy) <= SE3CQ_TAIL_LANES;
+			const int lim = few ? -1 : (tail && tail_tests < heavy_tests ? tail_tests : heavy_tests);
 			const int hmax = tail ? SE3CQ_HEAVY_TAIL_MAX : SE3CQ_HEAVY_MAX;
 			uint64_t dm = __ballot(idx >= 0 && tests > lim);
 			uint64_t taken = 0;
