@@ -586,11 +586,6 @@ __device__ bool coop_collide(const DevScene& s, const Loader& ld, int64_t i, con
 #ifndef SE3CQ_TAIL_LANES
 #define SE3CQ_TAIL_LANES 8
 #endif
-/* states walked together at most (the rest wait for the next group) */
-#ifndef SE3CQ_MANY_GROUP
-#define SE3CQ_MANY_GROUP 64
-#endif
-static_assert(SE3CQ_MANY_GROUP >= 1 && SE3CQ_MANY_GROUP <= 64, "a group's slots live one per lane");
 static_assert(SE3CQ_HEAVY_TAIL_MAX <= 64 && SE3CQ_HEAVY_MAX <= SE3CQ_HEAVY_TAIL_MAX, "set-aside states live one per lane");
 
 /* OR of a 64-bit value over the wavefront's lanes. */
@@ -614,9 +609,9 @@ __device__ __forceinline__ uint64_t wave_or64(uint64_t v)
  * lane, reads the entry's culling transform from its slot's lane (computed
  * once per slot), tests the pair, tests it exactly if both are leaves, and
  * pushes the children of an overlapping inner pair.  Entries of a slot
- * already found colliding are dropped.  Returns the mask of colliding slots; *ok is false when the
- * stack would overflow (the mask then holds only the collisions found).
- * Robot node ids must stay below 2^26.
+ * already found colliding are dropped.  Returns the mask of colliding
+ * slots; *ok is false when the stack would overflow (the mask then holds
+ * only the collisions found).  Robot node ids must stay below 2^26.
  */
 template <bool WIDE, class Loader>
 __device__ uint64_t coop_collide_many(const DevScene& s, const Loader& ld, int hv_lo, int hv_hi, int nh,
@@ -894,16 +889,10 @@ __device__ void states_refill(const DevScene& s, const Loader& ld, int64_t n, co
 	/* decided: slots whose answer is known; hit: those found colliding */
 	uint64_t decided = 0, hit = 0;
 	if (many) {
-		for (int g0 = 0; g0 < nheavy; g0 += SE3CQ_MANY_GROUP) {
-			const int gn = nheavy - g0 < SE3CQ_MANY_GROUP ? nheavy - g0 : SE3CQ_MANY_GROUP;
-			const int glo = __shfl(hv_lo, (lane + g0) & 63, 64), ghi = __shfl(hv_hi, (lane + g0) & 63, 64);
-			bool ok = false;
-			const uint64_t h = coop_collide_many<WIDE>(s, ld, glo, ghi, gn, stk, &ok);
-			const uint64_t gm = gn >= 64 ? ~0ull : ((1ull << gn) - 1ull);
-			hit |= h << g0;
-			decided |= (ok ? gm : h) << g0;
-			wave_lds_sync();
-		}
+		bool ok = false;
+		hit = coop_collide_many<WIDE>(s, ld, hv_lo, hv_hi, nheavy, stk, &ok);
+		decided = ok ? (nheavy >= 64 ? ~0ull : ((1ull << nheavy) - 1ull)) : hit;
+		wave_lds_sync();
 	}
 	if (lane < nheavy && ((decided >> lane) & 1ull)) {
 		const int64_t i = (int64_t)(((uint64_t)(uint32_t)hv_hi << 32) | (uint64_t)(uint32_t)hv_lo);
