@@ -1050,13 +1050,20 @@ __device__ __forceinline__ bool leaf_pair_x(const DevScene& s, const ExactTf& x,
  * exact binary64 tests on the queue.  The culling transform is rebuilt at
  * the start of every traversal phase so it is dead while the SATs run.
  *
- * Stack entries are node pairs known to overlap for the lanes of their mask
- * (the root pair is tested first).  A pop splits the pair by FCL's rule and
- * tests both children for those lanes in the same pass, then pushes the
- * overlapping inner children and queues the overlapping leaf pairs: the two
- * children's node loads overlap, a child that overlaps for no lane never
- * costs a pop (about half of all tested pairs on the touch workload), and
- * the pop's bookkeeping is paid once per two tests.
+ * Here a stack entry is a node pair to test for the lanes of its mask; a
+ * pop tests it and pushes both children of an overlapping inner pair.  In
+ * packet_collide_eager a stack entry is a pair known to overlap for its
+ * lanes (the root pair is tested first); a pop splits it by FCL's rule and
+ * tests both children in the same pass, then pushes the overlapping inner
+ * children and queues the overlapping leaf pairs: the two children's node
+ * loads overlap, a child that overlaps for no lane never costs a pop (about
+ * half of all tested pairs on the touch workload), and the pop's
+ * bookkeeping is paid once per two tests.  Measured on MI355X, the eager
+ * walk takes the visibility matrix from 2.78 to 2.50 s and OMPL motions
+ * from 112 to 108 ms, leaves the touch search and uneven segments within
+ * 1 %, and makes the plain transition walk of the shell scene slower (exact
+ * segment checks 595 -> 615 ms, prescreen re-verification 157 -> 182 ms),
+ * so the plain segment kernels keep this pair-at-a-time walk.
  *
  * With FIRST, pairs are taken in "focus" order: the topmost stack entry
  * holding the lowest lane that still has pending pairs.  Lanes are then
@@ -1075,6 +1082,10 @@ __device__ __forceinline__ bool leaf_pair_x(const DevScene& s, const ExactTf& x,
 #endif
 enum { PACKET_LEAVES = SE3CQ_PACKET_LEAVES };  /* at most 64: one lane per entry */
 static_assert(PACKET_LEAVES >= 2 && PACKET_LEAVES <= 64, "leaf entries live one per lane");
+/* the touch search walks with packet_collide_eager (1) or packet_collide (0) */
+#ifndef SE3CQ_TOUCH_EAGER
+#define SE3CQ_TOUCH_EAGER 1
+#endif
 
 /*
  * Where the traversal gets its pose.  The pose is read at the two places
@@ -1117,6 +1128,139 @@ __device__ __forceinline__ void stage_pose(double* pq, const double* q)
 
 template <bool FIRST, class Pose>
 __device__ uint64_t packet_collide(const DevScene& s, const Pose& pose, uint64_t live)
+{
+	const int lane = threadIdx.x & 63;
+	int va = 0, vb = 0, vml = 0, vmh = 0;
+	va = wrl(0, 0, va);
+	vb = wrl(0, 0, vb);
+	vml = wrl((int)(uint32_t)live, 0, vml);
+	vmh = wrl((int)(uint32_t)(live >> 32), 0, vmh);
+	int sp = 1;
+	uint64_t hit = 0;
+	int focus = __ffsll((long long)live) - 1;
+	const int focus_cap = 62 - s.stack_depth;
+	for (;;) {
+		int lte = 0, ltr = 0, lml = 0, lmh = 0;
+		int nl = 0;
+		{
+			QCull c;
+			{
+				double q[7];
+				pose.get(q);
+				make_qcull(s, q, c);
+			}
+			/* the robot box of the last robot node, reused while the env
+			 * side is split */
+			int cached_b = -1;
+			RobBox B;
+			while (sp > 0 && nl < PACKET_LEAVES) {
+				int p = sp - 1;
+				if (FIRST && sp <= focus_cap) {
+					/* lanes below `focus` have no pending pairs; lanes at
+					 * or above the lowest hit need none */
+					uint64_t hb = 0;
+					const int stop = hit ? __ffsll((long long)hit) - 1 : 64;
+					while (focus < stop) {
+						const int word = (focus < 32) ? vml : vmh;
+						hb = __ballot(lane < sp && ((word >> (focus & 31)) & 1));
+						if (hb)
+							break;
+						focus++;
+					}
+					if (!hb) {
+						sp = 0;
+						break;
+					}
+					p = 63 - __clzll((long long)hb);
+				}
+				const int a = rdl(va, p), b = rdl(vb, p);
+				uint64_t m = ((uint64_t)(uint32_t)rdl(vmh, p) << 32) | (uint64_t)(uint32_t)rdl(vml, p);
+				sp--;
+				if (p != sp) {
+					/* the top entry fills the hole */
+					va = wrl(rdl(va, sp), p, va);
+					vb = wrl(rdl(vb, sp), p, vb);
+					vml = wrl(rdl(vml, sp), p, vml);
+					vmh = wrl(rdl(vmh, sp), p, vmh);
+				}
+				m &= ~hit;
+				if (FIRST && hit)
+					m &= (hit & (~hit + 1ull)) - 1ull;
+				if (m == 0)
+					continue;
+				const GNode A = load_node_uniform(s.env.nodes, a);
+				if (b != cached_b) {
+					load_rob_box(load_node_uniform(s.rob.nodes, b), c, B);
+					cached_b = b;
+				}
+				const bool ov = ((m >> lane) & 1ull) && !obb_disjoint(A, B, c.eta);
+				const uint64_t mo = __ballot(ov);
+				if (mo == 0)
+					continue;
+				const int ca = __float_as_int(A.a2_child.w);
+				const int cb = B.child;
+				const bool la = ca < 0, lb = cb < 0;
+				const int ml = (int)(uint32_t)mo, mh = (int)(uint32_t)(mo >> 32);
+				if (la && lb) {
+					lte = wrl(~ca, nl, lte);
+					ltr = wrl(~cb, nl, ltr);
+					lml = wrl(ml, nl, lml);
+					lmh = wrl(mh, nl, lmh);
+					nl++;
+					continue;
+				}
+				if (sp + 2 > 64 || (!FIRST && sp + 2 > s.stack_depth)) {
+					if (lane == 0)
+						atomicOr(s.error_flag, 1u);
+					return live;
+				}
+				/* FCL's rule: split the env node unless it is a leaf or the
+				 * robot box is the larger one (squared extent norms). */
+				const float sa = fmaf(A.c_ex.w, A.c_ex.w, fmaf(A.a0_ey.w, A.a0_ey.w, A.a1_ez.w * A.a1_ez.w));
+				const float eb0 = B.e[0] - c.infl, eb1 = B.e[1] - c.infl, eb2 = B.e[2] - c.infl;
+				const float sb = fmaf(eb0, eb0, fmaf(eb1, eb1, eb2 * eb2));
+				int a1, b1, a2, b2;
+				if (!la && (lb || sa > sb)) {
+					a1 = ca + 1; b1 = b; a2 = ca; b2 = b;
+				} else {
+					a1 = a; b1 = cb + 1; a2 = a; b2 = cb;
+				}
+				va = wrl(a1, sp, va); vb = wrl(b1, sp, vb); vml = wrl(ml, sp, vml); vmh = wrl(mh, sp, vmh);
+				sp++;
+				va = wrl(a2, sp, va); vb = wrl(b2, sp, vb); vml = wrl(ml, sp, vml); vmh = wrl(mh, sp, vmh);
+				sp++;
+			}
+		}
+		if (nl > 0) {
+			ExactTf x;
+			{
+				double q[7];
+				pose.get(q);
+				make_exact(s, q, x);
+			}
+			for (int i = 0; i < nl; i++) {
+				uint64_t m = ((uint64_t)(uint32_t)rdl(lmh, i) << 32) | (uint64_t)(uint32_t)rdl(lml, i);
+				m &= ~hit;
+				if (FIRST && hit)
+					m &= (hit & (~hit + 1ull)) - 1ull;
+				if (m == 0)
+					continue;
+				const int te = rdl(lte, i), tr = rdl(ltr, i);
+				const bool h = ((m >> lane) & 1ull) && leaf_pair_x(s, x, te, tr);
+				hit |= __ballot(h);
+			}
+			if ((hit & live) == live || (FIRST && (hit & live & (~live + 1ull))))
+				return hit & live;
+		}
+		if (sp == 0)
+			return hit & live;
+	}
+}
+
+/* The same answer as packet_collide, walking pairs known to overlap and
+ * testing both children of each (see above). */
+template <bool FIRST, class Pose>
+__device__ uint64_t packet_collide_eager(const DevScene& s, const Pose& pose, uint64_t live)
 {
 	const int lane = threadIdx.x & 63;
 	int va = 0, vb = 0, vml = 0, vmh = 0;
@@ -1313,7 +1457,7 @@ struct SegShared {
 
 /* Wave-level check of one pose per lane (lanes outside `live` are ignored):
  * returns the mask of colliding lanes.  pq: the wavefront's pose rows. */
-template <bool FIRST>
+template <bool FIRST, bool EAGER>
 __device__ __forceinline__ uint64_t seg_poses_collide(const DevScene& s, const SegShared* sh, double* pq, double tau,
                                                       uint64_t live)
 {
@@ -1322,7 +1466,7 @@ __device__ __forceinline__ uint64_t seg_poses_collide(const DevScene& s, const S
 		se3_segment_at_fast(&sh->sg, tau, q);
 		stage_pose(pq, q);
 	}
-	return packet_collide<FIRST>(s, LdsPose{pq}, live);
+	return EAGER ? packet_collide_eager<FIRST>(s, LdsPose{pq}, live) : packet_collide<FIRST>(s, LdsPose{pq}, live);
 }
 
 /*
@@ -1374,7 +1518,7 @@ __device__ void bisect_wave(const DevScene& s, const SegShared* sh, double* pq, 
 			mid = 0.5 * (l0 + h0);
 		}
 		const uint64_t live = __ballot(act);
-		const uint64_t hitm = seg_poses_collide<false>(s, sh, pq, act ? mid : lo, live);
+		const uint64_t hitm = seg_poses_collide<false, SE3CQ_TOUCH_EAGER != 0>(s, sh, pq, act ? mid : lo, live);
 		ok = !((hitm >> lane) & 1ull);
 		const uint64_t vm = __ballot(act && ok);
 		int h = 1;
@@ -1787,7 +1931,7 @@ __device__ uint64_t walk_cands_collide(const DevScene& s, const se3_segment* sg,
 
 /* Colliding lanes of a walk chunk (lowest one exact): with SWEEP by
  * candidate lists when they fit, else by the packet traversal. */
-template <bool SWEEP>
+template <bool SWEEP, bool EAGER>
 __device__ __forceinline__ uint64_t walk_collide(const DevScene& s, const SegShared* sh, SweepShared* sw, double* pq,
                                                  double tau, uint64_t live)
 {
@@ -1799,7 +1943,7 @@ __device__ __forceinline__ uint64_t walk_collide(const DevScene& s, const SegSha
 		if (nc >= 0)
 			return walk_cands_collide(s, &sh->sg, sw, nc, tau, live);
 	}
-	return seg_poses_collide<true>(s, sh, pq, tau, live);
+	return seg_poses_collide<true, EAGER>(s, sh, pq, tau, live);
 }
 
 /* Wave-uniform outcome of one segment walk. */
@@ -1849,7 +1993,7 @@ __device__ __forceinline__ void seg_setup(SegShared* sh, const double* __restric
 	}
 }
 
-template <bool WIDE, int MODE, bool SWEEP = false>
+template <bool WIDE, int MODE, bool SWEEP = false, bool EAGER = (MODE == MODE_TOUCH && SE3CQ_TOUCH_EAGER)>
 __device__ SegResult segment_walk(const DevScene& s, const SegShared* sh, double* pq, int levels, BisectShared* bs,
                                   SweepShared* sw = nullptr)
 {
@@ -1910,7 +2054,7 @@ __device__ SegResult segment_walk(const DevScene& s, const SegShared* sh, double
 		}
 		const double tau = d * inv_dist;
 		const uint64_t am = __ballot(active);
-		const uint64_t cm = am ? walk_collide<SWEEP>(s, sh, sw, pq, active ? tau : sh->vd * inv_dist, am) : 0ull;
+		const uint64_t cm = am ? walk_collide<SWEEP, EAGER>(s, sh, sw, pq, active ? tau : sh->vd * inv_dist, am) : 0ull;
 		if (cm) {
 			const int f = __ffsll((long long)cm) - 1;
 			const double thi = shfl_d(tau, f);
@@ -2150,7 +2294,7 @@ __global__ void __launch_bounds__(256, SE3CQ_VIS_WAVES) k_visibility(DevScene s,
 		}
 		seg_setup<MODE_TRANSITION>(sh, qs0 + 7 * fi, qs1 + 7 * ti, ivd);
 		wave_lds_sync();
-		const SegResult r = segment_walk<WIDE, MODE_TRANSITION, SE3CQ_WALK_SWEEP != 0>(s, sh, pq, 0, nullptr, sw);
+		const SegResult r = segment_walk<WIDE, MODE_TRANSITION, SE3CQ_WALK_SWEEP != 0, true>(s, sh, pq, 0, nullptr, sw);
 		const uint8_t done = (uint8_t)r.done;
 		if (lane == 0) {
 			if (VIS == VIS_MATRIX) {
@@ -2250,7 +2394,7 @@ __global__ void __launch_bounds__(256, SE3CQ_OMPL_WAVES) k_ompl_motions(DevScene
 				const bool active = k <= K;
 				const uint64_t am = __ballot(active);
 				stage(active ? k : K);
-				if (packet_collide<true>(s, LdsPose{pq}, am))
+				if (packet_collide_eager<true>(s, LdsPose{pq}, am))
 					valid = 0;
 			}
 			/* then every sample, 64 neighbours per pass */
@@ -2259,7 +2403,7 @@ __global__ void __launch_bounds__(256, SE3CQ_OMPL_WAVES) k_ompl_motions(DevScene
 				const bool active = k <= K;
 				const uint64_t am = __ballot(active);
 				stage(active ? k : K);
-				if (packet_collide<true>(s, LdsPose{pq}, am))
+				if (packet_collide_eager<true>(s, LdsPose{pq}, am))
 					valid = 0;
 			}
 		}
