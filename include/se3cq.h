@@ -279,7 +279,7 @@ int se3cq_batch_set_config(se3cq_world* w, const se3cq_batch_config* cfg);
  *               captured with can overflow their traversal stacks, which is
  *               reported as an error).
  *   pool_chunk  states a wavefront of the state kernels takes per counter
- *               add (16-1024, default 128).
+ *               add (16-1024, default 64).
  *   queue_depth segments (or batch items) a wavefront of the segment and
  *               batch kernels takes per counter add (1-16, default 1). */
 typedef struct se3cq_tuning {

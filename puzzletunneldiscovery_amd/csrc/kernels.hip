@@ -415,7 +415,7 @@ struct StatesOmpl {
 #define SE3CQ_LEAF_PATIENCE 16
 #endif
 #ifndef SE3CQ_REFILL_CHUNK
-#define SE3CQ_REFILL_CHUNK 128
+#define SE3CQ_REFILL_CHUNK 64
 #endif
 enum { REFILL_MIN = SE3CQ_REFILL_MIN, LEAF_PATIENCE = SE3CQ_LEAF_PATIENCE, REFILL_CHUNK = SE3CQ_REFILL_CHUNK };
 

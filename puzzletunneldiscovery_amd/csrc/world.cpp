@@ -161,7 +161,7 @@ struct se3cq_world {
 	hipStream_t stream = nullptr; /* the world's own stream (tuning, error checks) */
 	DevBuf d_nodes[2], d_tris[2], d_tid[2], d_tpos[2];
 	int leaf_size = 1;     /* triangles per BVH leaf, at most */
-	int pool_chunk = 128;  /* states a wavefront takes per counter add (state kernels) */
+	int pool_chunk = 64;   /* states a wavefront takes per counter add (state kernels) */
 	int queue_depth = 1;   /* work items a wavefront takes per counter add (segments, batches) */
 	/* [0]: error bits of _device launches, kept until se3cq_check takes
 	 * them; [1]: where se3cq_check's kernel puts them */

@@ -29,7 +29,7 @@ def _free_starts(uw, n, seed):
 
 def test_host_calls_from_two_threads_overlap(torus_meshes, torus_oracle):
     """module.cc:138-184 releases the GIL on the const query methods, so
-    Python threads query one world at once.  Thread A walks 32 segments of 2^20
+    Python threads query one world at once.  Thread A walks 32 segments of 2^21
     steps each far outside the scene (each walk holds one wavefront for tens
     of milliseconds, the rest of the GPU stays free); thread B, started once A
     is inside its call, makes 20 state queries.  B's calls all end before A's
@@ -41,7 +41,7 @@ def test_host_calls_from_two_threads_overlap(torus_meshes, torus_oracle):
     far[:, :3] = np.floor(far[:, :3] * 64) / 64 + 5.0   # dyadic, so the 0.25 move is exact
     ends = far.copy()
     ends[:, 0] += 0.25
-    vd = 0.25 / 2 ** 20
+    vd = 0.25 / 2 ** 21
     qs = [W.random_states(10_000, seed=410 + k) for k in range(20)]
     want_b = [torus_oracle.valid_states(q).astype(bool) for q in qs]
     _flush("threads: reference walk alone")
@@ -73,7 +73,7 @@ def test_host_calls_from_two_threads_overlap(torus_meshes, torus_oracle):
     assert not ta.is_alive() and not tb.is_alive()
     for a, b in zip(t["ra"], alone):
         assert np.array_equal(a, b)
-    assert t["ra"][2].all() and (t["ra"][5] >= 2 ** 20 - 1).all()
+    assert t["ra"][2].all() and (t["ra"][5] >= 2 ** 21 - 1).all()
     for g, w_ in zip(got_b, want_b):
         assert np.array_equal(g, w_)
     assert t["b0"] < t["a1"] and t["b1"] < t["a1"], t

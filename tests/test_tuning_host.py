@@ -12,7 +12,7 @@ def world(torus_meshes):
 
 
 def test_leaf_size_rebuilds_the_trees(world):
-    assert world.tuning == {"leaf_size": 1, "pool_chunk": 128, "queue_depth": 1}
+    assert world.tuning == {"leaf_size": 1, "pool_chunk": 64, "queue_depth": 1}
     d1, n1 = world.bvh_info(0)
     assert n1 == 2 * 2048 - 1
     prev = (d1, n1)
