@@ -83,6 +83,7 @@ struct DevScene {
 	uint32_t* error_flag;  /* device word: bit 0 traversal overflow, bit 1 a segment with 2^31+ steps */
 	unsigned long long* queue; /* this launch's work counter (persistent kernels zero it first) */
 	const int32_t* order;  /* segment kernels: item k walks segment order[k] (null: segment k) */
+	const void* seg_rec;   /* segment kernels: segment k's precomputed record (null: computed in the walk) */
 	const uint32_t* n_items; /* segment kernels: device count of items (null: one per segment) */
 	int32_t num_cu;        /* compute units of the device (persistent grid size) */
 	int32_t pool_chunk;    /* state kernels: states a wavefront takes per counter add */

@@ -1960,11 +1960,9 @@ struct SegResult {
  * register saves go to scratch memory, so the other lanes sit out (their
  * saves would be 63 times the traffic for the same record). */
 template <int MODE>
-__device__ __forceinline__ void seg_setup(SegShared* sh, const double* __restrict__ from,
-                                          const double* __restrict__ to, double vd_in)
+__device__ __forceinline__ void seg_record(SegShared* sh, const double* __restrict__ from,
+                                           const double* __restrict__ to, double vd_in)
 {
-	if ((threadIdx.x & 63) != 0)
-		return;
 	double f[7], t[7];
 #pragma unroll
 	for (int k = 0; k < 7; k++) {
@@ -1992,6 +1990,35 @@ __device__ __forceinline__ void seg_setup(SegShared* sh, const double* __restric
 		sh->bad = (dist == dist) && !(vd >= dist) && !(dist / vd < 2147483648.0 && vd > 0.0);
 	}
 }
+
+/* Lane 0 computes the wavefront's segment record. */
+template <int MODE>
+__device__ __forceinline__ void seg_setup(SegShared* sh, const double* __restrict__ from,
+                                          const double* __restrict__ to, double vd_in)
+{
+	if ((threadIdx.x & 63) == 0)
+		seg_record<MODE>(sh, from, to, vd_in);
+}
+
+/*
+ * Every segment's record before the walk, one segment per lane.  In the
+ * walk a record is one dependent chain of correctly rounded functions on a
+ * single lane (about 54 us per segment on a wavefront of the touch search:
+ * 0.86 ms of wavefront time for 16 segments each); here 64 segments share
+ * each instruction.  The walk copies its record into LDS.  (Its register
+ * bound is the segment kernels': the out-of-line functions it shares with
+ * them are compiled once for all their callers.)
+ */
+template <int MODE>
+__global__ void __launch_bounds__(256, SE3CQ_SEG_WAVES) k_seg_records(const double* __restrict__ from, const double* __restrict__ to,
+                                                     int64_t n, double vd, SegShared* __restrict__ rec)
+{
+	const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n)
+		seg_record<MODE>(rec + i, from + 7 * i, to + 7 * i, vd);
+}
+enum { SEG_REC_WORDS = sizeof(SegShared) / 8 };
+static_assert(sizeof(SegShared) % 8 == 0 && SEG_REC_WORDS <= 64, "a record is copied one word per lane");
 
 template <bool WIDE, int MODE, bool SWEEP = false, bool EAGER = (MODE == MODE_TOUCH && SE3CQ_TOUCH_EAGER)>
 __device__ SegResult segment_walk(const DevScene& s, const SegShared* sh, double* pq, int levels, BisectShared* bs,
@@ -2169,7 +2196,14 @@ __global__ void __launch_bounds__(256, MODE == MODE_TOUCH ? SE3CQ_TOUCH_WAVES : 
 		const int64_t seg = s.order ? (int64_t)s.order[item] : item;
 		if (seg < 0 || seg >= n)
 			continue; /* an index list from the caller may hold anything */
-		seg_setup<MODE>(sh, from_all + 7 * seg, to_all + 7 * seg, vd);
+		if (s.seg_rec) {
+			const int lane = threadIdx.x & 63;
+			if (lane < SEG_REC_WORDS)
+				reinterpret_cast<uint64_t*>(sh)[lane] =
+				    reinterpret_cast<const uint64_t*>(static_cast<const SegShared*>(s.seg_rec) + seg)[lane];
+		} else {
+			seg_setup<MODE>(sh, from_all + 7 * seg, to_all + 7 * seg, vd);
+		}
 		wave_lds_sync();
 		const SegResult r = segment_walk<WIDE, MODE>(s, sh, pq, levels, bs);
 		seg_store(o, seg, r, sh, from_all + 7 * seg, to_all + 7 * seg);
@@ -2929,6 +2963,13 @@ static constexpr size_t order_bytes_for(int64_t n)
 	return 2 * ORDER_BINS * sizeof(uint32_t) + align256((size_t)n) + (size_t)n * sizeof(int32_t);
 }
 
+size_t segment_record_bytes(int64_t n)
+{
+	if (n <= 0 || n >= ((int64_t)1 << 31))
+		return 0;
+	return align256((size_t)n * sizeof(SegShared));
+}
+
 size_t segment_order_bytes(int64_t n)
 {
 	if (n <= 0 || n >= ((int64_t)1 << 31))
@@ -2989,7 +3030,7 @@ static hipError_t build_order(const double* from, const double* to, int64_t n, d
 
 template <int MODE>
 static hipError_t launch_seg_mode(const DevScene& s_in, const double* from, const double* to, int64_t n, double vd,
-                                  int levels, const SegOut& o, hipStream_t st, void* order_scratch)
+                                  int levels, const SegOut& o, hipStream_t st, void* order_scratch, void* rec_scratch)
 {
 	/* one wavefront per segment, four per block; each wavefront keeps its
 	 * pose rows in LDS, within a BisectShared record for the touch search */
@@ -3005,6 +3046,18 @@ static hipError_t launch_seg_mode(const DevScene& s_in, const double* from, cons
 		hipError_t e = build_order<MODE>(from, to, n, vd, order_scratch, &s.order, st);
 		if (e != hipSuccess)
 			return e;
+		/* many segments per wavefront: their records first, 64 per
+		 * wavefront instruction (uneven shell segments 300 -> 290 ms; the
+		 * touch search, whose set-up chains hide behind other wavefronts'
+		 * walks, measured no gain and computes them in the walk) */
+		if (MODE != MODE_TOUCH && rec_scratch && segment_record_bytes(n)) {
+			hipLaunchKernelGGL(k_seg_records<MODE>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, from, to, n, vd,
+			                   static_cast<SegShared*>(rec_scratch));
+			e = hipGetLastError();
+			if (e != hipSuccess)
+				return e;
+			s.seg_rec = rec_scratch;
+		}
 	}
 	hipError_t e = zero_queue(s, st);
 	if (e != hipSuccess)
@@ -3017,15 +3070,17 @@ static hipError_t launch_seg_mode(const DevScene& s_in, const double* from, cons
 }
 
 hipError_t launch_segments(const DevScene& s, int mode, const double* from, const double* to, int64_t n, double vd,
-                           int levels, const SegOut& o, hipStream_t st, void* order_scratch)
+                           int levels, const SegOut& o, hipStream_t st, void* order_scratch, void* rec_scratch)
 {
 	if (n <= 0)
 		return hipSuccess;
 	switch (mode) {
-	case MODE_TRANSIT: return launch_seg_mode<MODE_TRANSIT>(s, from, to, n, vd, levels, o, st, order_scratch);
+	case MODE_TRANSIT:
+		return launch_seg_mode<MODE_TRANSIT>(s, from, to, n, vd, levels, o, st, order_scratch, rec_scratch);
 	case MODE_TRANSITION:
-		return launch_seg_mode<MODE_TRANSITION>(s, from, to, n, vd, levels, o, st, order_scratch);
-	case MODE_TOUCH: return launch_seg_mode<MODE_TOUCH>(s, from, to, n, vd, levels, o, st, order_scratch);
+		return launch_seg_mode<MODE_TRANSITION>(s, from, to, n, vd, levels, o, st, order_scratch, rec_scratch);
+	case MODE_TOUCH:
+		return launch_seg_mode<MODE_TOUCH>(s, from, to, n, vd, levels, o, st, order_scratch, rec_scratch);
 	default: return hipErrorInvalidValue;
 	}
 }

@@ -27,11 +27,26 @@ hipError_t launch_valid_states(const DevScene& s, const double* q, int64_t n, ui
 /* mode: 0 transit (fixed verify_delta), 1 transition (min(d/2, ivd)), 2 touch (transit + bisection).
  * With order_scratch (segment_order_bytes(n) bytes of device memory, or
  * null), a launch of more segments than resident wavefronts first sorts
- * them longest first into that scratch and walks them in that order. */
+ * them longest first into that scratch and walks them in that order; with
+ * rec_scratch as well (segment_record_bytes(n) bytes, 256-byte aligned, or
+ * null), it also computes every segment's record beforehand, one per lane. */
 hipError_t launch_segments(const DevScene& s, int mode, const double* from, const double* to, int64_t n,
-                           double vd, int levels, const SegOut& o, hipStream_t st, void* order_scratch = nullptr);
+                           double vd, int levels, const SegOut& o, hipStream_t st, void* order_scratch = nullptr,
+                           void* rec_scratch = nullptr);
 /* Bytes of order scratch a launch of n segments uses (0: it walks in input order). */
 size_t segment_order_bytes(int64_t n);
+/* Bytes of record scratch a launch of n segments uses. */
+size_t segment_record_bytes(int64_t n);
+/* Order scratch followed by record scratch, and where the records start. */
+inline size_t segment_scratch_bytes(int64_t n)
+{
+	return segment_order_bytes(n) ? ((segment_order_bytes(n) + 255) & ~(size_t)255) + segment_record_bytes(n) : 0;
+}
+inline void* segment_records_in(void* scratch, int64_t n)
+{
+	return scratch && segment_record_bytes(n) ? (char*)scratch + ((segment_order_bytes(n) + 255) & ~(size_t)255)
+	                                          : nullptr;
+}
 
 /* The allocation-free touch search's scratch on the current device: the
  * work counter in the first NOALLOC_COUNTER_BYTES, then order scratch for

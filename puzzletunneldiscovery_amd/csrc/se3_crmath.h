@@ -58,6 +58,12 @@
 #define SE3_CR_NO_UNROLL
 #endif
 
+/* the fast paths inline on the device when SE3_CR_INLINE is set */
+#if defined(__HIP_DEVICE_COMPILE__) && defined(SE3_CR_INLINE) && SE3_CR_INLINE
+#define SE3_CR_FAST2 __attribute__((always_inline))
+#else
+#define SE3_CR_FAST2
+#endif
 #if defined(__GNUC__) || defined(__clang__)
 #define SE3_CR_NOINLINE __attribute__((unused))
 #else
@@ -650,7 +656,7 @@ SE3_HD static inline void se3_cr_sincos_small(se3_cr_dd s, se3_cr_dd* so, se3_cr
 
 /* sin (want_cos = 0) or cos of x, correctly rounded for |x| < 2^20; 0 when
  * |x| is out of that range (the caller falls back) */
-SE3_HD static inline int se3_cr_sincos(double x, int want_cos, double* out)
+SE3_HD static SE3_CR_FAST2 inline int se3_cr_sincos(double x, int want_cos, double* out)
 {
 	const double ax = fabs(x);
 	if (!(ax < 0x1p20))
@@ -770,7 +776,7 @@ SE3_HD static inline double se3_cr_atan_tiny_slow(double n, double d, double sig
 	return sign * v.h;
 }
 
-SE3_HD static inline double se3_cr_atan2(double y, double x)
+SE3_HD static SE3_CR_FAST2 inline double se3_cr_atan2(double y, double x)
 {
 	if (x != x || y != y)
 		return x + y;
@@ -824,7 +830,7 @@ SE3_HD static inline double se3_cr_atan2(double y, double x)
 	return se3_cr_atan2_slow(y, x);
 }
 
-SE3_HD static inline double se3_cr_acos(double x)
+SE3_HD static SE3_CR_FAST2 inline double se3_cr_acos(double x)
 {
 	if (x != x)
 		return x + x;

@@ -133,8 +133,15 @@ SE3_HD static inline double se3_cos_legacy(double x)
 
 /* The correctly rounded functions (se3_crmath.h); sin and cos beyond
  * |x| = 2^20 use the reduction above (deterministic on both targets, not
- * correctly rounded: slerp and apply never produce such arguments). */
-SE3_HD static inline double se3_sin(double x)
+ * correctly rounded: slerp and apply never produce such arguments).
+ * SE3_CR_INLINE forces their fast paths inline on the device (the slow
+ * paths stay calls); by default the compiler decides. */
+#if defined(__HIP_DEVICE_COMPILE__) && defined(SE3_CR_INLINE) && SE3_CR_INLINE
+#define SE3_CR_FAST __attribute__((always_inline))
+#else
+#define SE3_CR_FAST
+#endif
+SE3_HD static SE3_CR_FAST inline double se3_sin(double x)
 {
 	double r;
 	if (se3_cr_sincos(x, 0, &r))
@@ -142,7 +149,7 @@ SE3_HD static inline double se3_sin(double x)
 	return se3_sin_legacy(x);
 }
 
-SE3_HD static inline double se3_cos(double x)
+SE3_HD static SE3_CR_FAST inline double se3_cos(double x)
 {
 	double r;
 	if (se3_cr_sincos(x, 1, &r))
@@ -150,9 +157,9 @@ SE3_HD static inline double se3_cos(double x)
 	return se3_cos_legacy(x);
 }
 
-SE3_HD static inline double se3_acos(double x) { return se3_cr_acos(x); }
+SE3_HD static SE3_CR_FAST inline double se3_acos(double x) { return se3_cr_acos(x); }
 
-SE3_HD static inline double se3_atan2(double y, double x) { return se3_cr_atan2(y, x); }
+SE3_HD static SE3_CR_FAST inline double se3_atan2(double y, double x) { return se3_cr_atan2(y, x); }
 
 /* ---------------------------------------------------------------------- */
 /* Quaternion and state arithmetic (Eigen 3.3 semantics, see file header). */

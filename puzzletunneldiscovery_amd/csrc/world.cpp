@@ -1545,8 +1545,8 @@ static int seg_device(se3cq_world* w, int mode, const double* d_from, const doub
                       int levels, const SegOut& o, void* stream)
 {
 	hipStream_t st = pick_stream(w, stream);
-	return device_launch(w, stream, segment_order_bytes(n), [&](const DevScene& s, void* ord) {
-		return launch_segments(s, mode, d_from, d_to, n, vd, levels, o, st, ord);
+	return device_launch(w, stream, segment_scratch_bytes(n), [&](const DevScene& s, void* ord) {
+		return launch_segments(s, mode, d_from, d_to, n, vd, levels, o, st, ord, segment_records_in(ord, n));
 	});
 }
 
@@ -1589,8 +1589,8 @@ static int seg_host(se3cq_world* w, int mode, const double* from, const double* 
 	if (!rc)
 		rc = hc.h2d(dt, to, 7 * un * sizeof(double));
 	if (!rc)
-		rc = hc.launch(segment_order_bytes(n), [&](const DevScene& s, void* ord) {
-			return launch_segments(s, mode, df, dt, n, vd, levels, o, hc.stream(), ord);
+		rc = hc.launch(segment_scratch_bytes(n), [&](const DevScene& s, void* ord) {
+			return launch_segments(s, mode, df, dt, n, vd, levels, o, hc.stream(), ord, segment_records_in(ord, n));
 		});
 	if (!rc)
 		rc = hc.finish();
@@ -2206,11 +2206,12 @@ static int batch_launch(se3cq_world* w, const BatchArgs& a, const se3cq_batch_co
                         uint32_t* err)
 {
 	const bool split = a.n_states == 0 || c.split;
-	return with_launch(w, st, err, split ? segment_order_bytes(a.n_seg) : 0, [&](const DevScene& s, void* ord) {
+	return with_launch(w, st, err, split ? segment_scratch_bytes(a.n_seg) : 0, [&](const DevScene& s, void* ord) {
 		if (split) {
 			/* the persistent segment kernel (no traversal stacks in LDS),
 			 * then the state kernel */
-			hipError_t e = launch_segments(s, 0, a.seg_from, a.seg_to, a.n_seg, a.vd, 0, a.seg, st, ord);
+			hipError_t e = launch_segments(s, 0, a.seg_from, a.seg_to, a.n_seg, a.vd, 0, a.seg, st, ord,
+			                               segment_records_in(ord, a.n_seg));
 			if (e == hipSuccess && a.n_states > 0)
 				e = launch_valid_states(s, a.states, a.n_states, a.valid, st);
 			return e;
