@@ -50,12 +50,12 @@ VALU_ROOF = "fp32 VALU (vector ALUs; the kernel issues no matrix instructions)"
 # same counter reads for a 256 MiB device copy (FETCH_SIZE 131,094 KiB, i.e.
 # x2.000; WRITE_SIZE 262,144 KiB, x1.000); see
 # profiles/k_segments_touch_torus65536.md.  Measured, not computed here.
-TRAFFIC_FETCH_KIB = 182703
-TRAFFIC_WRITE_KIB = 603829
+TRAFFIC_FETCH_KIB = 166188
+TRAFFIC_WRITE_KIB = 490440
 # the same for one launch of the uneven-segment kernel (4M segments, shell
 # scene); see profiles/k_segments_transit_shell_uneven.md
-TRAFFIC_UNEVEN_FETCH_KIB = 80308493
-TRAFFIC_UNEVEN_WRITE_KIB = 142047204
+TRAFFIC_UNEVEN_FETCH_KIB = 64489375
+TRAFFIC_UNEVEN_WRITE_KIB = 122651438
 PEAK_HBM_GBS = 8000.0
 FETCH_SCALE = 262144 / 131094
 WRITE_SCALE = 1.0
