@@ -18,6 +18,7 @@
  */
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -50,6 +51,34 @@ __host__ __device__ inline float bf2f(uint16_t h)
 	float x;
 	std::memcpy(&x, &u, 4);
 	return x;
+}
+
+/*
+ * Position of element (row, col) of a [rows][K] bf16 matrix stored
+ * fragment-major (rows a multiple of 32, K of 16): the 32 x 16 blocks in
+ * row-block-major order, each as the 64 lanes x 8 values that
+ * v_mfma_f32_32x32x16_bf16 takes for one operand (lane r + 32h holds row r,
+ * columns 8h .. 8h + 7 of the block).  A wavefront's load of one fragment
+ * is then 1 KB of consecutive bytes, where a row-major load touches 32
+ * cache lines for 32 bytes each.
+ */
+__host__ __device__ inline int64_t frag_at(int64_t row, int64_t col, int64_t K)
+{
+	return (((row >> 5) * (K >> 4) + (col >> 4)) * 64 + (row & 31) + 32 * ((col >> 3) & 1)) * 8 + (col & 7);
+}
+
+/*
+ * The training kernel's activations and gradients ([features][rows], K =
+ * rows) use the same blocks with the 32 feature rows of each block in the
+ * order that its copy-out writes whole 128-byte runs: feature f sits in
+ * fragment row feat_pos(f), and fragment row p holds feature pos_feat(p)
+ * of the block.  The weight-gradient kernel maps its rows back.
+ */
+__host__ __device__ inline int feat_pos(int f) { return 8 * (f & 3) + ((f & 31) >> 2); }
+__host__ __device__ inline int pos_feat(int p) { return 4 * (p & 7) + (p >> 3); }
+__host__ __device__ inline int64_t act_at(int64_t f, int64_t s, int64_t rows)
+{
+	return (((f >> 5) * (rows >> 4) + (s >> 4)) * 64 + feat_pos((int)f) + 32 * ((s >> 3) & 1)) * 8 + (s & 7);
 }
 
 /* ---- GEMM ---------------------------------------------------------------- */
@@ -167,59 +196,6 @@ __global__ void __launch_bounds__(256) k_gemm_nt(Gemm g)
 	}
 }
 
-/*
- * Split-K form of the same product for long reductions (the weight
- * gradients reduce over the batch, K = rows, while M x N is only 256 x 256:
- * one wavefront per tile would leave the device nearly idle).  Block
- * (x, y, z) computes its four 32 x 32 tiles over k in [z SPLIT_K,
- * (z + 1) SPLIT_K) and stores the f32 partial P[z][m][n]; k_sum_parts adds
- * the partials in z order, so the sum does not depend on scheduling.
- */
-enum { SPLIT_K = 256 };
-__global__ void __launch_bounds__(256) k_gemm_nt_part(Gemm g, float* __restrict__ P)
-{
-	const int wave = threadIdx.x >> 6;
-	const int lane = threadIdx.x & 63;
-	const int m0 = (blockIdx.y * 2 + (wave >> 1)) * 32;
-	const int n0 = (blockIdx.x * 2 + (wave & 1)) * 32;
-	if (m0 >= g.M || n0 >= g.N)
-		return;
-	const int k0 = blockIdx.z * SPLIT_K;
-	const int k1 = (g.K - k0 < SPLIT_K) ? g.K : k0 + SPLIT_K;
-	const int r = lane & 31, kq = (lane >> 5) * 8;
-	f32x16 acc = {};
-	tile_k(g.A + (int64_t)(m0 + r) * g.lda + kq, g.Bt + (int64_t)(n0 + r) * g.ldb + kq, k0, k1, acc);
-	float* out = P + (int64_t)blockIdx.z * g.M * g.N;
-	const int n = n0 + r;
-#pragma unroll
-	for (int i = 0; i < 16; i++) {
-		const int m = m0 + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
-		out[(int64_t)m * g.N + n] = acc[i];
-	}
-}
-
-/* C[m][n] = sum over z < parts of P[z][m][n], in z order; four consecutive
- * elements per thread (M x N and N multiples of 4), eight partials' loads in
- * flight. */
-__global__ void __launch_bounds__(64) k_sum_parts(const float* __restrict__ P, int parts, int M, int N,
-                                                  float* __restrict__ C, int64_t ldc)
-{
-	const int64_t mn = (int64_t)M * N;
-	const int64_t e = 4 * ((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
-	if (e >= mn)
-		return;
-	float4 s = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-#pragma unroll 8
-	for (int z = 0; z < parts; z++) {
-		const float4 v = *reinterpret_cast<const float4*>(P + (int64_t)z * mn + e);
-		s.x += v.x;
-		s.y += v.y;
-		s.z += v.z;
-		s.w += v.w;
-	}
-	*reinterpret_cast<float4*>(C + (e / N) * ldc + e % N) = s;
-}
-
 /* ---- features, loss, reductions, optimiser ---------------------------------- */
 /* Rotation of the normalised quaternion (Eigen toRotationMatrix on q/|q|) and
  * translation - 0.5; rows >= n (padding up to the tile size) are zero. */
@@ -262,68 +238,6 @@ __global__ void k_features(const double* __restrict__ q, int64_t n, int64_t rows
 	}
 }
 
-/* Two-class softmax cross-entropy on columns 0 (free) and 1 (colliding) of
- * the output logits; label 1 = free.  Writes the gradient with respect to the
- * logits (scaled by 1/n) as bf16, row-major and transposed, and the per-row
- * loss. */
-__global__ void k_xent(const float* __restrict__ Z, int64_t n, int64_t rows, const uint8_t* __restrict__ label,
-                       uint16_t* __restrict__ dZ, uint16_t* __restrict__ dZT, float* __restrict__ loss)
-{
-	const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= rows)
-		return;
-	float g0 = 0.0f, g1 = 0.0f, l = 0.0f;
-	if (i < n) {
-		const float z0 = Z[i * OUT], z1 = Z[i * OUT + 1];
-		const float d = z1 - z0; /* log-odds of "colliding" */
-		const float p0 = 1.0f / (1.0f + expf(d)); /* P(free) */
-		const float y0 = label[i] ? 1.0f : 0.0f;
-		const float inv = 1.0f / (float)n;
-		g0 = (p0 - y0) * inv;
-		g1 = -g0;
-		/* -log p(label), computed stably */
-		const float t = y0 > 0.0f ? d : -d;
-		l = t > 0.0f ? t + log1pf(expf(-t)) : log1pf(expf(t));
-	}
-	for (int k = 0; k < OUT; k++) {
-		const float v = k == 0 ? g0 : (k == 1 ? g1 : 0.0f);
-		const uint16_t h = f2bf(v);
-		dZ[i * OUT + k] = h;
-		dZT[(int64_t)k * rows + i] = h;
-	}
-	loss[i] = l;
-}
-
-/* out[j] = sum_i XT[j][i] over i < rows (bias gradient; rows a multiple of
- * 8): one block per j, each thread summing runs of eight, then a tree of
- * fixed order in LDS. */
-__global__ void __launch_bounds__(256) k_rowsum_bf16(const uint16_t* __restrict__ XT, int64_t rows, int cols,
-                                                     float* __restrict__ out)
-{
-	__shared__ float part[256];
-	const int j = blockIdx.x;
-	if (j >= cols)
-		return;
-	const uint16_t* x = XT + (int64_t)j * rows;
-	float s = 0.0f;
-	for (int64_t i = 8 * (int64_t)threadIdx.x; i < rows; i += 8 * 256) {
-		const uint4 v = *reinterpret_cast<const uint4*>(x + i);
-		const uint32_t w[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-		for (int k = 0; k < 4; k++)
-			s += bf2f((uint16_t)(w[k] & 0xffffu)) + bf2f((uint16_t)(w[k] >> 16));
-	}
-	part[threadIdx.x] = s;
-	__syncthreads();
-	for (int off = 128; off > 0; off >>= 1) {
-		if ((int)threadIdx.x < off)
-			part[threadIdx.x] += part[threadIdx.x + off];
-		__syncthreads();
-	}
-	if (threadIdx.x == 0)
-		out[j] = part[0];
-}
-
 __global__ void k_sum_f32(const float* __restrict__ x, int64_t n, float* __restrict__ out)
 {
 	__shared__ float part[256];
@@ -342,9 +256,10 @@ __global__ void k_sum_f32(const float* __restrict__ x, int64_t n, float* __restr
 }
 
 /* Adam on the fp32 master copies of every parameter tensor of the network
- * in one launch (tensor t holds elements [start[t], start[t + 1]) of the
- * grid); refreshes the bf16 W[rows x cols] and its transpose WT[cols x rows]
- * of the weight tensors.  Entries outside `used` rows stay zero. */
+ * in one launch (grid row t takes tensor t); refreshes the bf16 copies of the weight tensors: W [rows][cols]
+ * row-major (inference), and W and its transpose fragment-major (training).  Entries outside `used` rows stay zero.  The
+ * gradient of element i is the sum of `parts` partials g[t][z * size + i]
+ * (size = rows x cols), added in z order. */
 enum { ADAM_TENSORS = 2 * LAYERS };
 struct AdamSet {
 	float* w[ADAM_TENSORS];
@@ -352,25 +267,27 @@ struct AdamSet {
 	float* m[ADAM_TENSORS];
 	float* v[ADAM_TENSORS];
 	uint16_t* W[ADAM_TENSORS];
-	uint16_t* WT[ADAM_TENSORS];
+	uint16_t* Wf[ADAM_TENSORS];
+	uint16_t* WTf[ADAM_TENSORS];
 	int rows[ADAM_TENSORS], cols[ADAM_TENSORS], used[ADAM_TENSORS];
-	int64_t start[ADAM_TENSORS + 1];
+	int parts;
 };
 
 __global__ void k_adam(AdamSet a, float lr, float b1, float b2, float eps, float c1, float c2)
 {
-	const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-	if (e >= a.start[ADAM_TENSORS])
-		return;
-	int t = 0;
-	while (e >= a.start[t + 1])
-		t++;
-	const int64_t i = e - a.start[t];
+	const int t = blockIdx.y;
 	const int cols = a.cols[t];
-	const int r = (int)(i / cols), c = (int)(i % cols);
+	const int size = a.rows[t] * cols;
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= size)
+		return;
+	const int r = i / cols, c = i - r * cols;
 	float* w = a.w[t];
 	if (r < a.used[t]) {
-		const float gi = a.g[t][i];
+		const float* gp = a.g[t] + i;
+		float gi = gp[0];
+		for (int z = 1; z < a.parts; z++)
+			gi += gp[z * size];
 		const float mi = b1 * a.m[t][i] + (1.0f - b1) * gi;
 		const float vi = b2 * a.v[t][i] + (1.0f - b2) * gi * gi;
 		a.m[t][i] = mi;
@@ -380,21 +297,22 @@ __global__ void k_adam(AdamSet a, float lr, float b1, float b2, float eps, float
 	if (a.W[t]) {
 		const uint16_t h = f2bf(w[i]);
 		a.W[t][i] = h;
-		if (a.WT[t])
-			a.WT[t][(int64_t)c * a.rows[t] + r] = h;
+		a.Wf[t][frag_at(r, c, cols)] = h;
+		a.WTf[t][frag_at(c, r, a.rows[t])] = h;
 	}
 }
 
 __global__ void k_cast(const float* __restrict__ w, int rows, int cols, uint16_t* __restrict__ W,
-                       uint16_t* __restrict__ WT)
+                       uint16_t* __restrict__ Wf, uint16_t* __restrict__ WTf)
 {
 	const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= (int64_t)rows * cols)
 		return;
 	const uint16_t h = f2bf(w[i]);
+	const int r = (int)(i / cols), c = (int)(i % cols);
 	W[i] = h;
-	if (WT)
-		WT[(int64_t)(i % cols) * rows + i / cols] = h;
+	Wf[frag_at(r, c, cols)] = h;
+	WTf[frag_at(c, r, rows)] = h;
 }
 
 /* ---- fused inference ------------------------------------------------------ */
@@ -688,6 +606,551 @@ __global__ void __launch_bounds__(256, 2) k_mlp_infer(const double* __restrict__
 	}
 }
 
+/* ---- fused training pass ---------------------------------------------------- */
+/*
+ * Forward, loss and backward of one training batch down to every layer's
+ * pre-activation gradient, 64 rows per block of four wavefronts, in one
+ * launch (a batch of 16,384 rows is 256 blocks, one per CU).  All four
+ * hidden activation tiles of the block stay in LDS (hb[l], 64 rows x 256
+ * features, rows padded as in k_mlp_infer; 141 KB with the output
+ * gradient): layer l reads hb[l - 1] and writes hb[l], so a layer's
+ * epilogue needs no barrier before it, and the gradient pass that produces
+ * D_l reads the ReLU mask [H_l > 0] from hb[l] and overwrites it with D_l,
+ * which the next gradient pass reads.  The features sit in hb[1] until
+ * layer 1 writes it.  Wave w computes features
+ * [64w, 64w + 64) of a layer for all 64 rows (2 x 2 MFMA tiles, weights as
+ * the A operand, so lane (r, half) of tile (h, t) holds features 64w + 32h +
+ * 8g + 4 half + [0, 4) of row 32t + r in elements 4g + [0, 4)).
+ *
+ * The nine matrix passes (four hidden layers, the head, then the gradient
+ * through W4, W3, W2 and W1) read their A fragments from wf[16][2], one per
+ * k-step and feature half; the moment a pass has used fragment j it loads
+ * fragment j of the next pass into the same registers, so every weight load
+ * has a whole pass to arrive.
+ *
+ * Written for the weight gradients: the bf16 features XT [IN][rows], hidden
+ * activations HT[l] [HID][rows] (after the ReLU) and pre-activation
+ * gradients DT[l] [kOut[l]][rows], laid out by act_at, plus the per-row
+ * loss.  The weights are read fragment-major too.  The loss is the two-class softmax cross-entropy on logit columns 0 (free)
+ * and 1 (colliding), label 1 = free, averaged over the n real rows; rows in
+ * [n, rows) get zero gradient.
+ */
+enum { TM = 64, DZLD = OUT + 8 };
+static const size_t TRAIN_LDS =
+    (size_t)(LAYERS - 1) * TM * ALD * 2 + (size_t)TM * DZLD * 2 + 2 * 64 * 4 + (size_t)(LAYERS - 1) * HID * 4;
+
+struct TrainArgs {
+	const double* q;
+	const uint8_t* label;
+	int64_t n, rows;
+	const uint16_t* Wf[LAYERS];  /* [kOut][kIn], fragment-major */
+	const uint16_t* WTf[LAYERS]; /* [kIn][kOut], fragment-major */
+	const float* b[LAYERS];
+	uint16_t* XT;
+	uint16_t* HT[LAYERS - 1];
+	uint16_t* DT[LAYERS];
+	float* loss;
+};
+
+/* lane (r, half) of bias tile h: features nb + 32h + 8g + 4 half + [0, 4)
+ * in elements 4g + [0, 4) (b in LDS, so that a pass never waits for a
+ * global load issued just before it) */
+__device__ __forceinline__ void bias_tile(const float* b, int nb, int lane, f32x16 (&pb)[2])
+{
+	const float* bl = b + nb + 4 * (lane >> 5);
+#pragma unroll
+	for (int h = 0; h < 2; h++) {
+#pragma unroll
+		for (int g = 0; g < 4; g++) {
+			const float4 v = *reinterpret_cast<const float4*>(bl + 32 * h + 8 * g);
+			pb[h][4 * g] = v.x;
+			pb[h][4 * g + 1] = v.y;
+			pb[h][4 * g + 2] = v.z;
+			pb[h][4 * g + 3] = v.w;
+		}
+	}
+}
+
+/*
+ * c[h][t] += A_h . in^T over NK k-steps, A_h's fragments in wf[j][h], the
+ * B operand rows 32t + r of the LDS tile `in` (row stride ld).  After using
+ * wf[j] the lane loads fragment j of the next pass from nx0 / nx1 (this
+ * lane's part of the next pass's first fragment of each half, in a
+ * fragment-major matrix, so fragment j is 512 values further); fragments
+ * the next pass has beyond NK are loaded first.  H_NEXT = 1 loads only nx0.
+ */
+template <int NK, int NK_NEXT, int H_NEXT>
+__device__ __forceinline__ void train_pass(bf16x8 (&wf)[16][2], f32x16 (&c)[2][2], const uint16_t* in, int ld,
+                                           const uint16_t* nx0, const uint16_t* nx1, int r, int kq)
+{
+#pragma unroll
+	for (int j = NK; j < NK_NEXT; j++) {
+		wf[j][0] = ld8(nx0 + 512 * j);
+		if (H_NEXT == 2)
+			wf[j][1] = ld8(nx1 + 512 * j);
+	}
+	/* the B fragments of step j + 1 are read from LDS before step j's MFMAs */
+	bf16x8 a0 = ld8(in + r * ld + kq), a1 = ld8(in + (32 + r) * ld + kq);
+#pragma unroll
+	for (int j = 0; j < NK; j++) {
+		bf16x8 n0 = a0, n1 = a1;
+		if (j + 1 < NK) {
+			n0 = ld8(in + r * ld + 16 * (j + 1) + kq);
+			n1 = ld8(in + (32 + r) * ld + 16 * (j + 1) + kq);
+		}
+		c[0][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wf[j][0], a0, c[0][0], 0, 0, 0);
+		c[0][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wf[j][0], a1, c[0][1], 0, 0, 0);
+		c[1][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wf[j][1], a0, c[1][0], 0, 0, 0);
+		c[1][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wf[j][1], a1, c[1][1], 0, 0, 0);
+		if (j < NK_NEXT) {
+			wf[j][0] = ld8(nx0 + 512 * j);
+			if (H_NEXT == 2)
+				wf[j][1] = ld8(nx1 + 512 * j);
+		}
+		a0 = n0;
+		a1 = n1;
+	}
+}
+
+/* The block's 64 rows x NF features of an LDS tile (row stride ld) to the
+ * [features][rows] matrix FT laid out by act_at.  An item is 8 rows x 4
+ * features: eight 8-byte LDS reads, transposed with byte permutes into four
+ * 16-byte stores of 8 rows of one feature; store j of the 8 items of a
+ * 32-feature block goes to fragment rows 8j .. 8j + 7, one 128-byte run.
+ * The trip count is a constant, so the compiler counts the stores exactly
+ * and does not wait for the next pass's weight fragments along with them. */
+template <int NF>
+__device__ __forceinline__ void copy_out(const uint16_t* tile, int ld, uint16_t* FT, int64_t rows, int64_t row0)
+{
+	constexpr int GROUPS = NF / 4, ITEMS = GROUPS * (TM / 8);
+#pragma unroll
+	for (int k = 0; k < (ITEMS + 255) / 256; k++) {
+		const int q = threadIdx.x + 256 * k;
+		if (ITEMS % 256 == 0 || q < ITEMS) {
+			const int fg = q % GROUPS, s0 = 8 * (q / GROUPS);
+			uint2 x[8];
+#pragma unroll
+			for (int i = 0; i < 8; i++)
+				x[i] = *reinterpret_cast<const uint2*>(tile + (s0 + i) * ld + 4 * fg);
+#pragma unroll
+			for (int j = 0; j < 4; j++) {
+				/* feature 4 fg + j: the low (j even) or high half of dword j / 2
+				 * of each row; rows 2i and 2i + 1 form output dword i */
+				const uint32_t sel = (j & 1) ? 0x07060302u : 0x05040100u;
+				uint32_t w[4];
+#pragma unroll
+				for (int i = 0; i < 4; i++) {
+					const uint32_t lo = (j >> 1) ? x[2 * i].y : x[2 * i].x;
+					const uint32_t hi = (j >> 1) ? x[2 * i + 1].y : x[2 * i + 1].x;
+					w[i] = __builtin_amdgcn_perm(hi, lo, sel);
+				}
+				*reinterpret_cast<uint4*>(FT + act_at(4 * fg + j, row0 + s0, rows)) = make_uint4(w[0], w[1], w[2], w[3]);
+			}
+		}
+	}
+}
+
+/* hidden layer epilogue: bf16 rounding and ReLU as in k_mlp_infer, the
+ * activations to the layer's LDS tile */
+__device__ __forceinline__ void train_fwd_epi(const f32x16 (&c)[2][2], uint16_t* hl, int wave, int lane)
+{
+	const int r = lane & 31, half = lane >> 5;
+#pragma unroll
+	for (int h = 0; h < 2; h++) {
+#pragma unroll
+		for (int t = 0; t < 2; t++) {
+			const f32x16& v = c[h][t];
+#pragma unroll
+			for (int g = 0; g < 4; g++) {
+				const bf16x2 p0 = {(__bf16)v[4 * g], (__bf16)v[4 * g + 1]};
+				const bf16x2 p1 = {(__bf16)v[4 * g + 2], (__bf16)v[4 * g + 3]};
+				uint2 u;
+				std::memcpy(&u.x, &p0, 4);
+				std::memcpy(&u.y, &p1, 4);
+				asm("v_pk_max_i16 %0, %0, 0" : "+v"(u.x));
+				asm("v_pk_max_i16 %0, %0, 0" : "+v"(u.y));
+				*reinterpret_cast<uint2*>(hl + (32 * t + r) * ALD + 64 * wave + 32 * h + 8 * g + 4 * half) = u;
+			}
+		}
+	}
+}
+
+/* 0xffff in each 16-bit half of x that is not zero (x holds two bf16 after
+ * the ReLU, so each half is in [0, 0x7fff]): min(half, 1), then 0 - that */
+__device__ __forceinline__ uint32_t nonzero_halves(uint32_t x)
+{
+	uint32_t m;
+	asm("v_pk_min_u16 %0, %1, %2" : "=v"(m) : "v"(x), "v"(0x00010001u));
+	asm("v_pk_sub_u16 %0, 0, %0" : "+v"(m));
+	return m;
+}
+
+/* gradient epilogue: round to bf16, zero where the layer's activation in hl
+ * is not positive, and overwrite the activation with the gradient */
+__device__ __forceinline__ void train_bwd_epi(const f32x16 (&c)[2][2], uint16_t* hl, int wave, int lane)
+{
+	const int r = lane & 31, half = lane >> 5;
+#pragma unroll
+	for (int h = 0; h < 2; h++) {
+#pragma unroll
+		for (int t = 0; t < 2; t++) {
+			const f32x16& v = c[h][t];
+#pragma unroll
+			for (int g = 0; g < 4; g++) {
+				uint2* p = reinterpret_cast<uint2*>(hl + (32 * t + r) * ALD + 64 * wave + 32 * h + 8 * g + 4 * half);
+				const uint2 act = *p;
+				const bf16x2 p0 = {(__bf16)v[4 * g], (__bf16)v[4 * g + 1]};
+				const bf16x2 p1 = {(__bf16)v[4 * g + 2], (__bf16)v[4 * g + 3]};
+				uint2 u;
+				std::memcpy(&u.x, &p0, 4);
+				std::memcpy(&u.y, &p1, 4);
+				u.x &= nonzero_halves(act.x);
+				u.y &= nonzero_halves(act.y);
+				*p = u;
+			}
+		}
+	}
+}
+
+__global__ void __launch_bounds__(256, 1) k_mlp_train(TrainArgs a)
+{
+	extern __shared__ __align__(16) uint16_t lds[];
+	uint16_t* hb[LAYERS - 1];
+#pragma unroll
+	for (int l = 0; l < LAYERS - 1; l++)
+		hb[l] = lds + l * TM * ALD;
+	uint16_t* dz = lds + (LAYERS - 1) * TM * ALD; /* output gradient, columns >= 2 zero */
+	float* head = reinterpret_cast<float*>(dz + TM * DZLD); /* [2][64]: the two logits of 32 rows per wave */
+	float* bias = head + 2 * 64;                           /* [4][HID]: the hidden layers' biases */
+	uint16_t* xs = hb[1];
+	const int wave = threadIdx.x >> 6;
+	const int lane = threadIdx.x & 63;
+	const int r = lane & 31, kq = (lane >> 5) * 8;
+	const int64_t rows = a.rows;
+	const int64_t row0 = (int64_t)blockIdx.x * TM;
+	/* this lane's part of the first fragment of rows 64 wave + 32h + [0, 32)
+	 * of a fragment-major [*][K] matrix */
+	auto arow = [&](const uint16_t* M, int K, int h) {
+		return M + ((int64_t)(2 * wave + h) * (K / 16) * 64 + lane) * 8;
+	};
+
+	bf16x8 wf[16][2];
+	f32x16 c[2][2], pb[2];
+#pragma unroll
+	for (int j = 0; j < IN / 16; j++) {
+		wf[j][0] = ld8(arow(a.Wf[0], IN, 0) + 512 * j);
+		wf[j][1] = ld8(arow(a.Wf[0], IN, 1) + 512 * j);
+	}
+#pragma unroll
+	for (int l = 0; l < LAYERS - 1; l++)
+		bias[l * HID + threadIdx.x] = a.b[l][threadIdx.x];
+	/* what the head needs from global memory, loaded now */
+	const float head_bias = a.b[LAYERS - 1][r];
+	const int64_t head_row = row0 + 32 * (wave & 1) + (lane & 31);
+	const bool head_real = wave < 2 && lane < 32 && head_row < a.n;
+	const float head_y = head_real && a.label[head_row] ? 1.0f : 0.0f;
+	for (int i = threadIdx.x; i < TM * DZLD / 2; i += 256)
+		reinterpret_cast<uint32_t*>(dz)[i] = 0u;
+	if (threadIdx.x < TM) {
+		/* features of row row0 + threadIdx.x, as k_features computes them */
+		const int64_t i = row0 + threadIdx.x;
+		float f[IN];
+#pragma unroll
+		for (int k = 0; k < IN; k++)
+			f[k] = 0.0f;
+		if (i < a.n) {
+			const double* s = a.q + 7 * i;
+			double qw = s[3], qx = s[4], qy = s[5], qz = s[6];
+			const double nn = sqrt((qx * qx + qz * qz) + (qy * qy + qw * qw));
+			if (nn > 0.0) {
+				qw /= nn; qx /= nn; qy /= nn; qz /= nn;
+			} else {
+				qw = 1.0;
+			}
+			const double tx = 2 * qx, ty = 2 * qy, tz = 2 * qz;
+			f[0] = (float)(1 - (ty * qy + tz * qz));
+			f[1] = (float)(tx * qy - tz * qw);
+			f[2] = (float)(tx * qz + ty * qw);
+			f[3] = (float)(tx * qy + tz * qw);
+			f[4] = (float)(1 - (tx * qx + tz * qz));
+			f[5] = (float)(ty * qz - tx * qw);
+			f[6] = (float)(tx * qz - ty * qw);
+			f[7] = (float)(ty * qz + tx * qw);
+			f[8] = (float)(1 - (tx * qx + ty * qy));
+#pragma unroll
+			for (int k = 0; k < 3; k++)
+				f[9 + k] = (float)(s[k] - 0.5);
+		}
+#pragma unroll
+		for (int k = 0; k < IN; k += 2) {
+			const bf16x2 p = {(__bf16)f[k], (__bf16)f[k + 1]};
+			uint32_t u;
+			std::memcpy(&u, &p, 4);
+			*reinterpret_cast<uint32_t*>(xs + threadIdx.x * ALD + k) = u;
+		}
+	}
+	__syncthreads();
+	copy_out<IN>(xs, ALD, a.XT, rows, row0);
+
+	/* hidden layers; layer 3 is followed by the head, whose B fragments
+	 * (output columns [0, 32) of W4) take wf[j][0].  Layer 1 overwrites the
+	 * features in hb[1], which layer 0 and the copy above have read before
+	 * the barrier that ends layer 0. */
+#pragma unroll
+	for (int l = 0; l < LAYERS - 1; l++) {
+		bias_tile(bias + l * HID, 64 * wave, lane, pb);
+#pragma unroll
+		for (int h = 0; h < 2; h++)
+			c[h][0] = c[h][1] = pb[h];
+		if (l == 0)
+			train_pass<IN / 16, HID / 16, 2>(wf, c, xs, ALD, arow(a.Wf[1], HID, 0), arow(a.Wf[1], HID, 1), r, kq);
+		else if (l < LAYERS - 2)
+			train_pass<HID / 16, HID / 16, 2>(wf, c, hb[l - 1], ALD, arow(a.Wf[l + 1], HID, 0),
+			                                  arow(a.Wf[l + 1], HID, 1), r, kq);
+		else
+			train_pass<HID / 16, HID / 16, 1>(wf, c, hb[l - 1], ALD, a.Wf[LAYERS - 1] + lane * 8, nullptr, r, kq);
+		train_fwd_epi(c, hb[l], wave, lane);
+		__syncthreads();
+		copy_out<HID>(hb[l], ALD, a.HT[l], rows, row0);
+	}
+
+	/* head and loss: waves 0 and 1 take rows 32 wave + [0, 32); every wave
+	 * then loads the A fragments of the first gradient pass (WT4, K = 32) */
+	if (wave < 2) {
+		f32x16 ch = {};
+		const uint16_t* h3 = hb[LAYERS - 2];
+#pragma unroll
+		for (int j = 0; j < HID / 16; j++) {
+			ch = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ld8(h3 + (32 * wave + r) * ALD + 16 * j + kq), wf[j][0], ch, 0,
+			                                             0, 0);
+			if (j < OUT / 16) {
+				wf[j][0] = ld8(arow(a.WTf[LAYERS - 1], OUT, 0) + 512 * j);
+				wf[j][1] = ld8(arow(a.WTf[LAYERS - 1], OUT, 1) + 512 * j);
+			}
+		}
+		float* zs = head + 64 * wave;
+		if (r < OUT_USED) {
+#pragma unroll
+			for (int e = 0; e < 16; e++)
+				zs[2 * (8 * (e >> 2) + 4 * (lane >> 5) + (e & 3)) + r] = ch[e] + head_bias;
+		}
+		__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+		__builtin_amdgcn_wave_barrier();
+		__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+		if (lane < 32) {
+			const int s = 32 * wave + lane;
+			const int64_t i = row0 + s;
+			float g0 = 0.0f, l = 0.0f;
+			if (head_real) {
+				const float z0 = zs[2 * lane], z1 = zs[2 * lane + 1];
+				const float d = z1 - z0;                   /* log-odds of "colliding" */
+				const float p0 = 1.0f / (1.0f + expf(d)); /* P(free) */
+				const float y0 = head_y;
+				g0 = (p0 - y0) * (1.0f / (float)a.n);
+				/* -log p(label), computed stably */
+				const float tt = y0 > 0.0f ? d : -d;
+				l = tt > 0.0f ? tt + log1pf(expf(-tt)) : log1pf(expf(tt));
+			}
+			dz[s * DZLD] = f2bf(g0);
+			dz[s * DZLD + 1] = f2bf(-g0);
+			a.loss[i] = l;
+		}
+	} else {
+#pragma unroll
+		for (int j = 0; j < OUT / 16; j++) {
+			wf[j][0] = ld8(arow(a.WTf[LAYERS - 1], OUT, 0) + 512 * j);
+			wf[j][1] = ld8(arow(a.WTf[LAYERS - 1], OUT, 1) + 512 * j);
+		}
+	}
+	__syncthreads();
+	copy_out<OUT>(dz, DZLD, a.DT[LAYERS - 1], rows, row0);
+
+	/* D3 = (D4 . W4) * [H3 > 0] from dz, into hb[3] */
+	c[0][0] = c[0][1] = c[1][0] = c[1][1] = f32x16{};
+	train_pass<OUT / 16, HID / 16, 2>(wf, c, dz, DZLD, arow(a.WTf[3], HID, 0), arow(a.WTf[3], HID, 1), r, kq);
+	train_bwd_epi(c, hb[3], wave, lane);
+	__syncthreads();
+	copy_out<HID>(hb[3], ALD, a.DT[3], rows, row0);
+	/* D_{l-1} = (D_l . W_l) * [H_{l-1} > 0] from hb[l] into hb[l - 1] */
+#pragma unroll
+	for (int l = 3; l >= 1; l--) {
+		c[0][0] = c[0][1] = c[1][0] = c[1][1] = f32x16{};
+		if (l > 1)
+			train_pass<HID / 16, HID / 16, 2>(wf, c, hb[l], ALD, arow(a.WTf[l - 1], HID, 0),
+			                                  arow(a.WTf[l - 1], HID, 1), r, kq);
+		else
+			train_pass<HID / 16, 0, 2>(wf, c, hb[l], ALD, nullptr, nullptr, r, kq);
+		train_bwd_epi(c, hb[l - 1], wave, lane);
+		__syncthreads();
+		copy_out<HID>(hb[l - 1], ALD, a.DT[l - 1], rows, row0);
+	}
+}
+
+/*
+ * Weight and bias gradients of every layer in one launch.  Layer l's
+ * gradient is gW[o][i] = sum over rows s of DT_l[o][s] XT_l[i][s] (XT_0 the
+ * features, XT_l = HT[l - 1]) and gb[o] = sum_s DT_l[o][s].  Block b takes a
+ * 64 x 64 tile of one layer (32-wide tiles at the narrow layers compute the
+ * missing half on duplicated rows and drop it) over slice z of WG_ROWS
+ * rows, each wavefront a quarter of the slice with WG_KCH k-steps'
+ * fragments in flight.  DT_l and XT_l are [features][rows] matrices laid
+ * out by act_at, so each fragment load reads 1 KB of consecutive bytes, and
+ * fragment row p of a block is feature pos_feat(p).  The four wavefronts
+ * add their tiles in LDS in wave order and write P_l[z][o][i]; blocks of
+ * tile column 0 also write the bias partials
+ * Pb_l[z][o].  k_adam adds the slices in z order, so the sums do not depend
+ * on scheduling.
+ */
+#ifndef SE3MLP_WG_XCD
+#define SE3MLP_WG_XCD 1
+#endif
+#ifndef SE3MLP_WG_KCH
+#define SE3MLP_WG_KCH 4 /* 8 took 17.0 against 16.4 us per step */
+#endif
+enum { WG_ROWS = 2048, WG_KCH = SE3MLP_WG_KCH };
+static const size_t WG_LDS = (size_t)4 * 64 * 64 * 4 + (size_t)4 * 64 * 4;
+
+struct WgradArgs {
+	const uint16_t* A[LAYERS]; /* DT_l: [M][rows], act_at layout */
+	const uint16_t* B[LAYERS]; /* XT_l: [N][rows], act_at layout */
+	float* P[LAYERS];          /* [parts][M][N] */
+	float* Pb[LAYERS];         /* [parts][M] */
+	int M[LAYERS], N[LAYERS];
+	int first[LAYERS + 1];     /* first block of each layer; first[LAYERS] = grid */
+	int64_t rows;
+};
+
+__global__ void __launch_bounds__(256, 2) k_wgrad(WgradArgs g)
+{
+	extern __shared__ __align__(16) float red[]; /* [4][64][64] tiles, then [4][64] bias sums */
+	int l = 0;
+	while ((int)blockIdx.x >= g.first[l + 1])
+		l++;
+	const int M = g.M[l], N = g.N[l];
+	const int tn = (N + 63) / 64, tiles = ((M + 63) / 64) * tn;
+	const int b = (int)blockIdx.x - g.first[l];
+#if SE3MLP_WG_XCD
+	/* consecutive blocks go to different XCDs, so with 8 slices every tile
+	 * of slice z runs on one XCD and reads the slice's rows from its L2 */
+	const int parts = (g.first[l + 1] - g.first[l]) / tiles;
+	const int z = b % parts, tile = b / parts;
+#else
+	const int z = b / tiles, tile = b % tiles;
+#endif
+	const int o0 = (tile / tn) * 64, i0 = (tile % tn) * 64;
+	const bool m2 = M - o0 >= 64, n2 = N - i0 >= 64, bias = i0 == 0;
+	const int wave = threadIdx.x >> 6;
+	const int lane = threadIdx.x & 63;
+	const int r = lane & 31, kq = (lane >> 5) * 8;
+	const int64_t rows = g.rows;
+	const int64_t s0 = (int64_t)z * WG_ROWS;
+	const int64_t s1 = rows - s0 < WG_ROWS ? rows : s0 + WG_ROWS;
+	const int64_t q = (s1 - s0) / 4; /* a multiple of 16: rows is a multiple of 64 */
+	const int64_t k0 = s0 + wave * q, k1 = k0 + q;
+	/* this lane's part of the fragments of row blocks o0 / 32 (+ 1) and
+	 * i0 / 32 (+ 1); the fragment of k-step k (a multiple of 16) is 32 k
+	 * values further */
+	const int64_t blk = rows / 16 * 64 * 8; /* one row block */
+	const uint16_t* pa[2];
+	const uint16_t* pb[2];
+	pa[0] = g.A[l] + (o0 / 32) * blk + lane * 8;
+	pa[1] = m2 ? pa[0] + blk : pa[0];
+	pb[0] = g.B[l] + (i0 / 32) * blk + lane * 8;
+	pb[1] = n2 ? pb[0] + blk : pb[0];
+	f32x16 c[2][2] = {};
+	float sa[2] = {0.0f, 0.0f};
+	/* one k-step on loaded fragments (A rows o0 + 32x + r, B rows i0 + 32y + r) */
+	auto step = [&](const uint4 (&va)[2], const uint4 (&vb)[2]) {
+		bf16x8 fa[2], fb[2];
+#pragma unroll
+		for (int x = 0; x < 2; x++) {
+			std::memcpy(&fa[x], &va[x], 16);
+			std::memcpy(&fb[x], &vb[x], 16);
+		}
+#pragma unroll
+		for (int x = 0; x < 2; x++)
+#pragma unroll
+			for (int y = 0; y < 2; y++)
+				c[x][y] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[x], fb[y], c[x][y], 0, 0, 0);
+		if (bias) {
+#pragma unroll
+			for (int x = 0; x < 2; x++) {
+				const uint32_t w4[4] = {va[x].x, va[x].y, va[x].z, va[x].w};
+#pragma unroll
+				for (int e = 0; e < 4; e++)
+					sa[x] += bf2f((uint16_t)(w4[e] & 0xffffu)) + bf2f((uint16_t)(w4[e] >> 16));
+			}
+		}
+	};
+	int64_t k = k0;
+	for (; k + 16 * WG_KCH <= k1; k += 16 * WG_KCH) {
+		uint4 va[WG_KCH][2], vb[WG_KCH][2];
+#pragma unroll
+		for (int j = 0; j < WG_KCH; j++) {
+#pragma unroll
+			for (int x = 0; x < 2; x++) {
+				va[j][x] = *reinterpret_cast<const uint4*>(pa[x] + 32 * (k + 16 * j));
+				vb[j][x] = *reinterpret_cast<const uint4*>(pb[x] + 32 * (k + 16 * j));
+			}
+		}
+#pragma unroll
+		for (int j = 0; j < WG_KCH; j++)
+			step(va[j], vb[j]);
+	}
+	for (; k < k1; k += 16) {
+		uint4 va[2], vb[2];
+#pragma unroll
+		for (int x = 0; x < 2; x++) {
+			va[x] = *reinterpret_cast<const uint4*>(pa[x] + 32 * k);
+			vb[x] = *reinterpret_cast<const uint4*>(pb[x] + 32 * k);
+		}
+		step(va, vb);
+	}
+	/* element e of lane (r, half) of tile (x, y) is fragment row 8(e/4) +
+	 * 4 half + e%4 of A block x against fragment row r of B block y */
+	float* mine = red + wave * 4096;
+	const int ir = pos_feat(r);
+#pragma unroll
+	for (int x = 0; x < 2; x++)
+#pragma unroll
+		for (int y = 0; y < 2; y++)
+#pragma unroll
+			for (int e = 0; e < 16; e++)
+				mine[(32 * x + pos_feat(8 * (e >> 2) + 4 * (lane >> 5) + (e & 3))) * 64 + 32 * y + ir] = c[x][y][e];
+	float* rb = red + 4 * 4096;
+	if (bias) {
+#pragma unroll
+		for (int x = 0; x < 2; x++)
+			sa[x] += __shfl_xor(sa[x], 32);
+		if (lane < 32) {
+			rb[wave * 64 + ir] = sa[0];
+			rb[wave * 64 + 32 + ir] = sa[1];
+		}
+	}
+	__syncthreads();
+	float* P = g.P[l] + (int64_t)z * M * N;
+#pragma unroll
+	for (int v = 0; v < 4; v++) {
+		const int e = 4 * (threadIdx.x + 256 * v); /* element of the 64 x 64 tile */
+		const int ol = e >> 6, il = e & 63;
+		float4 s = *reinterpret_cast<const float4*>(red + e);
+#pragma unroll
+		for (int w = 1; w < 4; w++) {
+			const float4 t = *reinterpret_cast<const float4*>(red + w * 4096 + e);
+			s.x += t.x;
+			s.y += t.y;
+			s.z += t.z;
+			s.w += t.w;
+		}
+		if ((ol < 32 || m2) && (il < 32 || n2))
+			*reinterpret_cast<float4*>(P + (int64_t)(o0 + ol) * N + i0 + il) = s;
+	}
+	if (bias && threadIdx.x < 64 && ((int)threadIdx.x < 32 || m2)) {
+		const float s = ((rb[threadIdx.x] + rb[64 + threadIdx.x]) + rb[128 + threadIdx.x]) + rb[192 + threadIdx.x];
+		g.Pb[l][(int64_t)z * M + o0 + threadIdx.x] = s;
+	}
+}
+
 __global__ void k_zero_u32(uint32_t* p)
 {
 	if (threadIdx.x == 0)
@@ -731,14 +1194,14 @@ using namespace se3mlp;
 struct se3mlp_net {
 	int64_t rows = 0; /* batch capacity, a multiple of 64 */
 	int step = 0;
-	Buf w[LAYERS], b[LAYERS], mw[LAYERS], vw[LAYERS], mb[LAYERS], vb[LAYERS], gw[LAYERS], gb[LAYERS];
-	Buf W[LAYERS], WT[LAYERS];
-	Buf X, XT;                  /* features */
-	Buf H[LAYERS], HT[LAYERS];  /* bf16 activations after layer l (H[4] unused) */
-	Buf Z;                      /* f32 output logits */
-	Buf D[LAYERS], DT[LAYERS];  /* bf16 gradient w.r.t. the pre-activation of layer l */
+	Buf w[LAYERS], b[LAYERS], mw[LAYERS], vw[LAYERS], mb[LAYERS], vb[LAYERS];
+	Buf W[LAYERS];              /* bf16 weights, row-major (inference) */
+	Buf Wf[LAYERS], WTf[LAYERS]; /* bf16 weights and their transposes, fragment-major (training) */
+	Buf XT;                     /* bf16 features, feature-major */
+	Buf HT[LAYERS - 1];         /* bf16 hidden activations, feature-major */
+	Buf DT[LAYERS];             /* bf16 pre-activation gradients, feature-major */
 	Buf loss, lsum;
-	Buf part;                   /* split-K partials of the weight gradients */
+	Buf part[LAYERS], bpart[LAYERS]; /* per row slice: weight and bias gradient partials */
 	hipStream_t stream = nullptr;
 	std::mutex mu;
 };
@@ -777,22 +1240,6 @@ hipError_t gemm(hipStream_t st, const uint16_t* A, int64_t lda, const uint16_t* 
 
 inline unsigned blocks(int64_t n, int t = 256) { return (unsigned)((n + t - 1) / t); }
 
-/* forward over rows (already featurised into X); fills H/HT and Z */
-int forward_rows(se3mlp_net* net, int64_t rows, bool keep_t, hipStream_t st)
-{
-	const uint16_t* in = net->X.as<uint16_t>();
-	int64_t ldin = IN;
-	for (int l = 0; l < LAYERS; l++) {
-		const bool last = l == LAYERS - 1;
-		HCHK(gemm(st, in, ldin, net->W[l].as<uint16_t>(), kIn[l], (int)rows, kOut[l], kIn[l], net->b[l].as<float>(),
-		          last ? 0 : 1, nullptr, 0, last ? net->Z.as<float>() : nullptr, OUT,
-		          last ? nullptr : net->H[l].as<uint16_t>(), HID,
-		          (last || !keep_t) ? nullptr : net->HT[l].as<uint16_t>(), rows));
-		in = net->H[l].as<uint16_t>();
-		ldin = HID;
-	}
-	return SE3MLP_OK;
-}
 } // namespace
 
 extern "C" {
@@ -809,16 +1256,17 @@ int se3mlp_create(int64_t max_batch, uint64_t seed, se3mlp_net** out)
 	auto* net = new se3mlp_net();
 	net->rows = (max_batch + 63) / 64 * 64;
 	const int64_t R = net->rows;
+	const int64_t slices = (R + WG_ROWS - 1) / WG_ROWS;
 	bool ok = true;
 	uint64_t s = seed;
 	for (int l = 0; l < LAYERS && ok; l++) {
 		const size_t nw = (size_t)kOut[l] * kIn[l];
 		ok = net->w[l].alloc(nw * 4) && net->b[l].alloc(kOut[l] * 4) && net->mw[l].alloc(nw * 4) &&
 		     net->vw[l].alloc(nw * 4) && net->mb[l].alloc(kOut[l] * 4) && net->vb[l].alloc(kOut[l] * 4) &&
-		     net->gw[l].alloc(nw * 4) && net->gb[l].alloc(kOut[l] * 4) && net->W[l].alloc(nw * 2) &&
-		     net->WT[l].alloc(nw * 2) && net->H[l].alloc((size_t)R * HID * 2) &&
-		     net->HT[l].alloc((size_t)R * HID * 2) && net->D[l].alloc((size_t)R * HID * 2) &&
-		     net->DT[l].alloc((size_t)R * HID * 2);
+		     net->W[l].alloc(nw * 2) && net->Wf[l].alloc(nw * 2) && net->WTf[l].alloc(nw * 2) &&
+		     (l == LAYERS - 1 || net->HT[l].alloc((size_t)R * HID * 2)) &&
+		     net->DT[l].alloc((size_t)R * kOut[l] * 2) && net->part[l].alloc((size_t)slices * nw * 4) &&
+		     net->bpart[l].alloc((size_t)slices * kOut[l] * 4);
 		if (!ok)
 			break;
 		/* Kaiming-uniform weights over the used fan-in; padding rows and
@@ -832,16 +1280,15 @@ int se3mlp_create(int64_t max_batch, uint64_t seed, se3mlp_net** out)
 			}
 		ok = hipMemcpy(net->w[l].p, hw.data(), nw * 4, hipMemcpyHostToDevice) == hipSuccess;
 	}
-	ok = ok && net->X.alloc((size_t)R * IN * 2) && net->XT.alloc((size_t)R * IN * 2) &&
-	     net->Z.alloc((size_t)R * OUT * 4) && net->loss.alloc((size_t)R * 4) && net->lsum.alloc(4) &&
-	     net->part.alloc((size_t)((R + SPLIT_K - 1) / SPLIT_K) * HID * HID * 4);
+	ok = ok && net->XT.alloc((size_t)R * IN * 2) && net->loss.alloc((size_t)R * 4) && net->lsum.alloc(4);
 	if (!ok) {
 		delete net;
 		return fail(SE3MLP_ENOMEM, "se3mlp_create: device allocation failed");
 	}
 	for (int l = 0; l < LAYERS; l++) {
 		hipLaunchKernelGGL(k_cast, dim3(blocks((int64_t)kOut[l] * kIn[l])), dim3(256), 0, nullptr, net->w[l].as<float>(),
-		                   kOut[l], kIn[l], net->W[l].as<uint16_t>(), net->WT[l].as<uint16_t>());
+		                   kOut[l], kIn[l], net->W[l].as<uint16_t>(), net->Wf[l].as<uint16_t>(),
+		                   net->WTf[l].as<uint16_t>());
 	}
 	if (hipDeviceSynchronize() != hipSuccess) {
 		delete net;
@@ -894,7 +1341,7 @@ int se3mlp_set_params(se3mlp_net* net, int layer, const float* w, const float* b
 		HCHK(hipMemcpy(net->b[layer].p, b, (size_t)kOut[layer] * 4, hipMemcpyHostToDevice));
 	hipLaunchKernelGGL(k_cast, dim3(blocks((int64_t)kOut[layer] * kIn[layer])), dim3(256), 0, nullptr,
 	                   net->w[layer].as<float>(), kOut[layer], kIn[layer], net->W[layer].as<uint16_t>(),
-	                   net->WT[layer].as<uint16_t>());
+	                   net->Wf[layer].as<uint16_t>(), net->WTf[layer].as<uint16_t>());
 	HCHK(hipDeviceSynchronize());
 	return SE3MLP_OK;
 }
@@ -982,75 +1429,76 @@ int se3mlp_train_step_device(se3mlp_net* net, const double* d_states, const uint
 	std::lock_guard<std::mutex> lk(net->mu);
 	hipStream_t st = (hipStream_t)stream;
 	const int64_t rows = (n + 63) / 64 * 64;
-	hipLaunchKernelGGL(k_features, dim3(blocks(rows)), dim3(256), 0, st, d_states, n, rows, net->X.as<uint16_t>(),
-	                   net->XT.as<uint16_t>());
-	HCHK(hipGetLastError());
-	int rc = forward_rows(net, rows, true, st);
-	if (rc)
-		return rc;
-	/* output gradient into D[4] / DT[4] (32 columns) */
-	hipLaunchKernelGGL(k_xent, dim3(blocks(rows)), dim3(256), 0, st, net->Z.as<float>(), n, rows, d_labels,
-	                   net->D[4].as<uint16_t>(), net->DT[4].as<uint16_t>(), net->loss.as<float>());
-	HCHK(hipGetLastError());
-	for (int l = LAYERS - 1; l >= 0; l--) {
-		const int no = kOut[l], ni = kIn[l];
-		const uint16_t* inT = l == 0 ? net->XT.as<uint16_t>() : net->HT[l - 1].as<uint16_t>();
-		/* weight gradient: gW[no x ni] = D_l^T (no x rows) . X_l (rows x ni),
-		 * split over the rows */
-		{
-			Gemm g{net->DT[l].as<uint16_t>(), inT, rows, rows, no, ni, (int)rows, nullptr, 0, nullptr, 0,
-			       nullptr, 0, nullptr, 0, nullptr, 0};
-			const int parts = (int)((rows + SPLIT_K - 1) / SPLIT_K);
-			hipLaunchKernelGGL(k_gemm_nt_part, dim3((unsigned)((ni + 63) / 64), (unsigned)((no + 63) / 64), (unsigned)parts),
-			                   dim3(256), 0, st, g, net->part.as<float>());
-			hipLaunchKernelGGL(k_sum_parts, dim3(blocks((int64_t)no * ni / 4, 64)), dim3(64), 0, st,
-			                   net->part.as<float>(), parts, no, ni, net->gw[l].as<float>(), (int64_t)ni);
-		}
-		hipLaunchKernelGGL(k_rowsum_bf16, dim3((unsigned)no), dim3(256), 0, st, net->DT[l].as<uint16_t>(), rows, no,
-		                   net->gb[l].as<float>());
-		HCHK(hipGetLastError());
-		if (l > 0) {
-			/* input gradient through ReLU: D_{l-1} = (D_l . W_l) * [H_{l-1} > 0] */
-			HCHK(gemm(st, net->D[l].as<uint16_t>(), no, net->WT[l].as<uint16_t>(), no, (int)rows, ni, no, nullptr, 0,
-			          net->H[l - 1].as<uint16_t>(), HID, nullptr, 0, net->D[l - 1].as<uint16_t>(), HID,
-			          net->DT[l - 1].as<uint16_t>(), rows));
-		}
+	/* forward, loss and pre-activation gradients */
+	TrainArgs ta{};
+	ta.q = d_states;
+	ta.label = d_labels;
+	ta.n = n;
+	ta.rows = rows;
+	for (int l = 0; l < LAYERS; l++) {
+		ta.Wf[l] = net->Wf[l].as<uint16_t>();
+		ta.WTf[l] = net->WTf[l].as<uint16_t>();
+		ta.b[l] = net->b[l].as<float>();
+		ta.DT[l] = net->DT[l].as<uint16_t>();
+		if (l < LAYERS - 1)
+			ta.HT[l] = net->HT[l].as<uint16_t>();
 	}
+	ta.XT = net->XT.as<uint16_t>();
+	ta.loss = net->loss.as<float>();
+	hipLaunchKernelGGL(k_mlp_train, dim3((unsigned)(rows / TM)), dim3(256), TRAIN_LDS, st, ta);
+	HCHK(hipGetLastError());
+	/* weight and bias gradient partials per slice of WG_ROWS rows */
+	const int parts = (int)((rows + WG_ROWS - 1) / WG_ROWS);
+	WgradArgs wg{};
+	wg.rows = rows;
+	int nblk = 0;
+	for (int l = 0; l < LAYERS; l++) {
+		wg.A[l] = net->DT[l].as<uint16_t>();
+		wg.B[l] = l == 0 ? net->XT.as<uint16_t>() : net->HT[l - 1].as<uint16_t>();
+		wg.P[l] = net->part[l].as<float>();
+		wg.Pb[l] = net->bpart[l].as<float>();
+		wg.M[l] = kOut[l];
+		wg.N[l] = kIn[l];
+		wg.first[l] = nblk;
+		nblk += ((kOut[l] + 63) / 64) * ((kIn[l] + 63) / 64) * parts;
+	}
+	wg.first[LAYERS] = nblk;
+	hipLaunchKernelGGL(k_wgrad, dim3((unsigned)nblk), dim3(256), WG_LDS, st, wg);
+	HCHK(hipGetLastError());
 	net->step++;
 	const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
 	const float c1 = 1.0f / (1.0f - std::pow(b1, (float)net->step));
 	const float c2 = 1.0f / (1.0f - std::pow(b2, (float)net->step));
 	{
 		AdamSet a{};
-		int64_t at = 0;
+		int64_t biggest = 0;
 		for (int l = 0; l < LAYERS; l++) {
 			/* tensor 2l: the weights of layer l, tensor 2l + 1 its biases */
 			const int tw = 2 * l, tb = 2 * l + 1;
 			a.w[tw] = net->w[l].as<float>();
-			a.g[tw] = net->gw[l].as<float>();
+			a.g[tw] = net->part[l].as<float>();
 			a.m[tw] = net->mw[l].as<float>();
 			a.v[tw] = net->vw[l].as<float>();
 			a.W[tw] = net->W[l].as<uint16_t>();
-			a.WT[tw] = net->WT[l].as<uint16_t>();
+			a.Wf[tw] = net->Wf[l].as<uint16_t>();
+			a.WTf[tw] = net->WTf[l].as<uint16_t>();
 			a.rows[tw] = kOut[l];
 			a.cols[tw] = kIn[l];
 			a.used[tw] = kUsed[l];
-			a.start[tw] = at;
-			at += (int64_t)kOut[l] * kIn[l];
+			biggest = std::max(biggest, (int64_t)kOut[l] * kIn[l]);
 			a.w[tb] = net->b[l].as<float>();
-			a.g[tb] = net->gb[l].as<float>();
+			a.g[tb] = net->bpart[l].as<float>();
 			a.m[tb] = net->mb[l].as<float>();
 			a.v[tb] = net->vb[l].as<float>();
 			a.W[tb] = nullptr;
-			a.WT[tb] = nullptr;
+			a.Wf[tb] = nullptr;
+			a.WTf[tb] = nullptr;
 			a.rows[tb] = kOut[l];
 			a.cols[tb] = 1;
 			a.used[tb] = kUsed[l];
-			a.start[tb] = at;
-			at += kOut[l];
 		}
-		a.start[ADAM_TENSORS] = at;
-		hipLaunchKernelGGL(k_adam, dim3(blocks(at)), dim3(256), 0, st, a, lr, b1, b2, eps, c1, c2);
+		a.parts = parts;
+		hipLaunchKernelGGL(k_adam, dim3(blocks(biggest), ADAM_TENSORS), dim3(256), 0, st, a, lr, b1, b2, eps, c1, c2);
 		HCHK(hipGetLastError());
 	}
 	if (loss_out) {

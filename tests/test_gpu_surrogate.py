@@ -112,11 +112,14 @@ def test_forward_matches_reference(torch_cuda):
     assert np.abs(p.cpu().numpy() - pf).max() <= 1e-2
 
 
-def test_adam_step_follows_gradient_sign(torch_cuda):
+@pytest.mark.parametrize("n", [4096, 3001])
+def test_adam_step_follows_gradient_sign(torch_cuda, n):
+    # 3001 rows: a padded last block and a last slice of the weight-gradient
+    # reduction that is not a multiple of its chunk
     torch = torch_cuda
     net = Surrogate(max_batch=4096, seed=7)
     params = net.params()
-    qs = W.random_states(4096, seed=8)
+    qs = W.random_states(n, seed=8)
     env, rob = W.torus_scene(0)
     uw = pyosr.create_unit_world_from_arrays(env, rob)
     y = uw.is_valid_states(qs).astype(np.uint8)
@@ -129,8 +132,10 @@ def test_adam_step_follows_gradient_sign(torch_cuda):
     G = np.zeros((len(qs), params[-1][0].shape[0]))
     G[:, 0], G[:, 1] = g0, -g0
     grads = [None] * len(params)
+    bgrads = [None] * len(params)
     for l in range(len(params) - 1, -1, -1):
         grads[l] = G.T @ acts[l]
+        bgrads[l] = G.sum(axis=0)
         if l:
             G = (G @ bf16(params[l][0])) * (acts[l] > 0)
     dq = torch.from_numpy(qs).cuda()
@@ -149,6 +154,14 @@ def test_adam_step_follows_gradient_sign(torch_cuda):
         # padding stays zero
         r_used, c_used = net.shapes[l][2], net.shapes[l][3]
         assert not w1[r_used:].any() and not w1[:, c_used:].any()
+    for l, ((_, b0), (_, b1)) in enumerate(zip(params, after)):
+        g = bgrads[l]
+        big = np.abs(g) > 1e-2 * np.abs(g).max()
+        step = (b1 - b0)[big]
+        assert np.all(np.abs(np.abs(step) - lr) <= 1e-5), l
+        agree = np.mean(np.sign(step) == -np.sign(g[big]))
+        assert agree >= 0.99, (l, agree)
+        assert not b1[net.shapes[l][2]:].any()
 
 
 def test_training_learns_torus_labels(torch_cuda):
