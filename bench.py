@@ -876,6 +876,10 @@ def run_surrogate(args, torch, pyosr, W):
     sel = S.selected_mask(r, n_screen)
     recall, precision = S.recall_precision(r, truth)
     flops_sample = 2 * (12 * 256 + 3 * 256 * 256 + 256 * 2)
+    # one training row: forward, gradient back to layer 0's pre-activation
+    # (through the head and W3..W1) and the weight gradients, over the used
+    # (unpadded) layer sizes
+    flops_train_row = 2 * flops_sample + 2 * (256 * 2 + 3 * 256 * 256)
     rate_plain = free_plain / (plain_ms * 1e-3)
     rate_scr = free_scr / (scr_ms * 1e-3)
     extra = {
@@ -883,7 +887,11 @@ def run_surrogate(args, torch, pyosr, W):
         "cpu_baseline": None,
         "surrogate": {
             "labels": n_train, "label_ms": round(label_ms, 2), "train_epochs": args.train_epochs,
-            "train_steps": steps_done, "train_s": round(train_s, 2), "loss_first_epoch": round(losses[0], 4),
+            "train_steps": steps_done, "train_s": round(train_s, 3),
+            "train_step_us": round(train_s / max(steps_done, 1) * 1e6, 1),
+            "train_tflops": round(steps_done * 16384 * flops_train_row / max(train_s, 1e-9) / 1e12, 1),
+            "train_mfma_frac_bf16": round(steps_done * 16384 * flops_train_row / max(train_s, 1e-9) / 2.5e15, 4),
+            "loss_first_epoch": round(losses[0], 4),
             "loss_last_epoch": round(losses[-1], 4), "threshold": thr, "recall": round(recall, 4),
             "precision": round(precision, 4), "accepted": int(sel.sum().item()), "free_true": free_plain,
             "free_verified_screened": free_scr,

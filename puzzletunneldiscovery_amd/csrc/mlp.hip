@@ -238,7 +238,8 @@ __global__ void k_features(const double* __restrict__ q, int64_t n, int64_t rows
 	}
 }
 
-__global__ void k_sum_f32(const float* __restrict__ x, int64_t n, float* __restrict__ out)
+/* out[0] = scale * (x[0] + ... + x[n - 1]) */
+__global__ void k_sum_f32(const float* __restrict__ x, int64_t n, float scale, float* __restrict__ out)
 {
 	__shared__ float part[256];
 	float s = 0.0f;
@@ -252,7 +253,7 @@ __global__ void k_sum_f32(const float* __restrict__ x, int64_t n, float* __restr
 		__syncthreads();
 	}
 	if (threadIdx.x == 0)
-		out[0] = part[0];
+		out[0] = part[0] * scale;
 }
 
 /* Adam on the fp32 master copies of every parameter tensor of the network
@@ -609,33 +610,38 @@ __global__ void __launch_bounds__(256, 2) k_mlp_infer(const double* __restrict__
 /* ---- fused training pass ---------------------------------------------------- */
 /*
  * Forward, loss and backward of one training batch down to every layer's
- * pre-activation gradient, 64 rows per block of four wavefronts, in one
- * launch (a batch of 16,384 rows is 256 blocks, one per CU).  All four
- * hidden activation tiles of the block stay in LDS (hb[l], 64 rows x 256
- * features, rows padded as in k_mlp_infer; 141 KB with the output
- * gradient): layer l reads hb[l - 1] and writes hb[l], so a layer's
- * epilogue needs no barrier before it, and the gradient pass that produces
- * D_l reads the ReLU mask [H_l > 0] from hb[l] and overwrites it with D_l,
- * which the next gradient pass reads.  The features sit in hb[1] until
- * layer 1 writes it.  Wave w computes features
- * [64w, 64w + 64) of a layer for all 64 rows (2 x 2 MFMA tiles, weights as
- * the A operand, so lane (r, half) of tile (h, t) holds features 64w + 32h +
- * 8g + 4 half + [0, 4) of row 32t + r in elements 4g + [0, 4)).
+ * pre-activation gradient, TM = 64 rows per block of four wavefronts, in
+ * one launch (a batch of 16,384 rows is 256 blocks, one per CU).
+ *
+ * All four hidden activation tiles of the block stay in LDS (hb[l], 64 rows
+ * x 256 features, rows padded as in k_mlp_infer; 145 KB with the output
+ * gradient and the biases): layer l reads hb[l - 1] and writes hb[l], so a
+ * layer's epilogue needs no barrier before it, and the gradient pass that
+ * produces D_l reads the ReLU mask [H_l > 0] from hb[l] and overwrites it
+ * with D_l, which the next gradient pass reads.  The features sit in hb[1]
+ * until layer 1 writes it.  Wave w computes features [64w, 64w + 64) of a
+ * layer for all rows (2 x TM/32 MFMA tiles, weights as the A operand, so
+ * lane (r, half) of tile (h, t) holds features 64w + 32h + 8g + 4 half +
+ * [0, 4) of row 32t + r in elements 4g + [0, 4)).
  *
  * The nine matrix passes (four hidden layers, the head, then the gradient
  * through W4, W3, W2 and W1) read their A fragments from wf[16][2], one per
  * k-step and feature half; the moment a pass has used fragment j it loads
  * fragment j of the next pass into the same registers, so every weight load
- * has a whole pass to arrive.
+ * has a whole pass to arrive.  The weights are fragment-major (frag_at).
  *
- * Written for the weight gradients: the bf16 features XT [IN][rows], hidden
- * activations HT[l] [HID][rows] (after the ReLU) and pre-activation
- * gradients DT[l] [kOut[l]][rows], laid out by act_at, plus the per-row
- * loss.  The weights are read fragment-major too.  The loss is the two-class softmax cross-entropy on logit columns 0 (free)
- * and 1 (colliding), label 1 = free, averaged over the n real rows; rows in
- * [n, rows) get zero gradient.
+ * Written for the weight gradients, each copied out of LDS once its tile is
+ * complete: the bf16 features XT [IN][rows], hidden activations HT[l]
+ * [HID][rows] (after the ReLU) and pre-activation gradients DT[l]
+ * [kOut[l]][rows], laid out by act_at; and the per-row loss.  The loss is
+ * the two-class softmax cross-entropy on logit columns 0 (free) and 1
+ * (colliding), label 1 = free, averaged over the n real rows; rows in [n,
+ * rows) get zero gradient.
  */
-enum { TM = 64, DZLD = OUT + 8 };
+#ifndef SE3MLP_TRAIN_TM
+#define SE3MLP_TRAIN_TM 64 /* 32 (two blocks per CU) took 32.4 against 27.5 us: each block reads all weights */
+#endif
+enum { TM = SE3MLP_TRAIN_TM, TT = TM / 32, DZLD = OUT + 8 };
 static const size_t TRAIN_LDS =
     (size_t)(LAYERS - 1) * TM * ALD * 2 + (size_t)TM * DZLD * 2 + 2 * 64 * 4 + (size_t)(LAYERS - 1) * HID * 4;
 
@@ -680,7 +686,7 @@ __device__ __forceinline__ void bias_tile(const float* b, int nb, int lane, f32x
  * the next pass has beyond NK are loaded first.  H_NEXT = 1 loads only nx0.
  */
 template <int NK, int NK_NEXT, int H_NEXT>
-__device__ __forceinline__ void train_pass(bf16x8 (&wf)[16][2], f32x16 (&c)[2][2], const uint16_t* in, int ld,
+__device__ __forceinline__ void train_pass(bf16x8 (&wf)[16][2], f32x16 (&c)[2][TT], const uint16_t* in, int ld,
                                            const uint16_t* nx0, const uint16_t* nx1, int r, int kq)
 {
 #pragma unroll
@@ -689,26 +695,31 @@ __device__ __forceinline__ void train_pass(bf16x8 (&wf)[16][2], f32x16 (&c)[2][2
 		if (H_NEXT == 2)
 			wf[j][1] = ld8(nx1 + 512 * j);
 	}
-	/* the B fragments of step j + 1 are read from LDS before step j's MFMAs */
-	bf16x8 a0 = ld8(in + r * ld + kq), a1 = ld8(in + (32 + r) * ld + kq);
+	/* the B fragments of step j + 1 are read from LDS before step j's MFMAs
+	 * (reading them 2 or 3 steps ahead measured the same) */
+	bf16x8 b[TT];
+#pragma unroll
+	for (int t = 0; t < TT; t++)
+		b[t] = ld8(in + (32 * t + r) * ld + kq);
 #pragma unroll
 	for (int j = 0; j < NK; j++) {
-		bf16x8 n0 = a0, n1 = a1;
-		if (j + 1 < NK) {
-			n0 = ld8(in + r * ld + 16 * (j + 1) + kq);
-			n1 = ld8(in + (32 + r) * ld + 16 * (j + 1) + kq);
-		}
-		c[0][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wf[j][0], a0, c[0][0], 0, 0, 0);
-		c[0][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wf[j][0], a1, c[0][1], 0, 0, 0);
-		c[1][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wf[j][1], a0, c[1][0], 0, 0, 0);
-		c[1][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wf[j][1], a1, c[1][1], 0, 0, 0);
+		bf16x8 nb[TT];
+#pragma unroll
+		for (int t = 0; t < TT; t++)
+			nb[t] = j + 1 < NK ? ld8(in + (32 * t + r) * ld + 16 * (j + 1) + kq) : b[t];
+#pragma unroll
+		for (int h = 0; h < 2; h++)
+#pragma unroll
+			for (int t = 0; t < TT; t++)
+				c[h][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wf[j][h], b[t], c[h][t], 0, 0, 0);
 		if (j < NK_NEXT) {
 			wf[j][0] = ld8(nx0 + 512 * j);
 			if (H_NEXT == 2)
 				wf[j][1] = ld8(nx1 + 512 * j);
 		}
-		a0 = n0;
-		a1 = n1;
+#pragma unroll
+		for (int t = 0; t < TT; t++)
+			b[t] = nb[t];
 	}
 }
 
@@ -752,13 +763,13 @@ __device__ __forceinline__ void copy_out(const uint16_t* tile, int ld, uint16_t*
 
 /* hidden layer epilogue: bf16 rounding and ReLU as in k_mlp_infer, the
  * activations to the layer's LDS tile */
-__device__ __forceinline__ void train_fwd_epi(const f32x16 (&c)[2][2], uint16_t* hl, int wave, int lane)
+__device__ __forceinline__ void train_fwd_epi(const f32x16 (&c)[2][TT], uint16_t* hl, int wave, int lane)
 {
 	const int r = lane & 31, half = lane >> 5;
 #pragma unroll
 	for (int h = 0; h < 2; h++) {
 #pragma unroll
-		for (int t = 0; t < 2; t++) {
+		for (int t = 0; t < TT; t++) {
 			const f32x16& v = c[h][t];
 #pragma unroll
 			for (int g = 0; g < 4; g++) {
@@ -787,13 +798,13 @@ __device__ __forceinline__ uint32_t nonzero_halves(uint32_t x)
 
 /* gradient epilogue: round to bf16, zero where the layer's activation in hl
  * is not positive, and overwrite the activation with the gradient */
-__device__ __forceinline__ void train_bwd_epi(const f32x16 (&c)[2][2], uint16_t* hl, int wave, int lane)
+__device__ __forceinline__ void train_bwd_epi(const f32x16 (&c)[2][TT], uint16_t* hl, int wave, int lane)
 {
 	const int r = lane & 31, half = lane >> 5;
 #pragma unroll
 	for (int h = 0; h < 2; h++) {
 #pragma unroll
-		for (int t = 0; t < 2; t++) {
+		for (int t = 0; t < TT; t++) {
 			const f32x16& v = c[h][t];
 #pragma unroll
 			for (int g = 0; g < 4; g++) {
@@ -812,7 +823,7 @@ __device__ __forceinline__ void train_bwd_epi(const f32x16 (&c)[2][2], uint16_t*
 	}
 }
 
-__global__ void __launch_bounds__(256, 1) k_mlp_train(TrainArgs a)
+__global__ void __launch_bounds__(256, 64 / TM) k_mlp_train(TrainArgs a)
 {
 	extern __shared__ __align__(16) uint16_t lds[];
 	uint16_t* hb[LAYERS - 1];
@@ -835,7 +846,14 @@ __global__ void __launch_bounds__(256, 1) k_mlp_train(TrainArgs a)
 	};
 
 	bf16x8 wf[16][2];
-	f32x16 c[2][2], pb[2];
+	f32x16 c[2][TT], pb[2];
+	auto clear = [&]() {
+#pragma unroll
+		for (int h = 0; h < 2; h++)
+#pragma unroll
+			for (int t = 0; t < TT; t++)
+				c[h][t] = f32x16{};
+	};
 #pragma unroll
 	for (int j = 0; j < IN / 16; j++) {
 		wf[j][0] = ld8(arow(a.Wf[0], IN, 0) + 512 * j);
@@ -846,8 +864,8 @@ __global__ void __launch_bounds__(256, 1) k_mlp_train(TrainArgs a)
 		bias[l * HID + threadIdx.x] = a.b[l][threadIdx.x];
 	/* what the head needs from global memory, loaded now */
 	const float head_bias = a.b[LAYERS - 1][r];
-	const int64_t head_row = row0 + 32 * (wave & 1) + (lane & 31);
-	const bool head_real = wave < 2 && lane < 32 && head_row < a.n;
+	const int64_t head_row = row0 + 32 * wave + (lane & 31);
+	const bool head_real = wave < TT && lane < 32 && head_row < a.n;
 	const float head_y = head_real && a.label[head_row] ? 1.0f : 0.0f;
 	for (int i = threadIdx.x; i < TM * DZLD / 2; i += 256)
 		reinterpret_cast<uint32_t*>(dz)[i] = 0u;
@@ -901,7 +919,9 @@ __global__ void __launch_bounds__(256, 1) k_mlp_train(TrainArgs a)
 		bias_tile(bias + l * HID, 64 * wave, lane, pb);
 #pragma unroll
 		for (int h = 0; h < 2; h++)
-			c[h][0] = c[h][1] = pb[h];
+#pragma unroll
+			for (int t = 0; t < TT; t++)
+				c[h][t] = pb[h];
 		if (l == 0)
 			train_pass<IN / 16, HID / 16, 2>(wf, c, xs, ALD, arow(a.Wf[1], HID, 0), arow(a.Wf[1], HID, 1), r, kq);
 		else if (l < LAYERS - 2)
@@ -914,9 +934,9 @@ __global__ void __launch_bounds__(256, 1) k_mlp_train(TrainArgs a)
 		copy_out<HID>(hb[l], ALD, a.HT[l], rows, row0);
 	}
 
-	/* head and loss: waves 0 and 1 take rows 32 wave + [0, 32); every wave
-	 * then loads the A fragments of the first gradient pass (WT4, K = 32) */
-	if (wave < 2) {
+	/* head and loss: wave w < TT takes rows 32 w + [0, 32); every wave then
+	 * loads the A fragments of the first gradient pass (WT4, K = 32) */
+	if (wave < TT) {
 		f32x16 ch = {};
 		const uint16_t* h3 = hb[LAYERS - 2];
 #pragma unroll
@@ -966,7 +986,7 @@ __global__ void __launch_bounds__(256, 1) k_mlp_train(TrainArgs a)
 	copy_out<OUT>(dz, DZLD, a.DT[LAYERS - 1], rows, row0);
 
 	/* D3 = (D4 . W4) * [H3 > 0] from dz, into hb[3] */
-	c[0][0] = c[0][1] = c[1][0] = c[1][1] = f32x16{};
+	clear();
 	train_pass<OUT / 16, HID / 16, 2>(wf, c, dz, DZLD, arow(a.WTf[3], HID, 0), arow(a.WTf[3], HID, 1), r, kq);
 	train_bwd_epi(c, hb[3], wave, lane);
 	__syncthreads();
@@ -974,7 +994,7 @@ __global__ void __launch_bounds__(256, 1) k_mlp_train(TrainArgs a)
 	/* D_{l-1} = (D_l . W_l) * [H_{l-1} > 0] from hb[l] into hb[l - 1] */
 #pragma unroll
 	for (int l = 3; l >= 1; l--) {
-		c[0][0] = c[0][1] = c[1][0] = c[1][1] = f32x16{};
+		clear();
 		if (l > 1)
 			train_pass<HID / 16, HID / 16, 2>(wf, c, hb[l], ALD, arow(a.WTf[l - 1], HID, 0),
 			                                  arow(a.WTf[l - 1], HID, 1), r, kq);
@@ -1005,7 +1025,7 @@ __global__ void __launch_bounds__(256, 1) k_mlp_train(TrainArgs a)
 #define SE3MLP_WG_XCD 1
 #endif
 #ifndef SE3MLP_WG_KCH
-#define SE3MLP_WG_KCH 4 /* 8 took 17.0 against 16.4 us per step */
+#define SE3MLP_WG_KCH 4 /* 8 took 17.0 against 16.4 us per launch */
 #endif
 enum { WG_ROWS = 2048, WG_KCH = SE3MLP_WG_KCH };
 static const size_t WG_LDS = (size_t)4 * 64 * 64 * 4 + (size_t)4 * 64 * 4;
@@ -1421,13 +1441,12 @@ int se3mlp_screen_segments_device(se3mlp_net* net, const double* d_from, const d
 	return SE3MLP_OK;
 }
 
-int se3mlp_train_step_device(se3mlp_net* net, const double* d_states, const uint8_t* d_labels, int64_t n, float lr,
-                             float* loss_out, void* stream)
+namespace {
+/* Queues one training step on st; if d_loss is set, the batch's mean loss
+ * (before the step) goes there.  The caller holds net->mu. */
+int train_step(se3mlp_net* net, const double* d_states, const uint8_t* d_labels, int64_t n, float lr, float* d_loss,
+               hipStream_t st)
 {
-	if (!net || n <= 0 || n > net->rows || !d_states || !d_labels)
-		return fail(SE3MLP_EINVAL, "se3mlp_train_step_device: batch must hold 1..max_batch states");
-	std::lock_guard<std::mutex> lk(net->mu);
-	hipStream_t st = (hipStream_t)stream;
 	const int64_t rows = (n + 63) / 64 * 64;
 	/* forward, loss and pre-activation gradients */
 	TrainArgs ta{};
@@ -1501,12 +1520,45 @@ int se3mlp_train_step_device(se3mlp_net* net, const double* d_states, const uint
 		hipLaunchKernelGGL(k_adam, dim3(blocks(biggest), ADAM_TENSORS), dim3(256), 0, st, a, lr, b1, b2, eps, c1, c2);
 		HCHK(hipGetLastError());
 	}
-	if (loss_out) {
-		hipLaunchKernelGGL(k_sum_f32, dim3(1), dim3(256), 0, st, net->loss.as<float>(), n, net->lsum.as<float>());
+	if (d_loss) {
+		hipLaunchKernelGGL(k_sum_f32, dim3(1), dim3(256), 0, st, net->loss.as<float>(), n, 1.0f / (float)n, d_loss);
 		HCHK(hipGetLastError());
+	}
+	return SE3MLP_OK;
+}
+} // namespace
+
+int se3mlp_train_step_device(se3mlp_net* net, const double* d_states, const uint8_t* d_labels, int64_t n, float lr,
+                             float* loss_out, void* stream)
+{
+	if (!net || n <= 0 || n > net->rows || !d_states || !d_labels)
+		return fail(SE3MLP_EINVAL, "se3mlp_train_step_device: batch must hold 1..max_batch states");
+	std::lock_guard<std::mutex> lk(net->mu);
+	hipStream_t st = (hipStream_t)stream;
+	const int rc = train_step(net, d_states, d_labels, n, lr, loss_out ? net->lsum.as<float>() : nullptr, st);
+	if (rc)
+		return rc;
+	if (loss_out) {
 		HCHK(hipMemcpyAsync(loss_out, net->lsum.p, 4, hipMemcpyDeviceToHost, st));
 		HCHK(hipStreamSynchronize(st));
-		*loss_out /= (float)n;
+	}
+	return SE3MLP_OK;
+}
+
+int se3mlp_train_epoch_device(se3mlp_net* net, const double* d_states, const uint8_t* d_labels, int64_t n,
+                              int64_t batch, float lr, int loss_every, float* d_losses, void* stream)
+{
+	if (!net || batch <= 0 || batch > net->rows || n < batch || !d_states || !d_labels || (d_losses && loss_every <= 0))
+		return fail(SE3MLP_EINVAL, "se3mlp_train_epoch_device: need max_batch >= batch > 0, n >= batch and, "
+		                           "with a loss buffer, loss_every > 0");
+	std::lock_guard<std::mutex> lk(net->mu);
+	hipStream_t st = (hipStream_t)stream;
+	const int64_t steps = n / batch;
+	for (int64_t k = 0; k < steps; k++) {
+		float* slot = (d_losses && k % loss_every == 0) ? d_losses + k / loss_every : nullptr;
+		const int rc = train_step(net, d_states + 7 * k * batch, d_labels + k * batch, batch, lr, slot, st);
+		if (rc)
+			return rc;
 	}
 	return SE3MLP_OK;
 }

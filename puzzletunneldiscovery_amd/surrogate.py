@@ -89,23 +89,22 @@ class Surrogate:
         [n, 7], labels uint8 [n], 1 = free).  Returns the mean loss per epoch."""
         import torch
         n = d_states.shape[0]
-        batch = min(batch, self.max_batch)
+        batch = min(batch, self.max_batch, n)
         g = torch.Generator(device="cuda")
         g.manual_seed(seed)
         stream = torch.cuda.current_stream().cuda_stream
         losses = []
+        every = 16  # the loss of every 16th step enters the epoch's mean
         for ep in range(epochs):
             perm = torch.randperm(n, device="cuda", generator=g)
             xs = d_states[perm].contiguous()
             ys = d_labels[perm].contiguous()
-            tot, cnt = 0.0, 0
-            for off in range(0, n - batch + 1, batch):
-                want = (off // batch) % 16 == 0
-                l = self.train_step_device(xs[off:].data_ptr(), ys[off:].data_ptr(), batch, lr, want, stream)
-                if want:
-                    tot += l
-                    cnt += 1
-            losses.append(tot / max(cnt, 1))
+            steps = n // batch
+            d_loss = torch.empty((steps + every - 1) // every, dtype=torch.float32, device="cuda")
+            # the whole epoch is queued by the library, one host call
+            N.check_mlp(N.lib().se3mlp_train_epoch_device(self._h, xs.data_ptr(), ys.data_ptr(), n, batch,
+                                                        ctypes.c_float(lr), every, d_loss.data_ptr(), stream))
+            losses.append(float(d_loss.mean().item()))
             if log:
                 log(f"epoch {ep}: loss {losses[-1]:.4f}")
         torch.cuda.synchronize()

@@ -179,3 +179,30 @@ def test_training_learns_torus_labels(torch_cuda):
     acc = np.mean((p > 0.5) == (y[260_000:] == 1))
     assert acc > 0.85, acc
     assert np.array_equal(p > 0.5, m > 0)
+
+
+def test_train_epoch_equals_consecutive_steps(torch_cuda):
+    # the epoch entry point queues the same steps as calling the single-step
+    # one per batch, so parameters and sampled losses agree to the bit
+    torch = torch_cuda
+    import ctypes
+    from puzzletunneldiscovery_amd import _native as N
+    qs = W.random_states(3 * 2048 + 100, seed=21)
+    y = (np.random.default_rng(22).random(len(qs)) > 0.4).astype(np.uint8)
+    dq = torch.from_numpy(qs).cuda()
+    dy = torch.from_numpy(y).cuda()
+    a = Surrogate(max_batch=2048, seed=23)
+    b = Surrogate(max_batch=2048, seed=23)
+    d_loss = torch.full((2,), -1.0, dtype=torch.float32, device="cuda")
+    st = torch.cuda.current_stream().cuda_stream
+    N.check_mlp(N.lib().se3mlp_train_epoch_device(a._h, dq.data_ptr(), dy.data_ptr(), len(qs), 2048,
+                                                ctypes.c_float(2e-3), 2, d_loss.data_ptr(), st))
+    want = []
+    for k in range(3):
+        l = b.train_step_device(dq[k * 2048:].data_ptr(), dy[k * 2048:].data_ptr(), 2048, 2e-3, k % 2 == 0, st)
+        if k % 2 == 0:
+            want.append(l)
+    torch.cuda.synchronize()
+    assert np.array_equal(d_loss.cpu().numpy(), np.array(want, np.float32))
+    for (wa, ba), (wb, bb) in zip(a.params(), b.params()):
+        assert np.array_equal(wa, wb) and np.array_equal(ba, bb)
