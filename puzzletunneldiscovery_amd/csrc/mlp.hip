@@ -257,17 +257,17 @@ __global__ void k_sum_f32(const float* __restrict__ x, int64_t n, float scale, f
 }
 
 /* Adam on the fp32 master copies of every parameter tensor of the network
- * in one launch (grid row t takes tensor t); refreshes the bf16 copies of the weight tensors: W [rows][cols]
- * row-major (inference), and W and its transpose fragment-major (training).  Entries outside `used` rows stay zero.  The
- * gradient of element i is the sum of `parts` partials g[t][z * size + i]
- * (size = rows x cols), added in z order. */
+ * in one launch (grid row t takes tensor t); refreshes the bf16 copies of
+ * the weight tensors, W [rows][cols] and its transpose, both fragment-major.
+ * Entries outside `used` rows stay zero.  The gradient of element i is the
+ * sum of `parts` partials g[t][z * size + i] (size = rows x cols), added in
+ * z order. */
 enum { ADAM_TENSORS = 2 * LAYERS };
 struct AdamSet {
 	float* w[ADAM_TENSORS];
 	const float* g[ADAM_TENSORS];
 	float* m[ADAM_TENSORS];
 	float* v[ADAM_TENSORS];
-	uint16_t* W[ADAM_TENSORS];
 	uint16_t* Wf[ADAM_TENSORS];
 	uint16_t* WTf[ADAM_TENSORS];
 	int rows[ADAM_TENSORS], cols[ADAM_TENSORS], used[ADAM_TENSORS];
@@ -295,23 +295,23 @@ __global__ void k_adam(AdamSet a, float lr, float b1, float b2, float eps, float
 		a.v[t][i] = vi;
 		w[i] -= lr * (mi * c1) / (sqrtf(vi * c2) + eps);
 	}
-	if (a.W[t]) {
+	if (a.Wf[t]) {
 		const uint16_t h = f2bf(w[i]);
-		a.W[t][i] = h;
 		a.Wf[t][frag_at(r, c, cols)] = h;
 		a.WTf[t][frag_at(c, r, a.rows[t])] = h;
 	}
 }
 
-__global__ void k_cast(const float* __restrict__ w, int rows, int cols, uint16_t* __restrict__ W,
-                       uint16_t* __restrict__ Wf, uint16_t* __restrict__ WTf)
+/* the bf16 copies of one fp32 weight tensor [rows][cols]: W and its
+ * transpose, fragment-major */
+__global__ void k_cast(const float* __restrict__ w, int rows, int cols, uint16_t* __restrict__ Wf,
+                       uint16_t* __restrict__ WTf)
 {
 	const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= (int64_t)rows * cols)
 		return;
 	const uint16_t h = f2bf(w[i]);
 	const int r = (int)(i / cols), c = (int)(i % cols);
-	W[i] = h;
 	Wf[frag_at(r, c, cols)] = h;
 	WTf[frag_at(c, r, rows)] = h;
 }
@@ -342,7 +342,7 @@ enum { FM = 128, ALD = HID + 8 };
 static const size_t MLP_LDS = (size_t)FM * ALD * sizeof(uint16_t);
 
 struct MlpW {
-	const uint16_t* W[LAYERS];
+	const uint16_t* W[LAYERS]; /* [kOut][kIn], fragment-major (frag_at) */
 	const float* b[LAYERS];
 };
 
@@ -448,13 +448,17 @@ __global__ void __launch_bounds__(256, 2) k_mlp_infer(const double* __restrict__
 	 * its first two k-steps, loaded while the previous layer finishes */
 	f32x16 pbt[2];
 	bf16x8 pw[4];
+	/* this lane's part of the first fragment of weight rows nb + [0, 32) of
+	 * layer l (fragment-major: k-step k is 32 k values further, rows nb + 32
+	 * + [0, 32) are 32 K values further) */
+	auto wrow = [&](int l, int K) { return w.W[l] + (int64_t)nb * K + lane * 8; };
 	auto prefetch = [&](int l) {
 		const int K = l == 0 ? IN : HID;
-		const uint16_t* W0 = w.W[l] + (int64_t)(nb + r) * K + kq;
+		const uint16_t* W0 = wrow(l, K);
 		pw[0] = ld8(W0);
 		pw[1] = ld8(W0 + 32 * K);
-		pw[2] = ld8(W0 + 16); /* K >= 32 */
-		pw[3] = ld8(W0 + 32 * K + 16);
+		pw[2] = ld8(W0 + 32 * 16); /* K >= 32 */
+		pw[3] = ld8(W0 + 32 * K + 32 * 16);
 		const float* bl = w.b[l] + nb + 4 * (lane >> 5);
 #pragma unroll
 		for (int h = 0; h < 2; h++) {
@@ -471,7 +475,7 @@ __global__ void __launch_bounds__(256, 2) k_mlp_infer(const double* __restrict__
 	prefetch(0);
 	for (int l = 0; l < LAYERS - 1; l++) {
 		const int K = l == 0 ? IN : HID;
-		const uint16_t* W0 = w.W[l] + (int64_t)(nb + r) * K + kq;
+		const uint16_t* W0 = wrow(l, K);
 		const uint16_t* W1 = W0 + 32 * K;
 		/* c[h][t]: features nb + 32h + [0, 32) x states 32t + [0, 32).
 		 * The first k-step adds its products to the bias tile, so no
@@ -487,8 +491,8 @@ __global__ void __launch_bounds__(256, 2) k_mlp_infer(const double* __restrict__
 		for (int k = 16; k < K; k += 16) {
 			bf16x8 nw0 = w0, nw1 = w1;
 			if (k + 16 < K) {
-				nw0 = ld8(W0 + k + 16);
-				nw1 = ld8(W1 + k + 16);
+				nw0 = ld8(W0 + 32 * (k + 16));
+				nw1 = ld8(W1 + 32 * (k + 16));
 			}
 #pragma unroll
 			for (int t = 0; t < 4; t++) {
@@ -530,12 +534,12 @@ __global__ void __launch_bounds__(256, 2) k_mlp_infer(const double* __restrict__
 		__syncthreads();
 	}
 	{
-		const uint16_t* W = w.W[LAYERS - 1] + (int64_t)r * HID + kq;
+		const uint16_t* W = w.W[LAYERS - 1] + lane * 8;
 		/* all sixteen weight fragments at once: one load latency, not 16 */
 		bf16x8 hw[HID / 16];
 #pragma unroll
 		for (int j = 0; j < HID / 16; j++)
-			hw[j] = ld8(W + 16 * j);
+			hw[j] = ld8(W + 512 * j);
 		f32x16 c = {};
 #pragma unroll
 		for (int j = 0; j < HID / 16; j++)
@@ -1215,8 +1219,7 @@ struct se3mlp_net {
 	int64_t rows = 0; /* batch capacity, a multiple of 64 */
 	int step = 0;
 	Buf w[LAYERS], b[LAYERS], mw[LAYERS], vw[LAYERS], mb[LAYERS], vb[LAYERS];
-	Buf W[LAYERS];              /* bf16 weights, row-major (inference) */
-	Buf Wf[LAYERS], WTf[LAYERS]; /* bf16 weights and their transposes, fragment-major (training) */
+	Buf Wf[LAYERS], WTf[LAYERS]; /* bf16 weights and their transposes, fragment-major */
 	Buf XT;                     /* bf16 features, feature-major */
 	Buf HT[LAYERS - 1];         /* bf16 hidden activations, feature-major */
 	Buf DT[LAYERS];             /* bf16 pre-activation gradients, feature-major */
@@ -1283,7 +1286,7 @@ int se3mlp_create(int64_t max_batch, uint64_t seed, se3mlp_net** out)
 		const size_t nw = (size_t)kOut[l] * kIn[l];
 		ok = net->w[l].alloc(nw * 4) && net->b[l].alloc(kOut[l] * 4) && net->mw[l].alloc(nw * 4) &&
 		     net->vw[l].alloc(nw * 4) && net->mb[l].alloc(kOut[l] * 4) && net->vb[l].alloc(kOut[l] * 4) &&
-		     net->W[l].alloc(nw * 2) && net->Wf[l].alloc(nw * 2) && net->WTf[l].alloc(nw * 2) &&
+		     net->Wf[l].alloc(nw * 2) && net->WTf[l].alloc(nw * 2) &&
 		     (l == LAYERS - 1 || net->HT[l].alloc((size_t)R * HID * 2)) &&
 		     net->DT[l].alloc((size_t)R * kOut[l] * 2) && net->part[l].alloc((size_t)slices * nw * 4) &&
 		     net->bpart[l].alloc((size_t)slices * kOut[l] * 4);
@@ -1307,7 +1310,7 @@ int se3mlp_create(int64_t max_batch, uint64_t seed, se3mlp_net** out)
 	}
 	for (int l = 0; l < LAYERS; l++) {
 		hipLaunchKernelGGL(k_cast, dim3(blocks((int64_t)kOut[l] * kIn[l])), dim3(256), 0, nullptr, net->w[l].as<float>(),
-		                   kOut[l], kIn[l], net->W[l].as<uint16_t>(), net->Wf[l].as<uint16_t>(),
+		                   kOut[l], kIn[l], net->Wf[l].as<uint16_t>(),
 		                   net->WTf[l].as<uint16_t>());
 	}
 	if (hipDeviceSynchronize() != hipSuccess) {
@@ -1360,8 +1363,8 @@ int se3mlp_set_params(se3mlp_net* net, int layer, const float* w, const float* b
 	if (b)
 		HCHK(hipMemcpy(net->b[layer].p, b, (size_t)kOut[layer] * 4, hipMemcpyHostToDevice));
 	hipLaunchKernelGGL(k_cast, dim3(blocks((int64_t)kOut[layer] * kIn[layer])), dim3(256), 0, nullptr,
-	                   net->w[layer].as<float>(), kOut[layer], kIn[layer], net->W[layer].as<uint16_t>(),
-	                   net->Wf[layer].as<uint16_t>(), net->WTf[layer].as<uint16_t>());
+	                   net->w[layer].as<float>(), kOut[layer], kIn[layer], net->Wf[layer].as<uint16_t>(),
+	                   net->WTf[layer].as<uint16_t>());
 	HCHK(hipDeviceSynchronize());
 	return SE3MLP_OK;
 }
@@ -1376,7 +1379,7 @@ int se3mlp_forward_device(se3mlp_net* net, const double* d_states, int64_t n, fl
 		return SE3MLP_OK;
 	MlpW mw;
 	for (int l = 0; l < LAYERS; l++) {
-		mw.W[l] = net->W[l].as<uint16_t>();
+		mw.W[l] = net->Wf[l].as<uint16_t>();
 		mw.b[l] = net->b[l].as<float>();
 	}
 	/* one block per 128 states; grids above 2^31 blocks are split */
@@ -1405,7 +1408,7 @@ int se3mlp_screen_states_device(se3mlp_net* net, const double* d_states, int64_t
 		return SE3MLP_OK;
 	MlpW mw;
 	for (int l = 0; l < LAYERS; l++) {
-		mw.W[l] = net->W[l].as<uint16_t>();
+		mw.W[l] = net->Wf[l].as<uint16_t>();
 		mw.b[l] = net->b[l].as<float>();
 	}
 	const ScreenArgs sa{nullptr, 1, threshold, d_idx, d_count, 0};
@@ -1430,7 +1433,7 @@ int se3mlp_screen_segments_device(se3mlp_net* net, const double* d_from, const d
 		return SE3MLP_OK;
 	MlpW mw;
 	for (int l = 0; l < LAYERS; l++) {
-		mw.W[l] = net->W[l].as<uint16_t>();
+		mw.W[l] = net->Wf[l].as<uint16_t>();
 		mw.b[l] = net->b[l].as<float>();
 	}
 	const int64_t rows = n * points;
@@ -1498,7 +1501,6 @@ int train_step(se3mlp_net* net, const double* d_states, const uint8_t* d_labels,
 			a.g[tw] = net->part[l].as<float>();
 			a.m[tw] = net->mw[l].as<float>();
 			a.v[tw] = net->vw[l].as<float>();
-			a.W[tw] = net->W[l].as<uint16_t>();
 			a.Wf[tw] = net->Wf[l].as<uint16_t>();
 			a.WTf[tw] = net->WTf[l].as<uint16_t>();
 			a.rows[tw] = kOut[l];
@@ -1509,7 +1511,6 @@ int train_step(se3mlp_net* net, const double* d_states, const uint8_t* d_labels,
 			a.g[tb] = net->bpart[l].as<float>();
 			a.m[tb] = net->mb[l].as<float>();
 			a.v[tb] = net->vb[l].as<float>();
-			a.W[tb] = nullptr;
 			a.Wf[tb] = nullptr;
 			a.WTf[tb] = nullptr;
 			a.rows[tb] = kOut[l];
