@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -338,6 +339,9 @@ __global__ void k_cast(const float* __restrict__ w, int rows, int cols, uint16_t
  * activation fragments a step ahead, or weights two steps ahead, measured
  * slower.
  */
+#ifndef SE3MLP_INFER_AHEAD
+#define SE3MLP_INFER_AHEAD 3 /* 1 / 2 / 3 steps ahead: 5.73 / 5.73 / 5.72 ms per 16M samples (5.89 with a rolled k-loop) */
+#endif
 enum { FM = 128, ALD = HID + 8 };
 static const size_t MLP_LDS = (size_t)FM * ALD * sizeof(uint16_t);
 
@@ -445,20 +449,26 @@ __global__ void __launch_bounds__(256, 2) k_mlp_infer(const double* __restrict__
 	const int nb = 64 * wave;
 	/* layer l's bias tile (lane (r, half) of tile h holds features nb + 32h
 	 * + 8(i/4) + 4 half + (i%4) in element i) and the weight fragments of
-	 * its first two k-steps, loaded while the previous layer finishes */
+	 * its first PD + 1 k-steps, loaded while the previous layer finishes;
+	 * within a layer, the fragments of step j + PD + 1 are loaded into the
+	 * slot that step j has just used */
+	constexpr int PD = SE3MLP_INFER_AHEAD;
 	f32x16 pbt[2];
-	bf16x8 pw[4];
+	bf16x8 wq[PD + 1][2];
 	/* this lane's part of the first fragment of weight rows nb + [0, 32) of
-	 * layer l (fragment-major: k-step k is 32 k values further, rows nb + 32
+	 * layer l (fragment-major: k-step j is 512 values further, rows nb + 32
 	 * + [0, 32) are 32 K values further) */
 	auto wrow = [&](int l, int K) { return w.W[l] + (int64_t)nb * K + lane * 8; };
 	auto prefetch = [&](int l) {
 		const int K = l == 0 ? IN : HID;
 		const uint16_t* W0 = wrow(l, K);
-		pw[0] = ld8(W0);
-		pw[1] = ld8(W0 + 32 * K);
-		pw[2] = ld8(W0 + 32 * 16); /* K >= 32 */
-		pw[3] = ld8(W0 + 32 * K + 32 * 16);
+#pragma unroll
+		for (int j = 0; j <= PD; j++) {
+			if (16 * j < K) {
+				wq[j][0] = ld8(W0 + 512 * j);
+				wq[j][1] = ld8(W0 + 32 * K + 512 * j);
+			}
+		}
 		const float* bl = w.b[l] + nb + 4 * (lane >> 5);
 #pragma unroll
 		for (int h = 0; h < 2; h++) {
@@ -472,45 +482,35 @@ __global__ void __launch_bounds__(256, 2) k_mlp_infer(const double* __restrict__
 			}
 		}
 	};
-	prefetch(0);
-	for (int l = 0; l < LAYERS - 1; l++) {
-		const int K = l == 0 ? IN : HID;
-		const uint16_t* W0 = wrow(l, K);
-		const uint16_t* W1 = W0 + 32 * K;
-		/* c[h][t]: features nb + 32h + [0, 32) x states 32t + [0, 32).
-		 * The first k-step adds its products to the bias tile, so no
-		 * accumulator is initialised separately. */
-		f32x16 c[2][4];
-		bf16x8 w0 = pw[2], w1 = pw[3];
+	/* c[h][t]: features nb + 32h + [0, 32) x states 32t + [0, 32).  The
+	 * first k-step adds its products to the bias tile, so no accumulator is
+	 * initialised separately. */
+	f32x16 c[2][4];
+	auto layer = [&](auto nk_c, int l) {
+		constexpr int NK = decltype(nk_c)::value;
+		const uint16_t* W0 = wrow(l, 16 * NK);
+		const uint16_t* W1 = W0 + 32 * 16 * NK;
 #pragma unroll
-		for (int t = 0; t < 4; t++) {
-			const bf16x8 a = ld8(act + (32 * t + r) * ALD + kq);
-			c[0][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pw[0], a, pbt[0], 0, 0, 0);
-			c[1][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pw[1], a, pbt[1], 0, 0, 0);
-		}
-		for (int k = 16; k < K; k += 16) {
-			bf16x8 nw0 = w0, nw1 = w1;
-			if (k + 16 < K) {
-				nw0 = ld8(W0 + 32 * (k + 16));
-				nw1 = ld8(W1 + 32 * (k + 16));
-			}
+		for (int j = 0; j < NK; j++) {
+			const int sl = j % (PD + 1);
 #pragma unroll
 			for (int t = 0; t < 4; t++) {
-				const bf16x8 a = ld8(act + (32 * t + r) * ALD + k + kq);
-				c[0][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w0, a, c[0][t], 0, 0, 0);
-				c[1][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w1, a, c[1][t], 0, 0, 0);
+				const bf16x8 a = ld8(act + (32 * t + r) * ALD + 16 * j + kq);
+				c[0][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wq[sl][0], a, j == 0 ? pbt[0] : c[0][t], 0, 0, 0);
+				c[1][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wq[sl][1], a, j == 0 ? pbt[1] : c[1][t], 0, 0, 0);
 			}
-			w0 = nw0;
-			w1 = nw1;
+			if (j + PD + 1 < NK) {
+				wq[sl][0] = ld8(W0 + 512 * (j + PD + 1));
+				wq[sl][1] = ld8(W1 + 512 * (j + PD + 1));
+			}
 		}
-		if (l + 1 < LAYERS - 1)
-			prefetch(l + 1);
-		__syncthreads(); /* every wave has read this layer's input */
-		/* lane (r, half) of tile (h, t): state 32t + r.  Each pair is
-		 * rounded to bf16 (v_cvt_pk_bf16_f32, nearest even) and then put
-		 * through the ReLU as two int16: a bf16 with its sign bit set is a
-		 * negative int16, so max(., 0) maps it to +0 and keeps the rest
-		 * (the same values as rounding max(x, 0)). */
+	};
+	/* lane (r, half) of tile (h, t): state 32t + r.  Each pair is rounded to
+	 * bf16 (v_cvt_pk_bf16_f32, nearest even) and then put through the ReLU
+	 * as two int16: a bf16 with its sign bit set is a negative int16, so
+	 * max(., 0) maps it to +0 and keeps the rest (the same values as
+	 * rounding max(x, 0)). */
+	auto epilogue = [&]() {
 #pragma unroll
 		for (int h = 0; h < 2; h++) {
 #pragma unroll
@@ -531,6 +531,19 @@ __global__ void __launch_bounds__(256, 2) k_mlp_infer(const double* __restrict__
 				}
 			}
 		}
+	};
+	prefetch(0);
+	layer(std::integral_constant<int, IN / 16>{}, 0);
+	prefetch(1);
+	__syncthreads(); /* every wave has read this layer's input */
+	epilogue();
+	__syncthreads();
+	for (int l = 1; l < LAYERS - 1; l++) {
+		layer(std::integral_constant<int, HID / 16>{}, l);
+		if (l + 1 < LAYERS - 1)
+			prefetch(l + 1);
+		__syncthreads();
+		epilogue();
 		__syncthreads();
 	}
 	{
