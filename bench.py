@@ -800,7 +800,8 @@ def _screen_segments(args, torch, W, uw, net, sq, sy, timer, stream):
     state = {}
 
     def screened():
-        state["r"] = S.screen_segments(uw, net, starts, ends, VD, args.threshold, points=8, stream=sp)
+        state["r"] = S.screen_segments(uw, net, starts, ends, VD, args.threshold, points=8, stream=sp,
+                                       out=state.get("r"))
 
     _, scr_ms = timer.run(screened, args.steps, args.warmup)
     uw.check_device()
@@ -865,7 +866,7 @@ def run_surrogate(args, torch, pyosr, W):
     def screened():
         # the product's prescreen: selection fused into the inference kernel,
         # exact re-verification of the device-side list, no host round trip
-        state["r"] = S.screen_states(uw, net, sq, thr, stream=sp)
+        state["r"] = S.screen_states(uw, net, sq, thr, stream=sp, out=state.get("r"))
 
     wall_scr, scr_ms = timer.run(screened, args.steps, args.warmup)
     uw.check_device()

@@ -140,7 +140,26 @@ class _one_stream:
         return False
 
 
-def screen_states(uw, net, d_states, threshold=0.5, stream=None, d_pfree=None):
+def _outputs(out, n, device, sp):
+    """idx, count and free for n items, count and free cleared on stream sp:
+    the caller's buffers (out, a result of an earlier call with at least n
+    items) or new ones."""
+    import torch
+    with torch.cuda.stream(torch.cuda.ExternalStream(sp, device=device)):
+        if out is None:
+            return (torch.empty(max(n, 1), dtype=torch.int32, device=device),
+                    torch.zeros(1, dtype=torch.int32, device=device),
+                    torch.zeros(n, dtype=torch.uint8, device=device))
+        idx, count, free = out["idx"], out["count"], out["free"]
+        if idx.numel() < max(n, 1) or free.numel() < n:
+            raise ValueError("out holds buffers for fewer items than this call")
+        count.zero_()
+        free = free[:n]
+        free.zero_()
+        return idx, count, free
+
+
+def screen_states(uw, net, d_states, threshold=0.5, stream=None, d_pfree=None, out=None):
     """The prescreen of the sampler loops (src/GP/uw_random.py:46-52,
     src/GP/pipeline/keyconf.py:127-131): the network selects the states with
     P(free) > threshold in one kernel, and the exact engine re-verifies every
@@ -150,13 +169,11 @@ def screen_states(uw, net, d_states, threshold=0.5, stream=None, d_pfree=None):
     synchronisation.  d_states: float64 CUDA tensor [n, 7].  Returns a dict of
     CUDA tensors: idx (int32 [n], the first count entries are the selected
     states, in no fixed order), count (uint32 [1]) and free (uint8 [n], 1 for
-    a selected state that is exactly free, else 0)."""
-    import torch
+    a selected state that is exactly free, else 0).  A sampler loop passes
+    the previous result as `out` to reuse its buffers."""
     n = int(d_states.shape[0])
-    idx = torch.empty(max(n, 1), dtype=torch.int32, device=d_states.device)
-    count = torch.zeros(1, dtype=torch.int32, device=d_states.device)
-    free = torch.zeros(n, dtype=torch.uint8, device=d_states.device)
     with _one_stream(stream) as sp:
+        idx, count, free = _outputs(out, n, d_states.device, sp)
         net.screen_states_device(d_states.data_ptr(), n, threshold, idx.data_ptr(), count.data_ptr(),
                                  d_pfree.data_ptr() if d_pfree is not None else None, stream=sp)
         if n:
@@ -165,19 +182,16 @@ def screen_states(uw, net, d_states, threshold=0.5, stream=None, d_pfree=None):
     return {"idx": idx, "count": count, "free": free}
 
 
-def screen_segments(uw, net, d_from, d_to, initial_verify_delta, threshold=0.5, points=8, stream=None):
+def screen_segments(uw, net, d_from, d_to, initial_verify_delta, threshold=0.5, points=8, stream=None, out=None):
     """The prescreen of candidate edges (the visibility checks of
     src/GP/pipeline/solve2.py:164-233): a segment is selected when the
     network gives P(free) > threshold at `points` states along it, and every
     selected segment is re-verified exactly (UnitWorld::isValidTransition,
     lib/osr/unit_world.cc:477-486).  Returns idx, count and free as
-    screen_states does, per segment."""
-    import torch
+    screen_states does, per segment, and reuses `out` the same way."""
     n = int(d_from.shape[0])
-    idx = torch.empty(max(n, 1), dtype=torch.int32, device=d_from.device)
-    count = torch.zeros(1, dtype=torch.int32, device=d_from.device)
-    free = torch.zeros(n, dtype=torch.uint8, device=d_from.device)
     with _one_stream(stream) as sp:
+        idx, count, free = _outputs(out, n, d_from.device, sp)
         net.screen_segments_device(d_from.data_ptr(), d_to.data_ptr(), n, points, threshold, idx.data_ptr(),
                                    count.data_ptr(), stream=sp)
         if n:

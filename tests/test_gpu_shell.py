@@ -116,6 +116,18 @@ def test_shell_prescreen(shell):
     recall, precision = S.recall_precision(r, torch.from_numpy(truth).cuda())
     _flush(f"prescreen: states recall {recall:.4f} precision {precision:.4f}")
     assert recall >= 0.9 and precision > truth.mean()
+    # a second screen reusing the first one's buffers (cleared on its stream)
+    # selects the same states; a smaller batch reuses them too
+    idx0 = np.sort(r["idx"][:k].cpu().numpy())
+    r2 = S.screen_states(uw, net, dq, 0.5, out=r)
+    torch.cuda.synchronize()
+    assert int(r2["count"].item()) == k
+    assert np.array_equal(np.sort(r2["idx"][:k].cpu().numpy()), idx0)
+    assert np.array_equal(r2["free"].cpu().numpy().astype(bool), sel & truth)
+    r3 = S.screen_states(uw, net, dq[:1000], 0.5, out=r2)
+    torch.cuda.synchronize()
+    assert int(r3["count"].item()) == int(sel[:1000].sum())
+    assert np.array_equal(r3["free"].cpu().numpy().astype(bool), (sel & truth)[:1000])
     _flush("prescreen: 2,000 segments")
     a = _free(uw, 2000, 314)
     steps = W.lognormal_steps(len(a), seed=315)
