@@ -1054,7 +1054,10 @@ __global__ void __launch_bounds__(256, 64 / TM) k_mlp_train(TrainArgs a)
 #ifndef SE3MLP_WG_KCH
 #define SE3MLP_WG_KCH 4 /* 8 took 17.0 against 16.4 us per launch */
 #endif
-enum { WG_ROWS = 2048, WG_KCH = SE3MLP_WG_KCH };
+#ifndef SE3MLP_WG_ROWS
+#define SE3MLP_WG_ROWS 2048 /* 1024 / 2048 / 4096: 55.8 / 50.2 / 54.5 us per training step */
+#endif
+enum { WG_ROWS = SE3MLP_WG_ROWS, WG_KCH = SE3MLP_WG_KCH };
 static const size_t WG_LDS = (size_t)4 * 64 * 64 * 4 + (size_t)4 * 64 * 4;
 
 struct WgradArgs {
