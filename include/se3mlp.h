@@ -80,12 +80,15 @@ int se3mlp_screen_segments_device(se3mlp_net* net, const double* d_from, const d
 int se3mlp_train_step_device(se3mlp_net* net, const double* d_states, const uint8_t* d_labels, int64_t n, float lr,
                              float* loss_out, void* stream);
 /* n / batch such steps (batch <= max_batch) over the consecutive batches of
- * the given states, queued without waiting for the device.  If d_losses is
- * not NULL, step k with k % loss_every == 0 writes its batch's mean loss
- * (before the step) to d_losses[k / loss_every], device memory the caller
- * sizes to ceil((n / batch) / loss_every) floats. */
-int se3mlp_train_epoch_device(se3mlp_net* net, const double* d_states, const uint8_t* d_labels, int64_t n,
-                              int64_t batch, float lr, int loss_every, float* d_losses, void* stream);
+ * the given n states, queued without waiting for the device.  With d_perm
+ * (a permutation of 0 .. n-1, int64, on the device) batch k is the states
+ * d_perm[k * batch .. (k + 1) * batch), read in place (a shuffled epoch
+ * without a shuffled copy).  If d_losses is not NULL, step k with k %
+ * loss_every == 0 writes its batch's mean loss (before the step) to
+ * d_losses[k / loss_every], device memory the caller sizes to
+ * ceil((n / batch) / loss_every) floats. */
+int se3mlp_train_epoch_device(se3mlp_net* net, const double* d_states, const uint8_t* d_labels, const int64_t* d_perm,
+                              int64_t n, int64_t batch, float lr, int loss_every, float* d_losses, void* stream);
 
 /* The matrix-core kernel alone, for testing: C[M x N] (fp32, row stride ldc)
  * = A[M x K] . Bt[N x K]^T with bf16 A and Bt (row strides lda, ldb).  M and N

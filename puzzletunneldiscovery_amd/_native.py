@@ -124,7 +124,7 @@ MLP_SIGNATURES = {
     "se3mlp_screen_states_device": [vp, vp, i64, ctypes.c_float, vp, vp, vp, vp],
     "se3mlp_screen_segments_device": [vp, vp, vp, i64, cint, ctypes.c_float, vp, vp, vp],
     "se3mlp_train_step_device": [vp, vp, vp, i64, ctypes.c_float, c_float_p, vp],
-    "se3mlp_train_epoch_device": [vp, vp, vp, i64, i64, ctypes.c_float, ctypes.c_int, vp, vp],
+    "se3mlp_train_epoch_device": [vp, vp, vp, vp, i64, i64, ctypes.c_float, ctypes.c_int, vp, vp],
     "se3mlp_gemm_device": [vp, i64, vp, i64, cint, cint, cint, vp, i64, vp],
     "se3mlp_features_device": [vp, i64, vp, vp],
 }

@@ -239,24 +239,6 @@ __global__ void k_features(const double* __restrict__ q, int64_t n, int64_t rows
 	}
 }
 
-/* out[0] = scale * (x[0] + ... + x[n - 1]) */
-__global__ void k_sum_f32(const float* __restrict__ x, int64_t n, float scale, float* __restrict__ out)
-{
-	__shared__ float part[256];
-	float s = 0.0f;
-	for (int64_t i = threadIdx.x; i < n; i += 256)
-		s += x[i];
-	part[threadIdx.x] = s;
-	__syncthreads();
-	for (int off = 128; off > 0; off >>= 1) {
-		if ((int)threadIdx.x < off)
-			part[threadIdx.x] += part[threadIdx.x + off];
-		__syncthreads();
-	}
-	if (threadIdx.x == 0)
-		out[0] = part[0] * scale;
-}
-
 /* Adam on the fp32 master copies of every parameter tensor of the network
  * in one launch (grid row t takes tensor t); refreshes the bf16 copies of
  * the weight tensors, W [rows][cols] and its transpose, both fragment-major.
@@ -659,8 +641,8 @@ __global__ void __launch_bounds__(256, 2) k_mlp_infer(const double* __restrict__
 #define SE3MLP_TRAIN_TM 64 /* 32 (two blocks per CU) took 32.4 against 27.5 us: each block reads all weights */
 #endif
 enum { TM = SE3MLP_TRAIN_TM, TT = TM / 32, DZLD = OUT + 8 };
-static const size_t TRAIN_LDS =
-    (size_t)(LAYERS - 1) * TM * ALD * 2 + (size_t)TM * DZLD * 2 + 2 * 64 * 4 + (size_t)(LAYERS - 1) * HID * 4;
+static const size_t TRAIN_LDS = (size_t)(LAYERS - 1) * TM * ALD * 2 + (size_t)TM * DZLD * 2 + 2 * 64 * 4 +
+                                (size_t)(LAYERS - 1) * HID * 4 + 16;
 
 struct TrainArgs {
 	const double* q;
@@ -672,7 +654,8 @@ struct TrainArgs {
 	uint16_t* XT;
 	uint16_t* HT[LAYERS - 1];
 	uint16_t* DT[LAYERS];
-	float* loss;
+	const int64_t* perm; /* row i of the batch is state perm[i] (null: state i) */
+	float* loss_part;    /* null, or per block the sum of its rows' losses */
 };
 
 /* lane (r, half) of bias tile h: features nb + 32h + 8g + 4 half + [0, 4)
@@ -847,6 +830,7 @@ __global__ void __launch_bounds__(256, 64 / TM) k_mlp_train(TrainArgs a)
 	uint16_t* dz = lds + (LAYERS - 1) * TM * ALD;        /* output gradient, columns >= 2 zero */
 	float* head = reinterpret_cast<float*>(dz + TM * DZLD); /* [2][64]: the two logits of 32 rows per wave */
 	float* bias = head + 2 * 64;                           /* [4][HID]: the hidden layers' biases */
+	float* head_loss = bias + (LAYERS - 1) * HID;          /* [2]: the loss sums of the head's waves */
 	uint16_t* xs = hb(1);
 	const int wave = threadIdx.x >> 6;
 	const int lane = threadIdx.x & 63;
@@ -880,7 +864,7 @@ __global__ void __launch_bounds__(256, 64 / TM) k_mlp_train(TrainArgs a)
 	const float head_bias = a.b[LAYERS - 1][r];
 	const int64_t head_row = row0 + 32 * wave + (lane & 31);
 	const bool head_real = wave < TT && lane < 32 && head_row < a.n;
-	const float head_y = head_real && a.label[head_row] ? 1.0f : 0.0f;
+	const float head_y = head_real && a.label[a.perm ? a.perm[head_row] : head_row] ? 1.0f : 0.0f;
 	for (int i = threadIdx.x; i < TM * DZLD / 2; i += 256)
 		reinterpret_cast<uint32_t*>(dz)[i] = 0u;
 	if (threadIdx.x < TM) {
@@ -891,7 +875,7 @@ __global__ void __launch_bounds__(256, 64 / TM) k_mlp_train(TrainArgs a)
 		for (int k = 0; k < IN; k++)
 			f[k] = 0.0f;
 		if (i < a.n) {
-			const double* s = a.q + 7 * i;
+			const double* s = a.q + 7 * (a.perm ? a.perm[i] : i);
 			double qw = s[3], qx = s[4], qy = s[5], qz = s[6];
 			const double nn = sqrt((qx * qx + qz * qz) + (qy * qy + qw * qw));
 			if (nn > 0.0) {
@@ -980,10 +964,10 @@ __global__ void __launch_bounds__(256, 64 / TM) k_mlp_train(TrainArgs a)
 		__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
 		__builtin_amdgcn_wave_barrier();
 		__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+		float l = 0.0f;
 		if (lane < 32) {
 			const int s = 32 * wave + lane;
-			const int64_t i = row0 + s;
-			float g0 = 0.0f, l = 0.0f;
+			float g0 = 0.0f;
 			if (head_real) {
 				const float z0 = zs[2 * lane], z1 = zs[2 * lane + 1];
 				const float d = z1 - z0;                   /* log-odds of "colliding" */
@@ -996,8 +980,13 @@ __global__ void __launch_bounds__(256, 64 / TM) k_mlp_train(TrainArgs a)
 			}
 			dz[s * DZLD] = f2bf(g0);
 			dz[s * DZLD + 1] = f2bf(-g0);
-			a.loss[i] = l;
 		}
+		/* the wave's 32 losses added in a fixed butterfly order */
+#pragma unroll
+		for (int off = 16; off > 0; off >>= 1)
+			l += __shfl_xor(l, off);
+		if (lane == 0)
+			head_loss[wave] = l;
 	} else {
 #pragma unroll
 		for (int j = 0; j < OUT / 16; j++) {
@@ -1006,6 +995,8 @@ __global__ void __launch_bounds__(256, 64 / TM) k_mlp_train(TrainArgs a)
 		}
 	}
 	__syncthreads();
+	if (a.loss_part && threadIdx.x == 0)
+		a.loss_part[blockIdx.x] = TT == 2 ? head_loss[0] + head_loss[1] : head_loss[0];
 	copy_out<OUT>(dz, DZLD, a.DT[LAYERS - 1], rows, row0);
 
 	/* D3 = (D4 . W4) * [H3 > 0] from dz, into hb(3) */
@@ -1068,11 +1059,22 @@ struct WgradArgs {
 	int M[LAYERS], N[LAYERS];
 	int first[LAYERS + 1];     /* first block of each layer; first[LAYERS] = grid */
 	int64_t rows;
+	const float* loss_part; /* the training kernel's per-block loss sums */
+	int loss_blocks;
+	float loss_scale;
+	float* loss_out;        /* null, or where block 0 writes loss_scale x their sum */
 };
 
 __global__ void __launch_bounds__(256, 2) k_wgrad(WgradArgs g)
 {
 	extern __shared__ __align__(16) float red[]; /* [4][64][64] tiles, then [4][64] bias sums */
+	if (g.loss_out && blockIdx.x == 0 && threadIdx.x == 0) {
+		/* the batch's mean loss: the training blocks' sums in block order */
+		float sum = 0.0f;
+		for (int k = 0; k < g.loss_blocks; k++)
+			sum += g.loss_part[k];
+		*g.loss_out = sum * g.loss_scale;
+	}
 	int l = 0;
 	while ((int)blockIdx.x >= g.first[l + 1])
 		l++;
@@ -1249,7 +1251,7 @@ struct se3mlp_net {
 	Buf XT;                     /* bf16 features, feature-major */
 	Buf HT[LAYERS - 1];         /* bf16 hidden activations, feature-major */
 	Buf DT[LAYERS];             /* bf16 pre-activation gradients, feature-major */
-	Buf loss, lsum;
+	Buf lpart, lsum; /* per training block loss sums; a one-float loss slot */
 	Buf part[LAYERS], bpart[LAYERS]; /* per row slice: weight and bias gradient partials */
 	hipStream_t stream = nullptr;
 	std::mutex mu;
@@ -1329,7 +1331,7 @@ int se3mlp_create(int64_t max_batch, uint64_t seed, se3mlp_net** out)
 			}
 		ok = hipMemcpy(net->w[l].p, hw.data(), nw * 4, hipMemcpyHostToDevice) == hipSuccess;
 	}
-	ok = ok && net->XT.alloc((size_t)R * IN * 2) && net->loss.alloc((size_t)R * 4) && net->lsum.alloc(4);
+	ok = ok && net->XT.alloc((size_t)R * IN * 2) && net->lpart.alloc((size_t)(R / 32) * 4) && net->lsum.alloc(4);
 	if (!ok) {
 		delete net;
 		return fail(SE3MLP_ENOMEM, "se3mlp_create: device allocation failed");
@@ -1473,8 +1475,8 @@ int se3mlp_screen_segments_device(se3mlp_net* net, const double* d_from, const d
 namespace {
 /* Queues one training step on st; if d_loss is set, the batch's mean loss
  * (before the step) goes there.  The caller holds net->mu. */
-int train_step(se3mlp_net* net, const double* d_states, const uint8_t* d_labels, int64_t n, float lr, float* d_loss,
-               hipStream_t st)
+int train_step(se3mlp_net* net, const double* d_states, const uint8_t* d_labels, const int64_t* d_perm, int64_t n,
+               float lr, float* d_loss, hipStream_t st)
 {
 	const int64_t rows = (n + 63) / 64 * 64;
 	/* forward, loss and pre-activation gradients */
@@ -1492,7 +1494,8 @@ int train_step(se3mlp_net* net, const double* d_states, const uint8_t* d_labels,
 			ta.HT[l] = net->HT[l].as<uint16_t>();
 	}
 	ta.XT = net->XT.as<uint16_t>();
-	ta.loss = net->loss.as<float>();
+	ta.perm = d_perm;
+	ta.loss_part = d_loss ? net->lpart.as<float>() : nullptr;
 	hipLaunchKernelGGL(k_mlp_train, dim3((unsigned)(rows / TM)), dim3(256), TRAIN_LDS, st, ta);
 	HCHK(hipGetLastError());
 	/* weight and bias gradient partials per slice of WG_ROWS rows */
@@ -1511,6 +1514,10 @@ int train_step(se3mlp_net* net, const double* d_states, const uint8_t* d_labels,
 		nblk += ((kOut[l] + 63) / 64) * ((kIn[l] + 63) / 64) * parts;
 	}
 	wg.first[LAYERS] = nblk;
+	wg.loss_part = net->lpart.as<float>();
+	wg.loss_blocks = (int)(rows / TM);
+	wg.loss_scale = 1.0f / (float)n;
+	wg.loss_out = d_loss;
 	hipLaunchKernelGGL(k_wgrad, dim3((unsigned)nblk), dim3(256), WG_LDS, st, wg);
 	HCHK(hipGetLastError());
 	net->step++;
@@ -1547,10 +1554,6 @@ int train_step(se3mlp_net* net, const double* d_states, const uint8_t* d_labels,
 		hipLaunchKernelGGL(k_adam, dim3(blocks(biggest), ADAM_TENSORS), dim3(256), 0, st, a, lr, b1, b2, eps, c1, c2);
 		HCHK(hipGetLastError());
 	}
-	if (d_loss) {
-		hipLaunchKernelGGL(k_sum_f32, dim3(1), dim3(256), 0, st, net->loss.as<float>(), n, 1.0f / (float)n, d_loss);
-		HCHK(hipGetLastError());
-	}
 	return SE3MLP_OK;
 }
 } // namespace
@@ -1562,7 +1565,7 @@ int se3mlp_train_step_device(se3mlp_net* net, const double* d_states, const uint
 		return fail(SE3MLP_EINVAL, "se3mlp_train_step_device: batch must hold 1..max_batch states");
 	std::lock_guard<std::mutex> lk(net->mu);
 	hipStream_t st = (hipStream_t)stream;
-	const int rc = train_step(net, d_states, d_labels, n, lr, loss_out ? net->lsum.as<float>() : nullptr, st);
+	const int rc = train_step(net, d_states, d_labels, nullptr, n, lr, loss_out ? net->lsum.as<float>() : nullptr, st);
 	if (rc)
 		return rc;
 	if (loss_out) {
@@ -1572,8 +1575,8 @@ int se3mlp_train_step_device(se3mlp_net* net, const double* d_states, const uint
 	return SE3MLP_OK;
 }
 
-int se3mlp_train_epoch_device(se3mlp_net* net, const double* d_states, const uint8_t* d_labels, int64_t n,
-                              int64_t batch, float lr, int loss_every, float* d_losses, void* stream)
+int se3mlp_train_epoch_device(se3mlp_net* net, const double* d_states, const uint8_t* d_labels, const int64_t* d_perm,
+                              int64_t n, int64_t batch, float lr, int loss_every, float* d_losses, void* stream)
 {
 	if (!net || batch <= 0 || batch > net->rows || n < batch || !d_states || !d_labels || (d_losses && loss_every <= 0))
 		return fail(SE3MLP_EINVAL, "se3mlp_train_epoch_device: need max_batch >= batch > 0, n >= batch and, "
@@ -1583,7 +1586,8 @@ int se3mlp_train_epoch_device(se3mlp_net* net, const double* d_states, const uin
 	const int64_t steps = n / batch;
 	for (int64_t k = 0; k < steps; k++) {
 		float* slot = (d_losses && k % loss_every == 0) ? d_losses + k / loss_every : nullptr;
-		const int rc = train_step(net, d_states + 7 * k * batch, d_labels + k * batch, batch, lr, slot, st);
+		const int rc = d_perm ? train_step(net, d_states, d_labels, d_perm + k * batch, batch, lr, slot, st)
+		                      : train_step(net, d_states + 7 * k * batch, d_labels + k * batch, nullptr, batch, lr, slot, st);
 		if (rc)
 			return rc;
 	}

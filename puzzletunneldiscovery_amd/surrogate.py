@@ -95,15 +95,17 @@ class Surrogate:
         stream = torch.cuda.current_stream().cuda_stream
         losses = []
         every = 16  # the loss of every 16th step enters the epoch's mean
+        d_states = d_states.contiguous()
+        d_labels = d_labels.contiguous()
         for ep in range(epochs):
             perm = torch.randperm(n, device="cuda", generator=g)
-            xs = d_states[perm].contiguous()
-            ys = d_labels[perm].contiguous()
             steps = n // batch
             d_loss = torch.empty((steps + every - 1) // every, dtype=torch.float32, device="cuda")
-            # the whole epoch is queued by the library, one host call
-            N.check_mlp(N.lib().se3mlp_train_epoch_device(self._h, xs.data_ptr(), ys.data_ptr(), n, batch,
-                                                        ctypes.c_float(lr), every, d_loss.data_ptr(), stream))
+            # the whole epoch is queued by the library, one host call; the
+            # kernels read the states through the permutation
+            N.check_mlp(N.lib().se3mlp_train_epoch_device(self._h, d_states.data_ptr(), d_labels.data_ptr(),
+                                                        perm.data_ptr(), n, batch, ctypes.c_float(lr), every,
+                                                        d_loss.data_ptr(), stream))
             losses.append(float(d_loss.mean().item()))
             if log:
                 log(f"epoch {ep}: loss {losses[-1]:.4f}")

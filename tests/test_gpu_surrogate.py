@@ -195,7 +195,7 @@ def test_train_epoch_equals_consecutive_steps(torch_cuda):
     b = Surrogate(max_batch=2048, seed=23)
     d_loss = torch.full((2,), -1.0, dtype=torch.float32, device="cuda")
     st = torch.cuda.current_stream().cuda_stream
-    N.check_mlp(N.lib().se3mlp_train_epoch_device(a._h, dq.data_ptr(), dy.data_ptr(), len(qs), 2048,
+    N.check_mlp(N.lib().se3mlp_train_epoch_device(a._h, dq.data_ptr(), dy.data_ptr(), None, len(qs), 2048,
                                                 ctypes.c_float(2e-3), 2, d_loss.data_ptr(), st))
     want = []
     for k in range(3):
@@ -205,4 +205,21 @@ def test_train_epoch_equals_consecutive_steps(torch_cuda):
     torch.cuda.synchronize()
     assert np.array_equal(d_loss.cpu().numpy(), np.array(want, np.float32))
     for (wa, ba), (wb, bb) in zip(a.params(), b.params()):
+        assert np.array_equal(wa, wb) and np.array_equal(ba, bb)
+    # a shuffled epoch read through the permutation equals steps over the
+    # shuffled copy
+    perm = torch.randperm(len(qs), device="cuda", generator=torch.Generator(device="cuda").manual_seed(24))
+    c = Surrogate(max_batch=2048, seed=25)
+    d = Surrogate(max_batch=2048, seed=25)
+    N.check_mlp(N.lib().se3mlp_train_epoch_device(c._h, dq.data_ptr(), dy.data_ptr(), perm.data_ptr(), len(qs),
+                                                2048, ctypes.c_float(2e-3), 2, d_loss.data_ptr(), st))
+    gq, gy = dq[perm].contiguous(), dy[perm].contiguous()
+    want = []
+    for k in range(3):
+        l = d.train_step_device(gq[k * 2048:].data_ptr(), gy[k * 2048:].data_ptr(), 2048, 2e-3, k % 2 == 0, st)
+        if k % 2 == 0:
+            want.append(l)
+    torch.cuda.synchronize()
+    assert np.array_equal(d_loss.cpu().numpy(), np.array(want, np.float32))
+    for (wa, ba), (wb, bb) in zip(c.params(), d.params()):
         assert np.array_equal(wa, wb) and np.array_equal(ba, bb)
